@@ -1,0 +1,14 @@
+"""MI355X-native Vamana / FreshVamana index (gfx950 HIP kernels behind a C ABI).
+
+Import with importlib (the directory name is not an identifier):
+
+    vmn = importlib.import_module("cs5013-project_amd")
+    index = vmn.Vamana(base, mode="parth", R=32, L_build=64)
+    ids, dists = index.searchPoints(queries, k=10, L=40)
+"""
+from .vamana import (MODE_FRESH, MODE_PARTH, Params, Vamana, VamanaError, default_params,
+                     groundtruth, lib, recall_at_k)
+from . import datasets
+
+__all__ = ["Vamana", "VamanaError", "Params", "default_params", "groundtruth", "lib",
+           "recall_at_k", "datasets", "MODE_PARTH", "MODE_FRESH"]

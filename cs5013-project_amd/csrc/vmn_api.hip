@@ -1,0 +1,593 @@
+// Host runtime behind the C ABI (include/vamana_mi355x.h): the index handle,
+// device memory, the build / search orchestration and error reporting.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vamana_mi355x.h"
+#include "vmn_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(VMN_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+// Device allocation that grows on demand and is kept across calls.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t want) {
+        if (want <= bytes) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            bytes = 0;
+        }
+        size_t b = std::max<size_t>(want, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+int pad_dim(int d) { return (d + 31) / 32 * 32; }
+
+}  // namespace
+
+struct vmn_index {
+    vmn_params p{};
+    int DP = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    int64_t n = 0, cap = 0;
+    int64_t medoid = -1;
+    bool built = false;
+    DevBuf vecs, adj, del_bits;
+    // scratch
+    DevBuf qpad, out_ids, out_d, stat_hops, stat_ndist, counter;
+    DevBuf cand_ids, cand_d, vis_ids, vis_d, vis_cnt;
+    DevBuf pair_dst, pair_d, rcount, rfill, roff, rscan_tmp, rsrc, rd, rlist;
+    DevBuf centroid, medoid_key;
+    int64_t last_nq = 0;
+};
+
+namespace {
+
+int set_device(const vmn_index* idx) {
+    HIPCHK(hipSetDevice(idx->p.device));
+    return VMN_OK;
+}
+
+// Places `n` rows of dimension D (host or device pointer) as [n x DP] rows in
+// dst (device). Returns the device pointer holding padded rows (may be src
+// itself when it is a device pointer with D == DP).
+int stage_rows(vmn_index* idx, const float* src, int64_t n, int32_t mem, DevBuf& scratch,
+               const float** out) {
+    const int D = idx->p.dim, DP = idx->DP;
+    if (mem == VMN_MEM_DEVICE && D == DP) {
+        *out = src;
+        return VMN_OK;
+    }
+    HIPCHK(scratch.reserve((size_t)n * DP * sizeof(float)));
+    if (mem == VMN_MEM_DEVICE) {
+        HIPCHK(vmn::launch_pad_rows(src, n, D, scratch.as<float>(), DP, idx->stream));
+    } else if (D == DP) {
+        HIPCHK(hipMemcpyAsync(scratch.p, src, (size_t)n * D * sizeof(float), hipMemcpyHostToDevice,
+                              idx->stream));
+    } else {
+        std::vector<float> tmp((size_t)n * DP, 0.0f);
+        for (int64_t i = 0; i < n; ++i) std::memcpy(&tmp[i * DP], src + i * D, D * sizeof(float));
+        HIPCHK(hipMemcpyAsync(scratch.p, tmp.data(), tmp.size() * sizeof(float),
+                              hipMemcpyHostToDevice, idx->stream));
+        HIPCHK(hipStreamSynchronize(idx->stream));
+    }
+    *out = scratch.as<float>();
+    return VMN_OK;
+}
+
+int zero_counter(vmn_index* idx) {
+    HIPCHK(idx->counter.reserve(64));
+    HIPCHK(vmn::launch_fill_u64(idx->counter.as<unsigned long long>(), 1, 0ull, idx->stream));
+    return VMN_OK;
+}
+
+int compute_medoid(vmn_index* idx) {
+    HIPCHK(idx->centroid.reserve((size_t)idx->DP * sizeof(float)));
+    HIPCHK(idx->medoid_key.reserve(64));
+    HIPCHK(vmn::launch_medoid(idx->vecs.as<float>(), idx->n, idx->p.dim, idx->DP,
+                              idx->centroid.as<float>(), nullptr,
+                              idx->medoid_key.as<unsigned long long>(), idx->stream));
+    unsigned long long key = 0;
+    HIPCHK(hipMemcpyAsync(&key, idx->medoid_key.p, sizeof key, hipMemcpyDeviceToHost, idx->stream));
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    idx->medoid = key == ~0ull ? 0 : (int64_t)(uint32_t)(key & 0xffffffffull);
+    return VMN_OK;
+}
+
+int alloc_graph(vmn_index* idx, int64_t n) {
+    const int64_t cap = std::max<int64_t>(idx->p.capacity, n);
+    idx->cap = cap;
+    HIPCHK(idx->vecs.reserve((size_t)cap * idx->DP * sizeof(float)));
+    HIPCHK(idx->adj.reserve((size_t)cap * idx->p.R * sizeof(int32_t)));
+    HIPCHK(idx->del_bits.reserve((size_t)((cap + 31) / 32) * sizeof(uint32_t)));
+    HIPCHK(vmn::launch_fill_i32(idx->del_bits.as<int32_t>(), (cap + 31) / 32, 0, idx->stream));
+    return VMN_OK;
+}
+
+// Search kernel over `nq` padded query rows.
+int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int32_t* out_ids,
+               float* out_d, bool queries_are_nodes, int mode, bool want_vis) {
+    int rc = zero_counter(idx);
+    if (rc) return rc;
+    HIPCHK(idx->stat_hops.reserve((size_t)nq * sizeof(int32_t)));
+    HIPCHK(idx->stat_ndist.reserve((size_t)nq * sizeof(int32_t)));
+    vmn::SearchArgs a{};
+    a.mode = mode;
+    a.DP = idx->DP;
+    a.vecs = idx->vecs.as<float>();
+    a.adj = idx->adj.as<int32_t>();
+    a.del_bits = idx->del_bits.as<uint32_t>();
+    a.W = idx->p.R;
+    a.queries = q;
+    a.nq = nq;
+    a.query_is_node = queries_are_nodes ? 1 : 0;
+    a.node_base = 0;
+    a.start = (int32_t)idx->medoid;
+    a.L = L;
+    a.visited_cap = idx->p.visited_cap;
+    a.max_hops = idx->p.max_hops;
+    a.out_n = out_n;
+    a.out_ids = out_ids;
+    a.out_d = out_d;
+    if (want_vis) {
+        HIPCHK(idx->vis_ids.reserve((size_t)nq * idx->p.P * sizeof(int32_t)));
+        HIPCHK(idx->vis_d.reserve((size_t)nq * idx->p.P * sizeof(float)));
+        HIPCHK(idx->vis_cnt.reserve((size_t)nq * sizeof(int32_t)));
+        a.vis_ids = idx->vis_ids.as<int32_t>();
+        a.vis_d = idx->vis_d.as<float>();
+        a.vis_cnt = idx->vis_cnt.as<int32_t>();
+    }
+    a.P = idx->p.P;
+    a.stat_hops = idx->stat_hops.as<int32_t>();
+    a.stat_ndist = idx->stat_ndist.as<int32_t>();
+    a.counter = idx->counter.as<unsigned long long>();
+    const int64_t want = std::min<int64_t>(nq, (int64_t)idx->cus * 32);
+    HIPCHK(vmn::launch_search(a, (int)std::max<int64_t>(want, 1), idx->stream));
+    idx->last_nq = nq;
+    return VMN_OK;
+}
+
+int prune_grid(const vmn_index* idx, int64_t n) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)idx->cus * 16));
+}
+
+int build_parth(vmn_index* idx) {
+    const int64_t n = idx->n;
+    const int L = idx->p.L_build;
+    HIPCHK(vmn::launch_random_init(idx->adj.as<int32_t>(), n, idx->p.R, idx->p.R, idx->p.seed,
+                                   idx->stream));
+    HIPCHK(idx->cand_ids.reserve((size_t)n * L * sizeof(int32_t)));
+    HIPCHK(idx->cand_d.reserve((size_t)n * L * sizeof(float)));
+    for (int pass = 0; pass < idx->p.passes; ++pass) {
+        int rc = run_search(idx, idx->vecs.as<float>(), n, L, L, idx->cand_ids.as<int32_t>(),
+                            idx->cand_d.as<float>(), true, VMN_MODE_PARTH, false);
+        if (rc) return rc;
+        rc = zero_counter(idx);
+        if (rc) return rc;
+        vmn::PruneArgs a{};
+        a.mode = VMN_MODE_PARTH;
+        a.DP = idx->DP;
+        a.vecs = idx->vecs.as<float>();
+        a.adj = idx->adj.as<int32_t>();
+        a.W = idx->p.R;
+        a.R = idx->p.R;
+        a.alpha = idx->p.alpha[pass];
+        a.n = n;
+        a.node_base = 0;
+        a.cand_ids = idx->cand_ids.as<int32_t>();
+        a.cand_d = idx->cand_d.as<float>();
+        a.Lc = L;
+        a.counter = idx->counter.as<unsigned long long>();
+        HIPCHK(vmn::launch_parth_prune(a, prune_grid(idx, n), idx->stream));
+    }
+    return VMN_OK;
+}
+
+// One FRESH pass over nodes [base, base + cnt): search, out-prune emitting
+// reverse pairs, group pairs by destination, reverse prune.
+int fresh_pass(vmn_index* idx, int64_t base, int64_t cnt, float alpha) {
+    const int R = idx->p.R;
+    const int64_t N = idx->n;
+    int rc = run_search(idx, idx->vecs.as<float>() + base * idx->DP, cnt, idx->p.L_build, 0,
+                        nullptr, nullptr, true, VMN_MODE_FRESH, true);
+    if (rc) return rc;
+    HIPCHK(idx->pair_dst.reserve((size_t)cnt * R * sizeof(int32_t)));
+    HIPCHK(idx->pair_d.reserve((size_t)cnt * R * sizeof(float)));
+    rc = zero_counter(idx);
+    if (rc) return rc;
+    vmn::PruneArgs a{};
+    a.mode = VMN_MODE_FRESH;
+    a.DP = idx->DP;
+    a.vecs = idx->vecs.as<float>();
+    a.adj = idx->adj.as<int32_t>();
+    a.W = R;
+    a.R = R;
+    a.alpha = alpha;
+    a.n = cnt;
+    a.node_base = base;
+    a.vis_ids = idx->vis_ids.as<int32_t>();
+    a.vis_d = idx->vis_d.as<float>();
+    a.vis_cnt = idx->vis_cnt.as<int32_t>();
+    a.P = idx->p.P;
+    a.pair_dst = idx->pair_dst.as<int32_t>();
+    a.pair_d = idx->pair_d.as<float>();
+    a.M = idx->p.M;
+    a.counter = idx->counter.as<unsigned long long>();
+    HIPCHK(vmn::launch_fresh_out_prune(a, prune_grid(idx, cnt), idx->stream));
+
+    // reverse pairs -> CSR by destination
+    const int64_t npairs = cnt * R;
+    HIPCHK(idx->rcount.reserve((size_t)N * sizeof(int32_t)));
+    HIPCHK(idx->rfill.reserve((size_t)N * sizeof(int32_t)));
+    HIPCHK(idx->roff.reserve((size_t)(N + 1) * sizeof(int64_t)));
+    HIPCHK(idx->rscan_tmp.reserve((size_t)((N + 1023) / 1024 + 2) * sizeof(int64_t)));
+    HIPCHK(idx->rsrc.reserve((size_t)npairs * sizeof(int32_t)));
+    HIPCHK(idx->rd.reserve((size_t)npairs * sizeof(float)));
+    HIPCHK(idx->rlist.reserve((size_t)N * sizeof(int64_t)));
+    HIPCHK(vmn::launch_fill_i32(idx->rcount.as<int32_t>(), N, 0, idx->stream));
+    HIPCHK(vmn::launch_fill_i32(idx->rfill.as<int32_t>(), N, 0, idx->stream));
+    HIPCHK(vmn::launch_reverse_count(idx->pair_dst.as<int32_t>(), npairs, idx->rcount.as<int32_t>(),
+                                     idx->stream));
+    HIPCHK(vmn::launch_exclusive_scan(idx->rcount.as<int32_t>(), N, idx->roff.as<int64_t>(),
+                                      idx->rscan_tmp.as<int64_t>(), idx->stream));
+    HIPCHK(vmn::launch_reverse_scatter(idx->pair_dst.as<int32_t>(), idx->pair_d.as<float>(), npairs,
+                                       R, base, idx->roff.as<int64_t>(), idx->rfill.as<int32_t>(),
+                                       idx->rsrc.as<int32_t>(), idx->rd.as<float>(), idx->stream));
+    rc = zero_counter(idx);
+    if (rc) return rc;
+    HIPCHK(vmn::launch_compact_nonzero(idx->rcount.as<int32_t>(), N, idx->rlist.as<int64_t>(),
+                                       idx->counter.as<unsigned long long>(), idx->stream));
+    unsigned long long nlist = 0;
+    HIPCHK(hipMemcpyAsync(&nlist, idx->counter.p, sizeof nlist, hipMemcpyDeviceToHost, idx->stream));
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    rc = zero_counter(idx);
+    if (rc) return rc;
+    a.n = (int64_t)nlist;
+    a.node_list = idx->rlist.as<int64_t>();
+    a.rev_off = idx->roff.as<int64_t>();
+    a.rev_src = idx->rsrc.as<int32_t>();
+    a.rev_d = idx->rd.as<float>();
+    HIPCHK(vmn::launch_fresh_rev_prune(a, prune_grid(idx, (int64_t)nlist), idx->stream));
+    return VMN_OK;
+}
+
+int build_fresh(vmn_index* idx) {
+    HIPCHK(vmn::launch_random_init(idx->adj.as<int32_t>(), idx->n, idx->p.R, idx->p.R, idx->p.seed,
+                                   idx->stream));
+    for (int pass = 0; pass < idx->p.passes; ++pass) {
+        int rc = fresh_pass(idx, 0, idx->n, idx->p.alpha[pass]);
+        if (rc) return rc;
+    }
+    return VMN_OK;
+}
+
+int check_params(const vmn_params* p) {
+    if (!p) return fail(VMN_ERR_ARG, "params is NULL");
+    if (p->dim < 1 || p->dim > 256) return fail(VMN_ERR_ARG, "dim %d not in [1, 256]", p->dim);
+    if (p->R < 1 || p->R > 64) return fail(VMN_ERR_ARG, "R %d not in [1, 64]", p->R);
+    if (p->L_build < 1 || p->L_build > 512) return fail(VMN_ERR_ARG, "L_build %d not in [1, 512]", p->L_build);
+    if (p->passes < 0 || p->passes > 4) return fail(VMN_ERR_ARG, "passes %d not in [0, 4]", p->passes);
+    if (p->mode != VMN_MODE_PARTH && p->mode != VMN_MODE_FRESH) return fail(VMN_ERR_ARG, "bad mode %d", p->mode);
+    if (p->visited_cap < 1 || p->max_hops < 1) return fail(VMN_ERR_ARG, "visited_cap/max_hops must be >= 1");
+    if (p->P < 1 || p->P > 640) return fail(VMN_ERR_ARG, "P %d not in [1, 640]", p->P);
+    if (p->M < 1 || p->M > 512) return fail(VMN_ERR_ARG, "M %d not in [1, 512]", p->M);
+    if (p->mode == VMN_MODE_PARTH && p->L_build + p->R > 576)
+        return fail(VMN_ERR_ARG, "L_build + R must be <= 576");
+    return VMN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void vmn_default_params(vmn_params* p, int32_t mode) {
+    std::memset(p, 0, sizeof *p);
+    p->dim = 128;
+    p->mode = mode;
+    p->seed = 1234;
+    p->visited_cap = 128;
+    p->P = 600;
+    p->M = 500;
+    p->device = 0;
+    if (mode == VMN_MODE_FRESH) {
+        p->R = 64;
+        p->L_build = 128;
+        p->passes = 1;
+        p->alpha[0] = p->alpha[1] = p->alpha[2] = p->alpha[3] = 1.5f;
+        p->max_hops = 1 << 20;
+    } else {
+        p->R = 32;
+        p->L_build = 64;
+        p->passes = 2;
+        p->alpha[0] = p->alpha[1] = p->alpha[2] = p->alpha[3] = 1.2f;
+        p->max_hops = 200;
+    }
+}
+
+int vmn_create(const vmn_params* p, vmn_index** out) {
+    if (!out) return fail(VMN_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int rc = check_params(p);
+    if (rc) return rc;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (p->device < 0 || p->device >= ndev) return fail(VMN_ERR_ARG, "device %d of %d", p->device, ndev);
+    HIPCHK(hipSetDevice(p->device));
+    auto* idx = new vmn_index();
+    idx->p = *p;
+    idx->DP = pad_dim(p->dim);
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, p->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&idx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete idx;
+        return fail(VMN_ERR_HIP, "device setup: %s", hipGetErrorString(e));
+    }
+    idx->cus = prop.multiProcessorCount;
+    *out = idx;
+    return VMN_OK;
+}
+
+int vmn_destroy(vmn_index* idx) {
+    if (!idx) return VMN_OK;
+    (void)hipSetDevice(idx->p.device);
+    (void)hipStreamSynchronize(idx->stream);
+    DevBuf* bufs[] = {&idx->vecs, &idx->adj, &idx->del_bits, &idx->qpad, &idx->out_ids,
+                      &idx->out_d, &idx->stat_hops, &idx->stat_ndist, &idx->counter,
+                      &idx->cand_ids, &idx->cand_d, &idx->vis_ids, &idx->vis_d, &idx->vis_cnt,
+                      &idx->pair_dst, &idx->pair_d, &idx->rcount, &idx->rfill, &idx->roff,
+                      &idx->rscan_tmp, &idx->rsrc, &idx->rd, &idx->rlist, &idx->centroid,
+                      &idx->medoid_key};
+    for (DevBuf* b : bufs) b->release();
+    (void)hipStreamDestroy(idx->stream);
+    delete idx;
+    return VMN_OK;
+}
+
+int vmn_build(vmn_index* idx, const float* vecs, int64_t n, int32_t mem) {
+    if (!idx || !vecs) return fail(VMN_ERR_ARG, "NULL argument");
+    if (n <= idx->p.R) return fail(VMN_ERR_ARG, "n = %lld must exceed R = %d", (long long)n, idx->p.R);
+    if (n >= (int64_t)1 << 31) return fail(VMN_ERR_ARG, "n must be < 2^31");
+    if (idx->p.capacity && n > idx->p.capacity) return fail(VMN_ERR_CAPACITY, "n exceeds capacity");
+    int rc = set_device(idx);
+    if (rc) return rc;
+    idx->built = false;
+    rc = alloc_graph(idx, n);
+    if (rc) return rc;
+    idx->n = n;
+    const float* staged = nullptr;
+    rc = stage_rows(idx, vecs, n, mem, idx->vecs, &staged);
+    if (rc) return rc;
+    if (staged != idx->vecs.as<float>())
+        HIPCHK(hipMemcpyAsync(idx->vecs.p, staged, (size_t)n * idx->DP * sizeof(float),
+                              hipMemcpyDeviceToDevice, idx->stream));
+    rc = compute_medoid(idx);
+    if (rc) return rc;
+    rc = idx->p.mode == VMN_MODE_PARTH ? build_parth(idx) : build_fresh(idx);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    idx->built = true;
+    return VMN_OK;
+}
+
+int vmn_load_graph(vmn_index* idx, const float* vecs, int64_t n, const int32_t* adj,
+                   int32_t row_width, int64_t medoid, int32_t mem) {
+    if (!idx || !vecs || !adj) return fail(VMN_ERR_ARG, "NULL argument");
+    if (n < 1 || n >= (int64_t)1 << 31) return fail(VMN_ERR_ARG, "bad n");
+    if (row_width < 1 || row_width > idx->p.R) return fail(VMN_ERR_ARG, "row_width %d > R %d", row_width, idx->p.R);
+    if (medoid >= n) return fail(VMN_ERR_ARG, "medoid out of range");
+    int rc = set_device(idx);
+    if (rc) return rc;
+    rc = alloc_graph(idx, n);
+    if (rc) return rc;
+    idx->n = n;
+    const float* staged = nullptr;
+    rc = stage_rows(idx, vecs, n, mem, idx->vecs, &staged);
+    if (rc) return rc;
+    if (staged != idx->vecs.as<float>())
+        HIPCHK(hipMemcpyAsync(idx->vecs.p, staged, (size_t)n * idx->DP * sizeof(float),
+                              hipMemcpyDeviceToDevice, idx->stream));
+    // rows are widened to R with -1 on the host side of the copy
+    const int R = idx->p.R;
+    std::vector<int32_t> rows((size_t)n * R, -1);
+    std::vector<int32_t> tmp;
+    const int32_t* src = adj;
+    if (mem == VMN_MEM_DEVICE) {
+        tmp.resize((size_t)n * row_width);
+        HIPCHK(hipMemcpy(tmp.data(), adj, tmp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        src = tmp.data();
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        int w = 0;
+        for (int j = 0; j < row_width; ++j) {
+            const int32_t v = src[i * row_width + j];
+            if (v == -1) break;
+            if (v < 0 || v >= n) return fail(VMN_ERR_ARG, "adjacency id %d out of range", v);
+            bool dup = false;
+            for (int t = 0; t < w; ++t) dup |= rows[i * R + t] == v;
+            if (!dup) rows[i * R + w++] = v;  // rows hold distinct ids
+        }
+    }
+    HIPCHK(hipMemcpyAsync(idx->adj.p, rows.data(), rows.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice, idx->stream));
+    if (medoid < 0) {
+        rc = compute_medoid(idx);
+        if (rc) return rc;
+    } else {
+        idx->medoid = medoid;
+    }
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    idx->built = true;
+    return VMN_OK;
+}
+
+int vmn_search(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int32_t L,
+               int32_t* ids_out, float* dists_out, int32_t mem) {
+    if (!idx || !queries || !ids_out) return fail(VMN_ERR_ARG, "NULL argument");
+    if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");
+    if (nq < 0 || k < 1 || k > L || L < 1 || L > 512)
+        return fail(VMN_ERR_ARG, "need 1 <= k <= L <= 512 (k=%d, L=%d)", k, L);
+    if (nq == 0) return VMN_OK;
+    int rc = set_device(idx);
+    if (rc) return rc;
+    const float* q = nullptr;
+    rc = stage_rows(idx, queries, nq, mem, idx->qpad, &q);
+    if (rc) return rc;
+    int32_t* oi = ids_out;
+    float* od = dists_out;
+    if (mem == VMN_MEM_HOST) {
+        HIPCHK(idx->out_ids.reserve((size_t)nq * k * sizeof(int32_t)));
+        HIPCHK(idx->out_d.reserve((size_t)nq * k * sizeof(float)));
+        oi = idx->out_ids.as<int32_t>();
+        od = idx->out_d.as<float>();
+    }
+    rc = run_search(idx, q, nq, L, k, oi, od, false, idx->p.mode, false);
+    if (rc) return rc;
+    if (mem == VMN_MEM_HOST) {
+        HIPCHK(hipMemcpyAsync(ids_out, oi, (size_t)nq * k * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              idx->stream));
+        if (dists_out)
+            HIPCHK(hipMemcpyAsync(dists_out, od, (size_t)nq * k * sizeof(float),
+                                  hipMemcpyDeviceToHost, idx->stream));
+        HIPCHK(hipStreamSynchronize(idx->stream));
+    }
+    return VMN_OK;
+}
+
+int vmn_get_adjacency(const vmn_index* idx, int32_t* out, int32_t mem) {
+    if (!idx || !out) return fail(VMN_ERR_ARG, "NULL argument");
+    if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");
+    int rc = set_device(idx);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out, idx->adj.p, (size_t)idx->n * idx->p.R * sizeof(int32_t),
+                          mem == VMN_MEM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
+                          idx->stream));
+    if (mem == VMN_MEM_HOST) HIPCHK(hipStreamSynchronize(idx->stream));
+    return VMN_OK;
+}
+
+int64_t vmn_size(const vmn_index* idx) { return idx ? idx->n : -1; }
+int64_t vmn_medoid(const vmn_index* idx) { return idx ? idx->medoid : -1; }
+void* vmn_stream(const vmn_index* idx) { return idx ? (void*)idx->stream : nullptr; }
+
+int vmn_search_stats(const vmn_index* idx, int64_t* expansions, int64_t* dist_evals) {
+    if (!idx) return fail(VMN_ERR_ARG, "NULL index");
+    int rc = set_device(idx);
+    if (rc) return rc;
+    const int64_t nq = idx->last_nq;
+    std::vector<int32_t> h(nq), d(nq);
+    if (nq) {
+        HIPCHK(hipMemcpyAsync(h.data(), idx->stat_hops.p, nq * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              idx->stream));
+        HIPCHK(hipMemcpyAsync(d.data(), idx->stat_ndist.p, nq * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, idx->stream));
+        HIPCHK(hipStreamSynchronize(idx->stream));
+    }
+    int64_t sh = 0, sd = 0;
+    for (int64_t i = 0; i < nq; ++i) {
+        sh += h[i];
+        sd += d[i];
+    }
+    if (expansions) *expansions = sh;
+    if (dist_evals) *dist_evals = sd;
+    return VMN_OK;
+}
+
+int vmn_groundtruth(const float* base, int64_t n, int32_t dim, const float* queries, int64_t nq,
+                    int32_t k, int32_t* ids_out, double* dists_out, int32_t mem, int32_t device) {
+    if (!base || !queries || !ids_out) return fail(VMN_ERR_ARG, "NULL argument");
+    if (dim < 1 || dim > 256 || k < 1 || k > 128 || n < 1 || nq < 0)
+        return fail(VMN_ERR_ARG, "bad shape");
+    if (nq == 0) return VMN_OK;
+    HIPCHK(hipSetDevice(device));
+    vmn_params p;
+    vmn_default_params(&p, VMN_MODE_PARTH);
+    p.dim = dim;
+    p.device = device;
+    vmn_index* idx = nullptr;
+    int rc = vmn_create(&p, &idx);
+    if (rc) return rc;
+    struct Guard {
+        vmn_index* i;
+        ~Guard() { vmn_destroy(i); }
+    } guard{idx};
+    const float* b = nullptr;
+    const float* q = nullptr;
+    rc = stage_rows(idx, base, n, mem, idx->vecs, &b);
+    if (rc) return rc;
+    rc = stage_rows(idx, queries, nq, mem, idx->qpad, &q);
+    if (rc) return rc;
+    int splits = (int)std::max<int64_t>(1, std::min<int64_t>(64, (int64_t)idx->cus * 256 / std::max<int64_t>(nq, 1)));
+    splits = (int)std::min<int64_t>(splits, n);
+    const int KP = k <= 16 ? 16 : 128;
+    DevBuf pd, pi, oi, od;
+    HIPCHK(pd.reserve((size_t)nq * splits * KP * sizeof(double)));
+    HIPCHK(pi.reserve((size_t)nq * splits * KP * sizeof(int32_t)));
+    int32_t* dst_i = ids_out;
+    double* dst_d = dists_out;
+    if (mem == VMN_MEM_HOST) {
+        HIPCHK(oi.reserve((size_t)nq * k * sizeof(int32_t)));
+        HIPCHK(od.reserve((size_t)nq * k * sizeof(double)));
+        dst_i = oi.as<int32_t>();
+        dst_d = od.as<double>();
+    }
+    HIPCHK(vmn::launch_gt(b, n, idx->DP, nullptr, q, nq, k, splits, pd.as<double>(), pi.as<int32_t>(),
+                          dst_i, dst_d, idx->stream));
+    if (mem == VMN_MEM_HOST) {
+        HIPCHK(hipMemcpyAsync(ids_out, dst_i, (size_t)nq * k * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              idx->stream));
+        if (dists_out)
+            HIPCHK(hipMemcpyAsync(dists_out, dst_d, (size_t)nq * k * sizeof(double),
+                                  hipMemcpyDeviceToHost, idx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    pd.release();
+    pi.release();
+    oi.release();
+    od.release();
+    return VMN_OK;
+}
+
+const char* vmn_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

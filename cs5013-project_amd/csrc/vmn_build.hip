@@ -1,0 +1,600 @@
+// Graph construction kernels: random initial graph, medoid, alpha robust
+// prune (PARTH and FRESH forms) and reverse-edge insertion.
+#include "vmn_kernels.hpp"
+
+namespace vmn {
+
+// ---------------------------------------------------------------------------
+// Small utility kernels
+// ---------------------------------------------------------------------------
+__global__ void fill_u64_kernel(unsigned long long* p, int64_t n, unsigned long long v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+static int fill_grid(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    return (int)(g < 4096 ? (g < 1 ? 1 : g) : 4096);
+}
+hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s) {
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(fill_grid(n)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(fill_grid(n)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+// Copies [n x D] rows into [n x DP] rows, zero padding columns D..DP.
+__global__ void pad_rows_kernel(const float* src, int64_t n, int D, float* dst, int DP) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = n * DP;
+    for (; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i / DP;
+        const int c = (int)(i - r * DP);
+        dst[i] = c < D ? src[r * D + c] : 0.0f;
+    }
+}
+hipError_t launch_pad_rows(const float* src, int64_t n, int D, float* dst, int DP, hipStream_t s) {
+    hipLaunchKernelGGL(pad_rows_kernel, dim3(fill_grid(n * DP)), dim3(256), 0, s, src, n, D, dst, DP);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Random initial graph: parth/src/build_graph.cu:28-56. Per node a 64-bit LCG
+// (multiplier 6364136223846793005, increment 1442695040888963407) seeded with
+// seed + node id draws targets (state >> 32) % n, rejecting self and repeats,
+// until R distinct neighbours are found; the rest of the row is -1.
+// ---------------------------------------------------------------------------
+__global__ void random_init_kernel(int32_t* adj, int64_t n, int R, int W, uint64_t seed) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= n) return;
+    int32_t* row = adj + tid * W;
+    int32_t loc[64];
+    uint64_t state = seed + (uint64_t)tid;
+    int count = 0;
+    // n > R is checked on the host, so R distinct non-self targets exist; the
+    // draw count is still bounded so a degenerate generator cannot spin forever.
+    for (int64_t draws = 0; count < R && draws < ((int64_t)1 << 24); ++draws) {
+        state = state * 6364136223846793005ull + 1442695040888963407ull;
+        const int32_t target = (int32_t)((state >> 32) % (uint64_t)n);
+        if (target == (int32_t)tid) continue;
+        bool exists = false;
+        for (int i = 0; i < count; ++i) exists |= loc[i] == target;
+        if (!exists) loc[count++] = target;
+    }
+    for (int i = 0; i < W; ++i) row[i] = i < count ? loc[i] : -1;
+}
+hipError_t launch_random_init(int32_t* adj, int64_t n, int R, int W, uint64_t seed, hipStream_t s) {
+    hipLaunchKernelGGL(random_init_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, adj,
+                       n, R, W, seed);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Medoid: parth/src/build_graph.cu:326-343. The reference sums the centroid on
+// the host node by node in float; float addition is not associative, so the
+// same sequential order is kept here: one lane per dimension walks the rows in
+// order (loads are issued UNR rows ahead). Then every node's distance to the
+// centroid (separately rounded multiply and add, index order) and the lowest
+// id with the smallest distance.
+// ---------------------------------------------------------------------------
+__global__ void centroid_kernel(const float* __restrict__ vecs, int64_t n, int D, int DP,
+                                float* centroid) {
+#pragma clang fp contract(off)
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= D) return;
+    constexpr int UNR = 32;
+    float acc = 0.0f;
+    int64_t i = 0;
+    for (; i + UNR <= n; i += UNR) {
+        float v[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) v[u] = vecs[(i + u) * DP + d];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) acc = acc + v[u];
+    }
+    for (; i < n; ++i) acc = acc + vecs[i * DP + d];
+    centroid[d] = acc / (float)n;
+}
+
+__global__ void centroid_dist_kernel(const float* __restrict__ vecs, int64_t n, int D, int DP,
+                                     const float* __restrict__ centroid,
+                                     unsigned long long* best) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long key = ~0ull;
+    if (i < n) {
+        float s = 0.0f;
+        for (int k = 0; k < D; ++k) {
+            const float diff = vecs[i * DP + k] - centroid[k];
+            const float sq = diff * diff;
+            s = s + sq;
+        }
+        // reference starts from min_dist = 1e30f with strict <
+        if (s < 1e30f) key = ((unsigned long long)__float_as_uint(s) << 32) | (uint64_t)(uint32_t)i;
+    }
+    // wave min then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(key, off);
+        key = o < key ? o : key;
+    }
+    if (lane() == 0 && key != ~0ull) atomicMin(best, key);
+}
+
+hipError_t launch_medoid(const float* vecs, int64_t n, int D, int DP, float* centroid,
+                         float* node_d, unsigned long long* best, hipStream_t s) {
+    (void)node_d;
+    hipLaunchKernelGGL(centroid_kernel, dim3((D + 63) / 64), dim3(64), 0, s, vecs, n, D, DP, centroid);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = launch_fill_u64(best, 1, ~0ull, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(centroid_dist_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       vecs, n, D, DP, centroid, best);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Robust prune of a sorted candidate list held in LDS.
+//
+// Candidates are taken in list order; a candidate that is still kept becomes
+// p* and is added to the output, and every later kept candidate p' is dropped
+// when alpha * d(p*, p') < d(node, p') (STRICT, parth/src/build_graph.cu:
+// 241-258) or <= (vamana/include/kernels.cuh:329-353). The product alpha*d is
+// one float rounding, compared without fusion. Stops once R are kept. The
+// d(p*, p') of one round are computed by the whole wave (8 rows at a time).
+// ---------------------------------------------------------------------------
+template <int DP, bool STRICT>
+__device__ int robust_prune(const float* __restrict__ vecs, const uint32_t* c_id,
+                            const float* c_d, int cnt, float alpha, int R, uint8_t* s_kept,
+                            int32_t* s_act_id, int32_t* s_act_pos, float* s_act_d,
+                            int32_t* out_ids, float* out_d) {
+    const int l = lane(), sub = l & 7;
+    for (int i = l; i < cnt; i += 64) s_kept[i] = 1;
+    __syncthreads();
+    int nk = 0, from = 0;
+    while (nk < R) {
+        int p = -1;
+        for (int b = from; b < cnt; b += 64) {
+            const int j = b + l;
+            const uint64_t m = ballot(j < cnt && s_kept[j]);
+            if (m) {
+                p = b + __ffsll((unsigned long long)m) - 1;
+                break;
+            }
+        }
+        if (p < 0) break;
+        const int32_t ps = (int32_t)(c_id[p] & VMN_ID_MASK);
+        if (l == 0) {
+            out_ids[nk] = ps;
+            if (out_d) out_d[nk] = c_d[p];
+        }
+        ++nk;
+        if (nk >= R) break;
+        QReg<DP> pq;
+        pq.load(vecs + (int64_t)ps * DP, sub);
+        int nact = 0;
+        for (int b = p + 1; b < cnt; b += 64) {
+            const int j = b + l;
+            const bool f = j < cnt && s_kept[j];
+            const uint64_t m = ballot(f);
+            const int k = nact + __popcll((unsigned long long)(m & lanemask_lt()));
+            if (f) {
+                s_act_id[k] = (int32_t)(c_id[j] & VMN_ID_MASK);
+                s_act_pos[k] = j;
+            }
+            nact += __popcll((unsigned long long)m);
+        }
+        __syncthreads();
+        gather_dists<DP>(pq, vecs, s_act_id, s_act_d, nact);
+        __syncthreads();
+        for (int k = l; k < nact; k += 64) {
+            const int j = s_act_pos[k];
+            const float lhs = alpha * s_act_d[k];
+            const bool pr = STRICT ? (lhs < c_d[j]) : (lhs <= c_d[j]);
+            if (pr) s_kept[j] = 0;
+        }
+        __syncthreads();
+        from = p + 1;
+    }
+    __syncthreads();
+    return nk;
+}
+
+// Distances from `node` to its current row entries that pass `keep`, appended
+// (compacted, adjacency order) to e_id/e_d. Returns the count.
+template <int DP>
+__device__ __forceinline__ int row_extras(const float* __restrict__ vecs, const int32_t* adj, int W,
+                                          int64_t node, const QReg<DP>& q, bool pass,
+                                          int32_t nb, int32_t* e_id, float* e_d) {
+    const uint64_t m = ballot(pass);
+    const int ne = __popcll((unsigned long long)m);
+    if (pass) e_id[__popcll((unsigned long long)(m & lanemask_lt()))] = nb;
+    __syncthreads();
+    gather_dists<DP>(q, vecs, e_id, e_d, ne);
+    __syncthreads();
+    (void)adj; (void)W; (void)node;
+    return ne;
+}
+
+__device__ __forceinline__ int64_t next_item(unsigned long long* counter, int64_t* s_q) {
+    if (lane() == 0) *s_q = (int64_t)atomicAdd(counter, 1ull);
+    __syncthreads();
+    const int64_t v = *s_q;
+    __syncthreads();
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// PARTH prune: parth/src/build_graph.cu:173-264 (`robustPruneKernel`).
+// Candidates = the node's search list (minus -1 and the node itself, already
+// in distance order) followed by its current neighbours that are not in the
+// list (fresh distances); a stable sort by distance (the reference's bubble
+// sort) equals a stable merge of the two with list entries first on ties.
+// ---------------------------------------------------------------------------
+template <int DP, int CCAP>
+__global__ __launch_bounds__(64) void parth_prune_kernel(PruneArgs a) {
+    __shared__ __align__(16) uint32_t c_id[2][CCAP];
+    __shared__ __align__(16) float c_d[2][CCAP];
+    __shared__ int32_t e_id[64];
+    __shared__ float e_d[64];
+    __shared__ uint8_t kept[CCAP];
+    __shared__ int32_t act_id[CCAP];
+    __shared__ int32_t act_pos[CCAP];
+    __shared__ float act_d[CCAP];
+    __shared__ int32_t newrow[64];
+    __shared__ int64_t s_q;
+    const int l = threadIdx.x, sub = l & 7;
+    const int W = a.W, Lc = a.Lc;
+    for (;;) {
+        const int64_t i = next_item(a.counter, &s_q);
+        if (i >= a.n) return;
+        const int64_t node = a.node_base + i;
+        int m = 0;
+        for (int b = 0; b < Lc; b += 64) {
+            const int j = b + l;
+            const int32_t id = j < Lc ? a.cand_ids[i * Lc + j] : -1;
+            const float d = j < Lc ? a.cand_d[i * Lc + j] : 0.0f;
+            const bool f = id != -1 && id != (int32_t)node;
+            const uint64_t mk = ballot(f);
+            if (f) {
+                const int k = m + __popcll((unsigned long long)(mk & lanemask_lt()));
+                c_id[0][k] = (uint32_t)id;
+                c_d[0][k] = d;
+            }
+            m += __popcll((unsigned long long)mk);
+        }
+        const int32_t nb = l < W ? a.adj[node * W + l] : -1;
+        __syncthreads();
+        bool pass = nb != -1 && nb != (int32_t)node;
+        {
+            bool dup = false;
+            for (int t = 0; t < m; ++t) dup |= c_id[0][t] == (uint32_t)nb;
+            pass = pass && !dup;
+        }
+        QReg<DP> q;
+        q.load(a.vecs + node * DP, sub);
+        const int ne = row_extras<DP>(a.vecs, a.adj, W, node, q, pass, nb, e_id, e_d);
+        const uint32_t my_id = l < ne ? (uint32_t)e_id[l] : 0u;
+        const float my_d = l < ne ? e_d[l] : 0.0f;
+        const int cnt = merge_sorted<CCAP, false>(c_id[0], c_d[0], m, my_id, my_d, ne, c_id[1],
+                                                  c_d[1], CCAP);
+        __syncthreads();
+        const int nk = robust_prune<DP, true>(a.vecs, c_id[1], c_d[1], cnt, a.alpha, a.R, kept,
+                                              act_id, act_pos, act_d, newrow, nullptr);
+        for (int j = l; j < W; j += 64) a.adj[node * W + j] = j < nk ? newrow[j] : -1;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FRESH out-neighbour prune: vamana/include/kernels.cuh:358-500
+// (`computeOutNeighbors`) + `pruneOutNeighbors` :231-356. Candidates = the
+// visited set stably sorted by distance (expansion order on ties) merged with
+// the current neighbours stably sorted by distance (adjacency order on ties),
+// visited-set entries first on ties, capped at P; the node itself is left out
+// (the reference can select the medoid as its own p*). Prune with <=. Each
+// kept p* emits the reverse pair (p* <- node) with d(node, p*) (:292-301).
+// ---------------------------------------------------------------------------
+template <int DP, int CCAP>
+__global__ __launch_bounds__(64) void fresh_out_prune_kernel(PruneArgs a) {
+    __shared__ __align__(16) uint32_t c_id[2][CCAP];
+    __shared__ __align__(16) float c_d[2][CCAP];
+    __shared__ int32_t e_id[64];
+    __shared__ float e_d[64];
+    __shared__ uint8_t kept[CCAP];
+    __shared__ int32_t act_id[CCAP];
+    __shared__ int32_t act_pos[CCAP];
+    __shared__ float act_d[CCAP];
+    __shared__ int32_t newrow[64];
+    __shared__ float newd[64];
+    __shared__ int64_t s_q;
+    const int l = threadIdx.x, sub = l & 7;
+    const int W = a.W, P = a.P;
+    for (;;) {
+        const int64_t i = next_item(a.counter, &s_q);
+        if (i >= a.n) return;
+        const int64_t node = a.node_base + i;
+        const int vc = a.vis_cnt[i];
+        int cur = 0, cnt = 0;
+        for (int b = 0; b < vc; b += 64) {
+            const int j = b + l;
+            const int32_t id = j < vc ? a.vis_ids[i * P + j] : -1;
+            const float d = j < vc ? a.vis_d[i * P + j] : 0.0f;
+            const bool f = j < vc && id != (int32_t)node;
+            const uint64_t mk = ballot(f);
+            const int k = __popcll((unsigned long long)(mk & lanemask_lt()));
+            const int nn = __popcll((unsigned long long)mk);
+            if (f) {
+                e_id[k] = id;
+                e_d[k] = d;
+            }
+            __syncthreads();
+            const uint32_t my_id = l < nn ? (uint32_t)e_id[l] : 0u;
+            const float my_d = l < nn ? e_d[l] : 0.0f;
+            __syncthreads();
+            cnt = merge_sorted<CCAP, false>(c_id[cur], c_d[cur], cnt, my_id, my_d, nn,
+                                            c_id[cur ^ 1], c_d[cur ^ 1], CCAP);
+            cur ^= 1;
+            __syncthreads();
+        }
+        const int32_t nb = l < W ? a.adj[node * W + l] : -1;
+        const bool pass = nb != -1 && nb != (int32_t)node;
+        QReg<DP> q;
+        q.load(a.vecs + node * DP, sub);
+        const int ne = row_extras<DP>(a.vecs, a.adj, W, node, q, pass, nb, e_id, e_d);
+        const uint32_t my_id = l < ne ? (uint32_t)e_id[l] : 0u;
+        const float my_d = l < ne ? e_d[l] : 0.0f;
+        cnt = merge_sorted<CCAP, false>(c_id[cur], c_d[cur], cnt, my_id, my_d, ne, c_id[cur ^ 1],
+                                        c_d[cur ^ 1], P);
+        cur ^= 1;
+        __syncthreads();
+        const int nk = robust_prune<DP, false>(a.vecs, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
+                                               kept, act_id, act_pos, act_d, newrow, newd);
+        for (int j = l; j < W; j += 64) a.adj[node * W + j] = j < nk ? newrow[j] : -1;
+        for (int j = l; j < a.R; j += 64) {
+            a.pair_dst[i * a.R + j] = j < nk ? newrow[j] : -1;
+            a.pair_d[i * a.R + j] = j < nk ? newd[j] : 0.0f;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FRESH reverse-edge prune: vamana/include/kernels.cuh:745-903
+// (`computeReverseEdges`) + `mergeIntoReverseEdges` :566-624 +
+// `pruneReverseEdges` :626-743. For a node with reverse candidates: those
+// candidates ordered by (distance, source id) (the reference's order is
+// atomic-arrival dependent), first M kept, merged with the node's current
+// neighbours (stable by distance, reverse candidates first on ties), capped at
+// M, then pruned with <= into at most R neighbours. Only nodes that received
+// reverse candidates are visited (the compacted list the reference builds at
+// :539-564); re-pruning an unchanged row with the alpha it was pruned with
+// returns it unchanged.
+// ---------------------------------------------------------------------------
+template <int DP, int CCAP>
+__global__ __launch_bounds__(64) void fresh_rev_prune_kernel(PruneArgs a) {
+    __shared__ __align__(16) uint32_t c_id[2][CCAP];
+    __shared__ __align__(16) float c_d[2][CCAP];
+    __shared__ int32_t e_id[64];
+    __shared__ float e_d[64];
+    __shared__ uint8_t kept[CCAP];
+    __shared__ int32_t act_id[CCAP];
+    __shared__ int32_t act_pos[CCAP];
+    __shared__ float act_d[CCAP];
+    __shared__ int32_t newrow[64];
+    __shared__ int64_t s_q;
+    const int l = threadIdx.x, sub = l & 7;
+    const int W = a.W, M = a.M;
+    for (;;) {
+        const int64_t i = next_item(a.counter, &s_q);
+        if (i >= a.n) return;
+        const int64_t node = a.node_list[i];
+        const int64_t o0 = a.rev_off[node], o1 = a.rev_off[node + 1];
+        int cur = 0, cnt = 0;
+        for (int64_t b = o0; b < o1; b += 64) {
+            const int64_t j = b + l;
+            const bool f = j < o1;
+            const uint32_t my_id = f ? (uint32_t)a.rev_src[j] : 0u;
+            const float my_d = f ? a.rev_d[j] : 0.0f;
+            const int nn = (int)((o1 - b) < 64 ? (o1 - b) : 64);
+            cnt = merge_sorted<CCAP, true>(c_id[cur], c_d[cur], cnt, my_id, my_d, nn,
+                                           c_id[cur ^ 1], c_d[cur ^ 1], M);
+            cur ^= 1;
+            __syncthreads();
+        }
+        const int32_t nb = l < W ? a.adj[node * W + l] : -1;
+        const bool pass = nb != -1 && nb != (int32_t)node;
+        QReg<DP> q;
+        q.load(a.vecs + node * DP, sub);
+        const int ne = row_extras<DP>(a.vecs, a.adj, W, node, q, pass, nb, e_id, e_d);
+        const uint32_t my_id = l < ne ? (uint32_t)e_id[l] : 0u;
+        const float my_d = l < ne ? e_d[l] : 0.0f;
+        cnt = merge_sorted<CCAP, false>(c_id[cur], c_d[cur], cnt, my_id, my_d, ne, c_id[cur ^ 1],
+                                        c_d[cur ^ 1], M);
+        cur ^= 1;
+        __syncthreads();
+        const int nk = robust_prune<DP, false>(a.vecs, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
+                                               kept, act_id, act_pos, act_d, newrow, nullptr);
+        for (int j = l; j < W; j += 64) a.adj[node * W + j] = j < nk ? newrow[j] : -1;
+        __syncthreads();
+    }
+}
+
+#define VMN_DISPATCH_DP(DPV, CALL)          \
+    switch (DPV) {                           \
+        case 32: { constexpr int DPc = 32; CALL; } break;   \
+        case 64: { constexpr int DPc = 64; CALL; } break;   \
+        case 96: { constexpr int DPc = 96; CALL; } break;   \
+        case 128: { constexpr int DPc = 128; CALL; } break; \
+        case 160: { constexpr int DPc = 160; CALL; } break; \
+        case 192: { constexpr int DPc = 192; CALL; } break; \
+        case 256: { constexpr int DPc = 256; CALL; } break; \
+        default: return hipErrorInvalidValue;             \
+    }
+
+hipError_t launch_parth_prune(const PruneArgs& a, int grid, hipStream_t s) {
+    const int need = a.Lc + a.W;
+    if (need <= 128) {
+        VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((parth_prune_kernel<DPc, 128>), dim3(grid), dim3(64), 0, s, a));
+    } else if (need <= 256) {
+        VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((parth_prune_kernel<DPc, 256>), dim3(grid), dim3(64), 0, s, a));
+    } else if (need <= 576) {
+        VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((parth_prune_kernel<DPc, 576>), dim3(grid), dim3(64), 0, s, a));
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fresh_out_prune(const PruneArgs& a, int grid, hipStream_t s) {
+    // list capacity must hold min(P, visited + W) entries
+    const int need = a.P;
+    if (need <= 256) {
+        VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((fresh_out_prune_kernel<DPc, 256>), dim3(grid), dim3(64), 0, s, a));
+    } else if (need <= 640) {
+        VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((fresh_out_prune_kernel<DPc, 640>), dim3(grid), dim3(64), 0, s, a));
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fresh_rev_prune(const PruneArgs& a, int grid, hipStream_t s) {
+    const int need = a.M;
+    if (need <= 256) {
+        VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((fresh_rev_prune_kernel<DPc, 256>), dim3(grid), dim3(64), 0, s, a));
+    } else if (need <= 512) {
+        VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((fresh_rev_prune_kernel<DPc, 512>), dim3(grid), dim3(64), 0, s, a));
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Reverse-edge grouping: count pairs per destination (atomics), exclusive scan
+// to CSR offsets, scatter (src, d) into each destination's segment (slot from
+// an atomic per destination; the order inside a segment is then fixed by the
+// (distance, id) keyed merge of the prune kernel), and the compacted list of
+// destinations with at least one reverse candidate.
+// ---------------------------------------------------------------------------
+__global__ void reverse_count_kernel(const int32_t* pair_dst, int64_t npairs, int32_t* counts) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < npairs; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t d = pair_dst[i];
+        if (d >= 0) atomicAdd(&counts[d], 1);
+    }
+}
+hipError_t launch_reverse_count(const int32_t* pair_dst, int64_t npairs, int32_t* counts, hipStream_t s) {
+    hipLaunchKernelGGL(reverse_count_kernel, dim3(fill_grid(npairs)), dim3(256), 0, s, pair_dst, npairs, counts);
+    return hipGetLastError();
+}
+
+// Exclusive scan in three passes: per-1024-block sums, a single-block scan of
+// those sums, then each block's local scan plus its offset. off has n+1 entries.
+__global__ void scan_block_sums(const int32_t* in, int64_t n, int64_t* bsum) {
+    __shared__ int64_t red[256];
+    const int64_t base = (int64_t)blockIdx.x * 1024;
+    int64_t s = 0;
+    for (int k = threadIdx.x; k < 1024; k += 256) {
+        const int64_t i = base + k;
+        if (i < n) s += in[i];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = red[0];
+}
+__global__ void scan_sums_single(int64_t* bsum, int64_t nb) {
+    if (threadIdx.x != 0) return;
+    int64_t run = 0;
+    for (int64_t i = 0; i < nb; ++i) {
+        const int64_t v = bsum[i];
+        bsum[i] = run;
+        run += v;
+    }
+    bsum[nb] = run;
+}
+__global__ void scan_local(const int32_t* in, int64_t n, const int64_t* bsum, int64_t* out) {
+    __shared__ int64_t buf[1024];
+    const int64_t base = (int64_t)blockIdx.x * 1024;
+    for (int k = threadIdx.x; k < 1024; k += 256) {
+        const int64_t i = base + k;
+        buf[k] = i < n ? in[i] : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = bsum[blockIdx.x];
+        for (int k = 0; k < 1024; ++k) {
+            const int64_t v = buf[k];
+            buf[k] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 1024; k += 256) {
+        const int64_t i = base + k;
+        if (i < n) out[i] = buf[k];
+    }
+}
+__global__ void scan_tail(int64_t* out, int64_t n, const int64_t* bsum, int64_t nb) {
+    out[n] = bsum[nb];
+}
+hipError_t launch_exclusive_scan(const int32_t* counts, int64_t n, int64_t* off, int64_t* tmp,
+                                 hipStream_t s) {
+    const int64_t nb = (n + 1023) / 1024;
+    hipLaunchKernelGGL(scan_block_sums, dim3((unsigned)nb), dim3(256), 0, s, counts, n, tmp);
+    hipLaunchKernelGGL(scan_sums_single, dim3(1), dim3(64), 0, s, tmp, nb);
+    hipLaunchKernelGGL(scan_local, dim3((unsigned)nb), dim3(256), 0, s, counts, n, tmp, off);
+    hipLaunchKernelGGL(scan_tail, dim3(1), dim3(1), 0, s, off, n, tmp, nb);
+    return hipGetLastError();
+}
+
+__global__ void reverse_scatter_kernel(const int32_t* pair_dst, const float* pair_d, int64_t npairs,
+                                       int R, int64_t node_base, const int64_t* off, int32_t* fill,
+                                       int32_t* rev_src, float* rev_d) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < npairs; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t d = pair_dst[i];
+        if (d < 0) continue;
+        const int64_t pos = off[d] + atomicAdd(&fill[d], 1);
+        rev_src[pos] = (int32_t)(node_base + i / R);
+        rev_d[pos] = pair_d[i];
+    }
+}
+hipError_t launch_reverse_scatter(const int32_t* pair_dst, const float* pair_d, int64_t npairs,
+                                  int R, int64_t node_base, const int64_t* off, int32_t* fill,
+                                  int32_t* rev_src, float* rev_d, hipStream_t s) {
+    hipLaunchKernelGGL(reverse_scatter_kernel, dim3(fill_grid(npairs)), dim3(256), 0, s, pair_dst,
+                       pair_d, npairs, R, node_base, off, fill, rev_src, rev_d);
+    return hipGetLastError();
+}
+
+// getPrunableQueryIDs (vamana/include/kernels.cuh:539-564): wave-ballot
+// compaction of nodes with a non-zero count.
+__global__ void compact_nonzero_kernel(const int32_t* counts, int64_t n, int64_t* list,
+                                       unsigned long long* nlist) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool f = i < n && counts[i] != 0;
+    const uint64_t m = ballot(f);
+    unsigned long long base = 0;
+    if (lane() == 0 && m) base = atomicAdd(nlist, (unsigned long long)__popcll((unsigned long long)m));
+    base = __shfl(base, 0);
+    if (f) list[base + __popcll((unsigned long long)(m & lanemask_lt()))] = i;
+}
+hipError_t launch_compact_nonzero(const int32_t* counts, int64_t n, int64_t* list,
+                                  unsigned long long* nlist, hipStream_t s) {
+    hipLaunchKernelGGL(compact_nonzero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       counts, n, list, nlist);
+    return hipGetLastError();
+}
+
+}  // namespace vmn

@@ -1,0 +1,92 @@
+// Kernel argument blocks and launchers (internal to the library).
+#pragma once
+#include "vmn_device.hpp"
+
+#define VMN_MODE_PARTH 0
+#define VMN_MODE_FRESH 1
+
+namespace vmn {
+
+struct SearchArgs {
+    int mode;                 // VMN_MODE_PARTH / VMN_MODE_FRESH
+    int DP;                   // padded dimension
+    const float* vecs;        // [cap x DP]
+    const int32_t* adj;       // [cap x W]
+    const uint32_t* del_bits; // tombstone bitmap or null
+    int W;
+    const float* queries;     // [nq x DP]
+    int64_t nq;
+    int query_is_node;        // queries are graph nodes node_base + qi (skip self in FRESH)
+    int64_t node_base;
+    int32_t start;
+    int L, visited_cap, max_hops;
+    int out_n;                // entries of the final list written per query
+    int32_t* out_ids;         // [nq x out_n] or null
+    float* out_d;             // [nq x out_n] or null
+    int32_t* vis_ids;         // FRESH: expanded nodes [nq x P] or null
+    float* vis_d;
+    int32_t* vis_cnt;         // [nq]
+    int P;
+    int32_t* stat_hops;       // [nq] expansions
+    int32_t* stat_ndist;      // [nq] distance evaluations
+    unsigned long long* counter;  // work queue head, zero before launch
+};
+
+hipError_t launch_search(const SearchArgs& a, int grid, hipStream_t s);
+
+struct PruneArgs {
+    int mode;
+    int DP;
+    const float* vecs;
+    int32_t* adj;             // rows rewritten in place
+    int W, R;
+    float alpha;
+    int64_t n;                // nodes handled: node = node_base + i, i < n
+    int64_t node_base;
+    const int64_t* node_list; // optional explicit node list (overrides node_base)
+    // PARTH: search lists [n x Lc]
+    const int32_t* cand_ids;
+    const float* cand_d;
+    int Lc;
+    // FRESH out-prune: visited sets [n x P]
+    const int32_t* vis_ids;
+    const float* vis_d;
+    const int32_t* vis_cnt;
+    int P;
+    // FRESH out-prune: emitted reverse pairs [n x R] (dst = -1 when unused)
+    int32_t* pair_dst;
+    float* pair_d;
+    // FRESH reverse prune: CSR of reverse candidates per node
+    const int64_t* rev_off;   // [N+1]
+    const int32_t* rev_src;
+    const float* rev_d;
+    int M;                    // cap on merged candidates (reference 500)
+    unsigned long long* counter;
+};
+
+hipError_t launch_parth_prune(const PruneArgs& a, int grid, hipStream_t s);
+hipError_t launch_fresh_out_prune(const PruneArgs& a, int grid, hipStream_t s);
+hipError_t launch_fresh_rev_prune(const PruneArgs& a, int grid, hipStream_t s);
+
+hipError_t launch_random_init(int32_t* adj, int64_t n, int R, int W, uint64_t seed, hipStream_t s);
+hipError_t launch_medoid(const float* vecs, int64_t n, int D, int DP, float* centroid,
+                         float* node_d, unsigned long long* best, hipStream_t s);
+hipError_t launch_pad_rows(const float* src, int64_t n, int D, float* dst, int DP, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+hipError_t launch_reverse_count(const int32_t* pair_dst, int64_t npairs, int32_t* counts,
+                                hipStream_t s);
+hipError_t launch_exclusive_scan(const int32_t* counts, int64_t n, int64_t* off, int64_t* tmp,
+                                 hipStream_t s);
+hipError_t launch_reverse_scatter(const int32_t* pair_dst, const float* pair_d, int64_t npairs,
+                                  int R, int64_t node_base, const int64_t* off, int32_t* fill,
+                                  int32_t* rev_src, float* rev_d, hipStream_t s);
+hipError_t launch_compact_nonzero(const int32_t* counts, int64_t n, int64_t* list,
+                                  unsigned long long* nlist, hipStream_t s);
+
+// FP64 exact k-NN: partial top-k per (query, base split), then a merge.
+hipError_t launch_gt(const float* base, int64_t n, int DP, const uint32_t* del_bits,
+                     const float* queries, int64_t nq, int k, int splits, double* part_d,
+                     int32_t* part_i, int32_t* out_ids, double* out_d, hipStream_t s);
+
+}  // namespace vmn

@@ -1,0 +1,207 @@
+// Batched greedy beam search, one wavefront per query, persistent launch.
+//
+// Two traversal semantics share one kernel body:
+//
+// PARTH (parth/src/engine.cu:29-87 `greedy_search`, build_graph.cu:68-119
+//   `greedy_search_build`, CandidateList common.h:89-155). The reference inserts
+//   each neighbour of the expanded node one at a time into a sorted list of at
+//   most L entries, skipping ids already in the list, with equal distances
+//   placed after existing entries and rejected when the list is full. That
+//   sequence equals: keep the L first entries of every id ever offered,
+//   ordered by (distance, time first offered). An id that left the list can
+//   never re-enter it (the list's last distance only falls), so a per-entry
+//   "expanded" flag replaces the reference's visited list, and one hop is:
+//   drop neighbours already in the list, compute the rest, and stable-merge
+//   them (list entries first on ties, new ones in adjacency order). Search
+//   stops when no unexpanded entry is left, after `visited_cap` expansions
+//   (reference VISITED_LIST_SIZE = 128) or after `max_hops` iterations
+//   (reference 200).
+//
+// FRESH (vamana/include/search.cuh:20-223, G2). New neighbours are those not
+//   yet in the per-query bloom filter (bloom_filter.cuh hashes, one bit per
+//   cell in LDS); they are sorted by distance and merged into the worklist
+//   (worklist entries first on ties), truncated to L; the closest unexpanded
+//   entry is expanded next, until none is left. Decisions where the reference
+//   is racy or self-inconsistent are fixed as follows: every neighbour of one
+//   expanded node is tested against the filter state from before that node
+//   (what one warp of the reference's filterNeighbors does), the start node
+//   is entered into the filter (the reference omits it and can list the
+//   medoid twice), and the query's own node id is skipped only when the query
+//   is a graph node (the reference compares against the batch index,
+//   search.cuh:126). Expanded nodes (the "visited set", search.cuh:104-111,
+//   capped at P) are written out with their distances for the prune.
+//
+// Both modes skip tombstoned nodes while expanding (search.cuh:122,
+// engine.cu:76); PARTH gives a tombstoned start node distance 1e30
+// (engine.cu:45).
+#include "vmn_kernels.hpp"
+
+namespace vmn {
+
+template <int DP, int LCAP, int MODE>
+__global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
+    __shared__ __align__(16) uint32_t s_id[2][LCAP];
+    __shared__ __align__(16) float s_d[2][LCAP];
+    __shared__ int32_t s_sid[64];
+    __shared__ float s_sd[64];
+    __shared__ uint32_t s_bloom[MODE == VMN_MODE_FRESH ? VMN_BF_WORDS : 1];
+    __shared__ int64_t s_q;
+
+    const int l = threadIdx.x, sub = l & 7;
+    const float* __restrict__ vecs = a.vecs;
+    const int W = a.W;
+
+    for (;;) {
+        if (l == 0) s_q = (int64_t)atomicAdd(a.counter, 1ull);
+        __syncthreads();
+        const int64_t qi = s_q;
+        __syncthreads();
+        if (qi >= a.nq) return;
+
+        QReg<DP> q;
+        q.load(a.queries + qi * DP, sub);
+        const int32_t self = a.query_is_node ? (int32_t)(a.node_base + qi) : -1;
+
+        if (MODE == VMN_MODE_FRESH) {
+            for (int w = l; w < (int)VMN_BF_WORDS; w += 64) s_bloom[w] = 0u;
+        }
+        const int32_t st = a.start;
+        float sd = reduce8(q.partial(vecs + (int64_t)st * DP, sub));
+        if (MODE == VMN_MODE_PARTH && a.del_bits && bit_test(a.del_bits, (uint32_t)st)) sd = 1e30f;
+        __syncthreads();
+        if (l == 0) {
+            s_id[0][0] = (uint32_t)st;
+            s_d[0][0] = sd;
+            if (MODE == VMN_MODE_FRESH) {
+                uint32_t h1 = bf_hash1((uint32_t)st), h2 = bf_hash2((uint32_t)st);
+                s_bloom[h1 >> 5] |= 1u << (h1 & 31);
+                s_bloom[h2 >> 5] |= 1u << (h2 & 31);
+            }
+        }
+        __syncthreads();
+
+        int cur = 0, cnt = 1, nexp = 0, hops = 0, ndist = 1;
+        while (hops < a.max_hops) {
+            ++hops;
+            // closest unexpanded entry
+            int p = -1;
+            for (int b = 0; b < cnt; b += 64) {
+                const int i = b + l;
+                const bool f = i < cnt && !(s_id[cur][i] & VMN_VIS_BIT);
+                const uint64_t m = ballot(f);
+                if (m) {
+                    p = b + __ffsll((unsigned long long)m) - 1;
+                    break;
+                }
+            }
+            if (p < 0) break;
+            if (MODE == VMN_MODE_PARTH && nexp >= a.visited_cap) break;
+            ++nexp;
+            const uint32_t e = s_id[cur][p];
+            const int32_t node = (int32_t)(e & VMN_ID_MASK);
+            const float nd = s_d[cur][p];
+            __syncthreads();
+            if (l == 0) {
+                s_id[cur][p] = e | VMN_VIS_BIT;
+                if (MODE == VMN_MODE_FRESH && a.vis_ids && nexp <= a.P) {
+                    a.vis_ids[qi * a.P + nexp - 1] = node;
+                    a.vis_d[qi * a.P + nexp - 1] = nd;
+                }
+            }
+            // neighbours up to the first -1
+            int32_t nb = l < W ? a.adj[(int64_t)node * W + l] : -1;
+            const uint64_t endm = ballot(l < W && nb == -1);
+            const int deg = endm ? __ffsll((unsigned long long)endm) - 1 : W;
+            bool valid = l < deg;
+            if (valid && a.del_bits) valid = !bit_test(a.del_bits, (uint32_t)nb);
+            if (MODE == VMN_MODE_FRESH && valid && nb == self) valid = false;
+            __syncthreads();
+            if (MODE == VMN_MODE_PARTH) {
+                // skip ids already in the list
+                bool dup = false;
+                const uint32_t key = (uint32_t)nb;
+                int i = 0;
+                for (; i + 4 <= cnt; i += 4) {
+                    const uint4 v = *reinterpret_cast<const uint4*>(&s_id[cur][i]);
+                    dup |= ((v.x & VMN_ID_MASK) == key) | ((v.y & VMN_ID_MASK) == key) |
+                           ((v.z & VMN_ID_MASK) == key) | ((v.w & VMN_ID_MASK) == key);
+                }
+                for (; i < cnt; ++i) dup |= (s_id[cur][i] & VMN_ID_MASK) == key;
+                valid = valid && !dup;
+            } else {
+                uint32_t h1 = 0, h2 = 0;
+                bool seen = true;
+                if (valid) {
+                    h1 = bf_hash1((uint32_t)nb);
+                    h2 = bf_hash2((uint32_t)nb);
+                    seen = bit_test(s_bloom, h1) && bit_test(s_bloom, h2);
+                }
+                __syncthreads();
+                valid = valid && !seen;
+                if (valid) {
+                    atomicOr(&s_bloom[h1 >> 5], 1u << (h1 & 31));
+                    atomicOr(&s_bloom[h2 >> 5], 1u << (h2 & 31));
+                }
+            }
+            const uint64_t vm = ballot(valid);
+            const int nnew = __popcll((unsigned long long)vm);
+            const int slot = __popcll((unsigned long long)(vm & lanemask_lt()));
+            if (valid) s_sid[slot] = nb;
+            __syncthreads();
+            gather_dists<DP>(q, vecs, s_sid, s_sd, nnew);
+            __syncthreads();
+            ndist += nnew;
+            const uint32_t my_id = l < nnew ? (uint32_t)s_sid[l] : 0u;
+            const float my_d = l < nnew ? s_sd[l] : 0.0f;
+            cnt = merge_sorted<LCAP, false>(s_id[cur], s_d[cur], cnt, my_id, my_d, nnew,
+                                            s_id[cur ^ 1], s_d[cur ^ 1], a.L);
+            cur ^= 1;
+            __syncthreads();
+        }
+
+        if (a.out_ids) {
+            for (int j = l; j < a.out_n; j += 64) {
+                a.out_ids[qi * a.out_n + j] = j < cnt ? (int32_t)(s_id[cur][j] & VMN_ID_MASK) : -1;
+                if (a.out_d) a.out_d[qi * a.out_n + j] = j < cnt ? s_d[cur][j] : 1e30f;
+            }
+        }
+        if (l == 0) {
+            if (a.vis_cnt) a.vis_cnt[qi] = nexp < a.P ? nexp : a.P;
+            if (a.stat_hops) a.stat_hops[qi] = nexp;
+            if (a.stat_ndist) a.stat_ndist[qi] = ndist;
+        }
+        __syncthreads();
+    }
+}
+
+template <int DP, int LCAP>
+static hipError_t launch_search_l(const SearchArgs& a, int grid, hipStream_t s) {
+    if (a.mode == VMN_MODE_PARTH)
+        hipLaunchKernelGGL((search_kernel<DP, LCAP, VMN_MODE_PARTH>), dim3(grid), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((search_kernel<DP, LCAP, VMN_MODE_FRESH>), dim3(grid), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int DP>
+static hipError_t launch_search_d(const SearchArgs& a, int grid, hipStream_t s) {
+    if (a.L <= 64) return launch_search_l<DP, 64>(a, grid, s);
+    if (a.L <= 128) return launch_search_l<DP, 128>(a, grid, s);
+    if (a.L <= 256) return launch_search_l<DP, 256>(a, grid, s);
+    return launch_search_l<DP, 512>(a, grid, s);
+}
+
+hipError_t launch_search(const SearchArgs& a, int grid, hipStream_t s) {
+    switch (a.DP) {
+        case 32: return launch_search_d<32>(a, grid, s);
+        case 64: return launch_search_d<64>(a, grid, s);
+        case 96: return launch_search_d<96>(a, grid, s);
+        case 128: return launch_search_d<128>(a, grid, s);
+        case 160: return launch_search_d<160>(a, grid, s);
+        case 192: return launch_search_d<192>(a, grid, s);
+        case 256: return launch_search_d<256>(a, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace vmn

@@ -1,0 +1,247 @@
+"""Python mirror of the reference's index API over the C ABI (libvmn.so).
+
+`Vamana` follows the reference's `Vamana<T>` (vamana/include/vamana.cuh:36-146):
+the constructor builds the graph, `searchPoints` runs a batched search, and the
+graph can be read back. Inputs may be numpy arrays (host) or torch tensors on
+the GPU (device pointers, no copy). The HIP library is required: nothing here
+falls back to a CPU path; a missing or unloadable libvmn.so raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvmn.so")
+
+MODE_PARTH = 0
+MODE_FRESH = 1
+MEM_HOST = 0
+MEM_DEVICE = 1
+
+STATUS = {0: "VMN_OK", -1: "VMN_ERR_ARG", -2: "VMN_ERR_HIP", -3: "VMN_ERR_STATE",
+          -4: "VMN_ERR_CAPACITY", -5: "VMN_ERR_IO"}
+
+
+class VamanaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    """Mirror of `vmn_params` (include/vamana_mi355x.h)."""
+    _fields_ = [("dim", ctypes.c_int32), ("R", ctypes.c_int32), ("L_build", ctypes.c_int32),
+                ("passes", ctypes.c_int32), ("alpha", ctypes.c_float * 4),
+                ("seed", ctypes.c_uint64), ("mode", ctypes.c_int32),
+                ("visited_cap", ctypes.c_int32), ("max_hops", ctypes.c_int32),
+                ("P", ctypes.c_int32), ("M", ctypes.c_int32), ("capacity", ctypes.c_int64),
+                ("device", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libvmn.so (built by build_lib.py). Raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        P, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+        L.vmn_default_params.argtypes = [ctypes.POINTER(Params), i32]
+        L.vmn_default_params.restype = None
+        L.vmn_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(P)]
+        L.vmn_destroy.argtypes = [P]
+        L.vmn_build.argtypes = [P, P, i64, i32]
+        L.vmn_load_graph.argtypes = [P, P, i64, P, i32, i64, i32]
+        L.vmn_search.argtypes = [P, P, i64, i32, i32, P, P, i32]
+        L.vmn_get_adjacency.argtypes = [P, P, i32]
+        L.vmn_size.argtypes = [P]
+        L.vmn_size.restype = i64
+        L.vmn_medoid.argtypes = [P]
+        L.vmn_medoid.restype = i64
+        L.vmn_search_stats.argtypes = [P, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.vmn_stream.argtypes = [P]
+        L.vmn_stream.restype = P
+        L.vmn_groundtruth.argtypes = [P, i64, i32, P, i64, i32, P, P, i32, i32]
+        L.vmn_last_error.argtypes = []
+        L.vmn_last_error.restype = ctypes.c_char_p
+        for name in ("vmn_create", "vmn_destroy", "vmn_build", "vmn_load_graph", "vmn_search",
+                     "vmn_get_adjacency", "vmn_search_stats", "vmn_groundtruth"):
+            getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise VamanaError(rc, lib().vmn_last_error().decode())
+
+
+def default_params(mode=MODE_PARTH):
+    p = Params()
+    lib().vmn_default_params(ctypes.byref(p), mode)
+    return p
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _arg(x, dtype):
+    """(pointer, mem kind, keepalive) for a numpy array or a torch tensor."""
+    if _is_torch(x):
+        import torch
+        tdt = {np.float32: torch.float32, np.int32: torch.int32}[dtype]
+        assert x.dtype == tdt and x.is_contiguous(), "tensor must be contiguous " + str(tdt)
+        mem = MEM_DEVICE if x.is_cuda else MEM_HOST
+        return ctypes.c_void_p(x.data_ptr()), mem, x
+    a = np.ascontiguousarray(x, dtype=dtype)
+    return a.ctypes.data_as(ctypes.c_void_p), MEM_HOST, a
+
+
+class Vamana:
+    """Vamana / FreshVamana index resident on one GPU.
+
+    Vamana(vectors, mode="parth"|"fresh", **params) builds the graph (as the
+    reference's constructor does). Vamana.from_graph(vectors, adj, medoid)
+    installs an existing graph instead.
+    """
+
+    def __init__(self, vectors=None, mode="parth", *, dim=None, R=None, L_build=None,
+                 alphas=None, passes=None, seed=None, visited_cap=None, max_hops=None, P=None,
+                 M=None, capacity=0, device=0, _build=True):
+        m = {"parth": MODE_PARTH, "fresh": MODE_FRESH}[mode] if isinstance(mode, str) else mode
+        p = default_params(m)
+        if vectors is not None:
+            p.dim = int(vectors.shape[1])
+        if dim is not None:
+            p.dim = dim
+        for name, v in (("R", R), ("L_build", L_build), ("passes", passes), ("seed", seed),
+                        ("visited_cap", visited_cap), ("max_hops", max_hops), ("P", P), ("M", M)):
+            if v is not None:
+                setattr(p, name, v)
+        if alphas is not None:
+            al = list(alphas)
+            if passes is None:
+                p.passes = len(al)
+            for i in range(4):
+                p.alpha[i] = al[min(i, len(al) - 1)]
+        p.capacity = capacity
+        p.device = device
+        self.params = p
+        self._h = ctypes.c_void_p()
+        _check(lib().vmn_create(ctypes.byref(p), ctypes.byref(self._h)))
+        if vectors is not None and _build:
+            ptr, mem, keep = _arg(vectors, np.float32)
+            _check(lib().vmn_build(self._h, ptr, int(vectors.shape[0]), mem))
+
+    @classmethod
+    def from_graph(cls, vectors, adj, medoid=-1, mode="parth", R=None, **kw):
+        R = R if R is not None else int(adj.shape[1])
+        self = cls(vectors, mode=mode, R=R, _build=False, **kw)
+        vp, vm, vk = _arg(vectors, np.float32)
+        ap, am, ak = _arg(adj, np.int32)
+        assert vm == am
+        _check(lib().vmn_load_graph(self._h, vp, int(vectors.shape[0]), ap, int(adj.shape[1]),
+                                    int(medoid), vm))
+        return self
+
+    def close(self):
+        if self._h:
+            lib().vmn_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def size(self):
+        return int(lib().vmn_size(self._h))
+
+    @property
+    def medoid(self):
+        return int(lib().vmn_medoid(self._h))
+
+    @property
+    def stream(self):
+        return lib().vmn_stream(self._h)
+
+    def searchPoints(self, queries, k=10, L=None, ids_out=None, dists_out=None, sync=True):
+        """Search a batch; returns (ids [nq,k] int32, dists [nq,k] float32).
+
+        numpy in -> numpy out. CUDA tensor in -> CUDA tensors out (written on
+        the index's stream; sync=False leaves the call asynchronous)."""
+        L = k if L is None else L
+        nq = int(queries.shape[0])
+        qp, mem, keep = _arg(queries, np.float32)
+        if mem == MEM_DEVICE:
+            import torch
+            ids = ids_out if ids_out is not None else torch.empty((nq, k), dtype=torch.int32,
+                                                                  device=queries.device)
+            dists = dists_out if dists_out is not None else torch.empty(
+                (nq, k), dtype=torch.float32, device=queries.device)
+            _check(lib().vmn_search(self._h, qp, nq, k, L, ctypes.c_void_p(ids.data_ptr()),
+                                    ctypes.c_void_p(dists.data_ptr()), mem))
+            if sync:
+                self.synchronize()
+            return ids, dists
+        ids = np.empty((nq, k), dtype=np.int32)
+        dists = np.empty((nq, k), dtype=np.float32)
+        _check(lib().vmn_search(self._h, qp, nq, k, L, ids.ctypes.data_as(ctypes.c_void_p),
+                                dists.ctypes.data_as(ctypes.c_void_p), mem))
+        return ids, dists
+
+    search = searchPoints
+
+    def synchronize(self):
+        import torch
+        torch.cuda.ExternalStream(self.stream).synchronize()
+
+    @property
+    def adjacency(self):
+        out = np.empty((self.size, self.params.R), dtype=np.int32)
+        _check(lib().vmn_get_adjacency(self._h, out.ctypes.data_as(ctypes.c_void_p), MEM_HOST))
+        return out
+
+    def search_stats(self):
+        e, d = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().vmn_search_stats(self._h, ctypes.byref(e), ctypes.byref(d)))
+        return int(e.value), int(d.value)
+
+
+def groundtruth(base, queries, k, device=0):
+    """Exact FP64 k-NN on the GPU: (ids [nq,k] int32, dists [nq,k] float64)."""
+    bp, bm, bk = _arg(base, np.float32)
+    qp, qm, qk = _arg(queries, np.float32)
+    assert bm == qm
+    nq = int(queries.shape[0])
+    if bm == MEM_DEVICE:
+        import torch
+        ids = torch.empty((nq, k), dtype=torch.int32, device=queries.device)
+        d = torch.empty((nq, k), dtype=torch.float64, device=queries.device)
+        _check(lib().vmn_groundtruth(bp, int(base.shape[0]), int(base.shape[1]), qp, nq, k,
+                                     ctypes.c_void_p(ids.data_ptr()),
+                                     ctypes.c_void_p(d.data_ptr()), bm, device))
+        return ids, d
+    ids = np.empty((nq, k), dtype=np.int32)
+    d = np.empty((nq, k), dtype=np.float64)
+    _check(lib().vmn_groundtruth(bp, int(base.shape[0]), int(base.shape[1]), qp, nq, k,
+                                 ids.ctypes.data_as(ctypes.c_void_p),
+                                 d.ctypes.data_as(ctypes.c_void_p), bm, device))
+    return ids, d
+
+
+def recall_at_k(found, gt_ids, k=10):
+    """Mean |found[:, :k] ∩ gt[:, :k]| / k (engine.cu:506-528 counts the same)."""
+    found = np.asarray(found)[:, :k]
+    gt = np.asarray(gt_ids)[:, :k]
+    hits = 0
+    for f, g in zip(found, gt):
+        hits += len(set(int(x) for x in f if x >= 0) & set(int(x) for x in g))
+    return hits / (k * found.shape[0])

@@ -1,0 +1,118 @@
+/*
+ * vamana_mi355x.h -- C ABI of the MI355X-native Vamana / FreshVamana index.
+ *
+ * Drop-in boundary for the reference's index API (VishankSingh/CS5013-Project):
+ * the C++ class `Vamana<T>` of vamana/include/vamana.cuh:36-146 (G2, mirrored
+ * at vamana.cu:1245-1527) and the deterministic build/search engine of
+ * parth/src/build_graph.cu and parth/src/engine.cu. Plain pointers and sizes,
+ * no C++ or torch types. Every call returns VMN_OK (0) or a negative status;
+ * vmn_last_error() describes the last failure on the calling thread. Nothing
+ * exits the process (the reference calls exit() through gpuErrchk,
+ * vamana/include/utils.cuh:68-78).
+ *
+ * Memory: arguments tagged `mem` are host pointers (VMN_MEM_HOST; the call is
+ * synchronous) or device pointers on the index's device (VMN_MEM_DEVICE; the
+ * call is asynchronous on the index's stream, vmn_stream(), and the caller
+ * synchronises before reading outputs). Outputs are caller-owned (the
+ * reference returns callee-allocated device buffers the caller must free,
+ * vamana/include/vamana.cuh:96,127).
+ *
+ * Threading: all state lives in the handle (the reference keeps process-global
+ * state, vamana/include/globals.cuh:4-14); calls on one handle must be
+ * serialised by the caller; separate handles are independent.
+ */
+#ifndef VAMANA_MI355X_H
+#define VAMANA_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VMN_OK 0
+#define VMN_ERR_ARG (-1)      /* invalid argument or unsupported shape */
+#define VMN_ERR_HIP (-2)      /* HIP runtime or kernel launch failure */
+#define VMN_ERR_STATE (-3)    /* call not valid in the index's state (e.g. search before build) */
+#define VMN_ERR_CAPACITY (-4) /* capacity exceeded */
+#define VMN_ERR_IO (-5)       /* file open / short read / write failure */
+
+/* Traversal and prune semantics. */
+#define VMN_MODE_PARTH 0 /* parth/src/build_graph.cu + engine.cu (deterministic; bit-exact pinned) */
+#define VMN_MODE_FRESH 1 /* vamana/include/{search,kernels}.cuh (G2: bloom filter, reverse edges) */
+
+#define VMN_MEM_HOST 0
+#define VMN_MEM_DEVICE 1
+
+typedef struct vmn_index vmn_index;
+
+typedef struct vmn_params {
+    int32_t dim;         /* vector dimension D (any D <= 256) */
+    int32_t R;           /* max out-degree = adjacency row width (<= 64) */
+    int32_t L_build;     /* beam width of the build searches */
+    int32_t passes;      /* build passes (parth: 2, build_graph.cu:345; G2: 1) */
+    float alpha[4];      /* alpha of each pass (reference: one alpha, 1.2 parth / 1.5 G2) */
+    uint64_t seed;       /* random initial graph seed (build_graph.cu:317 uses 1234) */
+    int32_t mode;        /* VMN_MODE_PARTH / VMN_MODE_FRESH */
+    int32_t visited_cap; /* PARTH: expansions per search (common.h:16 VISITED_LIST_SIZE = 128) */
+    int32_t max_hops;    /* PARTH: iterations per search (engine.cu:50, 200); FRESH: safety bound */
+    int32_t P;           /* FRESH: visited-set cap (constants.cuh:13, 600) */
+    int32_t M;           /* FRESH: reverse candidate cap (constants.cuh:20, 500) */
+    int64_t capacity;    /* node capacity for inserts (0: the build size) */
+    int32_t device;      /* HIP device ordinal */
+} vmn_params;
+
+/* Reference defaults for `mode` (parth: R=32, L=64, alpha 1.2 x 2 passes). */
+void vmn_default_params(vmn_params* p, int32_t mode);
+
+/* Create an empty index on p->device. Replaces the constructor half of
+ * `Vamana<T>::Vamana(std::unique_ptr<GraphT>)` (vamana_impl.cuh:10-20). */
+int vmn_create(const vmn_params* p, vmn_index** out);
+int vmn_destroy(vmn_index* idx);
+
+/* Build the graph over n vectors [n x dim]. Replaces `runVamana`
+ * (vamana_impl.cuh:289-360) and parth `build_graph` (build_graph.cu:282-378):
+ * random initial graph, medoid, then `passes` x (search all, prune all). */
+int vmn_build(vmn_index* idx, const float* vecs, int64_t n, int32_t mem);
+
+/* Install an existing graph ([n x dim] vectors, [n x row_width] adjacency,
+ * -1 padded, row_width <= R) with entry point `medoid` (-1: compute it).
+ * Replaces `initGraph` (utils.cuh:113-148) / parth load_data (engine.cu:338-357). */
+int vmn_load_graph(vmn_index* idx, const float* vecs, int64_t n, const int32_t* adj,
+                   int32_t row_width, int64_t medoid, int32_t mem);
+
+/* Batched greedy search of nq queries [nq x dim] from the medoid with beam L;
+ * writes the k best ids (-1 padded) and squared L2 distances (may be NULL) per
+ * query. Replaces `searchPoints` (vamana.cuh:96, vamana_impl.cuh:55-96) and
+ * parth `searchKernel` (engine.cu:100-120). Tombstoned nodes are skipped. */
+int vmn_search(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int32_t L,
+               int32_t* ids_out, float* dists_out, int32_t mem);
+
+/* Copy the adjacency [size x R] (int32, -1 padded). */
+int vmn_get_adjacency(const vmn_index* idx, int32_t* out, int32_t mem);
+
+int64_t vmn_size(const vmn_index* idx);
+int64_t vmn_medoid(const vmn_index* idx);
+
+/* Sum over the queries of the last vmn_search: expansions (hops) and distance
+ * evaluations. Synchronises the index's stream. */
+int vmn_search_stats(const vmn_index* idx, int64_t* expansions, int64_t* dist_evals);
+
+/* The hipStream_t all of the index's work is queued on. */
+void* vmn_stream(const vmn_index* idx);
+
+/* Exact k nearest neighbours (k <= 128) of nq queries over n base vectors in
+ * FP64, ids ordered by (distance, id); dists_out may be NULL. Replaces
+ * parth `bruteForceKernel` (engine.cu:253-279) and the external DiskANN
+ * compute_groundtruth (vamana.cu:1644). Runs on `device`; synchronous. */
+int vmn_groundtruth(const float* base, int64_t n, int32_t dim, const float* queries, int64_t nq,
+                    int32_t k, int32_t* ids_out, double* dists_out, int32_t mem, int32_t device);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* vmn_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VAMANA_MI355X_H */

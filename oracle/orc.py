@@ -1,0 +1,130 @@
+"""ctypes wrapper over the CPU oracle (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the package smoke check and by
+bench.py's cpu_baseline leg, always as the checker or the CPU baseline, never
+as the thing measured or shipped.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liborc.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i32, i64, u64, f32 = ctypes.c_int, ctypes.c_int64, ctypes.c_uint64, ctypes.c_float
+        _lib.orc_l2sq.argtypes = [P, P, i32, i32]
+        _lib.orc_l2sq.restype = f32
+        _lib.orc_random_init.argtypes = [i64, i32, i32, u64, P]
+        _lib.orc_medoid.argtypes = [P, i64, i32]
+        _lib.orc_medoid.restype = i64
+        _lib.orc_parth_search.argtypes = [P, i64, i32, P, i32, P, P, i64, i32, i32, i32, i32,
+                                          i32, i32, P, P, P, P]
+        _lib.orc_parth_prune.argtypes = [P, i64, i32, P, i32, i32, f32, P, P, i32, i32]
+        _lib.orc_parth_build.argtypes = [P, i64, i32, i32, i32, i32, P, i32, u64, i32, i32,
+                                         i32, P]
+        _lib.orc_parth_build.restype = i64
+        _lib.orc_bruteforce_fp64.argtypes = [P, i64, i32, P, P, i64, i32, P, P]
+        _lib.orc_num_threads.restype = i32
+        _lib.orc_set_num_threads.argtypes = [i32]
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def l2sq(a, b, dist_mode=0):
+    a, b = _f32(a), _f32(b)
+    return lib().orc_l2sq(_p(a), _p(b), a.shape[0], dist_mode)
+
+
+def random_init(n, R, row_width, seed=1234):
+    adj = np.empty((n, row_width), dtype=np.int32)
+    lib().orc_random_init(n, R, row_width, seed, _p(adj))
+    return adj
+
+
+def medoid(vecs):
+    vecs = _f32(vecs)
+    return int(lib().orc_medoid(_p(vecs), vecs.shape[0], vecs.shape[1]))
+
+
+def parth_search(vecs, adj, queries, start, L, out_n=10, visited_cap=128, max_hops=200,
+                 dist_mode=0, deleted=None):
+    """Returns (ids [nq,out_n] int32, dists [nq,out_n] f32, hops [nq], ndist [nq])."""
+    vecs, queries = _f32(vecs), _f32(queries)
+    adj = np.ascontiguousarray(adj, dtype=np.int32)
+    nq = queries.shape[0]
+    ids = np.empty((nq, out_n), dtype=np.int32)
+    dists = np.empty((nq, out_n), dtype=np.float32)
+    hops = np.empty(nq, dtype=np.int32)
+    nd = np.empty(nq, dtype=np.int32)
+    dm = None if deleted is None else np.ascontiguousarray(deleted, dtype=np.uint8)
+    lib().orc_parth_search(_p(vecs), vecs.shape[0], vecs.shape[1], _p(adj), adj.shape[1],
+                           _p(dm), _p(queries), nq, int(start), L, visited_cap, max_hops,
+                           dist_mode, out_n, _p(ids), _p(dists), _p(hops), _p(nd))
+    return ids, dists, hops, nd
+
+
+def parth_prune(vecs, adj, R, alpha, cand_ids, cand_dists, dist_mode=0):
+    """Prunes every row of `adj` in place from the [n, L] candidate lists."""
+    vecs = _f32(vecs)
+    assert adj.dtype == np.int32 and adj.flags.c_contiguous
+    cand_ids = np.ascontiguousarray(cand_ids, dtype=np.int32)
+    cand_dists = _f32(cand_dists)
+    lib().orc_parth_prune(_p(vecs), vecs.shape[0], vecs.shape[1], _p(adj), adj.shape[1], R,
+                          float(alpha), _p(cand_ids), _p(cand_dists), cand_ids.shape[1],
+                          dist_mode)
+    return adj
+
+
+def parth_build(vecs, R, L, alphas=(1.2, 1.2), row_width=None, seed=1234, visited_cap=128,
+                max_hops=200, dist_mode=0):
+    """Returns (adj [n,row_width] int32, medoid)."""
+    vecs = _f32(vecs)
+    n, d = vecs.shape
+    rw = R if row_width is None else row_width
+    adj = np.empty((n, rw), dtype=np.int32)
+    al = np.asarray(alphas, dtype=np.float32)
+    med = lib().orc_parth_build(_p(vecs), n, d, R, rw, L, _p(al), len(al), seed, visited_cap,
+                                max_hops, dist_mode, _p(adj))
+    return adj, int(med)
+
+
+def bruteforce_fp64(vecs, queries, k, deleted=None):
+    vecs, queries = _f32(vecs), _f32(queries)
+    nq = queries.shape[0]
+    ids = np.empty((nq, k), dtype=np.int32)
+    dists = np.empty((nq, k), dtype=np.float64)
+    dm = None if deleted is None else np.ascontiguousarray(deleted, dtype=np.uint8)
+    lib().orc_bruteforce_fp64(_p(vecs), vecs.shape[0], vecs.shape[1], _p(dm), _p(queries), nq,
+                              k, _p(ids), _p(dists))
+    return ids, dists
+
+
+def num_threads():
+    return int(lib().orc_num_threads())
+
+
+def set_num_threads(t):
+    lib().orc_set_num_threads(int(t))

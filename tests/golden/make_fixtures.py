@@ -1,0 +1,85 @@
+"""Extract the reference's data fixtures into small, self-contained .npz files.
+
+Reads DATA files only (vectors, graphs, ground truth) from the reference
+checkout; no reference code is imported or executed. Run once in a container
+where the reference is mounted:
+
+    python tests/golden/make_fixtures.py /root/reference
+
+Outputs (committed):
+  sift10k.npz  base (10000x128 u8), query (100x128 u8), gt100 (100x100 i32,
+               siftsmall_groundtruth.ivecs), diskann_gt_ids/dists (first 100
+               rows of data/sift10k/sift10k_groundtruth.bin, 10 per row),
+               parth_adj (10000x32 i32, parth/data/sift_initial_graph.bin),
+               custom_adj (5000x32 i32, parth/data/custom_sift_initial_graph.bin)
+  toy2d.npz    toy_vecs/toy_adj (20 nodes) and vis_vecs/vis_adj (50 nodes),
+               parth/data/{toy,vis}_initial_graph.bin (float 2-D data)
+
+SIFT10K is integer-valued (0..180); storing it as uint8 is lossless, which is
+asserted below.
+"""
+import os
+import struct
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def read_fvecs(path):
+    raw = np.fromfile(path, dtype=np.int32)
+    d = raw[0]
+    return raw.reshape(-1, d + 1)[:, 1:].view(np.float32).copy()
+
+
+def read_ivecs(path):
+    raw = np.fromfile(path, dtype=np.int32)
+    d = raw[0]
+    return raw.reshape(-1, d + 1)[:, 1:].copy()
+
+
+def read_parth_graph(path):
+    """parth graph file: int32 N, DIM, MAX_DEGREE, float[N*DIM], int32[N*MAX_DEGREE]
+    (format written by parth/src/build_graph.cu:361-369)."""
+    with open(path, "rb") as f:
+        n, d, md = struct.unpack("iii", f.read(12))
+        vecs = np.frombuffer(f.read(n * d * 4), dtype=np.float32).reshape(n, d).copy()
+        adj = np.frombuffer(f.read(n * md * 4), dtype=np.int32).reshape(n, md).copy()
+    return vecs, adj
+
+
+def to_u8(a):
+    assert np.all(a == np.round(a)) and a.min() >= 0 and a.max() <= 255
+    return a.astype(np.uint8)
+
+
+def main(ref):
+    base = read_fvecs(os.path.join(ref, "siftsmall/siftsmall_base.fvecs"))
+    query = read_fvecs(os.path.join(ref, "siftsmall/siftsmall_query.fvecs"))
+    gt100 = read_ivecs(os.path.join(ref, "siftsmall/siftsmall_groundtruth.ivecs"))
+    pv, padj = read_parth_graph(os.path.join(ref, "parth/data/sift_initial_graph.bin"))
+    cv, cadj = read_parth_graph(os.path.join(ref, "parth/data/custom_sift_initial_graph.bin"))
+    assert np.array_equal(pv, base), "parth sift graph vectors differ from SIFT10K base"
+    assert np.array_equal(cv, base[:5000]), "custom graph vectors differ from base[:5000]"
+    # DiskANN truthset: uint32 N, K, ids[N*K], float dists[N*K]
+    tpath = os.path.join(ref, "data/sift10k/sift10k_groundtruth.bin")
+    hdr = np.fromfile(tpath, dtype=np.uint32, count=2)
+    n_t, k_t = int(hdr[0]), int(hdr[1])
+    body = np.fromfile(tpath, dtype=np.uint32, offset=8)
+    t_ids = body[: n_t * k_t].reshape(n_t, k_t)[:100].astype(np.int32)
+    t_d = body[n_t * k_t: 2 * n_t * k_t].view(np.float32).reshape(n_t, k_t)[:100].copy()
+    np.savez_compressed(
+        os.path.join(HERE, "sift10k.npz"),
+        base=to_u8(base), query=to_u8(query), gt100=gt100.astype(np.int32),
+        diskann_gt_ids=t_ids, diskann_gt_dists=t_d,
+        parth_adj=padj, custom_adj=cadj)
+    tv, tadj = read_parth_graph(os.path.join(ref, "parth/data/toy_initial_graph.bin"))
+    vv, vadj = read_parth_graph(os.path.join(ref, "parth/data/vis_initial_graph.bin"))
+    np.savez_compressed(os.path.join(HERE, "toy2d.npz"),
+                        toy_vecs=tv, toy_adj=tadj, vis_vecs=vv, vis_adj=vadj)
+    print("wrote", os.listdir(HERE))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

@@ -1,0 +1,53 @@
+"""The C ABI library: loads, exports every declared entry point, validates
+arguments on the host (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "vamana_mi355x.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(vmn_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    for s in ("vmn_create", "vmn_build", "vmn_search", "vmn_groundtruth", "vmn_last_error"):
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol(vmn):
+    L = vmn.lib()
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_param_validation_without_gpu(vmn):
+    L = vmn.lib()
+    p = vmn.default_params(vmn.MODE_PARTH)
+    p.R = 65  # rows wider than a wavefront are rejected before touching the device
+    h = ctypes.c_void_p()
+    rc = L.vmn_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc == -1 and b"R" in L.vmn_last_error()
+    p = vmn.default_params(vmn.MODE_FRESH)
+    p.P = 10_000
+    assert L.vmn_create(ctypes.byref(p), ctypes.byref(h)) == -1
+
+
+def test_default_params_match_reference(vmn):
+    p = vmn.default_params(vmn.MODE_PARTH)
+    assert (p.R, p.L_build, p.passes, p.seed, p.visited_cap, p.max_hops) == (32, 64, 2, 1234, 128, 200)
+    assert abs(p.alpha[0] - 1.2) < 1e-7
+    f = vmn.default_params(vmn.MODE_FRESH)
+    assert (f.R, f.P, f.M) == (64, 600, 500) and abs(f.alpha[0] - 1.5) < 1e-7
+
+
+def test_null_arguments_are_errors(vmn):
+    L = vmn.lib()
+    assert L.vmn_build(None, None, 10, 0) == -1
+    assert L.vmn_search(None, None, 1, 1, 1, None, None, 0) == -1

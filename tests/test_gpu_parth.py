@@ -1,0 +1,103 @@
+"""GPU (HIP) parth-mode path against the reference fixtures and the oracle.
+
+Bit-exact bar: adjacency and returned ids/distances equal the reference's
+shipped graphs and the CPU oracle on the same inputs (integer-valued data, so
+FP32 distances are exact in any summation order).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_gpu_build_matches_reference_sift10k(vmn, sift, gpu):
+    base = sift["base"].astype(np.float32)
+    idx = vmn.Vamana(base, mode="parth", R=32, L_build=64, alphas=(1.2, 1.2))
+    assert idx.medoid == 3732
+    assert np.array_equal(idx.adjacency, sift["parth_adj"])
+
+
+def test_gpu_build_matches_reference_5k_subset(vmn, sift, gpu):
+    base = sift["base"][:5000].astype(np.float32)
+    idx = vmn.Vamana(base, mode="parth", R=32, L_build=64, alphas=(1.2, 1.2))
+    assert np.array_equal(idx.adjacency, sift["custom_adj"])
+
+
+@pytest.mark.parametrize("L", [10, 20, 40, 64, 100, 128])
+def test_gpu_search_matches_oracle_on_reference_graph(vmn, orc, sift, gpu, L):
+    base = sift["base"].astype(np.float32)
+    q = sift["query"].astype(np.float32)
+    idx = vmn.Vamana.from_graph(base, sift["parth_adj"], medoid=3732)
+    ids, d = idx.searchPoints(q, k=10, L=L)
+    oi, od, oh, on = orc.parth_search(base, sift["parth_adj"], q, 3732, L=L, out_n=10)
+    assert np.array_equal(ids, oi)
+    assert np.array_equal(d, od)
+    hops, nd = idx.search_stats()
+    assert hops == int(oh.sum())
+    assert nd == int(on.sum())
+
+
+def test_gpu_build_and_search_match_oracle_config1(vmn, orc, gpu):
+    ds = vmn.datasets
+    base = ds.integer_mixture(10_000, 128, k=64, seed=1234, stream=0)
+    q = ds.integer_mixture(100, 128, k=64, seed=1234, stream=1)
+    idx = vmn.Vamana(base, mode="parth", R=32, L_build=64, alphas=(1.2, 1.2))
+    oadj, omed = orc.parth_build(base, R=32, L=64, alphas=(1.2, 1.2))
+    assert idx.medoid == omed
+    assert np.array_equal(idx.adjacency, oadj)
+    for L in (10, 20, 40, 80):
+        ids, d = idx.searchPoints(q, k=10, L=L)
+        oi, od, _, _ = orc.parth_search(base, oadj, q, omed, L=L, out_n=10)
+        assert np.array_equal(ids, oi) and np.array_equal(d, od)
+
+
+def test_gpu_alpha_schedule_matches_oracle(vmn, orc, gpu):
+    ds = vmn.datasets
+    base = ds.integer_mixture(4000, 128, k=32, seed=7, stream=0)
+    idx = vmn.Vamana(base, mode="parth", R=64, L_build=128, alphas=(1.0, 1.2))
+    oadj, omed = orc.parth_build(base, R=64, L=128, alphas=(1.0, 1.2))
+    assert idx.medoid == omed
+    assert np.array_equal(idx.adjacency, oadj)
+
+
+def test_gpu_groundtruth_matches_oracle_and_ivecs(vmn, orc, sift, gpu):
+    base = sift["base"].astype(np.float32)
+    q = sift["query"].astype(np.float32)
+    ids, d = vmn.groundtruth(base, q, 100)
+    oi, od = orc.bruteforce_fp64(base, q, 100)
+    assert np.array_equal(ids, oi) and np.array_equal(d, od)
+    assert all(set(ids[i, :10]) == set(sift["gt100"][i, :10]) for i in range(100))
+    ids10, d10 = vmn.groundtruth(base, q, 10)
+    assert np.array_equal(ids10, oi[:, :10])
+
+
+def test_gpu_search_device_tensors(vmn, sift, gpu):
+    import torch
+    base = torch.from_numpy(sift["base"].astype(np.float32)).to(gpu)
+    q = torch.from_numpy(sift["query"].astype(np.float32)).to(gpu)
+    idx = vmn.Vamana.from_graph(sift["base"].astype(np.float32), sift["parth_adj"], medoid=3732)
+    ids_h, d_h = idx.searchPoints(sift["query"].astype(np.float32), k=10, L=40)
+    ids, d = idx.searchPoints(q, k=10, L=40)
+    assert np.array_equal(ids.cpu().numpy(), ids_h) and np.array_equal(d.cpu().numpy(), d_h)
+    del base
+
+
+def test_gpu_edge_cases(vmn, orc, gpu):
+    rng = np.random.default_rng(3)
+    v = rng.integers(0, 20, size=(40, 32)).astype(np.float32)
+    idx = vmn.Vamana(v, mode="parth", R=8, L_build=16)
+    oadj, omed = orc.parth_build(v, R=8, L=16, alphas=(1.2, 1.2))
+    assert np.array_equal(idx.adjacency, oadj)
+    # k > number reachable within the beam -> -1 padded like engine.cu:84-86
+    q = rng.integers(0, 20, size=(3, 32)).astype(np.float32)
+    ids, d = idx.searchPoints(q, k=16, L=16)
+    oi, od, _, _ = orc.parth_search(v, oadj, q, omed, L=16, out_n=16)
+    assert np.array_equal(ids, oi) and np.array_equal(d, od)
+    # empty batch
+    e_ids, _ = idx.searchPoints(np.zeros((0, 32), np.float32), k=4, L=8)
+    assert e_ids.shape == (0, 4)
+    # odd dimension is zero padded internally
+    v5 = rng.integers(0, 9, size=(50, 5)).astype(np.float32)
+    idx5 = vmn.Vamana(v5, mode="parth", R=6, L_build=12)
+    a5, m5 = orc.parth_build(v5, R=6, L=12, alphas=(1.2, 1.2))
+    assert idx5.medoid == m5 and np.array_equal(idx5.adjacency, a5)
