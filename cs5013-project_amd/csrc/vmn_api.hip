@@ -146,7 +146,8 @@ int alloc_graph(vmn_index* idx, int64_t n) {
 
 // Search kernel over `nq` padded query rows.
 int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int32_t* out_ids,
-               float* out_d, bool queries_are_nodes, int mode, bool want_vis) {
+               float* out_d, bool queries_are_nodes, int mode, bool want_vis,
+               int64_t node_base = 0) {
     int rc = zero_counter(idx);
     if (rc) return rc;
     HIPCHK(idx->stat_hops.reserve((size_t)nq * sizeof(int32_t)));
@@ -161,7 +162,7 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
     a.queries = q;
     a.nq = nq;
     a.query_is_node = queries_are_nodes ? 1 : 0;
-    a.node_base = 0;
+    a.node_base = node_base;
     a.start = (int32_t)idx->medoid;
     a.L = L;
     a.visited_cap = idx->p.visited_cap;
@@ -229,7 +230,7 @@ int fresh_pass(vmn_index* idx, int64_t base, int64_t cnt, float alpha) {
     const int R = idx->p.R;
     const int64_t N = idx->n;
     int rc = run_search(idx, idx->vecs.as<float>() + base * idx->DP, cnt, idx->p.L_build, 0,
-                        nullptr, nullptr, true, VMN_MODE_FRESH, true);
+                        nullptr, nullptr, true, VMN_MODE_FRESH, true, base);
     if (rc) return rc;
     HIPCHK(idx->pair_dst.reserve((size_t)cnt * R * sizeof(int32_t)));
     HIPCHK(idx->pair_d.reserve((size_t)cnt * R * sizeof(float)));

@@ -324,6 +324,259 @@ int64_t orc_parth_build(const float* vecs, int64_t n, int d, int R, int row_widt
 }
 
 // ---------------------------------------------------------------------------
+// fresh mode (G2: vamana/include/search.cuh, kernels.cuh, bloom_filter.cuh)
+// ---------------------------------------------------------------------------
+
+// vamana/include/bloom_filter.cuh:11-30
+uint32_t orc_bf_hash1(uint32_t x) {
+    uint64_t h = 0xcbf29ce4ull;
+    h = (h ^ (x & 0xffu)) * 0x01000193ull;
+    h = (h ^ ((x >> 8) & 0xffu)) * 0x01000193ull;
+    h = (h ^ ((x >> 16) & 0xffu)) * 0x01000193ull;
+    h = (h ^ ((x >> 24) & 0xffu)) * 0x01000193ull;
+    return (uint32_t)(h % 39988u);
+}
+uint32_t orc_bf_hash2(uint32_t x) {
+    uint64_t h = 0x84222325ull;
+    h = (h ^ (x & 0xffu)) * 0x1B3ull;
+    h = (h ^ ((x >> 8) & 0xffu)) * 0x1B3ull;
+    h = (h ^ ((x >> 16) & 0xffu)) * 0x1B3ull;
+    h = (h ^ ((x >> 24) & 0xffu)) * 0x1B3ull;
+    return (uint32_t)(h % 39988u);
+}
+
+struct Item {
+    int32_t id;
+    float d;
+    bool vis;
+};
+
+// Greedy search of vamana/include/search.cuh:226-367 for one query, in the
+// canonical order: the worklist (cap L) starts with the start node; each
+// iteration expands the closest unexpanded entry (search.cuh:205-221), records
+// it in the visited set (cap P, :104-111), takes its neighbours in adjacency
+// order, drops tombstoned ones (:122), the query's own node (:126; only when
+// the query is a graph node) and those whose two bloom bits are both set in
+// the filter state from before this expansion (:130), sets the bits of the
+// rest, sorts them stably by distance (sortByDistance) and merges them into
+// the worklist with worklist entries first on ties (mergeIntoWorklist
+// :140-223), truncating to L. The start node is entered into the filter.
+static void fresh_search_one(const float* vecs, int d, const int32_t* adj, int W,
+                             const uint8_t* del, const float* q, int32_t self, int32_t start,
+                             int L, int max_hops, int P, int dist_mode, int out_n,
+                             int32_t* out_ids, float* out_d, int32_t* vis_ids, float* vis_d,
+                             int32_t* vis_cnt, int32_t* out_hops, int32_t* out_ndist) {
+    std::vector<uint8_t> bloom(39988, 0);
+    bloom[orc_bf_hash1((uint32_t)start)] = 1;
+    bloom[orc_bf_hash2((uint32_t)start)] = 1;
+    std::vector<Item> wl;
+    wl.push_back({start, l2sq_m(q, vecs + (int64_t)start * d, d, dist_mode), false});
+    int nexp = 0, hops = 0, ndist = 1;
+    std::vector<int32_t> news;
+    while (hops < max_hops) {
+        ++hops;
+        int p = -1;
+        for (size_t i = 0; i < wl.size(); ++i)
+            if (!wl[i].vis) { p = (int)i; break; }
+        if (p < 0) break;
+        wl[p].vis = true;
+        const int32_t node = wl[p].id;
+        ++nexp;
+        if (vis_ids && nexp <= P) {
+            vis_ids[nexp - 1] = node;
+            vis_d[nexp - 1] = wl[p].d;
+        }
+        news.clear();
+        const int32_t* nb = adj + (int64_t)node * W;
+        for (int j = 0; j < W; ++j) {
+            const int32_t v = nb[j];
+            if (v == -1) break;
+            if (del && del[v]) continue;
+            if (v == self) continue;
+            if (bloom[orc_bf_hash1((uint32_t)v)] && bloom[orc_bf_hash2((uint32_t)v)]) continue;
+            news.push_back(v);
+        }
+        std::vector<Item> add;
+        for (int32_t v : news) {
+            bloom[orc_bf_hash1((uint32_t)v)] = 1;
+            bloom[orc_bf_hash2((uint32_t)v)] = 1;
+            add.push_back({v, l2sq_m(q, vecs + (int64_t)v * d, d, dist_mode), false});
+            ++ndist;
+        }
+        std::stable_sort(add.begin(), add.end(),
+                         [](const Item& a, const Item& b) { return a.d < b.d; });
+        std::vector<Item> merged;
+        merged.reserve(wl.size() + add.size());
+        size_t a = 0, b = 0;
+        while ((a < wl.size() || b < add.size()) && (int)merged.size() < L) {
+            if (b >= add.size() || (a < wl.size() && wl[a].d <= add[b].d)) merged.push_back(wl[a++]);
+            else merged.push_back(add[b++]);
+        }
+        wl.swap(merged);
+    }
+    for (int i = 0; i < out_n; ++i) {
+        out_ids[i] = i < (int)wl.size() ? wl[i].id : -1;
+        if (out_d) out_d[i] = i < (int)wl.size() ? wl[i].d : 1e30f;
+    }
+    if (vis_cnt) *vis_cnt = std::min(nexp, P);
+    if (out_hops) *out_hops = nexp;
+    if (out_ndist) *out_ndist = ndist;
+}
+
+void orc_fresh_search(const float* vecs, int64_t n, int d, const int32_t* adj, int W,
+                      const uint8_t* del, const float* queries, int64_t nq, int queries_are_nodes,
+                      int32_t start, int L, int max_hops, int P, int dist_mode, int out_n,
+                      int32_t* out_ids, float* out_dists, int32_t* vis_ids, float* vis_d,
+                      int32_t* vis_cnt, int32_t* out_hops, int32_t* out_ndist) {
+    (void)n;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t qi = 0; qi < nq; ++qi) {
+        fresh_search_one(vecs, d, adj, W, del, queries + qi * d,
+                         queries_are_nodes ? (int32_t)qi : -1, start, L, max_hops, P, dist_mode,
+                         out_n, out_ids ? out_ids + qi * out_n : nullptr,
+                         out_dists ? out_dists + qi * out_n : nullptr,
+                         vis_ids ? vis_ids + qi * P : nullptr, vis_d ? vis_d + qi * P : nullptr,
+                         vis_cnt ? vis_cnt + qi : nullptr, out_hops ? out_hops + qi : nullptr,
+                         out_ndist ? out_ndist + qi : nullptr);
+    }
+}
+
+// Robust prune with `<=` (vamana/include/kernels.cuh:231-356, 626-743) over a
+// sorted candidate list: the first still-INIT candidate becomes p* (while
+// fewer than R are kept), then every INIT candidate p' with
+// d(p*, p') * alpha <= d(node, p') is PRUNED. Returns kept ids and dists.
+static void prune_le(const float* vecs, int d, const std::vector<Item>& c, float alpha, int R,
+                     int dist_mode, std::vector<int32_t>& out, std::vector<float>& outd) {
+    out.clear();
+    outd.clear();
+    std::vector<uint8_t> st(c.size(), 0);  // 0 INIT, 1 PRUNED, 2 NEIGHBOR
+    while ((int)out.size() < R) {
+        int p = -1;
+        for (size_t i = 0; i < c.size(); ++i)
+            if (st[i] == 0) { p = (int)i; break; }
+        if (p < 0) break;
+        st[p] = 2;
+        out.push_back(c[p].id);
+        outd.push_back(c[p].d);
+        const float* pv = vecs + (int64_t)c[p].id * d;
+        for (size_t j = 0; j < c.size(); ++j) {
+            if (st[j] != 0) continue;
+            float dp = l2sq_m(pv, vecs + (int64_t)c[j].id * d, d, dist_mode);
+            float lhs = dp * alpha;
+            if (lhs <= c[j].d) st[j] = 1;
+        }
+    }
+}
+
+// Out-neighbour prune of vamana/include/kernels.cuh:358-500 for nodes
+// [base, base+cnt): candidates = the visited set stably sorted by distance,
+// merged (visited entries first on ties, mergeIntoVisitedSet :171-229) with
+// the current neighbours stably sorted by distance, capped at P; the node
+// itself is left out. Writes the new rows and, per kept p*, the reverse pair
+// (p* <- node, d(node, p*)) into pair_dst/pair_d [cnt x R] (-1 unused).
+void orc_fresh_out_prune(const float* vecs, int d, int32_t* adj, int W, int R, float alpha,
+                         int64_t base, int64_t cnt, const int32_t* vis_ids, const float* vis_d,
+                         const int32_t* vis_cnt, int P, int dist_mode, int32_t* pair_dst,
+                         float* pair_d) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < cnt; ++i) {
+        const int64_t node = base + i;
+        std::vector<Item> A, B, C;
+        for (int j = 0; j < vis_cnt[i]; ++j) {
+            int32_t v = vis_ids[i * P + j];
+            if (v != (int32_t)node) A.push_back({v, vis_d[i * P + j], false});
+        }
+        std::stable_sort(A.begin(), A.end(), [](const Item& a, const Item& b) { return a.d < b.d; });
+        int32_t* row = adj + node * W;
+        const float* nv = vecs + node * d;
+        for (int j = 0; j < W; ++j) {
+            int32_t v = row[j];
+            if (v == -1 || v == (int32_t)node) continue;
+            B.push_back({v, l2sq_m(nv, vecs + (int64_t)v * d, d, dist_mode), false});
+        }
+        std::stable_sort(B.begin(), B.end(), [](const Item& a, const Item& b) { return a.d < b.d; });
+        size_t a = 0, b = 0;
+        while ((a < A.size() || b < B.size()) && (int)C.size() < P) {
+            if (b >= B.size() || (a < A.size() && A[a].d <= B[b].d)) C.push_back(A[a++]);
+            else C.push_back(B[b++]);
+        }
+        std::vector<int32_t> out;
+        std::vector<float> outd;
+        prune_le(vecs, d, C, alpha, R, dist_mode, out, outd);
+        for (int j = 0; j < W; ++j) row[j] = j < (int)out.size() ? out[j] : -1;
+        for (int j = 0; j < R; ++j) {
+            pair_dst[i * R + j] = j < (int)out.size() ? out[j] : -1;
+            pair_d[i * R + j] = j < (int)out.size() ? outd[j] : 0.0f;
+        }
+    }
+}
+
+// Reverse-edge pass of vamana/include/kernels.cuh:745-903 for every node that
+// received at least one reverse pair: reverse candidates ordered by
+// (distance, source id) and capped at M, merged with the node's current
+// neighbours (stable by distance; reverse candidates first on ties,
+// mergeIntoReverseEdges :566-624), capped at M, pruned with <= into R.
+void orc_fresh_reverse(const float* vecs, int64_t n, int d, int32_t* adj, int W, int R,
+                       float alpha, int64_t base, int64_t cnt, const int32_t* pair_dst,
+                       const float* pair_d, int M, int dist_mode) {
+    std::vector<std::vector<Item>> rev(n);
+    for (int64_t i = 0; i < cnt; ++i)
+        for (int j = 0; j < R; ++j) {
+            int32_t dst = pair_dst[i * R + j];
+            if (dst >= 0) rev[dst].push_back({(int32_t)(base + i), pair_d[i * R + j], false});
+        }
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t node = 0; node < n; ++node) {
+        std::vector<Item>& A = rev[node];
+        if (A.empty()) continue;
+        std::sort(A.begin(), A.end(), [](const Item& a, const Item& b) {
+            return a.d < b.d || (a.d == b.d && a.id < b.id);
+        });
+        if ((int)A.size() > M) A.resize(M);
+        std::vector<Item> B, C;
+        int32_t* row = adj + node * W;
+        const float* nv = vecs + node * d;
+        for (int j = 0; j < W; ++j) {
+            int32_t v = row[j];
+            if (v == -1 || v == (int32_t)node) continue;
+            B.push_back({v, l2sq_m(nv, vecs + (int64_t)v * d, d, dist_mode), false});
+        }
+        std::stable_sort(B.begin(), B.end(), [](const Item& a, const Item& b) { return a.d < b.d; });
+        size_t a = 0, b = 0;
+        while ((a < A.size() || b < B.size()) && (int)C.size() < M) {
+            if (b >= B.size() || (a < A.size() && A[a].d <= B[b].d)) C.push_back(A[a++]);
+            else C.push_back(B[b++]);
+        }
+        std::vector<int32_t> out;
+        std::vector<float> outd;
+        prune_le(vecs, d, C, alpha, R, dist_mode, out, outd);
+        for (int j = 0; j < W; ++j) row[j] = j < (int)out.size() ? out[j] : -1;
+    }
+}
+
+// FreshVamana build (vamana_impl.cuh:289-360 runVamana) from the parth random
+// initial graph and the computed medoid, `passes` times: search every node,
+// out-prune every node, reverse-edge prune the nodes that received pairs.
+int64_t orc_fresh_build(const float* vecs, int64_t n, int d, int R, int L, const float* alphas,
+                        int passes, uint64_t seed, int P, int M, int max_hops, int dist_mode,
+                        int32_t* adj) {
+    orc_random_init(n, R, R, seed, adj);
+    int64_t medoid = orc_medoid(vecs, n, d);
+    std::vector<int32_t> vis((size_t)n * P), vcnt(n), pdst((size_t)n * R);
+    std::vector<float> visd((size_t)n * P), pd((size_t)n * R);
+    for (int p = 0; p < passes; ++p) {
+        orc_fresh_search(vecs, n, d, adj, R, nullptr, vecs, n, 1, (int32_t)medoid, L, max_hops, P,
+                         dist_mode, 0, nullptr, nullptr, vis.data(), visd.data(), vcnt.data(),
+                         nullptr, nullptr);
+        orc_fresh_out_prune(vecs, d, adj, R, R, alphas[p], 0, n, vis.data(), visd.data(),
+                            vcnt.data(), P, dist_mode, pdst.data(), pd.data());
+        orc_fresh_reverse(vecs, n, d, adj, R, R, alphas[p], 0, n, pdst.data(), pd.data(), M,
+                          dist_mode);
+    }
+    return medoid;
+}
+
+// ---------------------------------------------------------------------------
 // Exact k-NN ground truth in FP64 (replaces parth/src/engine.cu:253-279
 // `bruteForceKernel` (FP32) and the external DiskANN compute_groundtruth called
 // at vamana.cu:1644). Each distance is sum_k (double(q_k) - double(x_k))^2 in

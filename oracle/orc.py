@@ -40,6 +40,19 @@ def lib():
                                          i32, P]
         _lib.orc_parth_build.restype = i64
         _lib.orc_bruteforce_fp64.argtypes = [P, i64, i32, P, P, i64, i32, P, P]
+        _lib.orc_bf_hash1.argtypes = [ctypes.c_uint32]
+        _lib.orc_bf_hash1.restype = ctypes.c_uint32
+        _lib.orc_bf_hash2.argtypes = [ctypes.c_uint32]
+        _lib.orc_bf_hash2.restype = ctypes.c_uint32
+        _lib.orc_fresh_search.argtypes = [P, i64, i32, P, i32, P, P, i64, i32, i32, i32, i32,
+                                          i32, i32, i32, P, P, P, P, P, P, P]
+        _lib.orc_fresh_out_prune.argtypes = [P, i32, P, i32, i32, f32, i64, i64, P, P, P, i32,
+                                             i32, P, P]
+        _lib.orc_fresh_reverse.argtypes = [P, i64, i32, P, i32, i32, f32, i64, i64, P, P, i32,
+                                           i32]
+        _lib.orc_fresh_build.argtypes = [P, i64, i32, i32, i32, P, i32, u64, i32, i32, i32, i32,
+                                         P]
+        _lib.orc_fresh_build.restype = i64
         _lib.orc_num_threads.restype = i32
         _lib.orc_set_num_threads.argtypes = [i32]
     return _lib
@@ -108,6 +121,45 @@ def parth_build(vecs, R, L, alphas=(1.2, 1.2), row_width=None, seed=1234, visite
     al = np.asarray(alphas, dtype=np.float32)
     med = lib().orc_parth_build(_p(vecs), n, d, R, rw, L, _p(al), len(al), seed, visited_cap,
                                 max_hops, dist_mode, _p(adj))
+    return adj, int(med)
+
+
+def bf_hash(x):
+    return int(lib().orc_bf_hash1(x)), int(lib().orc_bf_hash2(x))
+
+
+def fresh_search(vecs, adj, queries, start, L, out_n=10, P=600, max_hops=1 << 20, dist_mode=0,
+                 deleted=None, queries_are_nodes=False, want_visited=False):
+    """G2 greedy search. Returns (ids, dists, hops, ndist[, vis_ids, vis_d, vis_cnt])."""
+    vecs, queries = _f32(vecs), _f32(queries)
+    adj = np.ascontiguousarray(adj, dtype=np.int32)
+    nq = queries.shape[0]
+    ids = np.empty((nq, out_n), dtype=np.int32)
+    dists = np.empty((nq, out_n), dtype=np.float32)
+    hops = np.empty(nq, dtype=np.int32)
+    nd = np.empty(nq, dtype=np.int32)
+    vi = np.full((nq, P), -1, dtype=np.int32) if want_visited else None
+    vd = np.zeros((nq, P), dtype=np.float32) if want_visited else None
+    vc = np.zeros(nq, dtype=np.int32) if want_visited else None
+    dm = None if deleted is None else np.ascontiguousarray(deleted, dtype=np.uint8)
+    lib().orc_fresh_search(_p(vecs), vecs.shape[0], vecs.shape[1], _p(adj), adj.shape[1], _p(dm),
+                           _p(queries), nq, int(queries_are_nodes), int(start), L, max_hops, P,
+                           dist_mode, out_n, _p(ids), _p(dists), _p(vi), _p(vd), _p(vc),
+                           _p(hops), _p(nd))
+    if want_visited:
+        return ids, dists, hops, nd, vi, vd, vc
+    return ids, dists, hops, nd
+
+
+def fresh_build(vecs, R=64, L=128, alphas=(1.5,), seed=1234, P=600, M=500, max_hops=1 << 20,
+                dist_mode=0):
+    """Returns (adj [n,R] int32, medoid)."""
+    vecs = _f32(vecs)
+    n, d = vecs.shape
+    adj = np.empty((n, R), dtype=np.int32)
+    al = np.asarray(alphas, dtype=np.float32)
+    med = lib().orc_fresh_build(_p(vecs), n, d, R, L, _p(al), len(al), seed, P, M, max_hops,
+                                dist_mode, _p(adj))
     return adj, int(med)
 
 

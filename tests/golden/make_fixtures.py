@@ -12,6 +12,9 @@ Outputs (committed):
                rows of data/sift10k/sift10k_groundtruth.bin, 10 per row),
                parth_adj (10000x32 i32, parth/data/sift_initial_graph.bin),
                custom_adj (5000x32 i32, parth/data/custom_sift_initial_graph.bin)
+  g1_graph.npz adj (10000x64 i32, -1 padded past each node's degree) and
+               degree of vamana/vout/vamana.out, the G1 GPU build of
+               data/sift10k/sift10k_randomgraph.bin (R=64, alpha 1.5, medoid 5000)
   toy2d.npz    toy_vecs/toy_adj (20 nodes) and vis_vecs/vis_adj (50 nodes),
                parth/data/{toy,vis}_initial_graph.bin (float 2-D data)
 
@@ -74,6 +77,13 @@ def main(ref):
         base=to_u8(base), query=to_u8(query), gt100=gt100.astype(np.int32),
         diskann_gt_ids=t_ids, diskann_gt_dists=t_d,
         parth_adj=padj, custom_adj=cadj)
+    # FreshVamana 772-byte records: float vec[128], uint32 degree, uint32 nbr[64]
+    rec = np.fromfile(os.path.join(ref, "vamana/vout/vamana.out"), dtype=np.uint8).reshape(-1, 772)
+    assert np.array_equal(rec[:, :512].copy().view(np.float32), base)
+    deg = rec[:, 512:516].copy().view(np.uint32).ravel().astype(np.int32)
+    nbr = rec[:, 516:].copy().view(np.uint32).astype(np.int64)
+    g1 = np.where(np.arange(64)[None, :] < deg[:, None], nbr, -1).astype(np.int32)
+    np.savez_compressed(os.path.join(HERE, "g1_graph.npz"), adj=g1, degree=deg)
     tv, tadj = read_parth_graph(os.path.join(ref, "parth/data/toy_initial_graph.bin"))
     vv, vadj = read_parth_graph(os.path.join(ref, "parth/data/vis_initial_graph.bin"))
     np.savez_compressed(os.path.join(HERE, "toy2d.npz"),

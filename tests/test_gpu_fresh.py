@@ -1,0 +1,58 @@
+"""GPU (HIP) FreshVamana-mode path (bloom-filter search, <= prune, reverse
+edges) against the CPU oracle's restatement of vamana/include/{search,kernels}.cuh.
+Bit-exact on integer-valued data."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("L", [10, 40, 150])
+def test_gpu_fresh_search_on_reference_g1_graph(vmn, orc, sift, gpu, L):
+    """Search the reference's own G1-built graph (vamana/vout/vamana.out)."""
+    import os
+    g1 = np.load(os.path.join(os.path.dirname(__file__), "golden", "g1_graph.npz"))
+    base = sift["base"].astype(np.float32)
+    q = sift["query"].astype(np.float32)
+    idx = vmn.Vamana.from_graph(base, g1["adj"], medoid=5000, mode="fresh")
+    ids, d = idx.searchPoints(q, k=10, L=L)
+    oi, od, oh, on = orc.fresh_search(base, g1["adj"], q, 5000, L=L)
+    assert np.array_equal(ids, oi) and np.array_equal(d, od)
+    hops, nd = idx.search_stats()
+    assert hops == int(oh.sum()) and nd == int(on.sum())
+
+
+def test_gpu_fresh_build_matches_oracle_sift10k(vmn, orc, sift, gpu):
+    base = sift["base"].astype(np.float32)
+    idx = vmn.Vamana(base, mode="fresh", R=64, L_build=150, alphas=(1.5,))
+    oadj, omed = orc.fresh_build(base, R=64, L=150, alphas=(1.5,))
+    assert idx.medoid == omed == 3732
+    adj = idx.adjacency
+    same = (adj == oadj).all(axis=1)
+    assert same.all(), f"{(~same).sum()} rows differ, first {np.flatnonzero(~same)[:5]}"
+    q = sift["query"].astype(np.float32)
+    for L in (10, 40):
+        ids, d = idx.searchPoints(q, k=10, L=L)
+        oi, od, _, _ = orc.fresh_search(base, oadj, q, omed, L=L)
+        assert np.array_equal(ids, oi) and np.array_equal(d, od)
+
+
+def test_gpu_fresh_two_pass_schedule_matches_oracle(vmn, orc, gpu):
+    ds = vmn.datasets
+    base = ds.integer_mixture(8000, 128, k=64, seed=1234, stream=0)
+    idx = vmn.Vamana(base, mode="fresh", R=64, L_build=128, alphas=(1.0, 1.2))
+    oadj, omed = orc.fresh_build(base, R=64, L=128, alphas=(1.0, 1.2))
+    assert idx.medoid == omed
+    assert np.array_equal(idx.adjacency, oadj)
+
+
+def test_gpu_fresh_graph_invariants(vmn, gpu):
+    ds = vmn.datasets
+    base = ds.integer_mixture(6000, 128, k=32, seed=5, stream=0)
+    idx = vmn.Vamana(base, mode="fresh", R=32, L_build=64, alphas=(1.2,))
+    adj = idx.adjacency
+    for i, row in enumerate(adj):
+        r = row[row >= 0]
+        assert i not in r and len(set(r.tolist())) == len(r)
+        # -1 only as a tail
+        assert (row[len(r):] == -1).all()
