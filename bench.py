@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--nq", type=int, default=10_000)
-    ap.add_argument("--mode", default="parth", choices=["parth", "fresh"])
+    ap.add_argument("--mode", default="fresh", choices=["parth", "fresh"])
     ap.add_argument("--R", type=int, default=64)
     ap.add_argument("--L-build", type=int, default=128)
     ap.add_argument("--alphas", default="1.0,1.2")
@@ -164,15 +164,14 @@ def cpu_baseline(idx, base, qs, gt, L, seconds, mode):
     adj = idx.adjacency
     threads = orc.num_threads()
     med = idx.medoid
-    if mode != "parth":
-        return None
+    search = orc.parth_search if mode == "parth" else orc.fresh_search
     n0 = min(64, qs.shape[0])
     t = time.time()
-    orc.parth_search(base, adj, qs[:n0], med, L=L, out_n=10)
+    search(base, adj, qs[:n0], med, L=L, out_n=10)
     per_q = (time.time() - t) / n0
     ns = int(min(qs.shape[0], max(n0, seconds / max(per_q, 1e-6))))
     t = time.time()
-    ids, _, _, _ = orc.parth_search(base, adj, qs[:ns], med, L=L, out_n=10)
+    ids, _, _, _ = search(base, adj, qs[:ns], med, L=L, out_n=10)
     el = time.time() - t
     log(f"cpu baseline: {ns} queries in {el:.1f} s on {threads} threads")
     return {"value": round(ns / el, 1), "unit": "queries/s", "cores": threads, "kind": "port",
