@@ -11,6 +11,7 @@
 
 #include "../../include/vamana_mi355x.h"
 #include "vmn_kernels.hpp"
+#include "vmn_update.hpp"
 
 namespace {
 
@@ -79,6 +80,8 @@ struct vmn_index {
     DevBuf pair_dst, pair_d, rcount, rfill, roff, rscan_tmp, rsrc, rd, rlist;
     DevBuf centroid, medoid_key;
     int64_t last_nq = 0;
+    int64_t n_deleted = 0;  // tombstones set since the build (upper bound)
+    DevBuf staging, find_out, bad;
 };
 
 namespace {
@@ -125,7 +128,8 @@ int compute_medoid(vmn_index* idx) {
     HIPCHK(idx->centroid.reserve((size_t)idx->DP * sizeof(float)));
     HIPCHK(idx->medoid_key.reserve(64));
     HIPCHK(vmn::launch_medoid(idx->vecs.as<float>(), idx->n, idx->p.dim, idx->DP,
-                              idx->centroid.as<float>(), nullptr,
+                              idx->n_deleted ? idx->del_bits.as<uint32_t>() : nullptr,
+                              idx->centroid.as<float>(),
                               idx->medoid_key.as<unsigned long long>(), idx->stream));
     unsigned long long key = 0;
     HIPCHK(hipMemcpyAsync(&key, idx->medoid_key.p, sizeof key, hipMemcpyDeviceToHost, idx->stream));
@@ -137,6 +141,7 @@ int compute_medoid(vmn_index* idx) {
 int alloc_graph(vmn_index* idx, int64_t n) {
     const int64_t cap = std::max<int64_t>(idx->p.capacity, n);
     idx->cap = cap;
+    idx->n_deleted = 0;
     HIPCHK(idx->vecs.reserve((size_t)cap * idx->DP * sizeof(float)));
     HIPCHK(idx->adj.reserve((size_t)cap * idx->p.R * sizeof(int32_t)));
     HIPCHK(idx->del_bits.reserve((size_t)((cap + 31) / 32) * sizeof(uint32_t)));
@@ -157,7 +162,7 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
     a.DP = idx->DP;
     a.vecs = idx->vecs.as<float>();
     a.adj = idx->adj.as<int32_t>();
-    a.del_bits = idx->del_bits.as<uint32_t>();
+    a.del_bits = idx->n_deleted ? idx->del_bits.as<uint32_t>() : nullptr;
     a.W = idx->p.R;
     a.queries = q;
     a.nq = nq;
@@ -252,6 +257,7 @@ int fresh_pass(vmn_index* idx, int64_t base, int64_t cnt, float alpha) {
     a.P = idx->p.P;
     a.pair_dst = idx->pair_dst.as<int32_t>();
     a.pair_d = idx->pair_d.as<float>();
+    a.del_bits = idx->n_deleted ? idx->del_bits.as<uint32_t>() : nullptr;
     a.M = idx->p.M;
     a.counter = idx->counter.as<unsigned long long>();
     HIPCHK(vmn::launch_fresh_out_prune(a, prune_grid(idx, cnt), idx->stream));
@@ -299,6 +305,19 @@ int build_fresh(vmn_index* idx) {
         int rc = fresh_pass(idx, 0, idx->n, idx->p.alpha[pass]);
         if (rc) return rc;
     }
+    return VMN_OK;
+}
+
+int set_tombstones(vmn_index* idx, const int32_t* dev_ids, int64_t m) {
+    HIPCHK(idx->bad.reserve(64));
+    HIPCHK(vmn::launch_fill_u64(idx->bad.as<unsigned long long>(), 1, 0ull, idx->stream));
+    HIPCHK(vmn::launch_tombstone(dev_ids, m, idx->n, idx->del_bits.as<uint32_t>(),
+                                 idx->bad.as<unsigned long long>(), idx->stream));
+    unsigned long long bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, idx->bad.p, sizeof bad, hipMemcpyDeviceToHost, idx->stream));
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    idx->n_deleted += m - (int64_t)bad;
+    if (bad) return fail(VMN_ERR_ARG, "%llu ids outside [0, %lld) were ignored", bad, (long long)idx->n);
     return VMN_OK;
 }
 
@@ -378,7 +397,7 @@ int vmn_destroy(vmn_index* idx) {
                       &idx->cand_ids, &idx->cand_d, &idx->vis_ids, &idx->vis_d, &idx->vis_cnt,
                       &idx->pair_dst, &idx->pair_d, &idx->rcount, &idx->rfill, &idx->roff,
                       &idx->rscan_tmp, &idx->rsrc, &idx->rd, &idx->rlist, &idx->centroid,
-                      &idx->medoid_key};
+                      &idx->medoid_key, &idx->staging, &idx->find_out, &idx->bad};
     for (DevBuf* b : bufs) b->release();
     (void)hipStreamDestroy(idx->stream);
     delete idx;
@@ -586,6 +605,166 @@ int vmn_groundtruth(const float* base, int64_t n, int32_t dim, const float* quer
     pi.release();
     oi.release();
     od.release();
+    return VMN_OK;
+}
+
+int vmn_insert(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out, int32_t mem) {
+    if (!idx || (!vecs && n)) return fail(VMN_ERR_ARG, "NULL argument");
+    if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");
+    if (n < 0) return fail(VMN_ERR_ARG, "negative count");
+    if (n == 0) return VMN_OK;
+    if (idx->n + n > idx->cap)
+        return fail(VMN_ERR_CAPACITY, "insert of %lld exceeds capacity %lld (size %lld)",
+                    (long long)n, (long long)idx->cap, (long long)idx->n);
+    int rc = set_device(idx);
+    if (rc) return rc;
+    const int64_t base = idx->n;
+    const int DP = idx->DP;
+    float* dst = idx->vecs.as<float>() + base * DP;
+    if (mem == VMN_MEM_DEVICE) {
+        HIPCHK(vmn::launch_pad_rows(vecs, n, idx->p.dim, dst, DP, idx->stream));
+    } else {
+        const float* staged = nullptr;
+        rc = stage_rows(idx, vecs, n, mem, idx->staging, &staged);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(dst, staged, (size_t)n * DP * sizeof(float), hipMemcpyDeviceToDevice,
+                              idx->stream));
+    }
+    HIPCHK(vmn::launch_fill_i32(idx->adj.as<int32_t>() + base * idx->p.R, n * idx->p.R, -1,
+                                idx->stream));
+    idx->n = base + n;
+    const int last = idx->p.passes > 0 ? idx->p.passes - 1 : 0;
+    rc = fresh_pass(idx, base, n, idx->p.alpha[last]);
+    if (rc) return rc;
+    std::vector<int32_t> ids(n);
+    for (int64_t i = 0; i < n; ++i) ids[i] = (int32_t)(base + i);
+    if (ids_out && mem == VMN_MEM_HOST) std::memcpy(ids_out, ids.data(), n * sizeof(int32_t));
+    if (ids_out && mem == VMN_MEM_DEVICE)
+        HIPCHK(hipMemcpyAsync(ids_out, ids.data(), n * sizeof(int32_t), hipMemcpyHostToDevice,
+                              idx->stream));
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    return VMN_OK;
+}
+
+int vmn_delete_ids(vmn_index* idx, const int32_t* ids, int64_t n, int32_t mem) {
+    if (!idx || (!ids && n)) return fail(VMN_ERR_ARG, "NULL argument");
+    if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");
+    if (n <= 0) return n == 0 ? VMN_OK : fail(VMN_ERR_ARG, "negative count");
+    int rc = set_device(idx);
+    if (rc) return rc;
+    const int32_t* d = ids;
+    if (mem == VMN_MEM_HOST) {
+        HIPCHK(idx->staging.reserve((size_t)n * sizeof(int32_t)));
+        HIPCHK(hipMemcpyAsync(idx->staging.p, ids, n * sizeof(int32_t), hipMemcpyHostToDevice,
+                              idx->stream));
+        d = idx->staging.as<int32_t>();
+    }
+    return set_tombstones(idx, d, n);
+}
+
+int vmn_delete_vectors(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out, int32_t mem) {
+    if (!idx || (!vecs && n)) return fail(VMN_ERR_ARG, "NULL argument");
+    if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");
+    if (n <= 0) return n == 0 ? VMN_OK : fail(VMN_ERR_ARG, "negative count");
+    if (n > 65535) return fail(VMN_ERR_ARG, "at most 65535 vectors per call");
+    int rc = set_device(idx);
+    if (rc) return rc;
+    const float* q = nullptr;
+    rc = stage_rows(idx, vecs, n, mem, idx->qpad, &q);
+    if (rc) return rc;
+    HIPCHK(idx->find_out.reserve((size_t)n * sizeof(int32_t)));
+    HIPCHK(vmn::launch_fill_i32(idx->find_out.as<int32_t>(), n, -1, idx->stream));
+    HIPCHK(vmn::launch_find_vectors(idx->vecs.as<float>(), idx->n, idx->p.dim, idx->DP,
+                                    idx->del_bits.as<uint32_t>(), q, n,
+                                    idx->find_out.as<uint32_t>(), idx->stream));
+    std::vector<int32_t> found(n);
+    HIPCHK(hipMemcpyAsync(found.data(), idx->find_out.p, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          idx->stream));
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    // not-found vectors are reported as -1 and not tombstoned (the reference
+    // appends the -1 to its delete list, vamana_impl.cuh:49-50)
+    std::vector<int32_t> hits;
+    for (int32_t v : found)
+        if (v >= 0) hits.push_back(v);
+    if (!hits.empty()) {
+        HIPCHK(idx->staging.reserve(hits.size() * sizeof(int32_t)));
+        HIPCHK(hipMemcpyAsync(idx->staging.p, hits.data(), hits.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice, idx->stream));
+        rc = set_tombstones(idx, idx->staging.as<int32_t>(), (int64_t)hits.size());
+        if (rc) return rc;
+    }
+    if (ids_out && mem == VMN_MEM_HOST) std::memcpy(ids_out, found.data(), n * sizeof(int32_t));
+    if (ids_out && mem == VMN_MEM_DEVICE) {
+        HIPCHK(hipMemcpyAsync(ids_out, found.data(), n * sizeof(int32_t), hipMemcpyHostToDevice,
+                              idx->stream));
+        HIPCHK(hipStreamSynchronize(idx->stream));
+    }
+    return VMN_OK;
+}
+
+int vmn_consolidate(vmn_index* idx) {
+    if (!idx) return fail(VMN_ERR_ARG, "NULL index");
+    if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");
+    if (idx->n_deleted == 0) return VMN_OK;
+    int rc = set_device(idx);
+    if (rc) return rc;
+    const int64_t N = idx->n;
+    const int R = idx->p.R;
+    HIPCHK(idx->rcount.reserve((size_t)N * sizeof(int32_t)));
+    HIPCHK(idx->rlist.reserve((size_t)N * sizeof(int64_t)));
+    HIPCHK(vmn::launch_cons_flags(idx->adj.as<int32_t>(), N, R, idx->del_bits.as<uint32_t>(),
+                                  idx->rcount.as<int32_t>(), idx->stream));
+    rc = zero_counter(idx);
+    if (rc) return rc;
+    HIPCHK(vmn::launch_compact_nonzero(idx->rcount.as<int32_t>(), N, idx->rlist.as<int64_t>(),
+                                       idx->counter.as<unsigned long long>(), idx->stream));
+    unsigned long long nlist = 0;
+    HIPCHK(hipMemcpyAsync(&nlist, idx->counter.p, sizeof nlist, hipMemcpyDeviceToHost, idx->stream));
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    if (nlist) {
+        rc = zero_counter(idx);
+        if (rc) return rc;
+        vmn::ConsArgs a{};
+        a.DP = idx->DP;
+        a.vecs = idx->vecs.as<float>();
+        a.adj = idx->adj.as<int32_t>();
+        a.del_bits = idx->del_bits.as<uint32_t>();
+        a.W = R;
+        a.R = R;
+        a.M = idx->p.M;
+        a.alpha = idx->p.alpha[idx->p.passes > 0 ? idx->p.passes - 1 : 0];
+        a.list = idx->rlist.as<int64_t>();
+        a.nlist = (int64_t)nlist;
+        a.counter = idx->counter.as<unsigned long long>();
+        HIPCHK(vmn::launch_consolidate(a, prune_grid(idx, (int64_t)nlist), idx->stream));
+    }
+    HIPCHK(vmn::launch_clear_deleted_rows(idx->adj.as<int32_t>(), N, R, idx->del_bits.as<uint32_t>(),
+                                          idx->stream));
+    // move the entry point off a tombstoned node
+    uint32_t word = 0;
+    HIPCHK(hipMemcpyAsync(&word, idx->del_bits.as<uint32_t>() + (idx->medoid >> 5), sizeof word,
+                          hipMemcpyDeviceToHost, idx->stream));
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    if ((word >> (idx->medoid & 31)) & 1u) {
+        rc = compute_medoid(idx);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    return VMN_OK;
+}
+
+int vmn_get_deleted(const vmn_index* idx, uint8_t* mask_out) {
+    if (!idx || !mask_out) return fail(VMN_ERR_ARG, "NULL argument");
+    int rc = set_device(idx);
+    if (rc) return rc;
+    const int64_t words = (idx->n + 31) / 32;
+    std::vector<uint32_t> bits(words);
+    if (words) {
+        HIPCHK(hipMemcpyAsync(bits.data(), idx->del_bits.p, words * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, idx->stream));
+        HIPCHK(hipStreamSynchronize(idx->stream));
+    }
+    for (int64_t i = 0; i < idx->n; ++i) mask_out[i] = (bits[i >> 5] >> (i & 31)) & 1u;
     return VMN_OK;
 }
 

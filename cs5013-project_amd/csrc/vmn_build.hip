@@ -1,6 +1,6 @@
 // Graph construction kernels: random initial graph, medoid, alpha robust
 // prune (PARTH and FRESH forms) and reverse-edge insertion.
-#include "vmn_kernels.hpp"
+#include "vmn_prune.hpp"
 
 namespace vmn {
 
@@ -83,31 +83,38 @@ hipError_t launch_random_init(int32_t* adj, int64_t n, int R, int W, uint64_t se
 // id with the smallest distance.
 // ---------------------------------------------------------------------------
 __global__ void centroid_kernel(const float* __restrict__ vecs, int64_t n, int D, int DP,
-                                float* centroid) {
+                                const uint32_t* del_bits, float* centroid) {
 #pragma clang fp contract(off)
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= D) return;
     constexpr int UNR = 32;
     float acc = 0.0f;
-    int64_t i = 0;
-    for (; i + UNR <= n; i += UNR) {
-        float v[UNR];
+    int64_t i = 0, live = 0;
+    if (!del_bits) {
+        for (; i + UNR <= n; i += UNR) {
+            float v[UNR];
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) v[u] = vecs[(i + u) * DP + d];
+            for (int u = 0; u < UNR; ++u) v[u] = vecs[(i + u) * DP + d];
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) acc = acc + v[u];
+            for (int u = 0; u < UNR; ++u) acc = acc + v[u];
+        }
+        live = i;
     }
-    for (; i < n; ++i) acc = acc + vecs[i * DP + d];
-    centroid[d] = acc / (float)n;
+    for (; i < n; ++i) {
+        if (del_bits && bit_test(del_bits, (uint32_t)i)) continue;
+        acc = acc + vecs[i * DP + d];
+        ++live;
+    }
+    centroid[d] = live ? acc / (float)live : 0.0f;
 }
 
 __global__ void centroid_dist_kernel(const float* __restrict__ vecs, int64_t n, int D, int DP,
-                                     const float* __restrict__ centroid,
+                                     const uint32_t* del_bits, const float* __restrict__ centroid,
                                      unsigned long long* best) {
 #pragma clang fp contract(off)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long key = ~0ull;
-    if (i < n) {
+    if (i < n && !(del_bits && bit_test(del_bits, (uint32_t)i))) {
         float s = 0.0f;
         for (int k = 0; k < D; ++k) {
             const float diff = vecs[i * DP + k] - centroid[k];
@@ -125,108 +132,17 @@ __global__ void centroid_dist_kernel(const float* __restrict__ vecs, int64_t n, 
     if (lane() == 0 && key != ~0ull) atomicMin(best, key);
 }
 
-hipError_t launch_medoid(const float* vecs, int64_t n, int D, int DP, float* centroid,
-                         float* node_d, unsigned long long* best, hipStream_t s) {
-    (void)node_d;
-    hipLaunchKernelGGL(centroid_kernel, dim3((D + 63) / 64), dim3(64), 0, s, vecs, n, D, DP, centroid);
+hipError_t launch_medoid(const float* vecs, int64_t n, int D, int DP, const uint32_t* del_bits,
+                         float* centroid, unsigned long long* best, hipStream_t s) {
+    hipLaunchKernelGGL(centroid_kernel, dim3((D + 63) / 64), dim3(64), 0, s, vecs, n, D, DP,
+                       del_bits, centroid);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = launch_fill_u64(best, 1, ~0ull, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(centroid_dist_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                       vecs, n, D, DP, centroid, best);
+                       vecs, n, D, DP, del_bits, centroid, best);
     return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-// Robust prune of a sorted candidate list held in LDS.
-//
-// Candidates are taken in list order; a candidate that is still kept becomes
-// p* and is added to the output, and every later kept candidate p' is dropped
-// when alpha * d(p*, p') < d(node, p') (STRICT, parth/src/build_graph.cu:
-// 241-258) or <= (vamana/include/kernels.cuh:329-353). The product alpha*d is
-// one float rounding, compared without fusion. Stops once R are kept. The
-// d(p*, p') of one round are computed by the whole wave (8 rows at a time).
-// ---------------------------------------------------------------------------
-template <int DP, bool STRICT>
-__device__ int robust_prune(const float* __restrict__ vecs, const uint32_t* c_id,
-                            const float* c_d, int cnt, float alpha, int R, uint8_t* s_kept,
-                            int32_t* s_act_id, int32_t* s_act_pos, float* s_act_d,
-                            int32_t* out_ids, float* out_d) {
-    const int l = lane(), sub = l & 7;
-    for (int i = l; i < cnt; i += 64) s_kept[i] = 1;
-    __syncthreads();
-    int nk = 0, from = 0;
-    while (nk < R) {
-        int p = -1;
-        for (int b = from; b < cnt; b += 64) {
-            const int j = b + l;
-            const uint64_t m = ballot(j < cnt && s_kept[j]);
-            if (m) {
-                p = b + __ffsll((unsigned long long)m) - 1;
-                break;
-            }
-        }
-        if (p < 0) break;
-        const int32_t ps = (int32_t)(c_id[p] & VMN_ID_MASK);
-        if (l == 0) {
-            out_ids[nk] = ps;
-            if (out_d) out_d[nk] = c_d[p];
-        }
-        ++nk;
-        if (nk >= R) break;
-        QReg<DP> pq;
-        pq.load(vecs + (int64_t)ps * DP, sub);
-        int nact = 0;
-        for (int b = p + 1; b < cnt; b += 64) {
-            const int j = b + l;
-            const bool f = j < cnt && s_kept[j];
-            const uint64_t m = ballot(f);
-            const int k = nact + __popcll((unsigned long long)(m & lanemask_lt()));
-            if (f) {
-                s_act_id[k] = (int32_t)(c_id[j] & VMN_ID_MASK);
-                s_act_pos[k] = j;
-            }
-            nact += __popcll((unsigned long long)m);
-        }
-        __syncthreads();
-        gather_dists<DP>(pq, vecs, s_act_id, s_act_d, nact);
-        __syncthreads();
-        for (int k = l; k < nact; k += 64) {
-            const int j = s_act_pos[k];
-            const float lhs = alpha * s_act_d[k];
-            const bool pr = STRICT ? (lhs < c_d[j]) : (lhs <= c_d[j]);
-            if (pr) s_kept[j] = 0;
-        }
-        __syncthreads();
-        from = p + 1;
-    }
-    __syncthreads();
-    return nk;
-}
-
-// Distances from `node` to its current row entries that pass `keep`, appended
-// (compacted, adjacency order) to e_id/e_d. Returns the count.
-template <int DP>
-__device__ __forceinline__ int row_extras(const float* __restrict__ vecs, const int32_t* adj, int W,
-                                          int64_t node, const QReg<DP>& q, bool pass,
-                                          int32_t nb, int32_t* e_id, float* e_d) {
-    const uint64_t m = ballot(pass);
-    const int ne = __popcll((unsigned long long)m);
-    if (pass) e_id[__popcll((unsigned long long)(m & lanemask_lt()))] = nb;
-    __syncthreads();
-    gather_dists<DP>(q, vecs, e_id, e_d, ne);
-    __syncthreads();
-    (void)adj; (void)W; (void)node;
-    return ne;
-}
-
-__device__ __forceinline__ int64_t next_item(unsigned long long* counter, int64_t* s_q) {
-    if (lane() == 0) *s_q = (int64_t)atomicAdd(counter, 1ull);
-    __syncthreads();
-    const int64_t v = *s_q;
-    __syncthreads();
-    return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -297,7 +213,8 @@ __global__ __launch_bounds__(64) void parth_prune_kernel(PruneArgs a) {
 // visited set stably sorted by distance (expansion order on ties) merged with
 // the current neighbours stably sorted by distance (adjacency order on ties),
 // visited-set entries first on ties, capped at P; the node itself is left out
-// (the reference can select the medoid as its own p*). Prune with <=. Each
+// (the reference can select the medoid as its own p*), and so are tombstoned
+// nodes (a tombstoned start node can be expanded). Prune with <=. Each
 // kept p* emits the reverse pair (p* <- node) with d(node, p*) (:292-301).
 // ---------------------------------------------------------------------------
 template <int DP, int CCAP>
@@ -325,7 +242,8 @@ __global__ __launch_bounds__(64) void fresh_out_prune_kernel(PruneArgs a) {
             const int j = b + l;
             const int32_t id = j < vc ? a.vis_ids[i * P + j] : -1;
             const float d = j < vc ? a.vis_d[i * P + j] : 0.0f;
-            const bool f = j < vc && id != (int32_t)node;
+            const bool f = j < vc && id != (int32_t)node &&
+                           !(a.del_bits && bit_test(a.del_bits, (uint32_t)id));
             const uint64_t mk = ballot(f);
             const int k = __popcll((unsigned long long)(mk & lanemask_lt()));
             const int nn = __popcll((unsigned long long)mk);
@@ -343,7 +261,8 @@ __global__ __launch_bounds__(64) void fresh_out_prune_kernel(PruneArgs a) {
             __syncthreads();
         }
         const int32_t nb = l < W ? a.adj[node * W + l] : -1;
-        const bool pass = nb != -1 && nb != (int32_t)node;
+        const bool pass = nb != -1 && nb != (int32_t)node &&
+                          !(a.del_bits && bit_test(a.del_bits, (uint32_t)nb));
         QReg<DP> q;
         q.load(a.vecs + node * DP, sub);
         const int ne = row_extras<DP>(a.vecs, a.adj, W, node, q, pass, nb, e_id, e_d);

@@ -39,6 +39,7 @@ struct PruneArgs {
     int DP;
     const float* vecs;
     int32_t* adj;             // rows rewritten in place
+    const uint32_t* del_bits; // FRESH out-prune: tombstoned ids are not candidates (or null)
     int W, R;
     float alpha;
     int64_t n;                // nodes handled: node = node_base + i, i < n
@@ -69,8 +70,8 @@ hipError_t launch_fresh_out_prune(const PruneArgs& a, int grid, hipStream_t s);
 hipError_t launch_fresh_rev_prune(const PruneArgs& a, int grid, hipStream_t s);
 
 hipError_t launch_random_init(int32_t* adj, int64_t n, int R, int W, uint64_t seed, hipStream_t s);
-hipError_t launch_medoid(const float* vecs, int64_t n, int D, int DP, float* centroid,
-                         float* node_d, unsigned long long* best, hipStream_t s);
+hipError_t launch_medoid(const float* vecs, int64_t n, int D, int DP, const uint32_t* del_bits,
+                         float* centroid, unsigned long long* best, hipStream_t s);
 hipError_t launch_pad_rows(const float* src, int64_t n, int D, float* dst, int DP, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);

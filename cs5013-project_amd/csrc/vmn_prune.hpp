@@ -1,0 +1,98 @@
+// Robust prune and candidate helpers shared by the build and update kernels.
+#pragma once
+#include "vmn_kernels.hpp"
+
+namespace vmn {
+
+// ---------------------------------------------------------------------------
+// Robust prune of a sorted candidate list held in LDS.
+//
+// Candidates are taken in list order; a candidate that is still kept becomes
+// p* and is added to the output, and every later kept candidate p' is dropped
+// when alpha * d(p*, p') < d(node, p') (STRICT, parth/src/build_graph.cu:
+// 241-258) or <= (vamana/include/kernels.cuh:329-353). The product alpha*d is
+// one float rounding, compared without fusion. Stops once R are kept. The
+// d(p*, p') of one round are computed by the whole wave (8 rows at a time).
+// ---------------------------------------------------------------------------
+template <int DP, bool STRICT>
+__device__ int robust_prune(const float* __restrict__ vecs, const uint32_t* c_id,
+                            const float* c_d, int cnt, float alpha, int R, uint8_t* s_kept,
+                            int32_t* s_act_id, int32_t* s_act_pos, float* s_act_d,
+                            int32_t* out_ids, float* out_d) {
+    const int l = lane(), sub = l & 7;
+    for (int i = l; i < cnt; i += 64) s_kept[i] = 1;
+    __syncthreads();
+    int nk = 0, from = 0;
+    while (nk < R) {
+        int p = -1;
+        for (int b = from; b < cnt; b += 64) {
+            const int j = b + l;
+            const uint64_t m = ballot(j < cnt && s_kept[j]);
+            if (m) {
+                p = b + __ffsll((unsigned long long)m) - 1;
+                break;
+            }
+        }
+        if (p < 0) break;
+        const int32_t ps = (int32_t)(c_id[p] & VMN_ID_MASK);
+        if (l == 0) {
+            out_ids[nk] = ps;
+            if (out_d) out_d[nk] = c_d[p];
+        }
+        ++nk;
+        if (nk >= R) break;
+        QReg<DP> pq;
+        pq.load(vecs + (int64_t)ps * DP, sub);
+        int nact = 0;
+        for (int b = p + 1; b < cnt; b += 64) {
+            const int j = b + l;
+            const bool f = j < cnt && s_kept[j];
+            const uint64_t m = ballot(f);
+            const int k = nact + __popcll((unsigned long long)(m & lanemask_lt()));
+            if (f) {
+                s_act_id[k] = (int32_t)(c_id[j] & VMN_ID_MASK);
+                s_act_pos[k] = j;
+            }
+            nact += __popcll((unsigned long long)m);
+        }
+        __syncthreads();
+        gather_dists<DP>(pq, vecs, s_act_id, s_act_d, nact);
+        __syncthreads();
+        for (int k = l; k < nact; k += 64) {
+            const int j = s_act_pos[k];
+            const float lhs = alpha * s_act_d[k];
+            const bool pr = STRICT ? (lhs < c_d[j]) : (lhs <= c_d[j]);
+            if (pr) s_kept[j] = 0;
+        }
+        __syncthreads();
+        from = p + 1;
+    }
+    __syncthreads();
+    return nk;
+}
+
+// Distances from `node` to its current row entries that pass `keep`, appended
+// (compacted, adjacency order) to e_id/e_d. Returns the count.
+template <int DP>
+__device__ __forceinline__ int row_extras(const float* __restrict__ vecs, const int32_t* adj, int W,
+                                          int64_t node, const QReg<DP>& q, bool pass,
+                                          int32_t nb, int32_t* e_id, float* e_d) {
+    const uint64_t m = ballot(pass);
+    const int ne = __popcll((unsigned long long)m);
+    if (pass) e_id[__popcll((unsigned long long)(m & lanemask_lt()))] = nb;
+    __syncthreads();
+    gather_dists<DP>(q, vecs, e_id, e_d, ne);
+    __syncthreads();
+    (void)adj; (void)W; (void)node;
+    return ne;
+}
+
+__device__ __forceinline__ int64_t next_item(unsigned long long* counter, int64_t* s_q) {
+    if (lane() == 0) *s_q = (int64_t)atomicAdd(counter, 1ull);
+    __syncthreads();
+    const int64_t v = *s_q;
+    __syncthreads();
+    return v;
+}
+
+}  // namespace vmn

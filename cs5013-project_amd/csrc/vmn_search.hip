@@ -32,8 +32,7 @@
 //   capped at P) are written out with their distances for the prune.
 //
 // Both modes skip tombstoned nodes while expanding (search.cuh:122,
-// engine.cu:76); PARTH gives a tombstoned start node distance 1e30
-// (engine.cu:45).
+// engine.cu:76) and give a tombstoned start node distance 1e30 (engine.cu:45).
 #include "vmn_kernels.hpp"
 
 namespace vmn {
@@ -67,7 +66,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
         }
         const int32_t st = a.start;
         float sd = reduce8(q.partial(vecs + (int64_t)st * DP, sub));
-        if (MODE == VMN_MODE_PARTH && a.del_bits && bit_test(a.del_bits, (uint32_t)st)) sd = 1e30f;
+        if (a.del_bits && bit_test(a.del_bits, (uint32_t)st)) sd = 1e30f;
         __syncthreads();
         if (l == 0) {
             s_id[0][0] = (uint32_t)st;

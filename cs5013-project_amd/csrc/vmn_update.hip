@@ -1,0 +1,170 @@
+// FreshVamana updates: tombstones, delete-by-vector, delete consolidation and
+// the entry-point move after deletes.
+#include "vmn_prune.hpp"
+#include "vmn_update.hpp"
+
+namespace vmn {
+
+// Tombstone a list of ids (vamana/include/delete_list.cuh keeps an append-only
+// id list scanned per neighbour; a bitmap gives the same membership answer in
+// one load). Ids outside [0, n) are counted in *bad and ignored.
+__global__ void tombstone_kernel(const int32_t* ids, int64_t m, int64_t n, uint32_t* del_bits,
+                                 unsigned long long* bad) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t id = ids[i];
+        if (id < 0 || id >= n) {
+            atomicAdd(bad, 1ull);
+            continue;
+        }
+        atomicOr(&del_bits[id >> 5], 1u << (id & 31));
+    }
+}
+hipError_t launch_tombstone(const int32_t* ids, int64_t m, int64_t n, uint32_t* del_bits,
+                            unsigned long long* bad, hipStream_t s) {
+    const int64_t g = (m + 255) / 256;
+    hipLaunchKernelGGL(tombstone_kernel, dim3((unsigned)(g < 1 ? 1 : (g > 4096 ? 4096 : g))),
+                       dim3(256), 0, s, ids, m, n, del_bits, bad);
+    return hipGetLastError();
+}
+
+// Exact-match lookup of query vectors among live nodes
+// (vamana/include/kernels.cuh:905-933 `findPointsKernel`): out[q] = the lowest
+// matching live id, or -1. out must hold 0xffffffff before the launch.
+__global__ void find_vectors_kernel(const float* __restrict__ vecs, int64_t n, int D, int DP,
+                                    const uint32_t* del_bits, const float* __restrict__ qv,
+                                    int64_t nq, uint32_t* out) {
+    const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t q = blockIdx.y;
+    if (node >= n || q >= nq) return;
+    if (bit_test(del_bits, (uint32_t)node)) return;
+    const float* a = vecs + node * DP;
+    const float* b = qv + q * DP;
+    for (int k = 0; k < D; ++k)
+        if (a[k] != b[k]) return;
+    atomicMin(&out[q], (uint32_t)node);
+}
+hipError_t launch_find_vectors(const float* vecs, int64_t n, int D, int DP, const uint32_t* del_bits,
+                               const float* qv, int64_t nq, uint32_t* out, hipStream_t s) {
+    dim3 g((unsigned)((n + 255) / 256), (unsigned)nq);
+    hipLaunchKernelGGL(find_vectors_kernel, g, dim3(256), 0, s, vecs, n, D, DP, del_bits, qv, nq, out);
+    return hipGetLastError();
+}
+
+// flags[p] = 1 for live nodes with at least one tombstoned out-neighbour.
+__global__ void cons_flags_kernel(const int32_t* adj, int64_t n, int W, const uint32_t* del_bits,
+                                  int32_t* flags) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int32_t f = 0;
+    if (!bit_test(del_bits, (uint32_t)p)) {
+        for (int j = 0; j < W; ++j) {
+            const int32_t v = adj[p * W + j];
+            if (v >= 0 && bit_test(del_bits, (uint32_t)v)) {
+                f = 1;
+                break;
+            }
+        }
+    }
+    flags[p] = f;
+}
+hipError_t launch_cons_flags(const int32_t* adj, int64_t n, int W, const uint32_t* del_bits,
+                             int32_t* flags, hipStream_t s) {
+    hipLaunchKernelGGL(cons_flags_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, adj,
+                       n, W, del_bits, flags);
+    return hipGetLastError();
+}
+
+// Delete consolidation (FreshDiskANN, Singh et al. 2021, Algorithm 4), one
+// wavefront per listed node p. The candidate pool is built row by row: p's
+// own row, then the row of each tombstoned neighbour in adjacency order; an
+// entry enters if it is live, not p and not already pooled; the pool is kept
+// as the first M entries by (distance, id) (an entry cut by the cap can never
+// return: the pool's M-th key only falls). Then robust prune with <= into R.
+template <int DP, int CCAP>
+__global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
+    __shared__ __align__(16) uint32_t c_id[2][CCAP];
+    __shared__ __align__(16) float c_d[2][CCAP];
+    __shared__ int32_t e_id[64];
+    __shared__ float e_d[64];
+    __shared__ uint8_t kept[CCAP];
+    __shared__ int32_t act_id[CCAP];
+    __shared__ int32_t act_pos[CCAP];
+    __shared__ float act_d[CCAP];
+    __shared__ int32_t newrow[64];
+    __shared__ int64_t s_q;
+    const int l = threadIdx.x, sub = l & 7;
+    const int W = a.W;
+    for (;;) {
+        const int64_t i = next_item(a.counter, &s_q);
+        if (i >= a.nlist) return;
+        const int64_t p = a.list[i];
+        QReg<DP> q;
+        q.load(a.vecs + p * DP, sub);
+        const int32_t own = l < W ? a.adj[p * W + l] : -1;
+        const uint64_t dmask = ballot(own >= 0 && bit_test(a.del_bits, (uint32_t)own));
+        int cur = 0, cnt = 0;
+        // source rows: j = -1 is p's own row, then each tombstoned neighbour
+        uint64_t rem = dmask;
+        int32_t src = -1;
+        for (;;) {
+            const int32_t v = src < 0 ? own : (l < W ? a.adj[(int64_t)src * W + l] : -1);
+            bool pass = v >= 0 && v != (int32_t)p && !bit_test(a.del_bits, (uint32_t)v);
+            {
+                bool dup = false;
+                for (int t = 0; t < cnt; ++t) dup |= c_id[cur][t] == (uint32_t)v;
+                pass = pass && !dup;
+            }
+            const int ne = row_extras<DP>(a.vecs, a.adj, W, p, q, pass, v, e_id, e_d);
+            const uint32_t my_id = l < ne ? (uint32_t)e_id[l] : 0u;
+            const float my_d = l < ne ? e_d[l] : 0.0f;
+            cnt = merge_sorted<CCAP, true>(c_id[cur], c_d[cur], cnt, my_id, my_d, ne,
+                                           c_id[cur ^ 1], c_d[cur ^ 1], a.M);
+            cur ^= 1;
+            __syncthreads();
+            if (!rem) break;
+            const int j = __ffsll((unsigned long long)rem) - 1;
+            rem &= rem - 1;
+            src = readlane_i(own, j);
+        }
+        const int nk = robust_prune<DP, false>(a.vecs, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
+                                               kept, act_id, act_pos, act_d, newrow, nullptr);
+        for (int j = l; j < W; j += 64) a.adj[p * W + j] = j < nk ? newrow[j] : -1;
+        __syncthreads();
+    }
+}
+
+hipError_t launch_consolidate(const ConsArgs& a, int grid, hipStream_t s) {
+    if (a.M > 512) return hipErrorInvalidValue;
+    switch (a.DP) {
+#define VMN_CONS_CASE(DPV)                                                                   \
+    case DPV:                                                                                \
+        hipLaunchKernelGGL((consolidate_kernel<DPV, 512>), dim3(grid), dim3(64), 0, s, a); \
+        break;
+        VMN_CONS_CASE(32)
+        VMN_CONS_CASE(64)
+        VMN_CONS_CASE(96)
+        VMN_CONS_CASE(128)
+        VMN_CONS_CASE(160)
+        VMN_CONS_CASE(192)
+        VMN_CONS_CASE(256)
+#undef VMN_CONS_CASE
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Rows of tombstoned nodes are emptied once consolidation no longer reads them.
+__global__ void clear_deleted_rows_kernel(int32_t* adj, int64_t n, int W, const uint32_t* del_bits) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * W) return;
+    if (bit_test(del_bits, (uint32_t)(i / W))) adj[i] = -1;
+}
+hipError_t launch_clear_deleted_rows(int32_t* adj, int64_t n, int W, const uint32_t* del_bits,
+                                     hipStream_t s) {
+    hipLaunchKernelGGL(clear_deleted_rows_kernel, dim3((unsigned)((n * W + 255) / 256)), dim3(256),
+                       0, s, adj, n, W, del_bits);
+    return hipGetLastError();
+}
+
+}  // namespace vmn

@@ -66,10 +66,17 @@ def lib():
         L.vmn_stream.argtypes = [P]
         L.vmn_stream.restype = P
         L.vmn_groundtruth.argtypes = [P, i64, i32, P, i64, i32, P, P, i32, i32]
+        L.vmn_insert.argtypes = [P, P, i64, P, i32]
+        L.vmn_delete_ids.argtypes = [P, P, i64, i32]
+        L.vmn_delete_vectors.argtypes = [P, P, i64, P, i32]
+        L.vmn_consolidate.argtypes = [P]
+        L.vmn_get_deleted.argtypes = [P, P]
         L.vmn_last_error.argtypes = []
         L.vmn_last_error.restype = ctypes.c_char_p
         for name in ("vmn_create", "vmn_destroy", "vmn_build", "vmn_load_graph", "vmn_search",
-                     "vmn_get_adjacency", "vmn_search_stats", "vmn_groundtruth"):
+                     "vmn_get_adjacency", "vmn_search_stats", "vmn_groundtruth", "vmn_insert",
+                     "vmn_delete_ids", "vmn_delete_vectors", "vmn_consolidate",
+                     "vmn_get_deleted"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -202,6 +209,47 @@ class Vamana:
     def synchronize(self):
         import torch
         torch.cuda.ExternalStream(self.stream).synchronize()
+
+    def insertPoints(self, vectors):
+        """Batched insert; returns the new ids (numpy int32)."""
+        n = int(vectors.shape[0])
+        ids = np.empty(n, dtype=np.int32)
+        vp, mem, keep = _arg(vectors, np.float32)
+        if mem == MEM_DEVICE:
+            import torch
+            dids = torch.empty(n, dtype=torch.int32, device=vectors.device)
+            _check(lib().vmn_insert(self._h, vp, n, ctypes.c_void_p(dids.data_ptr()), mem))
+            return dids.cpu().numpy()
+        _check(lib().vmn_insert(self._h, vp, n, ids.ctypes.data_as(ctypes.c_void_p), mem))
+        return ids
+
+    def deleteIds(self, ids):
+        ip, mem, keep = _arg(ids, np.int32)
+        _check(lib().vmn_delete_ids(self._h, ip, int(len(ids)), mem))
+
+    def deletePoints(self, vectors):
+        """Delete by exact vector match; returns the matched ids (-1: not found)."""
+        n = int(vectors.shape[0])
+        vp, mem, keep = _arg(vectors, np.float32)
+        out = np.empty(n, dtype=np.int32)
+        if mem == MEM_DEVICE:
+            import torch
+            d = torch.empty(n, dtype=torch.int32, device=vectors.device)
+            _check(lib().vmn_delete_vectors(self._h, vp, n, ctypes.c_void_p(d.data_ptr()), mem))
+            return d.cpu().numpy()
+        _check(lib().vmn_delete_vectors(self._h, vp, n, out.ctypes.data_as(ctypes.c_void_p), mem))
+        return out
+
+    def consolidate(self):
+        _check(lib().vmn_consolidate(self._h))
+
+    patchGraph = consolidate
+
+    @property
+    def deleted(self):
+        out = np.empty(self.size, dtype=np.uint8)
+        _check(lib().vmn_get_deleted(self._h, out.ctypes.data_as(ctypes.c_void_p)))
+        return out.astype(bool)
 
     @property
     def adjacency(self):

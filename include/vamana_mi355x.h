@@ -88,6 +88,35 @@ int vmn_load_graph(vmn_index* idx, const float* vecs, int64_t n, const int32_t* 
 int vmn_search(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int32_t L,
                int32_t* ids_out, float* dists_out, int32_t mem);
 
+/* Batched insert of n vectors [n x dim]: they become nodes size .. size+n-1
+ * (written to ids_out when not NULL). Each is searched from the entry point
+ * over the current graph (tombstones skipped), out-pruned from its visited
+ * set, and its reverse edges are merged into its out-neighbours, as the
+ * reference's patchGraph does for its staged insert list (vamana_impl.cuh:
+ * 182-265 with insertPoints, vamana.cuh:42,130). Needs capacity. */
+int vmn_insert(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out, int32_t mem);
+
+/* Tombstone ids: searches no longer expand or return them. Ids outside
+ * [0, size) give VMN_ERR_ARG (the valid ones are still applied). Replaces the
+ * delete list (delete_list.cuh:9-115) and parth deleteKernel (engine.cu:131-147). */
+int vmn_delete_ids(vmn_index* idx, const int32_t* ids, int64_t n, int32_t mem);
+
+/* Delete by exact vector match among live nodes (deletePoints, vamana.cuh:65,
+ * findPointsKernel kernels.cuh:905-933): ids_out[i] = the lowest matching id,
+ * or -1 when none matches (then nothing is tombstoned for it). n <= 65535. */
+int vmn_delete_vectors(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out,
+                       int32_t mem);
+
+/* Delete consolidation (FreshDiskANN Algorithm 4; the reference has none):
+ * every live node with a tombstoned out-neighbour is re-pruned from its live
+ * neighbours plus the live neighbours of its tombstoned neighbours; rows of
+ * tombstoned nodes are emptied; a tombstoned entry point moves to the medoid
+ * of the live nodes. Ids stay tombstoned (not reused). */
+int vmn_consolidate(vmn_index* idx);
+
+/* Tombstone mask [size] (1 = deleted) into host memory. */
+int vmn_get_deleted(const vmn_index* idx, uint8_t* mask_out);
+
 /* Copy the adjacency [size x R] (int32, -1 padded). */
 int vmn_get_adjacency(const vmn_index* idx, int32_t* out, int32_t mem);
 

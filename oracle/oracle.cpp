@@ -370,7 +370,9 @@ static void fresh_search_one(const float* vecs, int d, const int32_t* adj, int W
     bloom[orc_bf_hash1((uint32_t)start)] = 1;
     bloom[orc_bf_hash2((uint32_t)start)] = 1;
     std::vector<Item> wl;
-    wl.push_back({start, l2sq_m(q, vecs + (int64_t)start * d, d, dist_mode), false});
+    float sd = l2sq_m(q, vecs + (int64_t)start * d, d, dist_mode);
+    if (del && del[start]) sd = 1e30f;  // as parth/src/engine.cu:45
+    wl.push_back({start, sd, false});
     int nexp = 0, hops = 0, ndist = 1;
     std::vector<int32_t> news;
     while (hops < max_hops) {
@@ -425,14 +427,14 @@ static void fresh_search_one(const float* vecs, int d, const int32_t* adj, int W
 
 void orc_fresh_search(const float* vecs, int64_t n, int d, const int32_t* adj, int W,
                       const uint8_t* del, const float* queries, int64_t nq, int queries_are_nodes,
-                      int32_t start, int L, int max_hops, int P, int dist_mode, int out_n,
+                      int64_t node_base, int32_t start, int L, int max_hops, int P, int dist_mode, int out_n,
                       int32_t* out_ids, float* out_dists, int32_t* vis_ids, float* vis_d,
                       int32_t* vis_cnt, int32_t* out_hops, int32_t* out_ndist) {
     (void)n;
 #pragma omp parallel for schedule(dynamic, 4)
     for (int64_t qi = 0; qi < nq; ++qi) {
         fresh_search_one(vecs, d, adj, W, del, queries + qi * d,
-                         queries_are_nodes ? (int32_t)qi : -1, start, L, max_hops, P, dist_mode,
+                         queries_are_nodes ? (int32_t)(node_base + qi) : -1, start, L, max_hops, P, dist_mode,
                          out_n, out_ids ? out_ids + qi * out_n : nullptr,
                          out_dists ? out_dists + qi * out_n : nullptr,
                          vis_ids ? vis_ids + qi * P : nullptr, vis_d ? vis_d + qi * P : nullptr,
@@ -472,26 +474,26 @@ static void prune_le(const float* vecs, int d, const std::vector<Item>& c, float
 // [base, base+cnt): candidates = the visited set stably sorted by distance,
 // merged (visited entries first on ties, mergeIntoVisitedSet :171-229) with
 // the current neighbours stably sorted by distance, capped at P; the node
-// itself is left out. Writes the new rows and, per kept p*, the reverse pair
+// itself and tombstoned nodes are left out. Writes the new rows and, per kept p*, the reverse pair
 // (p* <- node, d(node, p*)) into pair_dst/pair_d [cnt x R] (-1 unused).
 void orc_fresh_out_prune(const float* vecs, int d, int32_t* adj, int W, int R, float alpha,
                          int64_t base, int64_t cnt, const int32_t* vis_ids, const float* vis_d,
                          const int32_t* vis_cnt, int P, int dist_mode, int32_t* pair_dst,
-                         float* pair_d) {
+                         float* pair_d, const uint8_t* del) {
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t i = 0; i < cnt; ++i) {
         const int64_t node = base + i;
         std::vector<Item> A, B, C;
         for (int j = 0; j < vis_cnt[i]; ++j) {
             int32_t v = vis_ids[i * P + j];
-            if (v != (int32_t)node) A.push_back({v, vis_d[i * P + j], false});
+            if (v != (int32_t)node && !(del && del[v])) A.push_back({v, vis_d[i * P + j], false});
         }
         std::stable_sort(A.begin(), A.end(), [](const Item& a, const Item& b) { return a.d < b.d; });
         int32_t* row = adj + node * W;
         const float* nv = vecs + node * d;
         for (int j = 0; j < W; ++j) {
             int32_t v = row[j];
-            if (v == -1 || v == (int32_t)node) continue;
+            if (v == -1 || v == (int32_t)node || (del && del[v])) continue;
             B.push_back({v, l2sq_m(nv, vecs + (int64_t)v * d, d, dist_mode), false});
         }
         std::stable_sort(B.begin(), B.end(), [](const Item& a, const Item& b) { return a.d < b.d; });
@@ -565,13 +567,114 @@ int64_t orc_fresh_build(const float* vecs, int64_t n, int d, int R, int L, const
     std::vector<int32_t> vis((size_t)n * P), vcnt(n), pdst((size_t)n * R);
     std::vector<float> visd((size_t)n * P), pd((size_t)n * R);
     for (int p = 0; p < passes; ++p) {
-        orc_fresh_search(vecs, n, d, adj, R, nullptr, vecs, n, 1, (int32_t)medoid, L, max_hops, P,
+        orc_fresh_search(vecs, n, d, adj, R, nullptr, vecs, n, 1, 0, (int32_t)medoid, L, max_hops, P,
                          dist_mode, 0, nullptr, nullptr, vis.data(), visd.data(), vcnt.data(),
                          nullptr, nullptr);
         orc_fresh_out_prune(vecs, d, adj, R, R, alphas[p], 0, n, vis.data(), visd.data(),
-                            vcnt.data(), P, dist_mode, pdst.data(), pd.data());
+                            vcnt.data(), P, dist_mode, pdst.data(), pd.data(), nullptr);
         orc_fresh_reverse(vecs, n, d, adj, R, R, alphas[p], 0, n, pdst.data(), pd.data(), M,
                           dist_mode);
+    }
+    return medoid;
+}
+
+// Batched FreshVamana insert (vamana_impl.cuh:182-265 `patchGraph` applied to
+// one batch): nodes [n_old, n_old + n_new) already hold their vectors and
+// empty rows; each is searched from `start` over the graph (tombstones
+// skipped, its own id skipped), out-pruned from its visited set, and the
+// reverse pairs are merged into their destinations. All searches of the batch
+// see the graph as it was before the batch.
+void orc_fresh_insert(const float* vecs, int64_t n_old, int64_t n_new, int d, int32_t* adj, int R,
+                      const uint8_t* del, int32_t start, int L, float alpha, int P, int M,
+                      int max_hops, int dist_mode) {
+    const int64_t N = n_old + n_new;
+    std::vector<int32_t> vis((size_t)n_new * P), vcnt(n_new), pdst((size_t)n_new * R);
+    std::vector<float> visd((size_t)n_new * P), pd((size_t)n_new * R);
+    orc_fresh_search(vecs, N, d, adj, R, del, vecs + n_old * d, n_new, 1, n_old, start, L,
+                     max_hops, P, dist_mode, 0, nullptr, nullptr, vis.data(), visd.data(),
+                     vcnt.data(), nullptr, nullptr);
+    orc_fresh_out_prune(vecs, d, adj, R, R, alpha, n_old, n_new, vis.data(), visd.data(),
+                        vcnt.data(), P, dist_mode, pdst.data(), pd.data(), del);
+    orc_fresh_reverse(vecs, N, d, adj, R, R, alpha, n_old, n_new, pdst.data(), pd.data(), M,
+                      dist_mode);
+}
+
+// Delete consolidation (FreshDiskANN, Singh et al. 2021, Algorithm 4; the
+// reference has none). For every live node p with a tombstoned out-neighbour:
+// candidates = live out-neighbours of p plus the live out-neighbours of each
+// tombstoned out-neighbour, without p; ordered by (d(p, c), id), the first M
+// kept; robust prune with <= and `alpha` into at most R. Then the rows of
+// tombstoned nodes are emptied. Returns the number of re-pruned nodes.
+int64_t orc_consolidate(const float* vecs, int64_t n, int d, int32_t* adj, int R,
+                        const uint8_t* del, float alpha, int M, int dist_mode) {
+    std::vector<int64_t> todo;
+    for (int64_t p = 0; p < n; ++p) {
+        if (del[p]) continue;
+        for (int j = 0; j < R; ++j) {
+            int32_t v = adj[p * R + j];
+            if (v >= 0 && del[v]) { todo.push_back(p); break; }
+        }
+    }
+    std::vector<std::vector<int32_t>> rows(todo.size());
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t t = 0; t < (int64_t)todo.size(); ++t) {
+        const int64_t p = todo[t];
+        std::vector<int32_t> ids;
+        auto add = [&](int32_t w) {
+            if (w < 0 || del[w] || w == (int32_t)p) return;
+            for (int32_t x : ids) if (x == w) return;
+            ids.push_back(w);
+        };
+        for (int j = 0; j < R; ++j) add(adj[p * R + j]);
+        for (int j = 0; j < R; ++j) {
+            int32_t v = adj[p * R + j];
+            if (v >= 0 && del[v])
+                for (int k = 0; k < R; ++k) add(adj[(int64_t)v * R + k]);
+        }
+        std::vector<Item> C;
+        for (int32_t w : ids)
+            C.push_back({w, l2sq_m(vecs + p * d, vecs + (int64_t)w * d, d, dist_mode), false});
+        std::sort(C.begin(), C.end(), [](const Item& a, const Item& b) {
+            return a.d < b.d || (a.d == b.d && a.id < b.id);
+        });
+        if ((int)C.size() > M) C.resize(M);
+        std::vector<int32_t> out;
+        std::vector<float> outd;
+        prune_le(vecs, d, C, alpha, R, dist_mode, out, outd);
+        rows[t] = out;
+    }
+    for (size_t t = 0; t < todo.size(); ++t)
+        for (int j = 0; j < R; ++j) adj[todo[t] * R + j] = j < (int)rows[t].size() ? rows[t][j] : -1;
+    for (int64_t p = 0; p < n; ++p)
+        if (del[p])
+            for (int j = 0; j < R; ++j) adj[p * R + j] = -1;
+    return (int64_t)todo.size();
+}
+
+// Entry point over live nodes: the medoid rule of parth/src/build_graph.cu:
+// 326-343 (sequential float centroid, nearest node, lowest id on ties)
+// restricted to nodes whose tombstone is clear; divides by the live count.
+int64_t orc_medoid_live(const float* vecs, int64_t n, int d, const uint8_t* del) {
+    std::vector<float> c(d, 0.0f);
+    int64_t live = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (del && del[i]) continue;
+        ++live;
+        for (int k = 0; k < d; ++k) c[k] += vecs[i * d + k];
+    }
+    if (live == 0) return -1;
+    for (int k = 0; k < d; ++k) c[k] /= (float)live;
+    int64_t medoid = -1;
+    float best = 1e30f;
+    for (int64_t i = 0; i < n; ++i) {
+        if (del && del[i]) continue;
+        float s = 0.0f;
+        for (int k = 0; k < d; ++k) {
+            float diff = vecs[i * d + k] - c[k];
+            float sq = diff * diff;
+            s = s + sq;
+        }
+        if (s < best) { best = s; medoid = i; }
     }
     return medoid;
 }

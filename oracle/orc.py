@@ -44,10 +44,16 @@ def lib():
         _lib.orc_bf_hash1.restype = ctypes.c_uint32
         _lib.orc_bf_hash2.argtypes = [ctypes.c_uint32]
         _lib.orc_bf_hash2.restype = ctypes.c_uint32
-        _lib.orc_fresh_search.argtypes = [P, i64, i32, P, i32, P, P, i64, i32, i32, i32, i32,
-                                          i32, i32, i32, P, P, P, P, P, P, P]
+        _lib.orc_fresh_search.argtypes = [P, i64, i32, P, i32, P, P, i64, i32, i64, i32, i32,
+                                          i32, i32, i32, i32, P, P, P, P, P, P, P]
+        _lib.orc_fresh_insert.argtypes = [P, i64, i64, i32, P, i32, P, i32, i32, f32, i32, i32,
+                                          i32, i32]
+        _lib.orc_consolidate.argtypes = [P, i64, i32, P, i32, P, f32, i32, i32]
+        _lib.orc_consolidate.restype = i64
+        _lib.orc_medoid_live.argtypes = [P, i64, i32, P]
+        _lib.orc_medoid_live.restype = i64
         _lib.orc_fresh_out_prune.argtypes = [P, i32, P, i32, i32, f32, i64, i64, P, P, P, i32,
-                                             i32, P, P]
+                                             i32, P, P, P]
         _lib.orc_fresh_reverse.argtypes = [P, i64, i32, P, i32, i32, f32, i64, i64, P, P, i32,
                                            i32]
         _lib.orc_fresh_build.argtypes = [P, i64, i32, i32, i32, P, i32, u64, i32, i32, i32, i32,
@@ -129,7 +135,7 @@ def bf_hash(x):
 
 
 def fresh_search(vecs, adj, queries, start, L, out_n=10, P=600, max_hops=1 << 20, dist_mode=0,
-                 deleted=None, queries_are_nodes=False, want_visited=False):
+                 deleted=None, queries_are_nodes=False, want_visited=False, node_base=0):
     """G2 greedy search. Returns (ids, dists, hops, ndist[, vis_ids, vis_d, vis_cnt])."""
     vecs, queries = _f32(vecs), _f32(queries)
     adj = np.ascontiguousarray(adj, dtype=np.int32)
@@ -143,7 +149,8 @@ def fresh_search(vecs, adj, queries, start, L, out_n=10, P=600, max_hops=1 << 20
     vc = np.zeros(nq, dtype=np.int32) if want_visited else None
     dm = None if deleted is None else np.ascontiguousarray(deleted, dtype=np.uint8)
     lib().orc_fresh_search(_p(vecs), vecs.shape[0], vecs.shape[1], _p(adj), adj.shape[1], _p(dm),
-                           _p(queries), nq, int(queries_are_nodes), int(start), L, max_hops, P,
+                           _p(queries), nq, int(queries_are_nodes), int(node_base), int(start),
+                           L, max_hops, P,
                            dist_mode, out_n, _p(ids), _p(dists), _p(vi), _p(vd), _p(vc),
                            _p(hops), _p(nd))
     if want_visited:
@@ -161,6 +168,33 @@ def fresh_build(vecs, R=64, L=128, alphas=(1.5,), seed=1234, P=600, M=500, max_h
     med = lib().orc_fresh_build(_p(vecs), n, d, R, L, _p(al), len(al), seed, P, M, max_hops,
                                 dist_mode, _p(adj))
     return adj, int(med)
+
+
+def fresh_insert(vecs, n_old, adj, start, L, alpha, deleted=None, P=600, M=500, max_hops=1 << 20,
+                 dist_mode=0):
+    """vecs [N,d] holds old and new nodes; adj [N,R] with -1 rows for the new
+    nodes is updated in place."""
+    vecs = _f32(vecs)
+    assert adj.dtype == np.int32 and adj.flags.c_contiguous
+    N, d = vecs.shape
+    dm = None if deleted is None else np.ascontiguousarray(deleted, dtype=np.uint8)
+    lib().orc_fresh_insert(_p(vecs), n_old, N - n_old, d, _p(adj), adj.shape[1], _p(dm),
+                           int(start), L, float(alpha), P, M, max_hops, dist_mode)
+    return adj
+
+
+def consolidate(vecs, adj, deleted, alpha, M=500, dist_mode=0):
+    vecs = _f32(vecs)
+    assert adj.dtype == np.int32 and adj.flags.c_contiguous
+    dm = np.ascontiguousarray(deleted, dtype=np.uint8)
+    return int(lib().orc_consolidate(_p(vecs), vecs.shape[0], vecs.shape[1], _p(adj),
+                                     adj.shape[1], _p(dm), float(alpha), M, dist_mode))
+
+
+def medoid_live(vecs, deleted):
+    vecs = _f32(vecs)
+    dm = np.ascontiguousarray(deleted, dtype=np.uint8)
+    return int(lib().orc_medoid_live(_p(vecs), vecs.shape[0], vecs.shape[1], _p(dm)))
 
 
 def bruteforce_fp64(vecs, queries, k, deleted=None):

@@ -1,0 +1,115 @@
+"""GPU FreshVamana updates (insert, delete, consolidate) bit-exact against the
+oracle, with searches checked after every step."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(vmn, n, seed=3, stream=0):
+    return vmn.datasets.integer_mixture(n, 128, k=32, seed=seed, stream=stream)
+
+
+def test_gpu_insert_matches_oracle(vmn, orc, gpu):
+    v = _data(vmn, 6000)
+    idx = vmn.Vamana(v[:5000], mode="fresh", R=32, L_build=64, alphas=(1.2,), capacity=6000)
+    oadj, med = orc.fresh_build(v[:5000], R=32, L=64, alphas=(1.2,))
+    full = np.full((6000, 32), -1, np.int32)
+    full[:5000] = oadj
+    assert np.array_equal(idx.adjacency, oadj)
+    ids = idx.insertPoints(v[5000:5400])
+    assert np.array_equal(ids, np.arange(5000, 5400))
+    orc.fresh_insert(v[:5400], 5000, full[:5400], med, L=64, alpha=1.2)
+    assert np.array_equal(idx.adjacency, full[:5400])
+    idx.insertPoints(v[5400:6000])
+    orc.fresh_insert(v, 5400, full, med, L=64, alpha=1.2)
+    assert np.array_equal(idx.adjacency, full)
+    q = _data(vmn, 50, stream=1)
+    gi, gd = idx.searchPoints(q, k=10, L=48)
+    oi, od, _, _ = orc.fresh_search(v, full, q, med, L=48)
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+    with pytest.raises(vmn.VamanaError):
+        idx.insertPoints(v[:1])  # capacity exhausted
+
+
+def test_gpu_delete_consolidate_matches_oracle(vmn, orc, gpu):
+    v = _data(vmn, 6000, seed=4)
+    q = _data(vmn, 80, seed=4, stream=1)
+    idx = vmn.Vamana(v, mode="fresh", R=32, L_build=64, alphas=(1.0, 1.2))
+    adj, med = orc.fresh_build(v, R=32, L=64, alphas=(1.0, 1.2))
+    rng = np.random.default_rng(5)
+    kill = rng.choice(6000, 900, replace=False).astype(np.int32)
+    kill[0] = med  # the entry point goes too
+    dele = np.zeros(6000, np.uint8)
+    dele[kill] = 1
+    idx.deleteIds(kill)
+    assert np.array_equal(idx.deleted, dele.astype(bool))
+    gi, gd = idx.searchPoints(q, k=10, L=64)
+    oi, od, _, _ = orc.fresh_search(v, adj, q, med, L=64, deleted=dele)
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+    idx.consolidate()
+    orc.consolidate(v, adj, dele, alpha=1.2)
+    assert np.array_equal(idx.adjacency, adj)
+    m2 = orc.medoid_live(v, dele)
+    assert idx.medoid == m2
+    gi, gd = idx.searchPoints(q, k=10, L=64)
+    oi, od, _, _ = orc.fresh_search(v, adj, q, m2, L=64, deleted=dele)
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+    assert not np.isin(gi[gi >= 0], kill).any()
+
+
+def test_gpu_stream_of_updates_matches_oracle(vmn, orc, gpu):
+    """delete -> insert (searches skip tombstones) -> consolidate, twice."""
+    v = _data(vmn, 7000, seed=6)
+    idx = vmn.Vamana(v[:5000], mode="fresh", R=32, L_build=64, alphas=(1.2,), capacity=7000)
+    adj5, med = orc.fresh_build(v[:5000], R=32, L=64, alphas=(1.2,))
+    full = np.full((7000, 32), -1, np.int32)
+    full[:5000] = adj5
+    dele = np.zeros(7000, np.uint8)
+    rng = np.random.default_rng(7)
+    n = 5000
+    for rnd in range(2):
+        live = np.flatnonzero(dele[:n] == 0)
+        kill = rng.choice(live, 250, replace=False).astype(np.int32)
+        dele[kill] = 1
+        idx.deleteIds(kill)
+        idx.insertPoints(v[n:n + 1000])
+        orc.fresh_insert(v[:n + 1000], n, full[:n + 1000], med, L=64, alpha=1.2,
+                         deleted=dele[:n + 1000])
+        n += 1000
+        assert np.array_equal(idx.adjacency, full[:n]), f"round {rnd} insert"
+        idx.consolidate()
+        orc.consolidate(v[:n], full[:n], dele[:n], alpha=1.2)
+        if dele[med]:
+            med = orc.medoid_live(v[:n], dele[:n])
+        assert idx.medoid == med
+        assert np.array_equal(idx.adjacency, full[:n]), f"round {rnd} consolidate"
+
+
+def test_gpu_delete_by_vector(vmn, gpu):
+    v = _data(vmn, 3000, seed=11)
+    idx = vmn.Vamana(v, mode="fresh", R=32, L_build=64, alphas=(1.2,))
+    probe = np.concatenate([v[[10, 20, 30]], np.full((1, 128), 999.0, np.float32)])
+    ids = idx.deletePoints(probe)
+    assert ids[3] == -1
+    for k, want in zip(ids[:3], (10, 20, 30)):
+        assert np.array_equal(v[k], v[want]) and idx.deleted[k]
+    # deleting the same vectors again finds no live match unless a duplicate exists
+    again = idx.deletePoints(probe[:3])
+    for a, b in zip(again, ids[:3]):
+        assert a == -1 or (a != b and np.array_equal(v[a], v[b]))
+
+
+def test_gpu_parth_delete_search_matches_oracle(vmn, orc, sift, gpu):
+    base = sift["base"].astype(np.float32)
+    q = sift["query"].astype(np.float32)
+    idx = vmn.Vamana.from_graph(base, sift["parth_adj"], medoid=3732)
+    kill = np.arange(0, 10000, 7, dtype=np.int32)
+    dele = np.zeros(10000, np.uint8)
+    dele[kill] = 1
+    idx.deleteIds(kill)
+    gi, gd = idx.searchPoints(q, k=10, L=64)
+    oi, od, _, _ = orc.parth_search(base, sift["parth_adj"], q, 3732, L=64, deleted=dele)
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+    with pytest.raises(vmn.VamanaError):
+        idx.deleteIds(np.array([10000], np.int32))
