@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--sweep", default="10,20,40,60,80,100,120,150,200")
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--rerank", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
 
@@ -69,7 +70,8 @@ def main():
 
     log(f"building ({args.mode}, R={args.R}, L={args.L_build}, alphas={alphas})")
     t = time.time()
-    idx = vmn.Vamana(base_t, mode=args.mode, R=args.R, L_build=args.L_build, alphas=alphas)
+    idx = vmn.Vamana(base_t, mode=args.mode, R=args.R, L_build=args.L_build, alphas=alphas,
+                     low_precision="both")
     torch.cuda.synchronize()
     build_s = time.time() - t
     log(f"build {build_s:.2f} s = {n / build_s:.0f} points/s, medoid {idx.medoid}")
@@ -83,75 +85,106 @@ def main():
     stream = torch.cuda.ExternalStream(idx.stream)
     ids = torch.empty((nq, 10), dtype=torch.int32, device=dev)
     dists = torch.empty((nq, 10), dtype=torch.float32, device=dev)
+    forms = {"fp32": ("fp32", 0), "bf16+rerank": ("bf16", args.rerank),
+             "fp8+rerank": ("fp8", args.rerank)}
 
-    def timed_search(L, reps):
-        idx.searchPoints(q_t, k=10, L=L, ids_out=ids, dists_out=dists, sync=True)
+    def run(form, L, reps, sync_first=True):
+        prec, rr = forms[form]
+        rr = min(rr, L)
+        if sync_first:
+            idx.searchPoints(q_t, k=10, L=L, ids_out=ids, dists_out=dists, sync=True, prec=prec,
+                             rerank=rr)
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(reps):
-            idx.searchPoints(q_t, k=10, L=L, ids_out=ids, dists_out=dists, sync=False)
+            idx.searchPoints(q_t, k=10, L=L, ids_out=ids, dists_out=dists, sync=False, prec=prec,
+                             rerank=rr)
         e1.record(stream)
         e1.synchronize()
         return e0.elapsed_time(e1) / reps
 
-    sweep = []
-    for L in [int(x) for x in args.sweep.split(",")]:
-        ms = timed_search(L, 2)
+    sweeps, chosen = {}, {}
+    for form in forms:
+        sw = []
+        for L in [int(x) for x in args.sweep.split(",")]:
+            ms = run(form, L, 2)
+            rec = vmn.recall_at_k(ids.cpu().numpy(), gt, 10)
+            hops, nd = idx.search_stats()
+            sw.append({"L": L, "recall@10": round(rec, 4), "qps": round(nq / (ms / 1e3), 1),
+                       "ms": round(ms, 3), "hops_per_q": round(hops / nq, 1),
+                       "dists_per_q": round(nd / nq, 1)})
+            log(f"{form:12s} L={L:4d} recall@10={rec:.4f} {nq / (ms / 1e3):,.0f} QPS ({ms:.2f} ms)")
+        ok = [x for x in sw if x["recall@10"] >= args.target]
+        chosen[form] = (ok[0] if ok else max(sw, key=lambda x: x["recall@10"]), bool(ok))
+        sweeps[form] = sw
+
+    # timed steps: every form at its chosen L (K steps each); the headline is BF16 + re-rank
+    timed = {}
+    for form in forms:
+        L = chosen[form][0]["L"]
+        prec, rr = forms[form]
+        for _ in range(args.warmup):
+            idx.searchPoints(q_t, k=10, L=L, ids_out=ids, dists_out=dists, sync=False, prec=prec,
+                             rerank=min(rr, L))
+        idx.synchronize()
+        torch.cuda.synchronize()
+        t = time.time()
+        kern_ms = run(form, L, args.steps, sync_first=False)
+        torch.cuda.synchronize()
+        wall = time.time() - t
         rec = vmn.recall_at_k(ids.cpu().numpy(), gt, 10)
         hops, nd = idx.search_stats()
-        sweep.append({"L": L, "recall@10": round(rec, 4), "qps": round(nq / (ms / 1e3), 1),
-                      "ms": round(ms, 3), "hops_per_q": round(hops / nq, 1),
-                      "dists_per_q": round(nd / nq, 1)})
-        log(f"L={L:4d} recall@10={rec:.4f} {nq / (ms / 1e3):,.0f} QPS ({ms:.2f} ms)")
-    ok = [s for s in sweep if s["recall@10"] >= args.target]
-    chosen = ok[0] if ok else max(sweep, key=lambda s: s["recall@10"])
-    L = chosen["L"]
+        row_b = {"fp32": d * 4, "bf16": d * 2 + 4, "fp8": d + 4}[prec]
+        bytes_launch = (d * 4) * nq + hops * 4 * args.R + nd * row_b + nq * min(rr, L) * d * 4 \
+            + nq * 10 * 8
+        timed[form] = {"L": L, "recall@10": round(rec, 4), "target_met": chosen[form][1],
+                       "qps": round(nq * args.steps / wall, 1), "ms_per_step": wall * 1e3 / args.steps,
+                       "kernel_ms": kern_ms, "bytes_per_launch": int(bytes_launch),
+                       "achieved_GBs": bytes_launch / (kern_ms / 1e3) / 1e9}
+        log(f"timed {form:12s} L={L} {timed[form]['qps']:,.0f} QPS recall {rec:.4f} kernel "
+            f"{kern_ms:.3f} ms {timed[form]['achieved_GBs']:.0f} GB/s")
 
-    log(f"timed: warmup {args.warmup}, steps {args.steps} at L={L}")
-    for _ in range(args.warmup):
-        idx.searchPoints(q_t, k=10, L=L, ids_out=ids, dists_out=dists, sync=False)
-    idx.synchronize()
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    t = time.time()
-    e0.record(stream)
-    for _ in range(args.steps):
-        idx.searchPoints(q_t, k=10, L=L, ids_out=ids, dists_out=dists, sync=False)
-    e1.record(stream)
-    e1.synchronize()
-    torch.cuda.synchronize()
-    wall = time.time() - t
-    kern_ms = e0.elapsed_time(e1) / args.steps
-    ms_step = wall * 1e3 / args.steps
-    qps = nq * args.steps / wall
-    rec = vmn.recall_at_k(ids.cpu().numpy(), gt, 10)
-    hops, nd = idx.search_stats()
-    R = args.R
-    bytes_q = (d * 4) * nq + hops * 4 * R + nd * d * 4 + nq * 10 * 8
-    achieved = bytes_q / (kern_ms / 1e3) / 1e9
-    log(f"{qps:,.0f} QPS, recall@10 {rec:.4f}, kernel {kern_ms:.3f} ms, {achieved:.0f} GB/s")
+    head = timed["bf16+rerank"]
+    # distance error of the low-precision rows against FP64 on a sample of pairs
+    err = {}
+    rng = np.random.default_rng(0)
+    pi = rng.integers(0, n, 4096)
+    qi = rng.integers(0, nq, 4096)
+    exact = ((base[pi].astype(np.float64) - qs[qi].astype(np.float64)) ** 2).sum(1)
+    for prec in ("bf16", "fp8"):
+        rows = idx.lowp_rows(prec)
+        approx = ((rows[pi].astype(np.float64) - qs[qi].astype(np.float64)) ** 2).sum(1)
+        rel = (approx - exact) / np.maximum(exact, 1.0)
+        err[prec] = {"rms_rel": float(np.sqrt(np.mean(rel ** 2))), "max_rel": float(np.abs(rel).max())}
+        del rows
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(idx, base, qs, gt, L, args.cpu_seconds, args.mode)
+        cpu = cpu_baseline(idx, base, qs, gt, chosen["fp32"][0]["L"], args.cpu_seconds, args.mode)
 
     out = {
-        "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "metric": METRIC, "value": head["qps"], "unit": "queries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
         "data": "synthetic: seeded 256-component Gaussian mixture, integer-valued in [0,255]",
         "config": {
-            "workload": f"Vamana search {n}x{d} fp32, {nq} queries per batch, k=10",
-            "mode": args.mode, "n": n, "nq": nq, "R": R, "L_build": args.L_build,
-            "alphas": list(alphas), "L_search": L, "recall@10": round(rec, 4),
-            "recall_target_met": bool(ok), "build_s": round(build_s, 3),
-            "build_points_per_s": round(n / build_s, 1), "sweep": sweep},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel_ms": round(kern_ms, 4),
-                     "bytes_per_launch": int(bytes_q)},
+            "workload": f"Vamana search {n}x{d}, BF16 rows on MFMA + FP32 re-rank of "
+                        f"{args.rerank}, {nq} queries per batch, k=10",
+            "mode": args.mode, "n": n, "nq": nq, "R": args.R, "L_build": args.L_build,
+            "alphas": list(alphas), "L_search": head["L"], "recall@10": head["recall@10"],
+            "recall_target_met": head["target_met"], "build_s": round(build_s, 3),
+            "build_points_per_s": round(n / build_s, 1),
+            "fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in timed["fp32"].items()},
+            "fp8+rerank": {k: (round(v, 4) if isinstance(v, float) else v)
+                           for k, v in timed["fp8+rerank"].items()},
+            "bf16_vs_fp32_qps": round(head["qps"] / timed["fp32"]["qps"], 3),
+            "distance_error_vs_fp64": err, "sweep": sweeps},
+        "roofline": {"bound": "hbm", "achieved": round(head["achieved_GBs"], 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(head["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_ms": round(head["kernel_ms"], 4),
+                     "bytes_per_launch": head["bytes_per_launch"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

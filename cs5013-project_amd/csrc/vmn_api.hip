@@ -82,6 +82,8 @@ struct vmn_index {
     int64_t last_nq = 0;
     int64_t n_deleted = 0;  // tombstones set since the build (upper bound)
     DevBuf staging, find_out, bad;
+    DevBuf bf16_rows, bf16_norms, fp8_rows, fp8_norms, fp8_scales, fp8_absmax;
+    bool fp8_scaled = false;
 };
 
 namespace {
@@ -149,10 +151,43 @@ int alloc_graph(vmn_index* idx, int64_t n) {
     return VMN_OK;
 }
 
+// (Re)computes the low-precision copies of rows [from, to). FP8 scales are
+// fixed from the rows present at the first call (later rows are clamped).
+int refresh_lowp(vmn_index* idx, int64_t from, int64_t to) {
+    const int DP = idx->DP;
+    const int64_t cap = idx->cap, cnt = to - from;
+    if (cnt <= 0) return VMN_OK;
+    const float* src = idx->vecs.as<float>() + from * DP;
+    if (idx->p.low_precision & VMN_STORE_BF16) {
+        HIPCHK(idx->bf16_rows.reserve((size_t)cap * DP * 2));
+        HIPCHK(idx->bf16_norms.reserve((size_t)cap * sizeof(float)));
+        HIPCHK(vmn::launch_to_bf16(src, cnt, DP, static_cast<char*>(idx->bf16_rows.p) + from * DP * 2,
+                                   idx->bf16_norms.as<float>() + from, idx->stream));
+    }
+    if (idx->p.low_precision & VMN_STORE_FP8) {
+        HIPCHK(idx->fp8_rows.reserve((size_t)cap * DP));
+        HIPCHK(idx->fp8_norms.reserve((size_t)cap * sizeof(float)));
+        HIPCHK(idx->fp8_scales.reserve((size_t)DP * sizeof(float)));
+        HIPCHK(idx->fp8_absmax.reserve((size_t)DP * sizeof(unsigned int)));
+        if (!idx->fp8_scaled) {
+            HIPCHK(vmn::launch_fp8_scales(idx->vecs.as<float>(), to, idx->p.dim, DP,
+                                          idx->fp8_absmax.as<unsigned int>(),
+                                          idx->fp8_scales.as<float>(), idx->stream));
+            idx->fp8_scaled = true;
+            from = 0;
+            src = idx->vecs.as<float>();
+        }
+        HIPCHK(vmn::launch_to_fp8(src, to - from, DP, idx->fp8_scales.as<float>(),
+                                  static_cast<char*>(idx->fp8_rows.p) + from * DP,
+                                  idx->fp8_norms.as<float>() + from, idx->stream));
+    }
+    return VMN_OK;
+}
+
 // Search kernel over `nq` padded query rows.
 int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int32_t* out_ids,
                float* out_d, bool queries_are_nodes, int mode, bool want_vis,
-               int64_t node_base = 0) {
+               int64_t node_base = 0, int prec = VMN_PREC_FP32, int krr = 0) {
     int rc = zero_counter(idx);
     if (rc) return rc;
     HIPCHK(idx->stat_hops.reserve((size_t)nq * sizeof(int32_t)));
@@ -187,6 +222,16 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
     a.stat_hops = idx->stat_hops.as<int32_t>();
     a.stat_ndist = idx->stat_ndist.as<int32_t>();
     a.counter = idx->counter.as<unsigned long long>();
+    a.prec = prec;
+    a.krr = krr;
+    if (prec == VMN_PREC_BF16) {
+        a.low_rows = idx->bf16_rows.p;
+        a.low_norms = idx->bf16_norms.as<float>();
+    } else if (prec == VMN_PREC_FP8) {
+        a.low_rows = idx->fp8_rows.p;
+        a.low_norms = idx->fp8_norms.as<float>();
+        a.fp8_scales = idx->fp8_scales.as<float>();
+    }
     const int64_t want = std::min<int64_t>(nq, (int64_t)idx->cus * 32);
     HIPCHK(vmn::launch_search(a, (int)std::max<int64_t>(want, 1), idx->stream));
     idx->last_nq = nq;
@@ -397,7 +442,9 @@ int vmn_destroy(vmn_index* idx) {
                       &idx->cand_ids, &idx->cand_d, &idx->vis_ids, &idx->vis_d, &idx->vis_cnt,
                       &idx->pair_dst, &idx->pair_d, &idx->rcount, &idx->rfill, &idx->roff,
                       &idx->rscan_tmp, &idx->rsrc, &idx->rd, &idx->rlist, &idx->centroid,
-                      &idx->medoid_key, &idx->staging, &idx->find_out, &idx->bad};
+                      &idx->medoid_key, &idx->staging, &idx->find_out, &idx->bad,
+                      &idx->bf16_rows, &idx->bf16_norms, &idx->fp8_rows, &idx->fp8_norms,
+                      &idx->fp8_scales, &idx->fp8_absmax};
     for (DevBuf* b : bufs) b->release();
     (void)hipStreamDestroy(idx->stream);
     delete idx;
@@ -424,6 +471,9 @@ int vmn_build(vmn_index* idx, const float* vecs, int64_t n, int32_t mem) {
     rc = compute_medoid(idx);
     if (rc) return rc;
     rc = idx->p.mode == VMN_MODE_PARTH ? build_parth(idx) : build_fresh(idx);
+    if (rc) return rc;
+    idx->fp8_scaled = false;
+    rc = refresh_lowp(idx, 0, n);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(idx->stream));
     idx->built = true;
@@ -476,17 +526,28 @@ int vmn_load_graph(vmn_index* idx, const float* vecs, int64_t n, const int32_t* 
     } else {
         idx->medoid = medoid;
     }
+    idx->fp8_scaled = false;
+    rc = refresh_lowp(idx, 0, n);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(idx->stream));
     idx->built = true;
     return VMN_OK;
 }
 
-int vmn_search(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int32_t L,
-               int32_t* ids_out, float* dists_out, int32_t mem) {
+int vmn_search_ex(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int32_t L,
+                  int32_t prec, int32_t k_rerank, int32_t* ids_out, float* dists_out,
+                  int32_t mem) {
     if (!idx || !queries || !ids_out) return fail(VMN_ERR_ARG, "NULL argument");
     if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");
     if (nq < 0 || k < 1 || k > L || L < 1 || L > 512)
         return fail(VMN_ERR_ARG, "need 1 <= k <= L <= 512 (k=%d, L=%d)", k, L);
+    if (prec == VMN_PREC_BF16 && !(idx->p.low_precision & VMN_STORE_BF16))
+        return fail(VMN_ERR_STATE, "index keeps no BF16 rows (low_precision)");
+    if (prec == VMN_PREC_FP8 && !(idx->p.low_precision & VMN_STORE_FP8))
+        return fail(VMN_ERR_STATE, "index keeps no FP8 rows (low_precision)");
+    if (prec < VMN_PREC_FP32 || prec > VMN_PREC_FP8) return fail(VMN_ERR_ARG, "bad precision %d", prec);
+    if (k_rerank != 0 && (k_rerank < k || k_rerank > 64 || k_rerank > L))
+        return fail(VMN_ERR_ARG, "need k <= k_rerank <= min(64, L) (k_rerank=%d)", k_rerank);
     if (nq == 0) return VMN_OK;
     int rc = set_device(idx);
     if (rc) return rc;
@@ -501,7 +562,8 @@ int vmn_search(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int3
         oi = idx->out_ids.as<int32_t>();
         od = idx->out_d.as<float>();
     }
-    rc = run_search(idx, q, nq, L, k, oi, od, false, idx->p.mode, false);
+    rc = run_search(idx, q, nq, L, k, oi, od, false, idx->p.mode, false, 0, prec,
+                    prec == VMN_PREC_FP32 ? 0 : k_rerank);
     if (rc) return rc;
     if (mem == VMN_MEM_HOST) {
         HIPCHK(hipMemcpyAsync(ids_out, oi, (size_t)nq * k * sizeof(int32_t), hipMemcpyDeviceToHost,
@@ -511,6 +573,43 @@ int vmn_search(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int3
                                   hipMemcpyDeviceToHost, idx->stream));
         HIPCHK(hipStreamSynchronize(idx->stream));
     }
+    return VMN_OK;
+}
+
+int vmn_search(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int32_t L,
+               int32_t* ids_out, float* dists_out, int32_t mem) {
+    return vmn_search_ex(idx, queries, nq, k, L, VMN_PREC_FP32, 0, ids_out, dists_out, mem);
+}
+
+int vmn_get_lowp_rows(const vmn_index* idx, int32_t prec, float* out) {
+    if (!idx || !out) return fail(VMN_ERR_ARG, "NULL argument");
+    const int DP = idx->DP, D = idx->p.dim;
+    const int64_t n = idx->n;
+    int rc = set_device(idx);
+    if (rc) return rc;
+    std::vector<float> rows((size_t)n * DP);
+    if (prec == VMN_PREC_BF16) {
+        if (!(idx->p.low_precision & VMN_STORE_BF16)) return fail(VMN_ERR_STATE, "no BF16 rows");
+        std::vector<uint16_t> h((size_t)n * DP);
+        HIPCHK(hipMemcpy(h.data(), idx->bf16_rows.p, h.size() * 2, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < h.size(); ++i) {
+            uint32_t w = (uint32_t)h[i] << 16;
+            std::memcpy(&rows[i], &w, 4);
+        }
+    } else if (prec == VMN_PREC_FP8) {
+        if (!(idx->p.low_precision & VMN_STORE_FP8)) return fail(VMN_ERR_STATE, "no FP8 rows");
+        DevBuf tmp;
+        HIPCHK(tmp.reserve(rows.size() * sizeof(float)));
+        HIPCHK(vmn::launch_fp8_dequant(idx->fp8_rows.p, n, DP, idx->fp8_scales.as<float>(),
+                                       tmp.as<float>(), idx->stream));
+        HIPCHK(hipMemcpyAsync(rows.data(), tmp.p, rows.size() * sizeof(float), hipMemcpyDeviceToHost,
+                              idx->stream));
+        HIPCHK(hipStreamSynchronize(idx->stream));
+        tmp.release();
+    } else {
+        return fail(VMN_ERR_ARG, "prec must be BF16 or FP8");
+    }
+    for (int64_t i = 0; i < n; ++i) std::memcpy(out + i * D, &rows[i * DP], D * sizeof(float));
     return VMN_OK;
 }
 
@@ -635,6 +734,8 @@ int vmn_insert(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out, i
     idx->n = base + n;
     const int last = idx->p.passes > 0 ? idx->p.passes - 1 : 0;
     rc = fresh_pass(idx, base, n, idx->p.alpha[last]);
+    if (rc) return rc;
+    rc = refresh_lowp(idx, base, base + n);
     if (rc) return rc;
     std::vector<int32_t> ids(n);
     for (int64_t i = 0; i < n; ++i) ids[i] = (int32_t)(base + i);

@@ -97,6 +97,161 @@ __device__ __forceinline__ void gather_dists(const QReg<DP>& q, const float* __r
     }
 }
 
+// ---------------------------------------------------------------------------
+// Low-precision candidate rows on MFMA.
+//
+// A wave computes q . x for 16 candidates per v_mfma_f32_16x16x32_{bf16,fp8}:
+// the query fills all 16 rows of A, candidate c's row is column c of B.
+// Lane l supplies, for k-block kk, elements 32*kk + 8*(l/16) .. +7 of the
+// query (A) and of candidate (l % 16) (B); A and B use the same k split, so
+// the products pair up whatever order the hardware sums them in, and column
+// n of the result (lane n % 16) holds q . x_n in every row.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+#define VMN_PREC_FP32 0
+#define VMN_PREC_BF16 1
+#define VMN_PREC_FP8 2
+
+template <int DP>
+struct QLow {
+    static constexpr int KB = DP / 32;  // k-blocks of 32
+    bf16x8_t a[KB];                     // BF16 query slice
+    long a8[KB];                        // FP8 query slice (8 x e4m3 per lane)
+    float qn;                           // ||q_hat||^2 (BF16) or ||q||^2 (FP8)
+    float t;                            // FP8: query scale
+};
+
+__device__ __forceinline__ __bf16 to_bf16(float f) { return (__bf16)f; }
+
+// Two floats -> two FP8 (hardware E4M3 encoding) packed in the low 16 bits.
+__device__ __forceinline__ uint32_t pk_fp8(float a, float b) {
+    return (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false) & 0xffffu;
+}
+
+__device__ __forceinline__ float fp8_to_f32(uint32_t word, int byte) {
+    switch (byte) {
+        case 0: return __builtin_amdgcn_cvt_f32_fp8((int)word, 0);
+        case 1: return __builtin_amdgcn_cvt_f32_fp8((int)word, 1);
+        case 2: return __builtin_amdgcn_cvt_f32_fp8((int)word, 2);
+        default: return __builtin_amdgcn_cvt_f32_fp8((int)word, 3);
+    }
+}
+
+// Query preparation (whole wave). qf: padded FP32 query row. BF16: q_hat =
+// bf16(q), qn = ||q_hat||^2. FP8: q'_j = q_j * s_j (per-dimension scales of
+// the stored rows), t = max|q'| / 448, A = e4m3(q' / t), qn = ||q||^2.
+template <int DP, int PREC>
+__device__ __forceinline__ void prep_query(QLow<DP>& Q, const float* qf, const float* scales) {
+    const int l = lane(), g = l >> 4;
+    if (PREC == VMN_PREC_BF16) {
+        float part = 0.0f;
+#pragma unroll
+        for (int kb = 0; kb < QLow<DP>::KB; ++kb) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const __bf16 h = to_bf16(qf[32 * kb + 8 * g + e]);
+                Q.a[kb][e] = h;
+                const float hf = (float)h;
+                part += hf * hf;
+            }
+        }
+        // lanes of one 16-lane group hold the same slice; sum the 4 groups
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        Q.qn = part;
+        Q.t = 1.0f;
+    } else {
+        float mx = 0.0f, part = 0.0f;
+#pragma unroll
+        for (int kb = 0; kb < QLow<DP>::KB; ++kb)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int j = 32 * kb + 8 * g + e;
+                const float v = qf[j] * scales[j];
+                mx = fmaxf(mx, fabsf(v));
+                part += qf[j] * qf[j];
+            }
+        mx = fmaxf(mx, __shfl_xor(mx, 16));
+        mx = fmaxf(mx, __shfl_xor(mx, 32));
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        const float t = mx > 0.0f ? mx / 448.0f : 1.0f;
+#pragma unroll
+        for (int kb = 0; kb < QLow<DP>::KB; ++kb) {
+            uint32_t w[4];
+#pragma unroll
+            for (int e = 0; e < 8; e += 2) {
+                const int j = 32 * kb + 8 * g + e;
+                const float v0 = fminf(fmaxf(qf[j] * scales[j] / t, -448.0f), 448.0f);
+                const float v1 = fminf(fmaxf(qf[j + 1] * scales[j + 1] / t, -448.0f), 448.0f);
+                w[e / 2] = pk_fp8(v0, v1);
+            }
+            const uint32_t lo = w[0] | (w[1] << 16), hi = w[2] | (w[3] << 16);
+            Q.a8[kb] = (long)(((unsigned long)hi << 32) | lo);
+        }
+        Q.qn = part;
+        Q.t = t;
+    }
+}
+
+// Distances from the prepared query to `cnt` candidates whose ids sit in LDS
+// ids[0..cnt); dist_out[s] (LDS) receives the distance of ids[s].
+// rows: [n x DP] BF16 (2 bytes) or FP8 (1 byte) rows; norms: ||x_hat||^2.
+template <int DP, int PREC>
+__device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* __restrict__ rows,
+                                                 const float* __restrict__ norms,
+                                                 const int32_t* ids, float* dist_out, int cnt) {
+    const int l = lane(), col = l & 15, g = l >> 4;
+    for (int base = 0; base < cnt; base += 16) {
+        const int c = base + col;
+        const int32_t id = c < cnt ? ids[c] : ids[0];
+        f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        float xn = 0.0f;
+        if (PREC == VMN_PREC_BF16) {
+            const uint4* r = reinterpret_cast<const uint4*>(
+                reinterpret_cast<const __bf16*>(rows) + (int64_t)id * DP);
+            uint4 v[QLow<DP>::KB];
+#pragma unroll
+            for (int kb = 0; kb < QLow<DP>::KB; ++kb) v[kb] = r[4 * kb + g];
+#pragma unroll
+            for (int kb = 0; kb < QLow<DP>::KB; ++kb) {
+                bf16x8_t b = __builtin_bit_cast(bf16x8_t, v[kb]);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Q.a[kb], b, acc, 0, 0, 0);
+            }
+            // ||x_hat||^2 from the same registers: 4 lanes (g = 0..3) hold the
+            // 4 k-slices of candidate `col`
+#pragma unroll
+            for (int kb = 0; kb < QLow<DP>::KB; ++kb) {
+                const uint32_t w[4] = {v[kb].x, v[kb].y, v[kb].z, v[kb].w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const bf16x2_t p = __builtin_bit_cast(bf16x2_t, w[e]);
+                    xn = __builtin_amdgcn_fdot2_f32_bf16(p, p, xn, false);
+                }
+            }
+            xn += __shfl_xor(xn, 16);
+            xn += __shfl_xor(xn, 32);
+        } else {
+            const unsigned long* r = reinterpret_cast<const unsigned long*>(
+                reinterpret_cast<const uint8_t*>(rows) + (int64_t)id * DP);
+            unsigned long v[QLow<DP>::KB];
+#pragma unroll
+            for (int kb = 0; kb < QLow<DP>::KB; ++kb) v[kb] = r[4 * kb + g];
+#pragma unroll
+            for (int kb = 0; kb < QLow<DP>::KB; ++kb)
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(Q.a8[kb], (long)v[kb], acc, 0, 0, 0);
+        }
+        if (g == 0 && c < cnt) {
+            const float dot = PREC == VMN_PREC_FP8 ? acc[0] * Q.t : acc[0];
+            const float nx = PREC == VMN_PREC_BF16 ? xn : norms[id];
+            dist_out[c] = (Q.qn + nx) - 2.0f * dot;
+        }
+    }
+}
+
 // FreshVamana bloom filter hashes (vamana/include/bloom_filter.cuh:11-30):
 // FNV-style over the 4 bytes of the id with a 64-bit accumulator, reduced
 // modulo the filter size (39,988 cells per query in the reference; here one

@@ -27,12 +27,25 @@ struct SearchArgs {
     float* vis_d;
     int32_t* vis_cnt;         // [nq]
     int P;
+    int prec;                 // VMN_PREC_FP32 / BF16 / FP8 distances during traversal
+    const void* low_rows;     // [cap x DP] BF16 or FP8 rows
+    const float* low_norms;   // [cap] ||x_hat||^2 of the low-precision rows
+    const float* fp8_scales;  // [DP] per-dimension FP8 scales
+    int krr;                  // > 0: exact FP32 re-rank of the first krr list entries (<= 64)
     int32_t* stat_hops;       // [nq] expansions
     int32_t* stat_ndist;      // [nq] distance evaluations
     unsigned long long* counter;  // work queue head, zero before launch
 };
 
 hipError_t launch_search(const SearchArgs& a, int grid, hipStream_t s);
+hipError_t launch_to_bf16(const float* vecs, int64_t n, int DP, void* rows, float* norms,
+                          hipStream_t s);
+hipError_t launch_fp8_scales(const float* vecs, int64_t n, int D, int DP, unsigned int* absmax_bits,
+                             float* scales, hipStream_t s);
+hipError_t launch_to_fp8(const float* vecs, int64_t n, int DP, const float* scales, void* rows,
+                         float* norms, hipStream_t s);
+hipError_t launch_fp8_dequant(const void* rows, int64_t n, int DP, const float* scales, float* out,
+                              hipStream_t s);
 
 struct PruneArgs {
     int mode;

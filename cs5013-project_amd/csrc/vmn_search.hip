@@ -33,11 +33,22 @@
 //
 // Both modes skip tombstoned nodes while expanding (search.cuh:122,
 // engine.cu:76) and give a tombstoned start node distance 1e30 (engine.cu:45).
+//
+// PREC selects the distance used while traversing: FP32 rows (lane-tree sum),
+// or BF16 / FP8 rows on MFMA (vmn_device.hpp). With krr > 0 the first krr
+// entries of the final list are re-ranked by exact FP32 distance (stable on
+// ties) before the k results are written; this is the re-rank step the
+// reference runs on the worklist (utils.cuh:199-285, merge_and_rerank_kernel),
+// done here on ids inside the search kernel.
 #include "vmn_kernels.hpp"
 
 namespace vmn {
 
-template <int DP, int LCAP, int MODE>
+__global__ void fill_zero_u32(unsigned int* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+template <int DP, int LCAP, int MODE, int PREC>
 __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
     __shared__ __align__(16) uint32_t s_id[2][LCAP];
     __shared__ __align__(16) float s_d[2][LCAP];
@@ -59,13 +70,24 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
 
         QReg<DP> q;
         q.load(a.queries + qi * DP, sub);
+        QLow<DP> ql;
+        if (PREC != VMN_PREC_FP32) prep_query<DP, PREC>(ql, a.queries + qi * DP, a.fp8_scales);
         const int32_t self = a.query_is_node ? (int32_t)(a.node_base + qi) : -1;
 
         if (MODE == VMN_MODE_FRESH) {
             for (int w = l; w < (int)VMN_BF_WORDS; w += 64) s_bloom[w] = 0u;
         }
         const int32_t st = a.start;
-        float sd = reduce8(q.partial(vecs + (int64_t)st * DP, sub));
+        float sd;
+        if (PREC == VMN_PREC_FP32) {
+            sd = reduce8(q.partial(vecs + (int64_t)st * DP, sub));
+        } else {
+            if (l == 0) s_sid[0] = st;
+            __syncthreads();
+            gather_dists_low<DP, PREC>(ql, a.low_rows, a.low_norms, s_sid, s_sd, 1);
+            __syncthreads();
+            sd = s_sd[0];
+        }
         if (a.del_bits && bit_test(a.del_bits, (uint32_t)st)) sd = 1e30f;
         __syncthreads();
         if (l == 0) {
@@ -147,17 +169,56 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
             const int slot = __popcll((unsigned long long)(vm & lanemask_lt()));
             if (valid) s_sid[slot] = nb;
             __syncthreads();
-            gather_dists<DP>(q, vecs, s_sid, s_sd, nnew);
+            if (PREC == VMN_PREC_FP32)
+                gather_dists<DP>(q, vecs, s_sid, s_sd, nnew);
+            else
+                gather_dists_low<DP, PREC>(ql, a.low_rows, a.low_norms, s_sid, s_sd, nnew);
             __syncthreads();
             ndist += nnew;
-            const uint32_t my_id = l < nnew ? (uint32_t)s_sid[l] : 0u;
-            const float my_d = l < nnew ? s_sd[l] : 0.0f;
-            cnt = merge_sorted<LCAP, false>(s_id[cur], s_d[cur], cnt, my_id, my_d, nnew,
+            // A full list keeps its entries on ties, so a candidate at or beyond
+            // its last distance can never enter: drop those before ranking.
+            const float thr = cnt == a.L ? s_d[cur][cnt - 1] : INFINITY;
+            uint32_t my_id = l < nnew ? (uint32_t)s_sid[l] : 0u;
+            float my_d = l < nnew ? s_sd[l] : 0.0f;
+            const bool keep = l < nnew && my_d < thr;
+            const uint64_t km = ballot(keep);
+            const int nkeep = __popcll((unsigned long long)km);
+            if (nkeep != nnew) {
+                __syncthreads();
+                if (keep) {
+                    const int k = __popcll((unsigned long long)(km & lanemask_lt()));
+                    s_sid[k] = (int32_t)my_id;
+                    s_sd[k] = my_d;
+                }
+                __syncthreads();
+                my_id = l < nkeep ? (uint32_t)s_sid[l] : 0u;
+                my_d = l < nkeep ? s_sd[l] : 0.0f;
+            }
+            cnt = merge_sorted<LCAP, false>(s_id[cur], s_d[cur], cnt, my_id, my_d, nkeep,
                                             s_id[cur ^ 1], s_d[cur ^ 1], a.L);
             cur ^= 1;
             __syncthreads();
         }
 
+        if (PREC != VMN_PREC_FP32 && a.krr > 0) {
+            // exact FP32 re-rank of the first krr entries, stable on ties
+            const int nr = cnt < a.krr ? cnt : a.krr;
+            if (l < nr) s_sid[l] = (int32_t)(s_id[cur][l] & VMN_ID_MASK);
+            __syncthreads();
+            gather_dists<DP>(q, vecs, s_sid, s_sd, nr);
+            __syncthreads();
+            const uint32_t rid = l < nr ? (uint32_t)s_sid[l] : 0u;
+            const float rd = l < nr ? s_sd[l] : 0.0f;
+            merge_sorted<LCAP, false>(s_id[cur], s_d[cur], 0, rid, rd, nr, s_id[cur ^ 1],
+                                      s_d[cur ^ 1], nr);
+            __syncthreads();
+            if (l < nr) {
+                s_id[cur][l] = s_id[cur ^ 1][l];
+                s_d[cur][l] = s_d[cur ^ 1][l];
+            }
+            __syncthreads();
+            if (cnt > nr) cnt = nr;  // entries past krr carry low-precision distances
+        }
         if (a.out_ids) {
             for (int j = l; j < a.out_n; j += 64) {
                 a.out_ids[qi * a.out_n + j] = j < cnt ? (int32_t)(s_id[cur][j] & VMN_ID_MASK) : -1;
@@ -173,13 +234,27 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
     }
 }
 
+template <int DP, int LCAP, int MODE>
+static hipError_t launch_search_p(const SearchArgs& a, int grid, hipStream_t s) {
+    switch (a.prec) {
+        case VMN_PREC_FP32:
+            hipLaunchKernelGGL((search_kernel<DP, LCAP, MODE, VMN_PREC_FP32>), dim3(grid), dim3(64), 0, s, a);
+            break;
+        case VMN_PREC_BF16:
+            hipLaunchKernelGGL((search_kernel<DP, LCAP, MODE, VMN_PREC_BF16>), dim3(grid), dim3(64), 0, s, a);
+            break;
+        case VMN_PREC_FP8:
+            hipLaunchKernelGGL((search_kernel<DP, LCAP, MODE, VMN_PREC_FP8>), dim3(grid), dim3(64), 0, s, a);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 template <int DP, int LCAP>
 static hipError_t launch_search_l(const SearchArgs& a, int grid, hipStream_t s) {
-    if (a.mode == VMN_MODE_PARTH)
-        hipLaunchKernelGGL((search_kernel<DP, LCAP, VMN_MODE_PARTH>), dim3(grid), dim3(64), 0, s, a);
-    else
-        hipLaunchKernelGGL((search_kernel<DP, LCAP, VMN_MODE_FRESH>), dim3(grid), dim3(64), 0, s, a);
-    return hipGetLastError();
+    if (a.mode == VMN_MODE_PARTH) return launch_search_p<DP, LCAP, VMN_MODE_PARTH>(a, grid, s);
+    return launch_search_p<DP, LCAP, VMN_MODE_FRESH>(a, grid, s);
 }
 
 template <int DP>
@@ -201,6 +276,96 @@ hipError_t launch_search(const SearchArgs& a, int grid, hipStream_t s) {
         case 256: return launch_search_d<256>(a, grid, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Low-precision copies of the rows.
+// ---------------------------------------------------------------------------
+__global__ void to_bf16_kernel(const float* __restrict__ vecs, int64_t n, int DP, __bf16* rows,
+                               float* norms) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int k = 0; k < DP; ++k) {
+        const __bf16 h = to_bf16(vecs[i * DP + k]);
+        rows[i * DP + k] = h;
+        const float f = (float)h;
+        s += f * f;
+    }
+    norms[i] = s;
+}
+hipError_t launch_to_bf16(const float* vecs, int64_t n, int DP, void* rows, float* norms,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, vecs, n,
+                       DP, reinterpret_cast<__bf16*>(rows), norms);
+    return hipGetLastError();
+}
+
+// Per-dimension absmax over the rows (non-negative float bits order as uints).
+__global__ void absmax_kernel(const float* __restrict__ vecs, int64_t n, int DP,
+                              unsigned int* absmax_bits) {
+    const int d = threadIdx.x;
+    unsigned int m = 0;
+    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const unsigned int b = __float_as_uint(fabsf(vecs[i * DP + d]));
+        m = b > m ? b : m;
+    }
+    atomicMax(&absmax_bits[d], m);
+}
+__global__ void scales_kernel(const unsigned int* absmax_bits, int DP, float* scales) {
+    const int d = threadIdx.x;
+    if (d >= DP) return;
+    const float m = __uint_as_float(absmax_bits[d]);
+    scales[d] = m > 0.0f ? m / 448.0f : 1.0f;
+}
+hipError_t launch_fp8_scales(const float* vecs, int64_t n, int D, int DP, unsigned int* absmax_bits,
+                             float* scales, hipStream_t s) {
+    (void)D;
+    hipLaunchKernelGGL(fill_zero_u32, dim3(1), dim3(256), 0, s, absmax_bits, DP);
+    const int64_t g = n < 2048 ? (n < 1 ? 1 : n) : 2048;
+    hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)g), dim3(DP), 0, s, vecs, n, DP, absmax_bits);
+    hipLaunchKernelGGL(scales_kernel, dim3(1), dim3(256), 0, s, absmax_bits, DP, scales);
+    return hipGetLastError();
+}
+
+// x~ = e4m3(clamp(x / s)), norms = ||s * x~||^2 with the hardware decode.
+__global__ void to_fp8_kernel(const float* __restrict__ vecs, int64_t n, int DP,
+                              const float* __restrict__ scales, uint8_t* rows, float* norms) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int k = 0; k < DP; k += 2) {
+        const float v0 = fminf(fmaxf(vecs[i * DP + k] / scales[k], -448.0f), 448.0f);
+        const float v1 = fminf(fmaxf(vecs[i * DP + k + 1] / scales[k + 1], -448.0f), 448.0f);
+        const uint32_t w = pk_fp8(v0, v1);
+        rows[i * DP + k] = (uint8_t)(w & 0xff);
+        rows[i * DP + k + 1] = (uint8_t)(w >> 8);
+        const float d0 = fp8_to_f32(w, 0) * scales[k];
+        const float d1 = fp8_to_f32(w, 1) * scales[k + 1];
+        s += d0 * d0;
+        s += d1 * d1;
+    }
+    norms[i] = s;
+}
+hipError_t launch_to_fp8(const float* vecs, int64_t n, int DP, const float* scales, void* rows,
+                         float* norms, hipStream_t s) {
+    hipLaunchKernelGGL(to_fp8_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, vecs, n,
+                       DP, scales, reinterpret_cast<uint8_t*>(rows), norms);
+    return hipGetLastError();
+}
+
+// Dequantised FP8 rows (s * x~), for error reports and tests.
+__global__ void fp8_dequant_kernel(const uint8_t* rows, int64_t n, int DP, const float* scales,
+                                   float* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * DP) return;
+    out[i] = fp8_to_f32(rows[i], 0) * scales[i % DP];
+}
+hipError_t launch_fp8_dequant(const void* rows, int64_t n, int DP, const float* scales, float* out,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(fp8_dequant_kernel, dim3((unsigned)((n * DP + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<const uint8_t*>(rows), n, DP, scales, out);
+    return hipGetLastError();
 }
 
 }  // namespace vmn

@@ -18,6 +18,9 @@ MODE_PARTH = 0
 MODE_FRESH = 1
 MEM_HOST = 0
 MEM_DEVICE = 1
+PREC = {"fp32": 0, "bf16": 1, "fp8": 2}
+STORE_BF16 = 1
+STORE_FP8 = 2
 
 STATUS = {0: "VMN_OK", -1: "VMN_ERR_ARG", -2: "VMN_ERR_HIP", -3: "VMN_ERR_STATE",
           -4: "VMN_ERR_CAPACITY", -5: "VMN_ERR_IO"}
@@ -36,7 +39,7 @@ class Params(ctypes.Structure):
                 ("seed", ctypes.c_uint64), ("mode", ctypes.c_int32),
                 ("visited_cap", ctypes.c_int32), ("max_hops", ctypes.c_int32),
                 ("P", ctypes.c_int32), ("M", ctypes.c_int32), ("capacity", ctypes.c_int64),
-                ("device", ctypes.c_int32)]
+                ("device", ctypes.c_int32), ("low_precision", ctypes.c_int32)]
 
 
 _lib = None
@@ -57,6 +60,8 @@ def lib():
         L.vmn_build.argtypes = [P, P, i64, i32]
         L.vmn_load_graph.argtypes = [P, P, i64, P, i32, i64, i32]
         L.vmn_search.argtypes = [P, P, i64, i32, i32, P, P, i32]
+        L.vmn_search_ex.argtypes = [P, P, i64, i32, i32, i32, i32, P, P, i32]
+        L.vmn_get_lowp_rows.argtypes = [P, i32, P]
         L.vmn_get_adjacency.argtypes = [P, P, i32]
         L.vmn_size.argtypes = [P]
         L.vmn_size.restype = i64
@@ -76,7 +81,7 @@ def lib():
         for name in ("vmn_create", "vmn_destroy", "vmn_build", "vmn_load_graph", "vmn_search",
                      "vmn_get_adjacency", "vmn_search_stats", "vmn_groundtruth", "vmn_insert",
                      "vmn_delete_ids", "vmn_delete_vectors", "vmn_consolidate",
-                     "vmn_get_deleted"):
+                     "vmn_get_deleted", "vmn_search_ex", "vmn_get_lowp_rows"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -119,7 +124,7 @@ class Vamana:
 
     def __init__(self, vectors=None, mode="parth", *, dim=None, R=None, L_build=None,
                  alphas=None, passes=None, seed=None, visited_cap=None, max_hops=None, P=None,
-                 M=None, capacity=0, device=0, _build=True):
+                 M=None, capacity=0, device=0, low_precision=0, _build=True):
         m = {"parth": MODE_PARTH, "fresh": MODE_FRESH}[mode] if isinstance(mode, str) else mode
         p = default_params(m)
         if vectors is not None:
@@ -138,6 +143,10 @@ class Vamana:
                 p.alpha[i] = al[min(i, len(al) - 1)]
         p.capacity = capacity
         p.device = device
+        if isinstance(low_precision, str):
+            low_precision = {"": 0, "bf16": STORE_BF16, "fp8": STORE_FP8,
+                             "both": STORE_BF16 | STORE_FP8}[low_precision]
+        p.low_precision = low_precision
         self.params = p
         self._h = ctypes.c_void_p()
         _check(lib().vmn_create(ctypes.byref(p), ctypes.byref(self._h)))
@@ -179,12 +188,16 @@ class Vamana:
     def stream(self):
         return lib().vmn_stream(self._h)
 
-    def searchPoints(self, queries, k=10, L=None, ids_out=None, dists_out=None, sync=True):
+    def searchPoints(self, queries, k=10, L=None, ids_out=None, dists_out=None, sync=True,
+                     prec="fp32", rerank=0):
         """Search a batch; returns (ids [nq,k] int32, dists [nq,k] float32).
 
+        prec: traversal distances "fp32" | "bf16" | "fp8"; rerank: exact FP32
+        re-rank of that many list entries (k <= rerank <= 64, 0 = none).
         numpy in -> numpy out. CUDA tensor in -> CUDA tensors out (written on
         the index's stream; sync=False leaves the call asynchronous)."""
         L = k if L is None else L
+        pc = PREC[prec] if isinstance(prec, str) else prec
         nq = int(queries.shape[0])
         qp, mem, keep = _arg(queries, np.float32)
         if mem == MEM_DEVICE:
@@ -193,15 +206,17 @@ class Vamana:
                                                                   device=queries.device)
             dists = dists_out if dists_out is not None else torch.empty(
                 (nq, k), dtype=torch.float32, device=queries.device)
-            _check(lib().vmn_search(self._h, qp, nq, k, L, ctypes.c_void_p(ids.data_ptr()),
-                                    ctypes.c_void_p(dists.data_ptr()), mem))
+            _check(lib().vmn_search_ex(self._h, qp, nq, k, L, pc, rerank,
+                                       ctypes.c_void_p(ids.data_ptr()),
+                                       ctypes.c_void_p(dists.data_ptr()), mem))
             if sync:
                 self.synchronize()
             return ids, dists
         ids = np.empty((nq, k), dtype=np.int32)
         dists = np.empty((nq, k), dtype=np.float32)
-        _check(lib().vmn_search(self._h, qp, nq, k, L, ids.ctypes.data_as(ctypes.c_void_p),
-                                dists.ctypes.data_as(ctypes.c_void_p), mem))
+        _check(lib().vmn_search_ex(self._h, qp, nq, k, L, pc, rerank,
+                                   ids.ctypes.data_as(ctypes.c_void_p),
+                                   dists.ctypes.data_as(ctypes.c_void_p), mem))
         return ids, dists
 
     search = searchPoints
@@ -255,6 +270,13 @@ class Vamana:
     def adjacency(self):
         out = np.empty((self.size, self.params.R), dtype=np.int32)
         _check(lib().vmn_get_adjacency(self._h, out.ctypes.data_as(ctypes.c_void_p), MEM_HOST))
+        return out
+
+    def lowp_rows(self, prec):
+        """Rows as the BF16 / FP8 traversal sees them, decoded to float32."""
+        out = np.empty((self.size, self.params.dim), dtype=np.float32)
+        pc = PREC[prec] if isinstance(prec, str) else prec
+        _check(lib().vmn_get_lowp_rows(self._h, pc, out.ctypes.data_as(ctypes.c_void_p)))
         return out
 
     def search_stats(self):

@@ -44,6 +44,15 @@ extern "C" {
 #define VMN_MEM_HOST 0
 #define VMN_MEM_DEVICE 1
 
+/* Distance precision during traversal (vmn_search_ex). */
+#define VMN_PREC_FP32 0 /* FP32 rows, lane-tree sums (the reference's arithmetic) */
+#define VMN_PREC_BF16 1 /* BF16 rows on MFMA, FP32 accumulation */
+#define VMN_PREC_FP8 2  /* FP8 E4M3 rows with per-dimension absmax scales on MFMA */
+
+/* vmn_params.low_precision bits: which low-precision copies the index keeps. */
+#define VMN_STORE_BF16 1
+#define VMN_STORE_FP8 2
+
 typedef struct vmn_index vmn_index;
 
 typedef struct vmn_params {
@@ -60,6 +69,7 @@ typedef struct vmn_params {
     int32_t M;           /* FRESH: reverse candidate cap (constants.cuh:20, 500) */
     int64_t capacity;    /* node capacity for inserts (0: the build size) */
     int32_t device;      /* HIP device ordinal */
+    int32_t low_precision; /* VMN_STORE_* bits: BF16 / FP8 row copies kept beside FP32 */
 } vmn_params;
 
 /* Reference defaults for `mode` (parth: R=32, L=64, alpha 1.2 x 2 passes). */
@@ -87,6 +97,19 @@ int vmn_load_graph(vmn_index* idx, const float* vecs, int64_t n, const int32_t* 
  * parth `searchKernel` (engine.cu:100-120). Tombstoned nodes are skipped. */
 int vmn_search(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int32_t L,
                int32_t* ids_out, float* dists_out, int32_t mem);
+
+/* vmn_search with the traversal distance in `prec` (VMN_PREC_*; BF16/FP8 need
+ * the matching VMN_STORE_* copy) and an exact FP32 re-rank of the first
+ * `k_rerank` (k <= k_rerank <= 64; 0 = none) list entries before the k
+ * results are taken (the reference re-ranks its worklist in
+ * merge_and_rerank_kernel, utils.cuh:199-285). */
+int vmn_search_ex(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int32_t L,
+                  int32_t prec, int32_t k_rerank, int32_t* ids_out, float* dists_out,
+                  int32_t mem);
+
+/* The rows as the low-precision traversal sees them, decoded to float
+ * ([size x dim], host): BF16 values, or FP8 values times their scales. */
+int vmn_get_lowp_rows(const vmn_index* idx, int32_t prec, float* out);
 
 /* Batched insert of n vectors [n x dim]: they become nodes size .. size+n-1
  * (written to ids_out when not NULL). Each is searched from the entry point

@@ -1,0 +1,98 @@
+"""BF16 / FP8 traversal on MFMA with exact FP32 re-rank.
+
+BF16 represents every integer 0..256 exactly and the MFMA accumulates the
+products exactly in FP32 below 2^24, so on the integer-valued data the BF16
+distances (||q||^2 + ||x||^2 - 2 q.x) equal the FP32 ones bit for bit and the
+search results must be identical. FP8 E4M3 is lossy: its decoded rows are
+checked against an exhaustive model of the encoding (nearest of the 256
+codes, ties to the even code), and its recall after re-rank against FP32."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def e4m3_values():
+    """All finite OCP E4M3 (e4m3fn) values: bias 7, no infinities, 0x7f/0xff NaN."""
+    vals = []
+    for code in range(256):
+        s = -1.0 if code & 0x80 else 1.0
+        e = (code >> 3) & 0xF
+        m = code & 7
+        if e == 15 and m == 7:
+            continue
+        v = (m / 8.0) * 2.0 ** -6 if e == 0 else (1 + m / 8.0) * 2.0 ** (e - 7)
+        vals.append((s * v, code))
+    return vals
+
+
+def e4m3_round(x):
+    vals = e4m3_values()
+    v = np.array([a for a, _ in vals])
+    codes = np.array([c for _, c in vals])
+    out = np.empty_like(x, dtype=np.float64)
+    for i, xi in enumerate(x.ravel()):
+        d = np.abs(v - xi)
+        best = np.flatnonzero(d == d.min())
+        pick = best[np.argmin(codes[best] & 1)] if len(best) > 1 else best[0]
+        out.ravel()[i] = v[pick]
+    return out
+
+
+@pytest.mark.parametrize("L", [10, 40, 100])
+def test_bf16_traversal_equals_fp32_on_integer_data(vmn, sift, gpu, L):
+    import os
+    g1 = np.load(os.path.join(os.path.dirname(__file__), "golden", "g1_graph.npz"))
+    base = sift["base"].astype(np.float32)
+    q = sift["query"].astype(np.float32)
+    idx = vmn.Vamana.from_graph(base, g1["adj"], medoid=5000, mode="fresh", low_precision="both")
+    assert np.array_equal(idx.lowp_rows("bf16"), base)
+    fi, fd = idx.searchPoints(q, k=10, L=L)
+    bi, bd = idx.searchPoints(q, k=10, L=L, prec="bf16")
+    assert np.array_equal(fi, bi) and np.array_equal(fd, bd)
+    ri, rd = idx.searchPoints(q, k=10, L=L, prec="bf16", rerank=min(40, L))
+    assert np.array_equal(fi, ri) and np.array_equal(fd, rd)
+    fh = idx.search_stats()
+    idx.searchPoints(q, k=10, L=L)
+    assert idx.search_stats() == fh
+
+
+def test_bf16_parth_mode_equals_fp32(vmn, sift, gpu):
+    base = sift["base"].astype(np.float32)
+    q = sift["query"].astype(np.float32)
+    idx = vmn.Vamana.from_graph(base, sift["parth_adj"], medoid=3732, low_precision="bf16")
+    for L in (20, 64):
+        fi, fd = idx.searchPoints(q, k=10, L=L)
+        bi, bd = idx.searchPoints(q, k=10, L=L, prec="bf16")
+        assert np.array_equal(fi, bi) and np.array_equal(fd, bd)
+
+
+def test_fp8_rows_follow_e4m3_rounding(vmn, gpu):
+    v = vmn.datasets.integer_mixture(3000, 128, k=16, seed=21, stream=0)
+    idx = vmn.Vamana(v, mode="fresh", R=32, L_build=64, alphas=(1.2,), low_precision="fp8")
+    dec = idx.lowp_rows("fp8")
+    absmax = np.abs(v).max(0)
+    scale = np.where(absmax > 0, absmax / np.float32(448.0), np.float32(1.0)).astype(np.float32)
+    sub = v[:64]
+    want = (e4m3_round((sub / scale).astype(np.float32)) * scale).astype(np.float32)
+    assert np.allclose(dec[:64], want, rtol=0, atol=1e-4 * scale.max())
+    rel = np.abs(dec - v) / np.maximum(np.abs(v), 1.0)
+    assert rel.max() <= 1.0 / 16 + 1e-6
+
+
+def test_fp8_rerank_recall_and_exact_distances(vmn, orc, gpu):
+    ds = vmn.datasets
+    base = ds.integer_mixture(20_000, 128, k=64, seed=1234, stream=0)
+    q = ds.integer_mixture(200, 128, k=64, seed=1234, stream=1)
+    idx = vmn.Vamana(base, mode="fresh", R=32, L_build=64, alphas=(1.0, 1.2),
+                     low_precision="both")
+    gt, _ = orc.bruteforce_fp64(base, q, 10)
+    fi, fd = idx.searchPoints(q, k=10, L=64)
+    ri, rd = idx.searchPoints(q, k=10, L=64, prec="fp8", rerank=40)
+    rf = vmn.recall_at_k(fi, gt)
+    r8 = vmn.recall_at_k(ri, gt)
+    assert r8 >= rf - 0.05, (r8, rf)
+    # re-ranked distances are the exact FP32 distances of the returned ids
+    exact = ((base[ri].astype(np.float64) - q[:, None, :]) ** 2).sum(-1).astype(np.float32)
+    assert np.array_equal(rd, exact)
+    assert (np.diff(rd, axis=1) >= 0).all()
