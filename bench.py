@@ -46,6 +46,11 @@ def parse():
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--rerank", type=int, default=40)
+    ap.add_argument("--workload", default="search", choices=["search", "streaming"],
+                    help="search: the headline metric; streaming: 10 rounds of 5%% deletes + "
+                         "5%% inserts with consolidation, recall@10 per round")
+    ap.add_argument("--rounds", type=int, default=10)
+    ap.add_argument("--L-eval", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
 
@@ -59,6 +64,8 @@ def main():
     import torch
     vmn = importlib.import_module("cs5013-project_amd")
     dev = torch.device("cuda:0")
+    if args.workload == "streaming":
+        return streaming(args, vmn, torch, dev)
     n, nq, d = args.n, args.nq, 128
     alphas = tuple(float(x) for x in args.alphas.split(","))
     log(f"generating {n}x{d} base + {nq} queries (256-component integer mixture)")
@@ -187,6 +194,103 @@ def main():
                      "bytes_per_launch": head["bytes_per_launch"]},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+def log_uniform_batches(rng, total, lo=1000, hi=50000):
+    """Split `total` into batch sizes drawn log-uniform in [lo, hi]."""
+    out = []
+    while total > 0:
+        b = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        b = min(b, total)
+        out.append(b)
+        total -= b
+    return out
+
+
+def streaming(args, vmn, torch, dev):
+    """FreshVamana streaming: start from the n x 128 FRESH index of the search
+    workload (R=64, L=128, alpha 1.0 then 1.2); updates prune with alpha 1.2;
+    each round deletes 5% of the live ids (uniform, seeded) and
+    inserts 5% new points from the same mixture, both in batches of
+    log-uniform size in [1k, 50k]; tombstones are consolidated at the end of
+    the round; recall@10 of 10k queries (FP32, L=L_eval) against FP64 truth
+    over the live points."""
+    n, nq, d = args.n, args.nq, 128
+    frac = 0.05
+    grow = int(n * frac) * args.rounds
+    base = vmn.datasets.integer_mixture(n, d, k=256, sigma=20.0, seed=1234, stream=0)
+    extra = vmn.datasets.integer_mixture(grow, d, k=256, sigma=20.0, seed=1234, stream=2)
+    qs = vmn.datasets.integer_mixture(nq, d, k=256, sigma=20.0, seed=1234, stream=1)
+    allv = torch.from_numpy(np.concatenate([base, extra])).to(dev)
+    q_t = torch.from_numpy(qs).to(dev)
+    t = time.time()
+    idx = vmn.Vamana(allv[:n], mode="fresh", R=args.R, L_build=args.L_build, alphas=(1.0, 1.2),
+                     capacity=n + grow)
+    torch.cuda.synchronize()
+    log(f"streaming: built {n} in {time.time() - t:.2f} s")
+    rng = np.random.default_rng(4321)
+    deleted = np.zeros(n + grow, dtype=bool)
+    size = n
+    rounds = []
+
+    def recall_now():
+        live = np.flatnonzero(~deleted[:size])
+        live_t = torch.from_numpy(live).to(dev)
+        gt_local, _ = vmn.groundtruth(allv[live_t].contiguous(), q_t, 10)
+        gt = live[gt_local.cpu().numpy()]
+        ids, _ = idx.searchPoints(q_t, k=10, L=args.L_eval)
+        found = ids.cpu().numpy()
+        assert not np.isin(found[found >= 0], np.flatnonzero(deleted)).any()
+        return vmn.recall_at_k(found, gt, 10)
+
+    r0 = recall_now()
+    log(f"round 0: recall@10 {r0:.4f} (L={args.L_eval})")
+    for rnd in range(1, args.rounds + 1):
+        live = np.flatnonzero(~deleted[:size])
+        n_del = int(len(live) * frac)
+        kill = rng.choice(live, n_del, replace=False).astype(np.int32)
+        n_ins = int(n * frac)
+        t_del = t_ins = 0.0
+        off = 0
+        for b in log_uniform_batches(rng, n_del):
+            torch.cuda.synchronize()
+            t = time.time()
+            idx.deleteIds(kill[off:off + b])
+            torch.cuda.synchronize()
+            t_del += time.time() - t
+            off += b
+        deleted[kill] = True
+        for b in log_uniform_batches(rng, n_ins):
+            torch.cuda.synchronize()
+            t = time.time()
+            idx.insertPoints(allv[size:size + b])
+            torch.cuda.synchronize()
+            t_ins += time.time() - t
+            size += b
+        t = time.time()
+        idx.consolidate()
+        torch.cuda.synchronize()
+        t_con = time.time() - t
+        rec = recall_now()
+        rounds.append({"round": rnd, "recall@10": round(rec, 4), "deleted": n_del,
+                       "inserted": n_ins, "live": int((~deleted[:size]).sum()),
+                       "delete_ops_per_s": round(n_del / max(t_del, 1e-9), 1),
+                       "insert_points_per_s": round(n_ins / t_ins, 1),
+                       "consolidate_s": round(t_con, 3)})
+        log(f"round {rnd}: recall@10 {rec:.4f}, insert {n_ins / t_ins:,.0f} pts/s, "
+            f"consolidate {t_con:.2f} s, live {rounds[-1]['live']}")
+    ins_rate = sum(r["inserted"] for r in rounds) / sum(r["inserted"] / r["insert_points_per_s"]
+                                                       for r in rounds)
+    out = {"metric": "FreshVamana streaming: insert points/s with recall@10 per round",
+           "value": round(ins_rate, 1), "unit": "points/s", "n_gpus": 1, "steps": args.rounds,
+           "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32",
+           "data": "synthetic: seeded 256-component Gaussian mixture, integer-valued in [0,255]",
+           "config": {"workload": f"FreshVamana {args.rounds} rounds of 5% delete + 5% insert "
+                                  f"on {n}x{d}, log-uniform batches [1k, 50k]",
+                      "R": args.R, "L_build": args.L_build, "alpha": 1.2,
+                      "L_eval": args.L_eval, "recall_round0": round(r0, 4), "rounds": rounds}}
     print(json.dumps(out), flush=True)
 
 

@@ -61,6 +61,22 @@ struct QReg {
 #pragma unroll
         for (int t = 0; t < T; ++t) v[t] = r[sub + 8 * t];
     }
+    __device__ __forceinline__ void partial2(const float* row0, const float* row1, int sub,
+                                             float& s0, float& s1) const {
+        const float4* r0 = reinterpret_cast<const float4*>(row0);
+        const float4* r1 = reinterpret_cast<const float4*>(row1);
+        float4 x0[T], x1[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) { x0[t] = r0[sub + 8 * t]; x1[t] = r1[sub + 8 * t]; }
+        s0 = 0.0f; s1 = 0.0f;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            float a = x0[t].x - v[t].x, b = x0[t].y - v[t].y, c = x0[t].z - v[t].z, d = x0[t].w - v[t].w;
+            s0 += a * a; s0 += b * b; s0 += c * c; s0 += d * d;
+            a = x1[t].x - v[t].x; b = x1[t].y - v[t].y; c = x1[t].z - v[t].z; d = x1[t].w - v[t].w;
+            s1 += a * a; s1 += b * b; s1 += c * c; s1 += d * d;
+        }
+    }
     // This lane's share of ||row - q||^2.
     __device__ __forceinline__ float partial(const float* row, int sub) const {
         const float4* r = reinterpret_cast<const float4*>(row);
@@ -85,15 +101,20 @@ template <int DP>
 __device__ __forceinline__ void gather_dists(const QReg<DP>& q, const float* __restrict__ vecs,
                                              const int32_t* ids, float* dist_out, int cnt) {
     const int l = lane(), sub = l & 7, grp = l >> 3;
-    for (int base = 0; base < cnt; base += 8) {
-        const int r = base + grp;
-        float acc = 0.0f;
-        if (r < cnt) {
-            const int32_t id = ids[r];
-            acc = q.partial(vecs + (int64_t)id * DP, sub);
+    for (int base = 0; base < cnt; base += 16) {
+        const int r0 = base + grp, r1 = base + 8 + grp;
+        float a0 = 0.0f, a1 = 0.0f;
+        const int32_t i0 = r0 < cnt ? ids[r0] : ids[0];
+        const int32_t i1 = r1 < cnt ? ids[r1] : ids[0];
+        if (base + 8 < cnt) {
+            q.partial2(vecs + (int64_t)i0 * DP, vecs + (int64_t)i1 * DP, sub, a0, a1);
+        } else {
+            a0 = q.partial(vecs + (int64_t)i0 * DP, sub);
         }
-        acc = reduce8(acc);
-        if (sub == 0 && r < cnt) dist_out[r] = acc;
+        a0 = reduce8(a0);
+        a1 = reduce8(a1);
+        if (sub == 0 && r0 < cnt) dist_out[r0] = a0;
+        if (sub == 0 && r1 < cnt) dist_out[r1] = a1;
     }
 }
 
@@ -198,56 +219,74 @@ __device__ __forceinline__ void prep_query(QLow<DP>& Q, const float* qf, const f
 }
 
 // Distances from the prepared query to `cnt` candidates whose ids sit in LDS
-// ids[0..cnt); dist_out[s] (LDS) receives the distance of ids[s].
-// rows: [n x DP] BF16 (2 bytes) or FP8 (1 byte) rows; norms: ||x_hat||^2.
+// ids[0..cnt); dist_out[s] (LDS) receives the distance of ids[s]. Two groups
+// of 16 candidates are loaded before either is multiplied, so one memory round
+// trip serves 32 candidates. BF16 row norms come from the loaded registers
+// (v_dot2_f32_bf16); FP8 norms (per-dimension scales) from `norms`.
 template <int DP, int PREC>
 __device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* __restrict__ rows,
                                                  const float* __restrict__ norms,
                                                  const int32_t* ids, float* dist_out, int cnt) {
     const int l = lane(), col = l & 15, g = l >> 4;
-    for (int base = 0; base < cnt; base += 16) {
-        const int c = base + col;
-        const int32_t id = c < cnt ? ids[c] : ids[0];
-        f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        float xn = 0.0f;
+    constexpr int KB = QLow<DP>::KB;
+    for (int base = 0; base < cnt; base += 32) {
+        const int c0 = base + col, c1 = base + 16 + col;
+        const int32_t id0 = c0 < cnt ? ids[c0] : ids[0];
+        const int32_t id1 = c1 < cnt ? ids[c1] : ids[0];
+        const bool two = base + 16 < cnt;
+        f32x4_t a0 = {0.0f, 0.0f, 0.0f, 0.0f}, a1 = {0.0f, 0.0f, 0.0f, 0.0f};
+        float x0 = 0.0f, x1 = 0.0f;
         if (PREC == VMN_PREC_BF16) {
-            const uint4* r = reinterpret_cast<const uint4*>(
-                reinterpret_cast<const __bf16*>(rows) + (int64_t)id * DP);
-            uint4 v[QLow<DP>::KB];
+            const uint4* r0 = reinterpret_cast<const uint4*>(reinterpret_cast<const __bf16*>(rows) + (int64_t)id0 * DP);
+            const uint4* r1 = reinterpret_cast<const uint4*>(reinterpret_cast<const __bf16*>(rows) + (int64_t)id1 * DP);
+            uint4 v0[KB], v1[KB];
 #pragma unroll
-            for (int kb = 0; kb < QLow<DP>::KB; ++kb) v[kb] = r[4 * kb + g];
+            for (int kb = 0; kb < KB; ++kb) v0[kb] = r0[4 * kb + g];
+            if (two) {
 #pragma unroll
-            for (int kb = 0; kb < QLow<DP>::KB; ++kb) {
-                bf16x8_t b = __builtin_bit_cast(bf16x8_t, v[kb]);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Q.a[kb], b, acc, 0, 0, 0);
+                for (int kb = 0; kb < KB; ++kb) v1[kb] = r1[4 * kb + g];
             }
-            // ||x_hat||^2 from the same registers: 4 lanes (g = 0..3) hold the
-            // 4 k-slices of candidate `col`
 #pragma unroll
-            for (int kb = 0; kb < QLow<DP>::KB; ++kb) {
-                const uint32_t w[4] = {v[kb].x, v[kb].y, v[kb].z, v[kb].w};
+            for (int kb = 0; kb < KB; ++kb) {
+                a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Q.a[kb], __builtin_bit_cast(bf16x8_t, v0[kb]), a0, 0, 0, 0);
+                const uint32_t w[4] = {v0[kb].x, v0[kb].y, v0[kb].z, v0[kb].w};
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const bf16x2_t p = __builtin_bit_cast(bf16x2_t, w[e]);
-                    xn = __builtin_amdgcn_fdot2_f32_bf16(p, p, xn, false);
+                for (int e = 0; e < 4; ++e) { const bf16x2_t p = __builtin_bit_cast(bf16x2_t, w[e]); x0 = __builtin_amdgcn_fdot2_f32_bf16(p, p, x0, false); }
+            }
+            if (two) {
+#pragma unroll
+                for (int kb = 0; kb < KB; ++kb) {
+                    a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Q.a[kb], __builtin_bit_cast(bf16x8_t, v1[kb]), a1, 0, 0, 0);
+                    const uint32_t w[4] = {v1[kb].x, v1[kb].y, v1[kb].z, v1[kb].w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) { const bf16x2_t p = __builtin_bit_cast(bf16x2_t, w[e]); x1 = __builtin_amdgcn_fdot2_f32_bf16(p, p, x1, false); }
                 }
             }
-            xn += __shfl_xor(xn, 16);
-            xn += __shfl_xor(xn, 32);
+            x0 += __shfl_xor(x0, 16); x0 += __shfl_xor(x0, 32);
+            x1 += __shfl_xor(x1, 16); x1 += __shfl_xor(x1, 32);
         } else {
-            const unsigned long* r = reinterpret_cast<const unsigned long*>(
-                reinterpret_cast<const uint8_t*>(rows) + (int64_t)id * DP);
-            unsigned long v[QLow<DP>::KB];
+            const unsigned long* r0 = reinterpret_cast<const unsigned long*>(reinterpret_cast<const uint8_t*>(rows) + (int64_t)id0 * DP);
+            const unsigned long* r1 = reinterpret_cast<const unsigned long*>(reinterpret_cast<const uint8_t*>(rows) + (int64_t)id1 * DP);
+            unsigned long v0[KB], v1[KB];
 #pragma unroll
-            for (int kb = 0; kb < QLow<DP>::KB; ++kb) v[kb] = r[4 * kb + g];
+            for (int kb = 0; kb < KB; ++kb) v0[kb] = r0[4 * kb + g];
+            if (two) {
 #pragma unroll
-            for (int kb = 0; kb < QLow<DP>::KB; ++kb)
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(Q.a8[kb], (long)v[kb], acc, 0, 0, 0);
+                for (int kb = 0; kb < KB; ++kb) v1[kb] = r1[4 * kb + g];
+            }
+            x0 = norms[id0];
+            x1 = two ? norms[id1] : 0.0f;
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb) a0 = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(Q.a8[kb], (long)v0[kb], a0, 0, 0, 0);
+            if (two) {
+#pragma unroll
+                for (int kb = 0; kb < KB; ++kb) a1 = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(Q.a8[kb], (long)v1[kb], a1, 0, 0, 0);
+            }
         }
-        if (g == 0 && c < cnt) {
-            const float dot = PREC == VMN_PREC_FP8 ? acc[0] * Q.t : acc[0];
-            const float nx = PREC == VMN_PREC_BF16 ? xn : norms[id];
-            dist_out[c] = (Q.qn + nx) - 2.0f * dot;
+        if (g == 0) {
+            const float t = PREC == VMN_PREC_FP8 ? Q.t : 1.0f;
+            if (c0 < cnt) dist_out[c0] = (Q.qn + x0) - 2.0f * (a0[0] * t);
+            if (c1 < cnt) dist_out[c1] = (Q.qn + x1) - 2.0f * (a1[0] * t);
         }
     }
 }
@@ -310,9 +349,15 @@ __device__ __forceinline__ int merge_sorted(const uint32_t* src_id, const float*
         bidv[j] = bi < cnt ? src_id[bi] : 0u;
         less[j] = 0;
     }
-    int rank = 0;
+    int rank = 0, ub = 0;
     for (int t = 0; t < nnew; ++t) {
         float dt = readlane_f(my_d, t);
+        if (!BY_ID) {
+            int c = 0;
+#pragma unroll
+            for (int j = 0; j < PER; ++j) c += __popcll((unsigned long long)ballot((l + 64 * j) < cnt && bdv[j] <= dt));
+            if (l == t) ub = c;
+        }
         if (BY_ID) {
             uint32_t kt = (uint32_t)readlane_i((int)my_key, t);
             rank += (dt < my_d) || (dt == my_d && kt < my_key);
@@ -336,7 +381,14 @@ __device__ __forceinline__ int merge_sorted(const uint32_t* src_id, const float*
             }
         }
     }
-    if (l < nnew) {
+    if (!BY_ID && l < nnew) {
+        const int pos = ub + rank;
+        if (pos < limit) {
+            dst_id[pos] = my_id;
+            dst_d[pos] = my_d;
+        }
+    }
+    if (BY_ID && l < nnew) {
         // list entries ordered before the new element
         int lo = 0, hi = cnt;
         while (lo < hi) {
