@@ -46,11 +46,15 @@ def parse():
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--rerank", type=int, default=40)
-    ap.add_argument("--workload", default="search", choices=["search", "streaming"],
+    ap.add_argument("--workload", default="search", choices=["search", "streaming", "serving"],
                     help="search: the headline metric; streaming: 10 rounds of 5%% deletes + "
                          "5%% inserts with consolidation, recall@10 per round")
     ap.add_argument("--rounds", type=int, default=10)
     ap.add_argument("--L-eval", type=int, default=60)
+    ap.add_argument("--serve-n", type=int, default=10_000_000)
+    ap.add_argument("--serve-rate", type=float, default=50_000.0, help="offered queries/s")
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-L", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
 
@@ -66,6 +70,8 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "streaming":
         return streaming(args, vmn, torch, dev)
+    if args.workload == "serving":
+        return serving(args, vmn, torch, dev)
     n, nq, d = args.n, args.nq, 128
     alphas = tuple(float(x) for x in args.alphas.split(","))
     log(f"generating {n}x{d} base + {nq} queries (256-component integer mixture)")
@@ -239,13 +245,16 @@ def streaming(args, vmn, torch, dev):
         live_t = torch.from_numpy(live).to(dev)
         gt_local, _ = vmn.groundtruth(allv[live_t].contiguous(), q_t, 10)
         gt = live[gt_local.cpu().numpy()]
-        ids, _ = idx.searchPoints(q_t, k=10, L=args.L_eval)
-        found = ids.cpu().numpy()
-        assert not np.isin(found[found >= 0], np.flatnonzero(deleted)).any()
-        return vmn.recall_at_k(found, gt, 10)
+        out = []
+        for L in (args.L_eval, args.L_build):
+            ids, _ = idx.searchPoints(q_t, k=10, L=L)
+            found = ids.cpu().numpy()
+            assert not np.isin(found[found >= 0], np.flatnonzero(deleted)).any()
+            out.append(vmn.recall_at_k(found, gt, 10))
+        return out
 
     r0 = recall_now()
-    log(f"round 0: recall@10 {r0:.4f} (L={args.L_eval})")
+    log(f"round 0: recall@10 {r0[0]:.4f} (L={args.L_eval}), {r0[1]:.4f} (L={args.L_build})")
     for rnd in range(1, args.rounds + 1):
         live = np.flatnonzero(~deleted[:size])
         n_del = int(len(live) * frac)
@@ -273,12 +282,13 @@ def streaming(args, vmn, torch, dev):
         torch.cuda.synchronize()
         t_con = time.time() - t
         rec = recall_now()
-        rounds.append({"round": rnd, "recall@10": round(rec, 4), "deleted": n_del,
+        rounds.append({"round": rnd, f"recall@10_L{args.L_eval}": round(rec[0], 4),
+                       f"recall@10_L{args.L_build}": round(rec[1], 4), "deleted": n_del,
                        "inserted": n_ins, "live": int((~deleted[:size]).sum()),
                        "delete_ops_per_s": round(n_del / max(t_del, 1e-9), 1),
                        "insert_points_per_s": round(n_ins / t_ins, 1),
                        "consolidate_s": round(t_con, 3)})
-        log(f"round {rnd}: recall@10 {rec:.4f}, insert {n_ins / t_ins:,.0f} pts/s, "
+        log(f"round {rnd}: recall@10 {rec[0]:.4f} / {rec[1]:.4f}, insert {n_ins / t_ins:,.0f} pts/s, "
             f"consolidate {t_con:.2f} s, live {rounds[-1]['live']}")
     ins_rate = sum(r["inserted"] for r in rounds) / sum(r["inserted"] / r["insert_points_per_s"]
                                                        for r in rounds)
@@ -290,7 +300,113 @@ def streaming(args, vmn, torch, dev):
            "config": {"workload": f"FreshVamana {args.rounds} rounds of 5% delete + 5% insert "
                                   f"on {n}x{d}, log-uniform batches [1k, 50k]",
                       "R": args.R, "L_build": args.L_build, "alpha": 1.2,
-                      "L_eval": args.L_eval, "recall_round0": round(r0, 4), "rounds": rounds}}
+                      "L_eval": [args.L_eval, args.L_build],
+                      "recall_round0": [round(r0[0], 4), round(r0[1], 4)], "rounds": rounds}}
+    print(json.dumps(out), flush=True)
+
+
+def serving(args, vmn, torch, dev):
+    """Served index: n x 96 unit-normalised 1,024-component mixture, FRESH graph
+    (R=64, L=100, alpha 1.0 then 1.2), BF16 rows with FP32 re-rank. Requests
+    arrive as a Poisson stream at the offered query rate: 90% searches (batch
+    log-uniform 1..256, k=10, L=64), 5% inserts, 5% deletes (batch log-uniform
+    1..256). Latency is arrival to completion on the server clock."""
+    n, d, nq_gt = args.serve_n, 96, 10_000
+    rng = np.random.default_rng(99)
+    log(f"serving: generating {n}x{d} unit mixture")
+    base = vmn.datasets.unit_mixture(n, d, k=1024, sigma=0.1, seed=1234, stream=0)
+    pool = vmn.datasets.unit_mixture(100_000, d, k=1024, sigma=0.1, seed=1234, stream=1)
+    extra = vmn.datasets.unit_mixture(200_000, d, k=1024, sigma=0.1, seed=1234, stream=2)
+    base_t = torch.from_numpy(base).to(dev)
+    t = time.time()
+    idx = vmn.Vamana(base_t, mode="fresh", R=64, L_build=100, alphas=(1.0, 1.2),
+                     capacity=n + len(extra), low_precision="bf16")
+    torch.cuda.synchronize()
+    build_s = time.time() - t
+    log(f"serving: built in {build_s:.1f} s ({n / build_s:,.0f} points/s)")
+    # request trace
+    mean_batch = 255.0 / np.log(256.0)
+    req_rate = args.serve_rate / (0.9 * mean_batch)
+    t_arr, kinds, sizes = [], [], []
+    tcur = 0.0
+    while tcur < args.serve_seconds:
+        tcur += rng.exponential(1.0 / req_rate)
+        r = rng.random()
+        kinds.append(vmn.REQ_SEARCH if r < 0.9 else (vmn.REQ_INSERT if r < 0.95 else vmn.REQ_DELETE))
+        sizes.append(int(np.exp(rng.uniform(0, np.log(256)))))
+        t_arr.append(tcur)
+    alive = np.ones(n, dtype=bool)
+    srv = vmn.Server(idx, k=10, L=args.serve_L, prec="bf16", rerank=40, max_batch=8192, n_slots=4)
+    ins_off = 0
+    outstanding, lat, done_q, results = [], [], 0, 0
+    t_first = None
+    start = srv.now() + 0.05
+    log(f"serving: {len(t_arr)} requests over {args.serve_seconds} s, offered "
+        f"{args.serve_rate:,.0f} queries/s")
+    for ta, kind, sz in zip(t_arr, kinds, sizes):
+        target = start + ta
+        while srv.now() < target:
+            pass
+        if kind == vmn.REQ_SEARCH:
+            data = pool[rng.integers(0, len(pool), sz)]
+        elif kind == vmn.REQ_INSERT:
+            data = extra[ins_off:ins_off + sz]
+            ins_off += sz
+        else:
+            ids = rng.choice(np.flatnonzero(alive[:n]), sz, replace=False).astype(np.int32)
+            alive[ids] = False
+            data = ids
+        outstanding.append((srv.submit(kind, data, t_arrival=target), kind, sz))
+        # reap finished requests from the front
+        while outstanding:
+            r = srv.poll(outstanding[0][0])
+            if r is None:
+                break
+            _, k_, s_ = outstanding.pop(0)
+            if k_ == vmn.REQ_SEARCH:
+                lat.append(r["t_done"] - r["t_arrival"])
+                done_q += s_
+    t_last_arrival = start + t_arr[-1]
+    while outstanding:
+        r = srv.poll(outstanding[0][0])
+        if r is None:
+            continue
+        _, k_, s_ = outstanding.pop(0)
+        if k_ == vmn.REQ_SEARCH:
+            lat.append(r["t_done"] - r["t_arrival"])
+            done_q += s_
+    t_end = srv.now()
+    srv.close()
+    lat_ms = np.array(lat) * 1e3
+    thr = done_q / (t_end - start)
+    log(f"serving: {done_q} queries, {thr:,.0f} q/s, p50 {np.percentile(lat_ms, 50):.2f} ms, "
+        f"p99 {np.percentile(lat_ms, 99):.2f} ms")
+    # recall on a 10k-query FP64 subset of the final index (live points)
+    q_gt = pool[:nq_gt]
+    deleted = idx.deleted
+    live = np.flatnonzero(~deleted)
+    allv = torch.from_numpy(np.concatenate([base, extra[:ins_off]]))
+    gt_local, _ = vmn.groundtruth(allv[torch.from_numpy(live)].to(dev).contiguous(),
+                                  torch.from_numpy(q_gt).to(dev), 10)
+    gt = live[gt_local.cpu().numpy()]
+    ids, _ = idx.searchPoints(torch.from_numpy(q_gt).to(dev), k=10, L=args.serve_L, prec="bf16",
+                              rerank=40)
+    rec = vmn.recall_at_k(ids.cpu().numpy(), gt, 10)
+    ids32, _ = idx.searchPoints(torch.from_numpy(q_gt).to(dev), k=10, L=args.serve_L)
+    rec32 = vmn.recall_at_k(ids32.cpu().numpy(), gt, 10)
+    log(f"serving: recall@10 {rec:.4f} (BF16+rerank), {rec32:.4f} (FP32)")
+    out = {"metric": "served index: queries/s with p50/p99 latency", "value": round(thr, 1),
+           "unit": "queries/s", "n_gpus": 1, "steps": len(t_arr), "warmup": 0,
+           "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "bf16", "data": "synthetic: seeded 1,024-component mixture, unit-normalised",
+           "config": {"workload": f"served Vamana {n}x{d}, Poisson {args.serve_rate:,.0f} "
+                                  f"offered queries/s, 90% search / 10% insert+delete",
+                      "build_s": round(build_s, 2), "requests": len(t_arr),
+                      "queries_done": done_q, "inserted": ins_off,
+                      "deleted": int((~alive).sum()), "L_search": args.serve_L,
+                      "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+                      "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+                      "recall@10_bf16_rerank": round(rec, 4), "recall@10_fp32": round(rec32, 4)}}
     print(json.dumps(out), flush=True)
 
 

@@ -6,9 +6,10 @@ Import with importlib (the directory name is not an identifier):
     index = vmn.Vamana(base, mode="parth", R=32, L_build=64)
     ids, dists = index.searchPoints(queries, k=10, L=40)
 """
-from .vamana import (MODE_FRESH, MODE_PARTH, Params, Vamana, VamanaError, default_params,
-                     groundtruth, lib, recall_at_k)
+from .vamana import (MODE_FRESH, MODE_PARTH, REQ_DELETE, REQ_INSERT, REQ_SEARCH, Params, Server,
+                     Vamana, VamanaError, default_params, groundtruth, lib, recall_at_k)
 from . import datasets
 
-__all__ = ["Vamana", "VamanaError", "Params", "default_params", "groundtruth", "lib",
-           "recall_at_k", "datasets", "MODE_PARTH", "MODE_FRESH"]
+__all__ = ["Vamana", "VamanaError", "Params", "Server", "default_params", "groundtruth", "lib",
+           "recall_at_k", "datasets", "MODE_PARTH", "MODE_FRESH", "REQ_SEARCH", "REQ_INSERT",
+           "REQ_DELETE"]

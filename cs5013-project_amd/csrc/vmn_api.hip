@@ -86,6 +86,15 @@ struct vmn_index {
     bool fp8_scaled = false;
 };
 
+// A search context: its own stream and per-launch scratch, so several search
+// batches can run on the device at once (the index is read-only meanwhile).
+struct vmn_ctx {
+    vmn_index* idx = nullptr;
+    hipStream_t stream = nullptr;
+    DevBuf counter, stat_hops, stat_ndist;
+    int64_t last_nq = 0;
+};
+
 namespace {
 
 int set_device(const vmn_index* idx) {
@@ -187,11 +196,16 @@ int refresh_lowp(vmn_index* idx, int64_t from, int64_t to) {
 // Search kernel over `nq` padded query rows.
 int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int32_t* out_ids,
                float* out_d, bool queries_are_nodes, int mode, bool want_vis,
-               int64_t node_base = 0, int prec = VMN_PREC_FP32, int krr = 0) {
-    int rc = zero_counter(idx);
-    if (rc) return rc;
-    HIPCHK(idx->stat_hops.reserve((size_t)nq * sizeof(int32_t)));
-    HIPCHK(idx->stat_ndist.reserve((size_t)nq * sizeof(int32_t)));
+               int64_t node_base = 0, int prec = VMN_PREC_FP32, int krr = 0,
+               vmn_ctx* ctx = nullptr) {
+    hipStream_t stream = ctx ? ctx->stream : idx->stream;
+    DevBuf& counter = ctx ? ctx->counter : idx->counter;
+    DevBuf& stat_hops = ctx ? ctx->stat_hops : idx->stat_hops;
+    DevBuf& stat_ndist = ctx ? ctx->stat_ndist : idx->stat_ndist;
+    HIPCHK(counter.reserve(64));
+    HIPCHK(vmn::launch_fill_u64(counter.as<unsigned long long>(), 1, 0ull, stream));
+    HIPCHK(stat_hops.reserve((size_t)nq * sizeof(int32_t)));
+    HIPCHK(stat_ndist.reserve((size_t)nq * sizeof(int32_t)));
     vmn::SearchArgs a{};
     a.mode = mode;
     a.DP = idx->DP;
@@ -219,9 +233,9 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
         a.vis_cnt = idx->vis_cnt.as<int32_t>();
     }
     a.P = idx->p.P;
-    a.stat_hops = idx->stat_hops.as<int32_t>();
-    a.stat_ndist = idx->stat_ndist.as<int32_t>();
-    a.counter = idx->counter.as<unsigned long long>();
+    a.stat_hops = stat_hops.as<int32_t>();
+    a.stat_ndist = stat_ndist.as<int32_t>();
+    a.counter = counter.as<unsigned long long>();
     a.prec = prec;
     a.krr = krr;
     if (prec == VMN_PREC_BF16) {
@@ -233,8 +247,9 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
         a.fp8_scales = idx->fp8_scales.as<float>();
     }
     const int64_t want = std::min<int64_t>(nq, (int64_t)idx->cus * 32);
-    HIPCHK(vmn::launch_search(a, (int)std::max<int64_t>(want, 1), idx->stream));
-    idx->last_nq = nq;
+    HIPCHK(vmn::launch_search(a, (int)std::max<int64_t>(want, 1), stream));
+    if (ctx) ctx->last_nq = nq;
+    else idx->last_nq = nq;
     return VMN_OK;
 }
 
@@ -867,6 +882,58 @@ int vmn_get_deleted(const vmn_index* idx, uint8_t* mask_out) {
     }
     for (int64_t i = 0; i < idx->n; ++i) mask_out[i] = (bits[i >> 5] >> (i & 31)) & 1u;
     return VMN_OK;
+}
+
+int vmn_ctx_create(vmn_index* idx, vmn_ctx** out) {
+    if (!idx || !out) return fail(VMN_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    int rc = set_device(idx);
+    if (rc) return rc;
+    auto* c = new vmn_ctx();
+    c->idx = idx;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(VMN_ERR_HIP, "stream: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return VMN_OK;
+}
+
+int vmn_ctx_destroy(vmn_ctx* c) {
+    if (!c) return VMN_OK;
+    (void)hipSetDevice(c->idx->p.device);
+    (void)hipStreamSynchronize(c->stream);
+    c->counter.release();
+    c->stat_hops.release();
+    c->stat_ndist.release();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return VMN_OK;
+}
+
+void* vmn_ctx_stream(const vmn_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int vmn_ctx_search(vmn_ctx* c, const float* queries, int64_t nq, int32_t k, int32_t L,
+                   int32_t prec, int32_t k_rerank, int32_t* ids_out, float* dists_out) {
+    if (!c || !queries || !ids_out) return fail(VMN_ERR_ARG, "NULL argument");
+    vmn_index* idx = c->idx;
+    if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");
+    if (idx->p.dim != idx->DP) return fail(VMN_ERR_ARG, "context search needs dim % 32 == 0");
+    if (nq < 0 || k < 1 || k > L || L < 1 || L > 512)
+        return fail(VMN_ERR_ARG, "need 1 <= k <= L <= 512 (k=%d, L=%d)", k, L);
+    if (prec == VMN_PREC_BF16 && !(idx->p.low_precision & VMN_STORE_BF16))
+        return fail(VMN_ERR_STATE, "index keeps no BF16 rows (low_precision)");
+    if (prec == VMN_PREC_FP8 && !(idx->p.low_precision & VMN_STORE_FP8))
+        return fail(VMN_ERR_STATE, "index keeps no FP8 rows (low_precision)");
+    if (prec < VMN_PREC_FP32 || prec > VMN_PREC_FP8) return fail(VMN_ERR_ARG, "bad precision %d", prec);
+    if (k_rerank != 0 && (k_rerank < k || k_rerank > 64 || k_rerank > L))
+        return fail(VMN_ERR_ARG, "need k <= k_rerank <= min(64, L) (k_rerank=%d)", k_rerank);
+    if (nq == 0) return VMN_OK;
+    int rc = set_device(idx);
+    if (rc) return rc;
+    return run_search(idx, queries, nq, L, k, ids_out, dists_out, false, idx->p.mode, false, 0, prec,
+                      prec == VMN_PREC_FP32 ? 0 : k_rerank, c);
 }
 
 const char* vmn_last_error(void) { return g_err.c_str(); }

@@ -42,6 +42,16 @@ class Params(ctypes.Structure):
                 ("device", ctypes.c_int32), ("low_precision", ctypes.c_int32)]
 
 
+class ServerParams(ctypes.Structure):
+    """Mirror of `vmn_server_params`."""
+    _fields_ = [("k", ctypes.c_int32), ("L", ctypes.c_int32), ("prec", ctypes.c_int32),
+                ("k_rerank", ctypes.c_int32), ("max_batch", ctypes.c_int32),
+                ("n_slots", ctypes.c_int32), ("max_queued_bytes", ctypes.c_int64),
+                ("consolidate_every", ctypes.c_int64), ("device", ctypes.c_int32)]
+
+
+REQ_SEARCH, REQ_INSERT, REQ_DELETE = 0, 1, 2
+
 _lib = None
 
 
@@ -76,12 +86,30 @@ def lib():
         L.vmn_delete_vectors.argtypes = [P, P, i64, P, i32]
         L.vmn_consolidate.argtypes = [P]
         L.vmn_get_deleted.argtypes = [P, P]
+        L.vmn_ctx_create.argtypes = [P, ctypes.POINTER(P)]
+        L.vmn_ctx_destroy.argtypes = [P]
+        L.vmn_ctx_stream.argtypes = [P]
+        L.vmn_ctx_stream.restype = P
+        L.vmn_ctx_search.argtypes = [P, P, i64, i32, i32, i32, i32, P, P]
+        L.vmn_default_server_params.argtypes = [ctypes.POINTER(ServerParams)]
+        L.vmn_default_server_params.restype = None
+        L.vmn_server_create.argtypes = [P, ctypes.POINTER(ServerParams), i32, ctypes.POINTER(P)]
+        L.vmn_server_now.argtypes = [P]
+        L.vmn_server_now.restype = ctypes.c_double
+        L.vmn_server_submit.argtypes = [P, i32, P, i64, ctypes.c_double, ctypes.POINTER(i64)]
+        L.vmn_server_poll.argtypes = [P, i64, P, P, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double)]
+        L.vmn_server_pending.argtypes = [P]
+        L.vmn_server_pending.restype = i64
+        L.vmn_server_destroy.argtypes = [P]
         L.vmn_last_error.argtypes = []
         L.vmn_last_error.restype = ctypes.c_char_p
         for name in ("vmn_create", "vmn_destroy", "vmn_build", "vmn_load_graph", "vmn_search",
                      "vmn_get_adjacency", "vmn_search_stats", "vmn_groundtruth", "vmn_insert",
                      "vmn_delete_ids", "vmn_delete_vectors", "vmn_consolidate",
-                     "vmn_get_deleted", "vmn_search_ex", "vmn_get_lowp_rows"):
+                     "vmn_get_deleted", "vmn_search_ex", "vmn_get_lowp_rows", "vmn_ctx_create",
+                     "vmn_ctx_destroy", "vmn_ctx_search", "vmn_server_create", "vmn_server_submit",
+                     "vmn_server_poll", "vmn_server_destroy"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -315,3 +343,74 @@ def recall_at_k(found, gt_ids, k=10):
     for f, g in zip(found, gt):
         hits += len(set(int(x) for x in f if x >= 0) & set(int(x) for x in g))
     return hits / (k * found.shape[0])
+
+
+class Server:
+    """Continuous-batching request server over an index (vmn_server_*).
+
+    submit() queues a request with its arrival time (server clock, now());
+    poll() returns None while pending, else a dict with the results and the
+    arrival / completion times. While the server runs, use the index only
+    through it."""
+
+    def __init__(self, index, k=10, L=64, prec="fp32", rerank=0, max_batch=4096, n_slots=3,
+                 max_queued_bytes=1 << 30, consolidate_every=0):
+        p = ServerParams()
+        lib().vmn_default_server_params(ctypes.byref(p))
+        p.k, p.L = k, L
+        p.prec = PREC[prec] if isinstance(prec, str) else prec
+        p.k_rerank, p.max_batch, p.n_slots = rerank, max_batch, n_slots
+        p.max_queued_bytes, p.consolidate_every = max_queued_bytes, consolidate_every
+        p.device = index.params.device
+        self.index, self.k, self.dim = index, k, int(index.params.dim)
+        self._h = ctypes.c_void_p()
+        rc = lib().vmn_server_create(index._h, ctypes.byref(p), self.dim, ctypes.byref(self._h))
+        if rc != 0:
+            raise VamanaError(rc, "vmn_server_create failed")
+        self._meta = {}
+
+    def now(self):
+        return float(lib().vmn_server_now(self._h))
+
+    def submit(self, kind, data, t_arrival=None):
+        t = self.now() if t_arrival is None else float(t_arrival)
+        dt = np.int32 if kind == REQ_DELETE else np.float32
+        a = np.ascontiguousarray(data, dtype=dt)
+        n = int(a.shape[0])
+        tk = ctypes.c_int64()
+        rc = lib().vmn_server_submit(self._h, kind, a.ctypes.data_as(ctypes.c_void_p), n, t,
+                                     ctypes.byref(tk))
+        if rc != 0:
+            raise VamanaError(rc, "submit rejected")
+        self._meta[tk.value] = (kind, n)
+        return tk.value
+
+    def poll(self, ticket):
+        kind, n = self._meta[ticket]
+        ids = np.empty((n, self.k) if kind == REQ_SEARCH else (n,), dtype=np.int32)
+        d = np.empty((n, self.k), dtype=np.float32) if kind == REQ_SEARCH else None
+        ta, td = ctypes.c_double(), ctypes.c_double()
+        rc = lib().vmn_server_poll(self._h, ticket, ids.ctypes.data_as(ctypes.c_void_p),
+                                   None if d is None else d.ctypes.data_as(ctypes.c_void_p),
+                                   ctypes.byref(ta), ctypes.byref(td))
+        if rc == 0:
+            return None
+        del self._meta[ticket]
+        if rc < 0:
+            raise VamanaError(rc, f"request {ticket} failed")
+        return {"kind": kind, "ids": ids if kind != REQ_DELETE else None, "dists": d,
+                "t_arrival": ta.value, "t_done": td.value}
+
+    def pending(self):
+        return int(lib().vmn_server_pending(self._h))
+
+    def close(self):
+        if self._h:
+            lib().vmn_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

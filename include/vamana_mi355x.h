@@ -54,6 +54,8 @@ extern "C" {
 #define VMN_STORE_FP8 2
 
 typedef struct vmn_index vmn_index;
+typedef struct vmn_ctx vmn_ctx;       /* search context: own stream + scratch */
+typedef struct vmn_server vmn_server; /* continuous-batching request server */
 
 typedef struct vmn_params {
     int32_t dim;         /* vector dimension D (any D <= 256) */
@@ -159,6 +161,60 @@ void* vmn_stream(const vmn_index* idx);
  * compute_groundtruth (vamana.cu:1644). Runs on `device`; synchronous. */
 int vmn_groundtruth(const float* base, int64_t n, int32_t dim, const float* queries, int64_t nq,
                     int32_t k, int32_t* ids_out, double* dists_out, int32_t mem, int32_t device);
+
+/* Search contexts. Each has its own HIP stream and scratch, so searches on
+ * different contexts run concurrently on the device. Searches only read the
+ * index: the caller must not run an update (build, insert, delete,
+ * consolidate) on the index while a context search is in flight.
+ * vmn_ctx_search takes device pointers (dim a multiple of 32) and is
+ * asynchronous on vmn_ctx_stream(ctx); otherwise as vmn_search_ex. No
+ * counterpart in the reference (its search is synchronous on the default
+ * stream, search.cuh:344). */
+int vmn_ctx_create(vmn_index* idx, vmn_ctx** out);
+int vmn_ctx_destroy(vmn_ctx* ctx);
+void* vmn_ctx_stream(const vmn_ctx* ctx);
+int vmn_ctx_search(vmn_ctx* ctx, const float* queries, int64_t nq, int32_t k, int32_t L,
+                   int32_t prec, int32_t k_rerank, int32_t* ids_out, float* dists_out);
+
+/* Continuous-batching server (no counterpart in the reference, whose request
+ * loops sort fixed batches into delete/insert/search phases,
+ * parth/src/engine.cu:407-568, vamana.cu:1782-1895). Requests are queued in
+ * arrival order; consecutive searches are packed into the batch of a free
+ * slot (a search context, up to max_batch queries) and launched while other
+ * slots' batches are still on the device; an insert or delete runs when it
+ * reaches the head of the queue, after the launched searches finish, so each
+ * search sees exactly the updates that arrived before it and returns the same
+ * result as when run alone. Submission fails with VMN_ERR_CAPACITY once the
+ * queued payload exceeds max_queued_bytes (admission by memory). Data is
+ * copied at submit; dim must be a multiple of 32. While a server runs, the
+ * index must only be used through it. */
+#define VMN_REQ_SEARCH 0 /* data: float [n x dim] queries; results [n x k] ids, dists */
+#define VMN_REQ_INSERT 1 /* data: float [n x dim] vectors; results: n assigned ids */
+#define VMN_REQ_DELETE 2 /* data: int32 [n] ids */
+
+typedef struct vmn_server_params {
+    int32_t k, L;              /* results per query, beam width */
+    int32_t prec, k_rerank;    /* traversal precision (VMN_PREC_*), FP32 re-rank count */
+    int32_t max_batch;         /* queries per search launch (per slot) */
+    int32_t n_slots;           /* search launches in flight at once (1..16) */
+    int64_t max_queued_bytes;  /* admission limit on queued request payload */
+    int64_t consolidate_every; /* consolidate after this many deletes (0: never) */
+    int32_t device;
+} vmn_server_params;
+
+void vmn_default_server_params(vmn_server_params* p);
+int vmn_server_create(vmn_index* idx, const vmn_server_params* p, int32_t dim, vmn_server** out);
+/* Seconds since the server was created (the clock t_arrival is measured on). */
+double vmn_server_now(const vmn_server* srv);
+int vmn_server_submit(vmn_server* srv, int32_t kind, const void* data, int64_t n,
+                      double t_arrival, int64_t* ticket);
+/* 1: done (results copied, ticket released), 0: pending, < 0: the request's
+ * error status (ticket released). */
+int vmn_server_poll(vmn_server* srv, int64_t ticket, int32_t* ids_out, float* dists_out,
+                    double* t_arrival, double* t_done);
+int64_t vmn_server_pending(const vmn_server* srv);
+/* Finishes the queued requests, then stops the engine. */
+int vmn_server_destroy(vmn_server* srv);
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* vmn_last_error(void);
