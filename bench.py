@@ -132,31 +132,47 @@ def main():
         chosen[form] = (ok[0] if ok else max(sw, key=lambda x: x["recall@10"]), bool(ok))
         sweeps[form] = sw
 
-    # timed steps: every form at its chosen L (K steps each); the headline is BF16 + re-rank
+    # timed steps: every form at its chosen L. The launch shape is autotuned,
+    # the batched search is captured once as a HIP graph and replayed.
     timed = {}
+    ctx = vmn.SearchContext(idx)
+    cst = torch.cuda.ExternalStream(ctx.stream)
     for form in forms:
         L = chosen[form][0]["L"]
         prec, rr = forms[form]
+        rr = min(rr, L)
+        tune = vmn.tuning.launch_shape(idx, q_t, k=10, L=L, prec=prec, rerank=rr, reps=3)
+        ctx.capture(q_t, ids, dists, k=10, L=L, prec=prec, rerank=rr)
         for _ in range(args.warmup):
-            idx.searchPoints(q_t, k=10, L=L, ids_out=ids, dists_out=dists, sync=False, prec=prec,
-                             rerank=min(rr, L))
-        idx.synchronize()
+            ctx.replay()
+        cst.synchronize()
         torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
         t = time.time()
-        kern_ms = run(form, L, args.steps, sync_first=False)
-        torch.cuda.synchronize()
+        e0.record(cst)
+        for _ in range(args.steps):
+            ctx.replay()
+        e1.record(cst)
+        e1.synchronize()
         wall = time.time() - t
+        kern_ms = e0.elapsed_time(e1) / args.steps
         rec = vmn.recall_at_k(ids.cpu().numpy(), gt, 10)
+        idx.searchPoints(q_t, k=10, L=L, prec=prec, rerank=rr)  # same traversal, for the counters
         hops, nd = idx.search_stats()
-        row_b = {"fp32": d * 4, "bf16": d * 2 + 4, "fp8": d + 4}[prec]
-        bytes_launch = (d * 4) * nq + hops * 4 * args.R + nd * row_b + nq * min(rr, L) * d * 4 \
+        row_b = {"fp32": d * 4, "bf16": d * 2, "fp8": d + 4}[prec]
+        bytes_launch = (d * 4) * nq + hops * 4 * args.R + nd * row_b + nq * rr * d * 4 \
             + nq * 10 * 8
         timed[form] = {"L": L, "recall@10": round(rec, 4), "target_met": chosen[form][1],
                        "qps": round(nq * args.steps / wall, 1), "ms_per_step": wall * 1e3 / args.steps,
                        "kernel_ms": kern_ms, "bytes_per_launch": int(bytes_launch),
-                       "achieved_GBs": bytes_launch / (kern_ms / 1e3) / 1e9}
+                       "achieved_GBs": bytes_launch / (kern_ms / 1e3) / 1e9,
+                       "launch": {"waves_per_cu": tune["waves_per_cu"], "list_cap": tune["list_cap"]},
+                       "hops_per_query": round(hops / nq, 1), "dists_per_query": round(nd / nq, 1)}
         log(f"timed {form:12s} L={L} {timed[form]['qps']:,.0f} QPS recall {rec:.4f} kernel "
-            f"{kern_ms:.3f} ms {timed[form]['achieved_GBs']:.0f} GB/s")
+            f"{kern_ms:.3f} ms {timed[form]['achieved_GBs']:.0f} GB/s (launch {tune['waves_per_cu']}"
+            f" waves/CU, cap {tune['list_cap']})")
+    ctx.close()
 
     head = timed["bf16+rerank"]
     # distance error of the low-precision rows against FP64 on a sample of pairs

@@ -6,10 +6,12 @@ Import with importlib (the directory name is not an identifier):
     index = vmn.Vamana(base, mode="parth", R=32, L_build=64)
     ids, dists = index.searchPoints(queries, k=10, L=40)
 """
-from .vamana import (MODE_FRESH, MODE_PARTH, REQ_DELETE, REQ_INSERT, REQ_SEARCH, Params, Server,
-                     Vamana, VamanaError, default_params, groundtruth, lib, recall_at_k)
-from . import datasets
+from .vamana import (MODE_FRESH, MODE_PARTH, REQ_DELETE, REQ_INSERT, REQ_SEARCH, Params,
+                     SearchContext, Server, Vamana, VamanaError, default_params, groundtruth, lib,
+                     recall_at_k)
+from . import datasets, tuning
 
-__all__ = ["Vamana", "VamanaError", "Params", "Server", "default_params", "groundtruth", "lib",
+__all__ = ["Vamana", "VamanaError", "Params", "SearchContext", "Server", "tuning",
+           "default_params", "groundtruth", "lib",
            "recall_at_k", "datasets", "MODE_PARTH", "MODE_FRESH", "REQ_SEARCH", "REQ_INSERT",
            "REQ_DELETE"]

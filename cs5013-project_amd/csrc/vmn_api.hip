@@ -84,6 +84,8 @@ struct vmn_index {
     DevBuf staging, find_out, bad;
     DevBuf bf16_rows, bf16_norms, fp8_rows, fp8_norms, fp8_scales, fp8_absmax;
     bool fp8_scaled = false;
+    int waves_per_cu = 32;  // search grid = CUs x waves_per_cu (persistent work queue)
+    int list_cap = 0;       // minimum beam-list capacity of the search kernel
 };
 
 // A search context: its own stream and per-launch scratch, so several search
@@ -93,6 +95,8 @@ struct vmn_ctx {
     hipStream_t stream = nullptr;
     DevBuf counter, stat_hops, stat_ndist;
     int64_t last_nq = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -246,7 +250,8 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
         a.low_norms = idx->fp8_norms.as<float>();
         a.fp8_scales = idx->fp8_scales.as<float>();
     }
-    const int64_t want = std::min<int64_t>(nq, (int64_t)idx->cus * 32);
+    a.lcap = idx->list_cap;
+    const int64_t want = std::min<int64_t>(nq, (int64_t)idx->cus * idx->waves_per_cu);
     HIPCHK(vmn::launch_search(a, (int)std::max<int64_t>(want, 1), stream));
     if (ctx) ctx->last_nq = nq;
     else idx->last_nq = nq;
@@ -904,6 +909,8 @@ int vmn_ctx_destroy(vmn_ctx* c) {
     if (!c) return VMN_OK;
     (void)hipSetDevice(c->idx->p.device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->exec) (void)hipGraphExecDestroy(c->exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     c->counter.release();
     c->stat_hops.release();
     c->stat_ndist.release();
@@ -934,6 +941,52 @@ int vmn_ctx_search(vmn_ctx* c, const float* queries, int64_t nq, int32_t k, int3
     if (rc) return rc;
     return run_search(idx, queries, nq, L, k, ids_out, dists_out, false, idx->p.mode, false, 0, prec,
                       prec == VMN_PREC_FP32 ? 0 : k_rerank, c);
+}
+
+int vmn_ctx_capture(vmn_ctx* c, const float* queries, int64_t nq, int32_t k, int32_t L,
+                    int32_t prec, int32_t k_rerank, int32_t* ids_out, float* dists_out) {
+    if (!c) return fail(VMN_ERR_ARG, "NULL context");
+    // one eager call sizes every scratch buffer, so nothing allocates during capture
+    int rc = vmn_ctx_search(c, queries, nq, k, L, prec, k_rerank, ids_out, dists_out);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->exec) {
+        HIPCHK(hipGraphExecDestroy(c->exec));
+        c->exec = nullptr;
+    }
+    if (c->graph) {
+        HIPCHK(hipGraphDestroy(c->graph));
+        c->graph = nullptr;
+    }
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = vmn_ctx_search(c, queries, nq, k, L, prec, k_rerank, ids_out, dists_out);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(VMN_ERR_HIP, "capture: %s", hipGetErrorString(e));
+    c->graph = g;
+    HIPCHK(hipGraphInstantiate(&c->exec, g, nullptr, nullptr, 0));
+    return VMN_OK;
+}
+
+int vmn_ctx_replay(vmn_ctx* c) {
+    if (!c || !c->exec) return fail(VMN_ERR_STATE, "no captured search on this context");
+    int rc = set_device(c->idx);
+    if (rc) return rc;
+    HIPCHK(hipGraphLaunch(c->exec, c->stream));
+    return VMN_OK;
+}
+
+int vmn_set_launch(vmn_index* idx, int32_t waves_per_cu, int32_t list_cap) {
+    if (!idx) return fail(VMN_ERR_ARG, "NULL index");
+    if (waves_per_cu < 1 || waves_per_cu > 64) return fail(VMN_ERR_ARG, "waves_per_cu in [1, 64]");
+    if (list_cap < 0 || list_cap > 512) return fail(VMN_ERR_ARG, "list_cap in [0, 512]");
+    idx->waves_per_cu = waves_per_cu;
+    idx->list_cap = list_cap;
+    return VMN_OK;
 }
 
 const char* vmn_last_error(void) { return g_err.c_str(); }

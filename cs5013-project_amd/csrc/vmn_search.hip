@@ -259,9 +259,10 @@ static hipError_t launch_search_l(const SearchArgs& a, int grid, hipStream_t s) 
 
 template <int DP>
 static hipError_t launch_search_d(const SearchArgs& a, int grid, hipStream_t s) {
-    if (a.L <= 64) return launch_search_l<DP, 64>(a, grid, s);
-    if (a.L <= 128) return launch_search_l<DP, 128>(a, grid, s);
-    if (a.L <= 256) return launch_search_l<DP, 256>(a, grid, s);
+    const int need = a.L > a.lcap ? a.L : a.lcap;
+    if (need <= 64) return launch_search_l<DP, 64>(a, grid, s);
+    if (need <= 128) return launch_search_l<DP, 128>(a, grid, s);
+    if (need <= 256) return launch_search_l<DP, 256>(a, grid, s);
     return launch_search_l<DP, 512>(a, grid, s);
 }
 

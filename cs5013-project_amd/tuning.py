@@ -1,0 +1,62 @@
+"""Search autotuner.
+
+beam_width(): the smallest beam width L whose recall@k against exact ground
+truth reaches a target (the reference hard-codes L_g = 199,
+vamana/include/constants.cuh:10). launch_shape(): times the batched search
+under each persistent-grid size (waves per CU) and beam-list capacity template
+on one search context, keeps the fastest, and installs it on the index.
+"""
+import numpy as np
+
+from .vamana import SearchContext, recall_at_k
+
+
+def _time(ctx, q, ids, d, k, L, prec, rerank, reps):
+    import torch
+    st = torch.cuda.ExternalStream(ctx.stream)
+    ctx.search(q, ids, d, k, L, prec, rerank)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        ctx.search(q, ids, d, k, L, prec, rerank)
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def beam_width(index, queries, gt_ids, target, widths, k=10, prec="fp32", rerank=0):
+    """(L, recall) of the smallest width reaching `target`, else the best seen."""
+    best = None
+    for L in widths:
+        ids, _ = index.searchPoints(queries, k=k, L=L, prec=prec, rerank=min(rerank, L))
+        r = recall_at_k(ids.cpu().numpy() if hasattr(ids, "cpu") else ids, gt_ids, k)
+        if r >= target:
+            return L, r
+        if best is None or r > best[1]:
+            best = (L, r)
+    return best
+
+
+def launch_shape(index, queries, k=10, L=64, prec="fp32", rerank=0,
+                 waves=(12, 16, 20, 24, 32), caps=(0, 128, 256), reps=3):
+    """Fastest (waves_per_cu, list_cap) for this batch; installs it and
+    returns {"waves_per_cu", "list_cap", "ms", "table"}."""
+    import torch
+    ctx = SearchContext(index)
+    nq = int(queries.shape[0])
+    ids = torch.empty((nq, k), dtype=torch.int32, device=queries.device)
+    d = torch.empty((nq, k), dtype=torch.float32, device=queries.device)
+    rr = min(rerank, L)
+    table = []
+    for cap in caps:
+        if cap and cap < L:
+            continue
+        for w in waves:
+            index.set_launch(w, cap)
+            table.append({"waves_per_cu": w, "list_cap": cap,
+                          "ms": round(_time(ctx, queries, ids, d, k, L, prec, rr, reps), 4)})
+    best = min(table, key=lambda t: t["ms"])
+    index.set_launch(best["waves_per_cu"], best["list_cap"])
+    ctx.close()
+    return dict(best, table=table)

@@ -91,6 +91,9 @@ def lib():
         L.vmn_ctx_stream.argtypes = [P]
         L.vmn_ctx_stream.restype = P
         L.vmn_ctx_search.argtypes = [P, P, i64, i32, i32, i32, i32, P, P]
+        L.vmn_ctx_capture.argtypes = [P, P, i64, i32, i32, i32, i32, P, P]
+        L.vmn_ctx_replay.argtypes = [P]
+        L.vmn_set_launch.argtypes = [P, i32, i32]
         L.vmn_default_server_params.argtypes = [ctypes.POINTER(ServerParams)]
         L.vmn_default_server_params.restype = None
         L.vmn_server_create.argtypes = [P, ctypes.POINTER(ServerParams), i32, ctypes.POINTER(P)]
@@ -108,7 +111,8 @@ def lib():
                      "vmn_get_adjacency", "vmn_search_stats", "vmn_groundtruth", "vmn_insert",
                      "vmn_delete_ids", "vmn_delete_vectors", "vmn_consolidate",
                      "vmn_get_deleted", "vmn_search_ex", "vmn_get_lowp_rows", "vmn_ctx_create",
-                     "vmn_ctx_destroy", "vmn_ctx_search", "vmn_server_create", "vmn_server_submit",
+                     "vmn_ctx_destroy", "vmn_ctx_search", "vmn_ctx_capture", "vmn_ctx_replay",
+                     "vmn_set_launch", "vmn_server_create", "vmn_server_submit",
                      "vmn_server_poll", "vmn_server_destroy"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
@@ -300,6 +304,9 @@ class Vamana:
         _check(lib().vmn_get_adjacency(self._h, out.ctypes.data_as(ctypes.c_void_p), MEM_HOST))
         return out
 
+    def set_launch(self, waves_per_cu=32, list_cap=0):
+        _check(lib().vmn_set_launch(self._h, waves_per_cu, list_cap))
+
     def lowp_rows(self, prec):
         """Rows as the BF16 / FP8 traversal sees them, decoded to float32."""
         out = np.empty((self.size, self.params.dim), dtype=np.float32)
@@ -343,6 +350,46 @@ def recall_at_k(found, gt_ids, k=10):
     for f, g in zip(found, gt):
         hits += len(set(int(x) for x in f if x >= 0) & set(int(x) for x in g))
     return hits / (k * found.shape[0])
+
+
+class SearchContext:
+    """A search context (own stream + scratch); searches on different contexts
+    run concurrently. capture() records one search as a HIP graph that
+    replay() launches again (device tensors only)."""
+
+    def __init__(self, index):
+        self.index = index
+        self._h = ctypes.c_void_p()
+        _check(lib().vmn_ctx_create(index._h, ctypes.byref(self._h)))
+
+    @property
+    def stream(self):
+        return lib().vmn_ctx_stream(self._h)
+
+    def _args(self, q, k, L, prec, rerank, ids, dists):
+        pc = PREC[prec] if isinstance(prec, str) else prec
+        return (self._h, ctypes.c_void_p(q.data_ptr()), int(q.shape[0]), k, L, pc, rerank,
+                ctypes.c_void_p(ids.data_ptr()), ctypes.c_void_p(dists.data_ptr()))
+
+    def search(self, q, ids, dists, k=10, L=64, prec="fp32", rerank=0):
+        _check(lib().vmn_ctx_search(*self._args(q, k, L, prec, rerank, ids, dists)))
+
+    def capture(self, q, ids, dists, k=10, L=64, prec="fp32", rerank=0):
+        _check(lib().vmn_ctx_capture(*self._args(q, k, L, prec, rerank, ids, dists)))
+
+    def replay(self):
+        _check(lib().vmn_ctx_replay(self._h))
+
+    def close(self):
+        if self._h:
+            lib().vmn_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:

@@ -216,6 +216,19 @@ int64_t vmn_server_pending(const vmn_server* srv);
 /* Finishes the queued requests, then stops the engine. */
 int vmn_server_destroy(vmn_server* srv);
 
+/* Capture one context search (fixed pointers and sizes) as a HIP graph of
+ * kernel nodes (the work-queue counter is cleared by a kernel, not a memset
+ * node), and launch it again on the context's stream. Replays re-read the
+ * query buffer, so new queries written there are searched. */
+int vmn_ctx_capture(vmn_ctx* ctx, const float* queries, int64_t nq, int32_t k, int32_t L,
+                    int32_t prec, int32_t k_rerank, int32_t* ids_out, float* dists_out);
+int vmn_ctx_replay(vmn_ctx* ctx);
+
+/* Search launch shape chosen by the autotuner: persistent grid of
+ * waves_per_cu x CUs workgroups (one wavefront each) and the minimum
+ * beam-list capacity template (0: the smallest of 64/128/256/512 >= L). */
+int vmn_set_launch(vmn_index* idx, int32_t waves_per_cu, int32_t list_cap);
+
 /* Message for the last failing call on this thread ("" if none). */
 const char* vmn_last_error(void);
 
