@@ -158,6 +158,7 @@ def main():
         wall = time.time() - t
         kern_ms = e0.elapsed_time(e1) / args.steps
         rec = vmn.recall_at_k(ids.cpu().numpy(), gt, 10)
+        kernel = idx.search_kernel_name(k=10, L=L, prec=prec, list_cap=tune["list_cap"])
         idx.searchPoints(q_t, k=10, L=L, prec=prec, rerank=rr)  # same traversal, for the counters
         hops, nd = idx.search_stats()
         row_b = {"fp32": d * 4, "bf16": d * 2, "fp8": d + 4}[prec]
@@ -168,6 +169,7 @@ def main():
                        "kernel_ms": kern_ms, "bytes_per_launch": int(bytes_launch),
                        "achieved_GBs": bytes_launch / (kern_ms / 1e3) / 1e9,
                        "launch": {"waves_per_cu": tune["waves_per_cu"], "list_cap": tune["list_cap"]},
+                       "kernel": kernel, "traffic": measured_traffic(form, n, nq, L, kernel),
                        "hops_per_query": round(hops / nq, 1), "dists_per_query": round(nd / nq, 1)}
         log(f"timed {form:12s} L={L} {timed[form]['qps']:,.0f} QPS recall {rec:.4f} kernel "
             f"{kern_ms:.3f} ms {timed[form]['achieved_GBs']:.0f} GB/s (launch {tune['waves_per_cu']}"
@@ -211,7 +213,7 @@ def main():
             "distance_error_vs_fp64": err, "sweep": sweeps},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_GBs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(head["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(head["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": head["traffic"],
                      "kernel_ms": round(head["kernel_ms"], 4),
                      "bytes_per_launch": head["bytes_per_launch"]},
         "cpu_baseline": cpu,
@@ -426,8 +428,29 @@ def serving(args, vmn, torch, dev):
     print(json.dumps(out), flush=True)
 
 
+def measured_traffic(form, n, nq, L, kernel):
+    """HBM bytes per launch of the search kernel for `form`, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (each scaled by what the counter
+    reads for a 1 GiB copy), when that profile covers this exact workload
+    (same n, nq, precision and L; the beam-list capacity of the instance
+    does not change the traversal); otherwise None."""
+    path = os.path.join(ROOT, "profiles", "search_1m_hbm_traffic.json")
+    try:
+        with open(path) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = prof.get("forms", {}).get(form)
+    same = lambda a, b: a.split(",")[0] == b.split(",")[0] and a.split(",")[2:] == b.split(",")[2:]
+    if not e or prof.get("n") != n or prof.get("nq") != nq or e.get("L") != L \
+            or not same(e.get("kernel", ""), kernel):
+        return None
+    return int(e["traffic_bytes"])
+
+
 def cpu_baseline(idx, base, qs, gt, L, seconds, mode):
-    """The CPU oracle's search (same graph, same L) on a bounded query sample."""
+    """The CPU oracle's search (same graph, same L) on a bounded query sample:
+    the query set is cycled until the sample holds about `seconds` of work."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
     adj = idx.adjacency
@@ -438,15 +461,19 @@ def cpu_baseline(idx, base, qs, gt, L, seconds, mode):
     t = time.time()
     search(base, adj, qs[:n0], med, L=L, out_n=10)
     per_q = (time.time() - t) / n0
-    ns = int(min(qs.shape[0], max(n0, seconds / max(per_q, 1e-6))))
+    # the 64-query probe runs at a fraction of the thread count, so it
+    # over-estimates the per-query time; size from it, then cap at 40x the set
+    ns = int(min(40 * qs.shape[0], max(n0, seconds / max(per_q / threads, 1e-7) / 4)))
+    sel = np.arange(ns) % qs.shape[0]
     t = time.time()
-    ids, _, _, _ = search(base, adj, qs[:ns], med, L=L, out_n=10)
+    ids, _, _, _ = search(base, adj, qs[sel], med, L=L, out_n=10)
     el = time.time() - t
     log(f"cpu baseline: {ns} queries in {el:.1f} s on {threads} threads")
     return {"value": round(ns / el, 1), "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"{ns} of the {qs.shape[0]} queries, same graph and L={L}, OpenMP over "
-                      f"queries", "recall@10_sample": round(
-                          float(np.mean([len(set(a) & set(b)) / 10 for a, b in zip(ids, gt[:ns])])), 4)}
+            "sample": f"{ns} searches ({el:.1f} s) cycling the {qs.shape[0]} queries, FP32 on the "
+                      f"same graph at L={L}, OpenMP over queries",
+            "recall@10_sample": round(float(np.mean(
+                [len(set(a) & set(b)) / 10 for a, b in zip(ids, gt[sel])])), 4)}
 
 
 if __name__ == "__main__":

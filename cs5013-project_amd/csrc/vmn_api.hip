@@ -61,7 +61,12 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
-int pad_dim(int d) { return (d + 31) / 32 * 32; }
+// Row stride in floats: the dimension rounded up to one of the instantiated
+// kernel widths {32, 64, 96, 128, 160, 192, 256}.
+int pad_dim(int d) {
+    const int p = (d + 31) / 32 * 32;
+    return p > 192 ? 256 : p;
+}
 
 }  // namespace
 

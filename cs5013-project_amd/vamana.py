@@ -307,6 +307,17 @@ class Vamana:
     def set_launch(self, waves_per_cu=32, list_cap=0):
         _check(lib().vmn_set_launch(self._h, waves_per_cu, list_cap))
 
+    def search_kernel_name(self, k, L, prec="fp32", list_cap=0):
+        """Name of the search kernel instance a search with these arguments
+        launches, as a profiler prints it (template arguments: padded
+        dimension, beam-list capacity, mode, precision)."""
+        d = (int(self.params.dim) + 31) // 32 * 32
+        dp = 256 if d > 192 else d
+        need = max(int(L), int(list_cap))
+        cap = next(c for c in (64, 128, 256, 512) if need <= c)
+        pc = PREC[prec] if isinstance(prec, str) else int(prec)
+        return f"void vmn::search_kernel<{dp}, {cap}, {int(self.params.mode)}, {pc}>(vmn::SearchArgs)"
+
     def lowp_rows(self, prec):
         """Rows as the BF16 / FP8 traversal sees them, decoded to float32."""
         out = np.empty((self.size, self.params.dim), dtype=np.float32)
