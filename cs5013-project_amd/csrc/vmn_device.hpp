@@ -20,6 +20,10 @@
 #include <stdint.h>
 
 #define VMN_WAVE 64
+// MFMA tiles of 16 candidates whose rows are in flight together
+#ifndef VMN_GATHER_TILES
+#define VMN_GATHER_TILES 2
+#endif
 #define VMN_VIS_BIT 0x80000000u
 #define VMN_ID_MASK 0x7fffffffu
 
@@ -218,75 +222,130 @@ __device__ __forceinline__ void prep_query(QLow<DP>& Q, const float* qf, const f
     }
 }
 
+// Moves a dword to the lane that uses it as an MFMA operand: lane L receives
+// the value lane src_lane(L) holds.
+__device__ __forceinline__ uint32_t bperm(int src_byte, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(src_byte, (int)v);
+}
+
 // Distances from the prepared query to `cnt` candidates whose ids sit in LDS
-// ids[0..cnt); dist_out[s] (LDS) receives the distance of ids[s]. Two groups
+// ids[0..cnt); dist_out[s] (LDS) receives the distance of ids[s]. Two tiles
 // of 16 candidates are loaded before either is multiplied, so one memory round
 // trip serves 32 candidates. BF16 row norms come from the loaded registers
 // (v_dot2_f32_bf16); FP8 norms (per-dimension scales) from `norms`.
-template <int DP, int PREC>
+//
+// Loads are coalesced per row: in load step s, lane l reads 16 bytes of tile
+// row l/4 at byte 64*s + 16*(l%4), so each 4-lane quad reads 64 contiguous
+// bytes of one row (one L1 access, against four when lanes are laid out as
+// the MFMA operand wants). ds_bpermute then hands every dword to the lane
+// that holds it as the B operand (lane L: column L%16, k-slice L/16).
+template <int DP, int PREC, int NG = 2>
 __device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* __restrict__ rows,
                                                  const float* __restrict__ norms,
                                                  const int32_t* ids, float* dist_out, int cnt) {
-    const int l = lane(), col = l & 15, g = l >> 4;
+    const int l = lane(), g = l >> 4, rr = l >> 2, sub = l & 3;
     constexpr int KB = QLow<DP>::KB;
-    for (int base = 0; base < cnt; base += 32) {
-        const int c0 = base + col, c1 = base + 16 + col;
-        const int32_t id0 = c0 < cnt ? ids[c0] : ids[0];
-        const int32_t id1 = c1 < cnt ? ids[c1] : ids[0];
-        const bool two = base + 16 < cnt;
-        f32x4_t a0 = {0.0f, 0.0f, 0.0f, 0.0f}, a1 = {0.0f, 0.0f, 0.0f, 0.0f};
-        float x0 = 0.0f, x1 = 0.0f;
+    for (int base = 0; base < cnt; base += 16 * NG) {
+        int32_t id[NG];
+#pragma unroll
+        for (int t = 0; t < NG; ++t) {
+            const int c = base + 16 * t + rr;
+            id[t] = c < cnt ? ids[c] : ids[0];
+        }
+        f32x4_t acc[NG];
+        float x[NG];
+#pragma unroll
+        for (int t = 0; t < NG; ++t) { acc[t] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f}; x[t] = 0.0f; }
         if (PREC == VMN_PREC_BF16) {
-            const uint4* r0 = reinterpret_cast<const uint4*>(reinterpret_cast<const __bf16*>(rows) + (int64_t)id0 * DP);
-            const uint4* r1 = reinterpret_cast<const uint4*>(reinterpret_cast<const __bf16*>(rows) + (int64_t)id1 * DP);
-            uint4 v0[KB], v1[KB];
+            // step kb holds k-block kb (64 bytes of every row)
+            const int src = (4 * (l & 15) + g) << 2;
+            uint4 v[NG][KB];
 #pragma unroll
-            for (int kb = 0; kb < KB; ++kb) v0[kb] = r0[4 * kb + g];
-            if (two) {
+            for (int t = 0; t < NG; ++t) {
+                if (t == 0 || base + 16 * t < cnt) {
+                    const uint4* r = reinterpret_cast<const uint4*>(reinterpret_cast<const __bf16*>(rows) + (int64_t)id[t] * DP);
 #pragma unroll
-                for (int kb = 0; kb < KB; ++kb) v1[kb] = r1[4 * kb + g];
-            }
-#pragma unroll
-            for (int kb = 0; kb < KB; ++kb) {
-                a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Q.a[kb], __builtin_bit_cast(bf16x8_t, v0[kb]), a0, 0, 0, 0);
-                const uint32_t w[4] = {v0[kb].x, v0[kb].y, v0[kb].z, v0[kb].w};
-#pragma unroll
-                for (int e = 0; e < 4; ++e) { const bf16x2_t p = __builtin_bit_cast(bf16x2_t, w[e]); x0 = __builtin_amdgcn_fdot2_f32_bf16(p, p, x0, false); }
-            }
-            if (two) {
-#pragma unroll
-                for (int kb = 0; kb < KB; ++kb) {
-                    a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Q.a[kb], __builtin_bit_cast(bf16x8_t, v1[kb]), a1, 0, 0, 0);
-                    const uint32_t w[4] = {v1[kb].x, v1[kb].y, v1[kb].z, v1[kb].w};
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) { const bf16x2_t p = __builtin_bit_cast(bf16x2_t, w[e]); x1 = __builtin_amdgcn_fdot2_f32_bf16(p, p, x1, false); }
+                    for (int kb = 0; kb < KB; ++kb) v[t][kb] = r[4 * kb + sub];
                 }
             }
-            x0 += __shfl_xor(x0, 16); x0 += __shfl_xor(x0, 32);
-            x1 += __shfl_xor(x1, 16); x1 += __shfl_xor(x1, 32);
-        } else {
-            const unsigned long* r0 = reinterpret_cast<const unsigned long*>(reinterpret_cast<const uint8_t*>(rows) + (int64_t)id0 * DP);
-            const unsigned long* r1 = reinterpret_cast<const unsigned long*>(reinterpret_cast<const uint8_t*>(rows) + (int64_t)id1 * DP);
-            unsigned long v0[KB], v1[KB];
 #pragma unroll
-            for (int kb = 0; kb < KB; ++kb) v0[kb] = r0[4 * kb + g];
-            if (two) {
+            for (int t = 0; t < NG; ++t) {
+                if (t == 0 || base + 16 * t < cnt) {
 #pragma unroll
-                for (int kb = 0; kb < KB; ++kb) v1[kb] = r1[4 * kb + g];
+                    for (int kb = 0; kb < KB; ++kb) {
+                        const uint32_t w[4] = {bperm(src, v[t][kb].x), bperm(src, v[t][kb].y),
+                                               bperm(src, v[t][kb].z), bperm(src, v[t][kb].w)};
+                        const uint4 u = {w[0], w[1], w[2], w[3]};
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Q.a[kb], __builtin_bit_cast(bf16x8_t, u), acc[t], 0, 0, 0);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) { const bf16x2_t p = __builtin_bit_cast(bf16x2_t, w[e]); x[t] = __builtin_amdgcn_fdot2_f32_bf16(p, p, x[t], false); }
+                    }
+                    x[t] += __shfl_xor(x[t], 16);
+                    x[t] += __shfl_xor(x[t], 32);
+                }
             }
-            x0 = norms[id0];
-            x1 = two ? norms[id1] : 0.0f;
+        } else if constexpr (DP % 64 == 0) {
+            // FP8: k-block kb is 32 bytes; step s = kb/2 holds k-blocks 2s, 2s+1.
+            // Lane L needs bytes 32*kb + 8*g .. +7 of row L%16: 16-byte chunk
+            // 2*kb + g/2 (lane 4*(L%16) + (2*kb + g/2) % 4 of step kb/2),
+            // dword pair 2*(g%2).
+            constexpr int NS = DP / 64 > 0 ? DP / 64 : 1;
+            uint4 v[NG][NS];
 #pragma unroll
-            for (int kb = 0; kb < KB; ++kb) a0 = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(Q.a8[kb], (long)v0[kb], a0, 0, 0, 0);
-            if (two) {
+            for (int t = 0; t < NG; ++t) {
+                if (t == 0 || base + 16 * t < cnt) {
+                    const uint4* r = reinterpret_cast<const uint4*>(reinterpret_cast<const uint8_t*>(rows) + (int64_t)id[t] * DP);
 #pragma unroll
-                for (int kb = 0; kb < KB; ++kb) a1 = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(Q.a8[kb], (long)v1[kb], a1, 0, 0, 0);
+                    for (int st = 0; st < NS; ++st) v[t][st] = r[4 * st + sub];
+                    // norm of tile column L%16: its id sits in lane 4*(L%16)
+                    x[t] = norms[__builtin_amdgcn_ds_bpermute((4 * (l & 15)) << 2, id[t])];
+                }
+            }
+            const bool hi = g & 1;
+#pragma unroll
+            for (int t = 0; t < NG; ++t) {
+                if (t == 0 || base + 16 * t < cnt) {
+#pragma unroll
+                    for (int kb = 0; kb < KB; ++kb) {
+                        const int src = (4 * (l & 15) + ((2 * kb + (g >> 1)) & 3)) << 2;
+                        const uint4 u = v[t][kb >> 1];
+                        const uint32_t lo0 = bperm(src, u.x), lo1 = bperm(src, u.y), hi0 = bperm(src, u.z), hi1 = bperm(src, u.w);
+                        const unsigned long op = ((unsigned long)(hi ? hi1 : lo1) << 32) | (hi ? hi0 : lo0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(Q.a8[kb], (long)op, acc[t], 0, 0, 0);
+                    }
+                }
+            }
+        } else {
+            // FP8 rows shorter than a multiple of 64 bytes: operand-shaped loads
+            const int col = l & 15;
+            unsigned long v[NG][KB];
+            int32_t oid[NG];
+#pragma unroll
+            for (int t = 0; t < NG; ++t) {
+                oid[t] = base + 16 * t + col < cnt ? ids[base + 16 * t + col] : ids[0];
+                if (t == 0 || base + 16 * t < cnt) {
+                    const unsigned long* r = reinterpret_cast<const unsigned long*>(reinterpret_cast<const uint8_t*>(rows) + (int64_t)oid[t] * DP);
+#pragma unroll
+                    for (int kb = 0; kb < KB; ++kb) v[t][kb] = r[4 * kb + g];
+                    x[t] = norms[oid[t]];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < NG; ++t) {
+                if (t == 0 || base + 16 * t < cnt) {
+#pragma unroll
+                    for (int kb = 0; kb < KB; ++kb)
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(Q.a8[kb], (long)v[t][kb], acc[t], 0, 0, 0);
+                }
             }
         }
         if (g == 0) {
-            const float t = PREC == VMN_PREC_FP8 ? Q.t : 1.0f;
-            if (c0 < cnt) dist_out[c0] = (Q.qn + x0) - 2.0f * (a0[0] * t);
-            if (c1 < cnt) dist_out[c1] = (Q.qn + x1) - 2.0f * (a1[0] * t);
+            const float ts = PREC == VMN_PREC_FP8 ? Q.t : 1.0f;
+#pragma unroll
+            for (int t = 0; t < NG; ++t) {
+                const int c = base + 16 * t + l;
+                if (c < cnt) dist_out[c] = (Q.qn + x[t]) - 2.0f * (acc[t][0] * ts);
+            }
         }
     }
 }

@@ -68,8 +68,10 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
         __syncthreads();
         if (qi >= a.nq) return;
 
+        // The FP32 query lives in registers through the walk only for FP32
+        // traversal; the low-precision forms load it for the re-rank.
         QReg<DP> q;
-        q.load(a.queries + qi * DP, sub);
+        if (PREC == VMN_PREC_FP32) q.load(a.queries + qi * DP, sub);
         QLow<DP> ql;
         if (PREC != VMN_PREC_FP32) prep_query<DP, PREC>(ql, a.queries + qi * DP, a.fp8_scales);
         const int32_t self = a.query_is_node ? (int32_t)(a.node_base + qi) : -1;
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
             if (PREC == VMN_PREC_FP32)
                 gather_dists<DP>(q, vecs, s_sid, s_sd, nnew);
             else
-                gather_dists_low<DP, PREC>(ql, a.low_rows, a.low_norms, s_sid, s_sd, nnew);
+                gather_dists_low<DP, PREC, VMN_GATHER_TILES>(ql, a.low_rows, a.low_norms, s_sid, s_sd, nnew);
             __syncthreads();
             ndist += nnew;
             // A full list keeps its entries on ties, so a candidate at or beyond
@@ -204,6 +206,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
             // exact FP32 re-rank of the first krr entries, stable on ties
             const int nr = cnt < a.krr ? cnt : a.krr;
             if (l < nr) s_sid[l] = (int32_t)(s_id[cur][l] & VMN_ID_MASK);
+            q.load(a.queries + qi * DP, sub);
             __syncthreads();
             gather_dists<DP>(q, vecs, s_sid, s_sd, nr);
             __syncthreads();
