@@ -52,9 +52,12 @@ def parse():
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--rerank", type=int, default=40)
-    ap.add_argument("--workload", default="search", choices=["search", "streaming", "serving"],
+    ap.add_argument("--workload", default="search",
+                    choices=["search", "streaming", "serving", "copy"],
                     help="search: the headline metric; streaming: 10 rounds of 5%% deletes + "
-                         "5%% inserts with consolidation, recall@10 per round")
+                         "5%% inserts with consolidation, recall@10 per round; serving: the "
+                         "continuous-batching request loop; copy: a 1 GiB device-to-device copy "
+                         "(the calibration kernel for the HBM traffic counters)")
     ap.add_argument("--rounds", type=int, default=10)
     ap.add_argument("--L-eval", type=int, default=60)
     ap.add_argument("--serve-n", type=int, default=10_000_000)
@@ -78,6 +81,8 @@ def main():
         return streaming(args, vmn, torch, dev)
     if args.workload == "serving":
         return serving(args, vmn, torch, dev)
+    if args.workload == "copy":
+        return copy_calibration(args, torch, dev)
     n, nq, d = args.n, args.nq, 128
     alphas = tuple(float(x) for x in args.alphas.split(","))
     log(f"generating {n}x{d} base + {nq} queries (256-component integer mixture)")
@@ -248,6 +253,32 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(args, torch, dev):
+    """1 GiB device-to-device copy (torch copy_, one runtime copy kernel per
+    step): its FETCH_SIZE / WRITE_SIZE per dispatch against the 2^30 bytes it
+    must read and write gives the scale that turns those counters into bytes."""
+    nbytes = 1 << 30
+    src = torch.full((nbytes // 4,), 1.0, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    print(json.dumps({"metric": "device copy bandwidth (read + write)", "value": round(2 * nbytes / ms / 1e6, 1),
+                      "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+                      "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": "copy 1 GiB device to device", "bytes": nbytes}}),
+          flush=True)
 
 
 def log_uniform_batches(rng, total, lo=1000, hi=50000):
