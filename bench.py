@@ -206,6 +206,18 @@ def main():
     for c in ctx:
         c.close()
 
+    # PCIe-inclusive rate of the headline form: host query buffer in, host ids
+    # and distances out (vmn_search with host pointers stages both ways)
+    Lh = chosen["bf16+rerank"][0]["L"]
+    rrh = min(args.rerank, Lh)
+    idx.set_launch(timed["bf16+rerank"]["launch"]["waves_per_cu"], timed["bf16+rerank"]["launch"]["list_cap"])
+    idx.searchPoints(qs, k=10, L=Lh, prec="bf16", rerank=rrh)
+    t = time.time()
+    for _ in range(args.steps):
+        idx.searchPoints(qs, k=10, L=Lh, prec="bf16", rerank=rrh)
+    host_ms = (time.time() - t) * 1e3 / args.steps
+    log(f"bf16+rerank with host buffers: {host_ms:.3f} ms per batch ({nq / host_ms * 1e3:,.0f} QPS)")
+
     head = timed["bf16+rerank"]
     # distance error of the low-precision rows against FP64 on a sample of pairs
     err = {}
@@ -242,6 +254,7 @@ def main():
             "bf16_vs_fp32_qps": round(head["qps"] / timed["fp32"]["qps"], 3),
             "bf16_vs_fp32_qps_serial": round(head["qps_serial"] / timed["fp32"]["qps_serial"], 3),
             "qps_serial": head["qps_serial"], "recall@10_set2": head["recall@10_set2"],
+            "qps_host_buffers": round(nq / host_ms * 1e3, 1),
             "batches": "K batches of 10,000 queries alternating over 2 search contexts "
                        "(query sets: mixture streams 1 and 2)",
             "distance_error_vs_fp64": err, "sweep": sweeps},
@@ -297,8 +310,8 @@ def streaming(args, vmn, torch, dev):
     workload (R=64, L=128, alpha 1.0 then 1.2); updates prune with alpha 1.2;
     each round deletes 5% of the live ids (uniform, seeded) and
     inserts 5% new points from the same mixture, both in batches of
-    log-uniform size in [1k, 50k]; tombstones are consolidated at the end of
-    the round; recall@10 of 10k queries (FP32, L=L_eval) against FP64 truth
+    log-uniform size in [1k, 50k] that arrive interleaved in a seeded order;
+    tombstones are consolidated at the end of the round; recall@10 of 10k queries (FP32, L=L_eval) against FP64 truth
     over the live points."""
     n, nq, d = args.n, args.nq, 128
     frac = 0.05
@@ -340,21 +353,25 @@ def streaming(args, vmn, torch, dev):
         n_ins = int(n * frac)
         t_del = t_ins = 0.0
         off = 0
-        for b in log_uniform_batches(rng, n_del):
+        # the round's delete and insert batches arrive interleaved (seeded order)
+        ops = [("del", b) for b in log_uniform_batches(rng, n_del)] + \
+              [("ins", b) for b in log_uniform_batches(rng, n_ins)]
+        for o in rng.permutation(len(ops)):
+            kind, b = ops[o]
             torch.cuda.synchronize()
             t = time.time()
-            idx.deleteIds(kill[off:off + b])
+            if kind == "del":
+                idx.deleteIds(kill[off:off + b])
+                off += b
+            else:
+                idx.insertPoints(allv[size:size + b])
+                size += b
             torch.cuda.synchronize()
-            t_del += time.time() - t
-            off += b
+            if kind == "del":
+                t_del += time.time() - t
+            else:
+                t_ins += time.time() - t
         deleted[kill] = True
-        for b in log_uniform_batches(rng, n_ins):
-            torch.cuda.synchronize()
-            t = time.time()
-            idx.insertPoints(allv[size:size + b])
-            torch.cuda.synchronize()
-            t_ins += time.time() - t
-            size += b
         t = time.time()
         idx.consolidate()
         torch.cuda.synchronize()

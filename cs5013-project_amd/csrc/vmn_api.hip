@@ -9,11 +9,11 @@
 #include <string>
 #include <vector>
 
-#include "../../include/vamana_mi355x.h"
+#include "vmn_internal.hpp"
 #include "vmn_kernels.hpp"
 #include "vmn_update.hpp"
 
-namespace {
+namespace vmn_rt {
 
 thread_local std::string g_err;
 
@@ -26,85 +26,6 @@ int fail(int code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
-
-#define HIPCHK(expr)                                                                        \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail(VMN_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                                \
-    } while (0)
-
-// Device allocation that grows on demand and is kept across calls.
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    hipError_t reserve(size_t want) {
-        if (want <= bytes) return hipSuccess;
-        if (p) {
-            hipError_t e = hipFree(p);
-            if (e != hipSuccess) return e;
-            p = nullptr;
-            bytes = 0;
-        }
-        size_t b = std::max<size_t>(want, 256);
-        hipError_t e = hipMalloc(&p, b);
-        if (e == hipSuccess) bytes = b;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <class T>
-    T* as() const { return static_cast<T*>(p); }
-};
-
-// Row stride in floats: the dimension rounded up to one of the instantiated
-// kernel widths {32, 64, 96, 128, 160, 192, 256}.
-int pad_dim(int d) {
-    const int p = (d + 31) / 32 * 32;
-    return p > 192 ? 256 : p;
-}
-
-}  // namespace
-
-struct vmn_index {
-    vmn_params p{};
-    int DP = 0;
-    int cus = 0;
-    hipStream_t stream = nullptr;
-    int64_t n = 0, cap = 0;
-    int64_t medoid = -1;
-    bool built = false;
-    DevBuf vecs, adj, del_bits;
-    // scratch
-    DevBuf qpad, out_ids, out_d, stat_hops, stat_ndist, counter;
-    DevBuf cand_ids, cand_d, vis_ids, vis_d, vis_cnt;
-    DevBuf pair_dst, pair_d, rcount, rfill, roff, rscan_tmp, rsrc, rd, rlist;
-    DevBuf centroid, medoid_key;
-    int64_t last_nq = 0;
-    int64_t n_deleted = 0;  // tombstones set since the build (upper bound)
-    DevBuf staging, find_out, bad;
-    DevBuf bf16_rows, bf16_norms, fp8_rows, fp8_norms, fp8_scales, fp8_absmax;
-    bool fp8_scaled = false;
-    int waves_per_cu = 32;  // search grid = CUs x waves_per_cu (persistent work queue)
-    int list_cap = 0;       // minimum beam-list capacity of the search kernel
-};
-
-// A search context: its own stream and per-launch scratch, so several search
-// batches can run on the device at once (the index is read-only meanwhile).
-struct vmn_ctx {
-    vmn_index* idx = nullptr;
-    hipStream_t stream = nullptr;
-    DevBuf counter, stat_hops, stat_ndist;
-    int64_t last_nq = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-};
-
-namespace {
 
 int set_device(const vmn_index* idx) {
     HIPCHK(hipSetDevice(idx->p.device));
@@ -267,11 +188,12 @@ int prune_grid(const vmn_index* idx, int64_t n) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)idx->cus * 16));
 }
 
-int build_parth(vmn_index* idx) {
+int build_parth(vmn_index* idx, bool random_init) {
     const int64_t n = idx->n;
     const int L = idx->p.L_build;
-    HIPCHK(vmn::launch_random_init(idx->adj.as<int32_t>(), n, idx->p.R, idx->p.R, idx->p.seed,
-                                   idx->stream));
+    if (random_init)
+        HIPCHK(vmn::launch_random_init(idx->adj.as<int32_t>(), n, idx->p.R, idx->p.R, idx->p.seed,
+                                       idx->stream));
     HIPCHK(idx->cand_ids.reserve((size_t)n * L * sizeof(int32_t)));
     HIPCHK(idx->cand_d.reserve((size_t)n * L * sizeof(float)));
     for (int pass = 0; pass < idx->p.passes; ++pass) {
@@ -332,15 +254,17 @@ int fresh_pass(vmn_index* idx, int64_t base, int64_t cnt, float alpha) {
     a.counter = idx->counter.as<unsigned long long>();
     HIPCHK(vmn::launch_fresh_out_prune(a, prune_grid(idx, cnt), idx->stream));
 
-    // reverse pairs -> CSR by destination
+    // reverse pairs -> CSR by destination; per-node arrays are sized for the
+    // index capacity so that growing the index does not reallocate them
     const int64_t npairs = cnt * R;
-    HIPCHK(idx->rcount.reserve((size_t)N * sizeof(int32_t)));
-    HIPCHK(idx->rfill.reserve((size_t)N * sizeof(int32_t)));
-    HIPCHK(idx->roff.reserve((size_t)(N + 1) * sizeof(int64_t)));
-    HIPCHK(idx->rscan_tmp.reserve((size_t)((N + 1023) / 1024 + 2) * sizeof(int64_t)));
+    const int64_t NC = std::max<int64_t>(idx->cap, N);
+    HIPCHK(idx->rcount.reserve((size_t)NC * sizeof(int32_t)));
+    HIPCHK(idx->rfill.reserve((size_t)NC * sizeof(int32_t)));
+    HIPCHK(idx->roff.reserve((size_t)(NC + 1) * sizeof(int64_t)));
+    HIPCHK(idx->rscan_tmp.reserve((size_t)((NC + 1023) / 1024 + 2) * sizeof(int64_t)));
     HIPCHK(idx->rsrc.reserve((size_t)npairs * sizeof(int32_t)));
     HIPCHK(idx->rd.reserve((size_t)npairs * sizeof(float)));
-    HIPCHK(idx->rlist.reserve((size_t)N * sizeof(int64_t)));
+    HIPCHK(idx->rlist.reserve((size_t)NC * sizeof(int64_t)));
     HIPCHK(vmn::launch_fill_i32(idx->rcount.as<int32_t>(), N, 0, idx->stream));
     HIPCHK(vmn::launch_fill_i32(idx->rfill.as<int32_t>(), N, 0, idx->stream));
     HIPCHK(vmn::launch_reverse_count(idx->pair_dst.as<int32_t>(), npairs, idx->rcount.as<int32_t>(),
@@ -368,9 +292,10 @@ int fresh_pass(vmn_index* idx, int64_t base, int64_t cnt, float alpha) {
     return VMN_OK;
 }
 
-int build_fresh(vmn_index* idx) {
-    HIPCHK(vmn::launch_random_init(idx->adj.as<int32_t>(), idx->n, idx->p.R, idx->p.R, idx->p.seed,
-                                   idx->stream));
+int build_fresh(vmn_index* idx, bool random_init) {
+    if (random_init)
+        HIPCHK(vmn::launch_random_init(idx->adj.as<int32_t>(), idx->n, idx->p.R, idx->p.R,
+                                       idx->p.seed, idx->stream));
     for (int pass = 0; pass < idx->p.passes; ++pass) {
         int rc = fresh_pass(idx, 0, idx->n, idx->p.alpha[pass]);
         if (rc) return rc;
@@ -391,6 +316,30 @@ int set_tombstones(vmn_index* idx, const int32_t* dev_ids, int64_t m) {
     return VMN_OK;
 }
 
+int normalize_rows(const int32_t* src, int64_t n, int row_width, int R, bool drop_self,
+                   int32_t mem, std::vector<int32_t>& rows) {
+    std::vector<int32_t> tmp;
+    if (mem == VMN_MEM_DEVICE) {
+        tmp.resize((size_t)n * row_width);
+        HIPCHK(hipMemcpy(tmp.data(), src, tmp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        src = tmp.data();
+    }
+    rows.assign((size_t)n * R, -1);
+    for (int64_t i = 0; i < n; ++i) {
+        int w = 0;
+        for (int j = 0; j < row_width && w < R; ++j) {
+            const int32_t v = src[i * row_width + j];
+            if (v == -1) break;
+            if (v < 0 || v >= n) return fail(VMN_ERR_ARG, "adjacency id %d out of range", v);
+            if (drop_self && v == i) continue;
+            bool dup = false;
+            for (int t = 0; t < w; ++t) dup |= rows[i * R + t] == v;
+            if (!dup) rows[i * R + w++] = v;
+        }
+    }
+    return VMN_OK;
+}
+
 int check_params(const vmn_params* p) {
     if (!p) return fail(VMN_ERR_ARG, "params is NULL");
     if (p->dim < 1 || p->dim > 256) return fail(VMN_ERR_ARG, "dim %d not in [1, 256]", p->dim);
@@ -406,7 +355,9 @@ int check_params(const vmn_params* p) {
     return VMN_OK;
 }
 
-}  // namespace
+}  // namespace vmn_rt
+
+using namespace vmn_rt;
 
 extern "C" {
 
@@ -476,11 +427,12 @@ int vmn_destroy(vmn_index* idx) {
     return VMN_OK;
 }
 
-int vmn_build(vmn_index* idx, const float* vecs, int64_t n, int32_t mem) {
-    if (!idx || !vecs) return fail(VMN_ERR_ARG, "NULL argument");
-    if (n <= idx->p.R) return fail(VMN_ERR_ARG, "n = %lld must exceed R = %d", (long long)n, idx->p.R);
-    if (n >= (int64_t)1 << 31) return fail(VMN_ERR_ARG, "n must be < 2^31");
-    if (idx->p.capacity && n > idx->p.capacity) return fail(VMN_ERR_CAPACITY, "n exceeds capacity");
+namespace vmn_rt {
+
+// vmn_build / vmn_build_from: `init` (host rows, -1 padded to R) replaces the
+// seeded random initial graph when not NULL.
+int build_common(vmn_index* idx, const float* vecs, int64_t n, int32_t mem,
+                 const std::vector<int32_t>* init) {
     int rc = set_device(idx);
     if (rc) return rc;
     idx->built = false;
@@ -493,15 +445,67 @@ int vmn_build(vmn_index* idx, const float* vecs, int64_t n, int32_t mem) {
     if (staged != idx->vecs.as<float>())
         HIPCHK(hipMemcpyAsync(idx->vecs.p, staged, (size_t)n * idx->DP * sizeof(float),
                               hipMemcpyDeviceToDevice, idx->stream));
+    if (init)
+        HIPCHK(hipMemcpyAsync(idx->adj.p, init->data(), init->size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice, idx->stream));
     rc = compute_medoid(idx);
     if (rc) return rc;
-    rc = idx->p.mode == VMN_MODE_PARTH ? build_parth(idx) : build_fresh(idx);
+    rc = idx->p.mode == VMN_MODE_PARTH ? build_parth(idx, !init) : build_fresh(idx, !init);
     if (rc) return rc;
     idx->fp8_scaled = false;
     rc = refresh_lowp(idx, 0, n);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(idx->stream));
     idx->built = true;
+    return VMN_OK;
+}
+
+int check_build_args(const vmn_index* idx, const float* vecs, int64_t n) {
+    if (!idx || !vecs) return fail(VMN_ERR_ARG, "NULL argument");
+    if (n <= idx->p.R) return fail(VMN_ERR_ARG, "n = %lld must exceed R = %d", (long long)n, idx->p.R);
+    if (n >= (int64_t)1 << 31) return fail(VMN_ERR_ARG, "n must be < 2^31");
+    if (idx->p.capacity && n > idx->p.capacity) return fail(VMN_ERR_CAPACITY, "n exceeds capacity");
+    return VMN_OK;
+}
+
+}  // namespace vmn_rt
+
+int vmn_build(vmn_index* idx, const float* vecs, int64_t n, int32_t mem) {
+    int rc = check_build_args(idx, vecs, n);
+    if (rc) return rc;
+    return build_common(idx, vecs, n, mem, nullptr);
+}
+
+int vmn_build_from(vmn_index* idx, const float* vecs, int64_t n, const int32_t* init_adj,
+                   int32_t row_width, int32_t mem) {
+    int rc = check_build_args(idx, vecs, n);
+    if (rc) return rc;
+    if (!init_adj) return build_common(idx, vecs, n, mem, nullptr);
+    if (row_width < 1) return fail(VMN_ERR_ARG, "row_width %d < 1", row_width);
+    rc = set_device(idx);
+    if (rc) return rc;
+    std::vector<int32_t> rows;
+    rc = normalize_rows(init_adj, n, row_width, idx->p.R, true, mem, rows);
+    if (rc) return rc;
+    return build_common(idx, vecs, n, mem, &rows);
+}
+
+int vmn_get_vectors(const vmn_index* idx, float* out, int32_t mem) {
+    if (!idx || !out) return fail(VMN_ERR_ARG, "NULL argument");
+    if (!idx->built) return fail(VMN_ERR_STATE, "index not built");
+    int rc = set_device(idx);
+    if (rc) return rc;
+    const int D = idx->p.dim, DP = idx->DP;
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    HIPCHK(hipMemcpy2D(out, (size_t)D * sizeof(float), idx->vecs.p, (size_t)DP * sizeof(float),
+                       (size_t)D * sizeof(float), (size_t)idx->n,
+                       mem == VMN_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+    return VMN_OK;
+}
+
+int vmn_get_params(const vmn_index* idx, vmn_params* out) {
+    if (!idx || !out) return fail(VMN_ERR_ARG, "NULL argument");
+    *out = idx->p;
     return VMN_OK;
 }
 
@@ -523,26 +527,9 @@ int vmn_load_graph(vmn_index* idx, const float* vecs, int64_t n, const int32_t* 
         HIPCHK(hipMemcpyAsync(idx->vecs.p, staged, (size_t)n * idx->DP * sizeof(float),
                               hipMemcpyDeviceToDevice, idx->stream));
     // rows are widened to R with -1 on the host side of the copy
-    const int R = idx->p.R;
-    std::vector<int32_t> rows((size_t)n * R, -1);
-    std::vector<int32_t> tmp;
-    const int32_t* src = adj;
-    if (mem == VMN_MEM_DEVICE) {
-        tmp.resize((size_t)n * row_width);
-        HIPCHK(hipMemcpy(tmp.data(), adj, tmp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-        src = tmp.data();
-    }
-    for (int64_t i = 0; i < n; ++i) {
-        int w = 0;
-        for (int j = 0; j < row_width; ++j) {
-            const int32_t v = src[i * row_width + j];
-            if (v == -1) break;
-            if (v < 0 || v >= n) return fail(VMN_ERR_ARG, "adjacency id %d out of range", v);
-            bool dup = false;
-            for (int t = 0; t < w; ++t) dup |= rows[i * R + t] == v;
-            if (!dup) rows[i * R + w++] = v;  // rows hold distinct ids
-        }
-    }
+    std::vector<int32_t> rows;
+    rc = normalize_rows(adj, n, row_width, idx->p.R, false, mem, rows);
+    if (rc) return rc;
     HIPCHK(hipMemcpyAsync(idx->adj.p, rows.data(), rows.size() * sizeof(int32_t),
                           hipMemcpyHostToDevice, idx->stream));
     if (medoid < 0) {

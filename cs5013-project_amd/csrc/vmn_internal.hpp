@@ -1,0 +1,117 @@
+// Internal view of the index handle shared by the runtime sources (not part
+// of the C ABI): device buffers, the vmn_index / vmn_ctx structs and the
+// runtime helpers the C entry points are built from.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/vamana_mi355x.h"
+
+namespace vmn_rt {
+
+// Records the message for vmn_last_error() and returns `code`.
+int fail(int code, const char* fmt, ...);
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(VMN_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+// Device allocation that grows on demand and is kept across calls.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t want) {
+        if (want <= bytes) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            bytes = 0;
+        }
+        // grow by at least half the old size: buffers sized by batch or node
+        // count are not freed and reallocated (a device-wide sync) on every
+        // slightly larger call
+        size_t b = std::max<size_t>(std::max<size_t>(want, 256), bytes + bytes / 2);
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Row stride in floats: the dimension rounded up to one of the instantiated
+// kernel widths {32, 64, 96, 128, 160, 192, 256}.
+inline int pad_dim(int d) {
+    const int p = (d + 31) / 32 * 32;
+    return p > 192 ? 256 : p;
+}
+
+}  // namespace vmn_rt
+
+struct vmn_index {
+    vmn_params p{};
+    int DP = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    int64_t n = 0, cap = 0;
+    int64_t medoid = -1;
+    bool built = false;
+    vmn_rt::DevBuf vecs, adj, del_bits;
+    // scratch
+    vmn_rt::DevBuf qpad, out_ids, out_d, stat_hops, stat_ndist, counter;
+    vmn_rt::DevBuf cand_ids, cand_d, vis_ids, vis_d, vis_cnt;
+    vmn_rt::DevBuf pair_dst, pair_d, rcount, rfill, roff, rscan_tmp, rsrc, rd, rlist;
+    vmn_rt::DevBuf centroid, medoid_key;
+    int64_t last_nq = 0;
+    int64_t n_deleted = 0;  // tombstones set since the build (upper bound)
+    vmn_rt::DevBuf staging, find_out, bad;
+    vmn_rt::DevBuf bf16_rows, bf16_norms, fp8_rows, fp8_norms, fp8_scales, fp8_absmax;
+    bool fp8_scaled = false;
+    int waves_per_cu = 32;  // search grid = CUs x waves_per_cu (persistent work queue)
+    int list_cap = 0;       // minimum beam-list capacity of the search kernel
+};
+
+// A search context: its own stream and per-launch scratch, so several search
+// batches can run on the device at once (the index is read-only meanwhile).
+struct vmn_ctx {
+    vmn_index* idx = nullptr;
+    hipStream_t stream = nullptr;
+    vmn_rt::DevBuf counter, stat_hops, stat_ndist;
+    int64_t last_nq = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace vmn_rt {
+
+int set_device(const vmn_index* idx);
+// Rows of dimension D (host or device) placed as [n x DP] rows on the device.
+int stage_rows(vmn_index* idx, const float* src, int64_t n, int32_t mem, DevBuf& scratch,
+               const float** out);
+int compute_medoid(vmn_index* idx);
+// Allocates rows, adjacency and tombstones for max(capacity, n) nodes.
+int alloc_graph(vmn_index* idx, int64_t n);
+// (Re)computes the BF16 / FP8 copies of rows [from, to).
+int refresh_lowp(vmn_index* idx, int64_t from, int64_t to);
+// Host copy of an adjacency given by a caller ([n x row_width], -1 padded):
+// each row keeps its entries up to the first -1, in order, without repeated
+// ids (and without i itself when drop_self), -1 padded to width R.
+int normalize_rows(const int32_t* src, int64_t n, int row_width, int R, bool drop_self,
+                   int32_t mem, std::vector<int32_t>& rows);
+int check_params(const vmn_params* p);
+
+}  // namespace vmn_rt

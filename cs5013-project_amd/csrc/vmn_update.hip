@@ -75,12 +75,18 @@ hipError_t launch_cons_flags(const int32_t* adj, int64_t n, int W, const uint32_
     return hipGetLastError();
 }
 
-// Delete consolidation (FreshDiskANN, Singh et al. 2021, Algorithm 4), one
-// wavefront per listed node p. The candidate pool is built row by row: p's
-// own row, then the row of each tombstoned neighbour in adjacency order; an
-// entry enters if it is live, not p and not already pooled; the pool is kept
-// as the first M entries by (distance, id) (an entry cut by the cap can never
-// return: the pool's M-th key only falls). Then robust prune with <= into R.
+// Delete consolidation, one wavefront per listed live node p with a tombstoned
+// out-neighbour (the reference has none; after FreshDiskANN, Singh et al.
+// 2021, Algorithm 4, with the live part of the row kept). The row's live
+// neighbours K all stay, in row order. Candidates for the slots the
+// tombstones free are the live out-neighbours of p's tombstoned neighbours
+// (rows in adjacency order; not p, not in K, not already pooled), kept as the
+// first M by (distance, id). Each K member drops the candidates c it occludes
+// (alpha * d(k, c) <= d(p, c)); the rest are taken in order as robust prune
+// does (each taken candidate drops the later ones it occludes) until the row
+// holds R. Re-pruning the whole pool instead lets the 2-hop candidates
+// occlude long edges the build chose from a full search, and recall decays
+// with every round of deletes.
 template <int DP, int CCAP>
 __global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
     __shared__ __align__(16) uint32_t c_id[2][CCAP];
@@ -91,6 +97,7 @@ __global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
     __shared__ int32_t act_id[CCAP];
     __shared__ int32_t act_pos[CCAP];
     __shared__ float act_d[CCAP];
+    __shared__ __align__(16) int32_t k_id[64];
     __shared__ int32_t newrow[64];
     __shared__ int64_t s_q;
     const int l = threadIdx.x, sub = l & 7;
@@ -102,18 +109,22 @@ __global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
         QReg<DP> q;
         q.load(a.vecs + p * DP, sub);
         const int32_t own = l < W ? a.adj[p * W + l] : -1;
-        const uint64_t dmask = ballot(own >= 0 && bit_test(a.del_bits, (uint32_t)own));
+        const bool own_dead = own >= 0 && bit_test(a.del_bits, (uint32_t)own);
+        const uint64_t dmask = ballot(own_dead);
+        const uint64_t kmask = ballot(own >= 0 && !own_dead);
+        const int nK = __popcll((unsigned long long)kmask);
+        if (own >= 0 && !own_dead) k_id[__popcll((unsigned long long)(kmask & lanemask_lt()))] = own;
+        __syncthreads();
         int cur = 0, cnt = 0;
-        // source rows: j = -1 is p's own row, then each tombstoned neighbour
-        uint64_t rem = dmask;
-        int32_t src = -1;
-        for (;;) {
-            const int32_t v = src < 0 ? own : (l < W ? a.adj[(int64_t)src * W + l] : -1);
+        for (uint64_t rem = dmask; rem; rem &= rem - 1) {
+            const int32_t src = readlane_i(own, __ffsll((unsigned long long)rem) - 1);
+            const int32_t v = l < W ? a.adj[(int64_t)src * W + l] : -1;
             bool pass = v >= 0 && v != (int32_t)p && !bit_test(a.del_bits, (uint32_t)v);
-            {
+            if (pass) {
                 bool dup = false;
+                for (int t = 0; t < nK; ++t) dup |= k_id[t] == v;
                 for (int t = 0; t < cnt; ++t) dup |= c_id[cur][t] == (uint32_t)v;
-                pass = pass && !dup;
+                pass = !dup;
             }
             const int ne = row_extras<DP>(a.vecs, a.adj, W, p, q, pass, v, e_id, e_d);
             const uint32_t my_id = l < ne ? (uint32_t)e_id[l] : 0u;
@@ -122,13 +133,78 @@ __global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
                                            c_id[cur ^ 1], c_d[cur ^ 1], a.M);
             cur ^= 1;
             __syncthreads();
-            if (!rem) break;
-            const int j = __ffsll((unsigned long long)rem) - 1;
-            rem &= rem - 1;
-            src = readlane_i(own, j);
         }
-        const int nk = robust_prune<DP, false>(a.vecs, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
-                                               kept, act_id, act_pos, act_d, newrow, nullptr);
+        // K stays; each member drops the candidates it occludes
+        for (int t = l; t < cnt; t += 64) kept[t] = 1;
+        if (l < nK) newrow[l] = k_id[l];
+        __syncthreads();
+        for (int t = 0; t < nK && cnt > 0; ++t) {
+            int nact = 0;
+            for (int b = 0; b < cnt; b += 64) {
+                const int j = b + l;
+                const bool f = j < cnt && kept[j];
+                const uint64_t m = ballot(f);
+                const int k = nact + __popcll((unsigned long long)(m & lanemask_lt()));
+                if (f) {
+                    act_id[k] = (int32_t)(c_id[cur][j] & VMN_ID_MASK);
+                    act_pos[k] = j;
+                }
+                nact += __popcll((unsigned long long)m);
+            }
+            if (nact == 0) break;
+            QReg<DP> kq;
+            kq.load(a.vecs + (int64_t)k_id[t] * DP, sub);
+            __syncthreads();
+            gather_dists<DP>(kq, a.vecs, act_id, act_d, nact);
+            __syncthreads();
+            for (int k = l; k < nact; k += 64) {
+                const int j = act_pos[k];
+                if (a.alpha * act_d[k] <= c_d[cur][j]) kept[j] = 0;
+            }
+            __syncthreads();
+        }
+        // then the remaining candidates in order, as robust prune takes them
+        int nk = nK, from = 0;
+        while (nk < a.R) {
+            int pi = -1;
+            for (int b = from; b < cnt; b += 64) {
+                const int j = b + l;
+                const uint64_t m = ballot(j < cnt && kept[j]);
+                if (m) {
+                    pi = b + __ffsll((unsigned long long)m) - 1;
+                    break;
+                }
+            }
+            if (pi < 0) break;
+            const int32_t ps = (int32_t)(c_id[cur][pi] & VMN_ID_MASK);
+            if (l == 0) newrow[nk] = ps;
+            ++nk;
+            if (nk >= a.R) break;
+            int nact = 0;
+            for (int b = pi + 1; b < cnt; b += 64) {
+                const int j = b + l;
+                const bool f = j < cnt && kept[j];
+                const uint64_t m = ballot(f);
+                const int k = nact + __popcll((unsigned long long)(m & lanemask_lt()));
+                if (f) {
+                    act_id[k] = (int32_t)(c_id[cur][j] & VMN_ID_MASK);
+                    act_pos[k] = j;
+                }
+                nact += __popcll((unsigned long long)m);
+            }
+            QReg<DP> pq;
+            pq.load(a.vecs + (int64_t)ps * DP, sub);
+            __syncthreads();
+            gather_dists<DP>(pq, a.vecs, act_id, act_d, nact);
+            __syncthreads();
+            for (int k = l; k < nact; k += 64) {
+                const int j = act_pos[k];
+                if (a.alpha * act_d[k] <= c_d[cur][j]) kept[j] = 0;
+            }
+            __syncthreads();
+            from = pi + 1;
+        }
+        __syncthreads();
         for (int j = l; j < W; j += 64) a.adj[p * W + j] = j < nk ? newrow[j] : -1;
         __syncthreads();
     }

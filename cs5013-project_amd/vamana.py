@@ -68,7 +68,17 @@ def lib():
         L.vmn_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(P)]
         L.vmn_destroy.argtypes = [P]
         L.vmn_build.argtypes = [P, P, i64, i32]
+        L.vmn_build_from.argtypes = [P, P, i64, P, i32, i32]
         L.vmn_load_graph.argtypes = [P, P, i64, P, i32, i64, i32]
+        cs = ctypes.c_char_p
+        L.vmn_save.argtypes = [P, cs]
+        L.vmn_load.argtypes = [cs, i32, ctypes.POINTER(P)]
+        L.vmn_save_fresh_graph.argtypes = [P, cs]
+        L.vmn_load_fresh_graph.argtypes = [cs, ctypes.POINTER(Params), i32, i64, ctypes.POINTER(P)]
+        L.vmn_save_parth_graph.argtypes = [P, cs]
+        L.vmn_load_parth_graph.argtypes = [cs, ctypes.POINTER(Params), i64, ctypes.POINTER(P)]
+        L.vmn_get_vectors.argtypes = [P, P, i32]
+        L.vmn_get_params.argtypes = [P, ctypes.POINTER(Params)]
         L.vmn_search.argtypes = [P, P, i64, i32, i32, P, P, i32]
         L.vmn_search_ex.argtypes = [P, P, i64, i32, i32, i32, i32, P, P, i32]
         L.vmn_get_lowp_rows.argtypes = [P, i32, P]
@@ -107,7 +117,10 @@ def lib():
         L.vmn_server_destroy.argtypes = [P]
         L.vmn_last_error.argtypes = []
         L.vmn_last_error.restype = ctypes.c_char_p
-        for name in ("vmn_create", "vmn_destroy", "vmn_build", "vmn_load_graph", "vmn_search",
+        for name in ("vmn_create", "vmn_destroy", "vmn_build", "vmn_build_from", "vmn_load_graph",
+                     "vmn_save", "vmn_load", "vmn_save_fresh_graph", "vmn_load_fresh_graph",
+                     "vmn_save_parth_graph", "vmn_load_parth_graph", "vmn_get_vectors",
+                     "vmn_get_params", "vmn_search",
                      "vmn_get_adjacency", "vmn_search_stats", "vmn_groundtruth", "vmn_insert",
                      "vmn_delete_ids", "vmn_delete_vectors", "vmn_consolidate",
                      "vmn_get_deleted", "vmn_search_ex", "vmn_get_lowp_rows", "vmn_ctx_create",
@@ -150,13 +163,15 @@ class Vamana:
     """Vamana / FreshVamana index resident on one GPU.
 
     Vamana(vectors, mode="parth"|"fresh", **params) builds the graph (as the
-    reference's constructor does). Vamana.from_graph(vectors, adj, medoid)
-    installs an existing graph instead.
+    reference's constructor does), from the seeded random initial graph or from
+    `init_adj` (an [n, w] -1-padded adjacency, e.g. the reference's random-graph
+    file). Vamana.from_graph(vectors, adj, medoid) installs an existing graph
+    instead; Vamana.load / load_fresh_graph / load_parth_graph read files.
     """
 
     def __init__(self, vectors=None, mode="parth", *, dim=None, R=None, L_build=None,
                  alphas=None, passes=None, seed=None, visited_cap=None, max_hops=None, P=None,
-                 M=None, capacity=0, device=0, low_precision=0, _build=True):
+                 M=None, capacity=0, device=0, low_precision=0, init_adj=None, _build=True):
         m = {"parth": MODE_PARTH, "fresh": MODE_FRESH}[mode] if isinstance(mode, str) else mode
         p = default_params(m)
         if vectors is not None:
@@ -184,7 +199,82 @@ class Vamana:
         _check(lib().vmn_create(ctypes.byref(p), ctypes.byref(self._h)))
         if vectors is not None and _build:
             ptr, mem, keep = _arg(vectors, np.float32)
-            _check(lib().vmn_build(self._h, ptr, int(vectors.shape[0]), mem))
+            if init_adj is None:
+                _check(lib().vmn_build(self._h, ptr, int(vectors.shape[0]), mem))
+            else:
+                ap, am, ak = _arg(init_adj, np.int32)
+                assert am == mem, "vectors and init_adj must both be host or both be device"
+                _check(lib().vmn_build_from(self._h, ptr, int(vectors.shape[0]), ap,
+                                            int(init_adj.shape[1]), mem))
+
+    @classmethod
+    def _wrap(cls, handle):
+        self = cls.__new__(cls)
+        self._h = handle
+        self.params = Params()
+        _check(lib().vmn_get_params(handle, ctypes.byref(self.params)))
+        return self
+
+    @classmethod
+    def load(cls, path, device=0):
+        """An index written by save() (vmn_load)."""
+        h = ctypes.c_void_p()
+        _check(lib().vmn_load(os.fsencode(path), device, ctypes.byref(h)))
+        return cls._wrap(h)
+
+    def save(self, path):
+        """Parameters, vectors, graph, tombstones, entry point, FP8 scales (vmn_save)."""
+        _check(lib().vmn_save(self._h, os.fsencode(path)))
+
+    @classmethod
+    def load_fresh_graph(cls, path, dim, R, build=False, medoid=-1, mode="fresh", **kw):
+        """The reference's FreshVamana record file (n x {float vec[dim]; uint32
+        degree; uint32 nbr[R]}): installed as the graph, or with build=True used
+        as the initial graph of a build (vmn_load_fresh_graph)."""
+        p = cls._params(mode, dim=dim, R=R, **kw)
+        h = ctypes.c_void_p()
+        _check(lib().vmn_load_fresh_graph(os.fsencode(path), ctypes.byref(p), int(bool(build)),
+                                          int(medoid), ctypes.byref(h)))
+        return cls._wrap(h)
+
+    def save_fresh_graph(self, path):
+        _check(lib().vmn_save_fresh_graph(self._h, os.fsencode(path)))
+
+    @classmethod
+    def load_parth_graph(cls, path, medoid=-1, mode="parth", **kw):
+        """The parth engine's graph file (header n, dim, max_degree; vectors;
+        -1-padded adjacency), installed with entry point `medoid` (-1: computed)."""
+        p = cls._params(mode, **kw)
+        h = ctypes.c_void_p()
+        _check(lib().vmn_load_parth_graph(os.fsencode(path), ctypes.byref(p), int(medoid),
+                                          ctypes.byref(h)))
+        return cls._wrap(h)
+
+    def save_parth_graph(self, path):
+        _check(lib().vmn_save_parth_graph(self._h, os.fsencode(path)))
+
+    @staticmethod
+    def _params(mode, dim=None, R=None, L_build=None, alphas=None, capacity=0, device=0,
+                low_precision=0):
+        m = {"parth": MODE_PARTH, "fresh": MODE_FRESH}[mode] if isinstance(mode, str) else mode
+        p = default_params(m)
+        if dim is not None:
+            p.dim = dim
+        if R is not None:
+            p.R = R
+        if L_build is not None:
+            p.L_build = L_build
+        if alphas is not None:
+            al = list(alphas)
+            p.passes = len(al)
+            for i in range(4):
+                p.alpha[i] = al[min(i, len(al) - 1)]
+        p.capacity, p.device = capacity, device
+        if isinstance(low_precision, str):
+            low_precision = {"": 0, "bf16": STORE_BF16, "fp8": STORE_FP8,
+                             "both": STORE_BF16 | STORE_FP8}[low_precision]
+        p.low_precision = low_precision
+        return p
 
     @classmethod
     def from_graph(cls, vectors, adj, medoid=-1, mode="parth", R=None, **kw):
@@ -297,6 +387,12 @@ class Vamana:
         out = np.empty(self.size, dtype=np.uint8)
         _check(lib().vmn_get_deleted(self._h, out.ctypes.data_as(ctypes.c_void_p)))
         return out.astype(bool)
+
+    @property
+    def vectors(self):
+        out = np.empty((self.size, self.params.dim), dtype=np.float32)
+        _check(lib().vmn_get_vectors(self._h, out.ctypes.data_as(ctypes.c_void_p), MEM_HOST))
+        return out
 
     @property
     def adjacency(self):

@@ -87,11 +87,49 @@ int vmn_destroy(vmn_index* idx);
  * random initial graph, medoid, then `passes` x (search all, prune all). */
 int vmn_build(vmn_index* idx, const float* vecs, int64_t n, int32_t mem);
 
+/* vmn_build starting from a given initial graph instead of the seeded random
+ * one: `init_adj` [n x row_width] (same `mem` as vecs), each row read up to its
+ * first -1, repeated ids and the node itself dropped, at most R kept. The
+ * reference's constructor builds from the random-graph file it is handed
+ * (`Vamana<T>(std::unique_ptr<GraphT>)`, vamana_impl.cuh:10-20, with
+ * initGraph, utils.cuh:113-148). init_adj == NULL is vmn_build. */
+int vmn_build_from(vmn_index* idx, const float* vecs, int64_t n, const int32_t* init_adj,
+                   int32_t row_width, int32_t mem);
+
 /* Install an existing graph ([n x dim] vectors, [n x row_width] adjacency,
  * -1 padded, row_width <= R) with entry point `medoid` (-1: compute it).
  * Replaces `initGraph` (utils.cuh:113-148) / parth load_data (engine.cu:338-357). */
 int vmn_load_graph(vmn_index* idx, const float* vecs, int64_t n, const int32_t* adj,
                    int32_t row_width, int64_t medoid, int32_t mem);
+
+/* Index file: parameters, vectors, adjacency, tombstones, entry point and the
+ * FP8 scales, so the loaded index searches and updates exactly as the saved
+ * one. vmn_load creates the index on `device`. Replaces `saveGraph`
+ * (vamana.cu:1346-1362), which writes only vectors and adjacency. */
+int vmn_save(const vmn_index* idx, const char* path);
+int vmn_load(const char* path, int32_t device, vmn_index** out);
+
+/* The reference's FreshVamana graph file: size() records of { float vec[dim];
+ * uint32 degree; uint32 nbr[R] } (772 bytes at dim 128, R 64), no header
+ * (constants.cuh:17-18; written by saveGraph, vamana.cu:1346-1362; read by
+ * initGraph, utils.cuh:113-148). Saving writes 0xffffffff past each degree.
+ * Loading takes dim and R (the record layout) and the index parameters from
+ * *p, the node count from the file size; build != 0 builds from the file's
+ * graph as the initial graph (the reference's constructor, vmn_build_from),
+ * build == 0 installs it with entry point `medoid` (-1: compute it; the
+ * reference uses the constant 5000, constants.cuh:12). */
+int vmn_save_fresh_graph(const vmn_index* idx, const char* path);
+int vmn_load_fresh_graph(const char* path, const vmn_params* p, int32_t build, int64_t medoid,
+                         vmn_index** out);
+
+/* The parth engine's graph file: int32 n, dim, max_degree; float
+ * vectors[n x dim]; int32 adjacency[n x max_degree], -1 padded (written by
+ * build_graph.cu:361-369, read by engine.cu:338-357). Saving writes R as
+ * max_degree. Loading takes dim and R from the file, the other parameters
+ * from *p, and installs the graph with entry point `medoid` (-1: compute it;
+ * the engine's searches start at node 0, engine.cu:476-477). */
+int vmn_save_parth_graph(const vmn_index* idx, const char* path);
+int vmn_load_parth_graph(const char* path, const vmn_params* p, int64_t medoid, vmn_index** out);
 
 /* Batched greedy search of nq queries [nq x dim] from the medoid with beam L;
  * writes the k best ids (-1 padded) and squared L2 distances (may be NULL) per
@@ -144,6 +182,13 @@ int vmn_get_deleted(const vmn_index* idx, uint8_t* mask_out);
 
 /* Copy the adjacency [size x R] (int32, -1 padded). */
 int vmn_get_adjacency(const vmn_index* idx, int32_t* out, int32_t mem);
+
+/* Copy the vectors [size x dim] (float32, as given to build / insert). The
+ * reference's graph records hold them (graphT.cuh:56-59). */
+int vmn_get_vectors(const vmn_index* idx, float* out, int32_t mem);
+
+/* The parameters the index was created with. */
+int vmn_get_params(const vmn_index* idx, vmn_params* out);
 
 int64_t vmn_size(const vmn_index* idx);
 int64_t vmn_medoid(const vmn_index* idx);

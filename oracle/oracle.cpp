@@ -119,6 +119,26 @@ void orc_random_init(int64_t n, int R, int row_width, uint64_t seed, int32_t* ad
     }
 }
 
+// A caller's initial graph [n x init_width] as rows of width `row_width`:
+// entries up to the first -1 in order, repeated ids and the node itself
+// dropped, at most R kept, -1 padded (the rule of vmn_build_from).
+void orc_init_rows(const int32_t* init, int64_t n, int init_width, int R, int row_width,
+                   int32_t* adj) {
+    for (int64_t i = 0; i < n; ++i) {
+        int32_t* row = adj + i * row_width;
+        for (int j = 0; j < row_width; ++j) row[j] = -1;
+        int w = 0;
+        for (int j = 0; j < init_width && w < R; ++j) {
+            const int32_t v = init[i * init_width + j];
+            if (v == -1) break;
+            if (v == (int32_t)i) continue;
+            bool dup = false;
+            for (int t = 0; t < w; ++t) dup |= row[t] == v;
+            if (!dup) row[w++] = v;
+        }
+    }
+}
+
 // parth/src/build_graph.cu:326-343: float centroid accumulated node by node,
 // divided by n, then the node nearest to it (separately rounded mul/add, strict
 // `<` so the lowest id wins ties).
@@ -308,8 +328,12 @@ void orc_parth_prune(const float* vecs, int64_t n, int d, int32_t* adj, int row_
 // The search of a pass completes for every node before any row is pruned.
 int64_t orc_parth_build(const float* vecs, int64_t n, int d, int R, int row_width, int L,
                         const float* alphas, int passes, uint64_t seed, int visited_cap,
-                        int max_hops, int dist_mode, int32_t* adj) {
-    orc_random_init(n, R, row_width, seed, adj);
+                        int max_hops, int dist_mode, int32_t* adj, const int32_t* init,
+                        int init_width) {
+    if (init)
+        orc_init_rows(init, n, init_width, R, row_width, adj);
+    else
+        orc_random_init(n, R, row_width, seed, adj);
     int64_t medoid = orc_medoid(vecs, n, d);
     std::vector<int32_t> ci((size_t)n * L);
     std::vector<float> cd((size_t)n * L);
@@ -561,8 +585,11 @@ void orc_fresh_reverse(const float* vecs, int64_t n, int d, int32_t* adj, int W,
 // out-prune every node, reverse-edge prune the nodes that received pairs.
 int64_t orc_fresh_build(const float* vecs, int64_t n, int d, int R, int L, const float* alphas,
                         int passes, uint64_t seed, int P, int M, int max_hops, int dist_mode,
-                        int32_t* adj) {
-    orc_random_init(n, R, R, seed, adj);
+                        int32_t* adj, const int32_t* init, int init_width) {
+    if (init)
+        orc_init_rows(init, n, init_width, R, R, adj);
+    else
+        orc_random_init(n, R, R, seed, adj);
     int64_t medoid = orc_medoid(vecs, n, d);
     std::vector<int32_t> vis((size_t)n * P), vcnt(n), pdst((size_t)n * R);
     std::vector<float> visd((size_t)n * P), pd((size_t)n * R);
@@ -599,12 +626,15 @@ void orc_fresh_insert(const float* vecs, int64_t n_old, int64_t n_new, int d, in
                       dist_mode);
 }
 
-// Delete consolidation (FreshDiskANN, Singh et al. 2021, Algorithm 4; the
-// reference has none). For every live node p with a tombstoned out-neighbour:
-// candidates = live out-neighbours of p plus the live out-neighbours of each
-// tombstoned out-neighbour, without p; ordered by (d(p, c), id), the first M
-// kept; robust prune with <= and `alpha` into at most R. Then the rows of
-// tombstoned nodes are emptied. Returns the number of re-pruned nodes.
+// Delete consolidation (the reference has none; after FreshDiskANN, Singh et
+// al. 2021, Algorithm 4, with the live part of each row kept). For every live
+// node p with a tombstoned out-neighbour: K = p's live out-neighbours in row
+// order, all kept; candidates = the live out-neighbours of p's tombstoned
+// out-neighbours (rows in adjacency order), without p and K, ordered by
+// (d(p, c), id), the first M kept; each candidate in order joins the row
+// unless a row member r occludes it (alpha * d(r, c) <= d(p, c)), until the
+// row holds R. Then the rows of tombstoned nodes are emptied. Returns the
+// number of repaired nodes.
 int64_t orc_consolidate(const float* vecs, int64_t n, int d, int32_t* adj, int R,
                         const uint8_t* del, float alpha, int M, int dist_mode) {
     std::vector<int64_t> todo;
@@ -619,17 +649,23 @@ int64_t orc_consolidate(const float* vecs, int64_t n, int d, int32_t* adj, int R
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t t = 0; t < (int64_t)todo.size(); ++t) {
         const int64_t p = todo[t];
+        std::vector<int32_t> row;
+        for (int j = 0; j < R; ++j) {
+            int32_t w = adj[p * R + j];
+            if (w >= 0 && !del[w]) row.push_back(w);
+        }
         std::vector<int32_t> ids;
-        auto add = [&](int32_t w) {
-            if (w < 0 || del[w] || w == (int32_t)p) return;
-            for (int32_t x : ids) if (x == w) return;
-            ids.push_back(w);
-        };
-        for (int j = 0; j < R; ++j) add(adj[p * R + j]);
         for (int j = 0; j < R; ++j) {
             int32_t v = adj[p * R + j];
-            if (v >= 0 && del[v])
-                for (int k = 0; k < R; ++k) add(adj[(int64_t)v * R + k]);
+            if (v < 0 || !del[v]) continue;
+            for (int k = 0; k < R; ++k) {
+                int32_t w = adj[(int64_t)v * R + k];
+                if (w < 0 || del[w] || w == (int32_t)p) continue;
+                bool seen = false;
+                for (int32_t x : row) seen |= x == w;
+                for (int32_t x : ids) seen |= x == w;
+                if (!seen) ids.push_back(w);
+            }
         }
         std::vector<Item> C;
         for (int32_t w : ids)
@@ -638,10 +674,16 @@ int64_t orc_consolidate(const float* vecs, int64_t n, int d, int32_t* adj, int R
             return a.d < b.d || (a.d == b.d && a.id < b.id);
         });
         if ((int)C.size() > M) C.resize(M);
-        std::vector<int32_t> out;
-        std::vector<float> outd;
-        prune_le(vecs, d, C, alpha, R, dist_mode, out, outd);
-        rows[t] = out;
+        for (const Item& c : C) {
+            if ((int)row.size() >= R) break;
+            bool occ = false;
+            for (int32_t r : row) {
+                float dr = l2sq_m(vecs + (int64_t)r * d, vecs + (int64_t)c.id * d, d, dist_mode);
+                if (dr * alpha <= c.d) { occ = true; break; }
+            }
+            if (!occ) row.push_back(c.id);
+        }
+        rows[t] = row;
     }
     for (size_t t = 0; t < todo.size(); ++t)
         for (int j = 0; j < R; ++j) adj[todo[t] * R + j] = j < (int)rows[t].size() ? rows[t][j] : -1;

@@ -37,7 +37,7 @@ def lib():
                                           i32, i32, P, P, P, P]
         _lib.orc_parth_prune.argtypes = [P, i64, i32, P, i32, i32, f32, P, P, i32, i32]
         _lib.orc_parth_build.argtypes = [P, i64, i32, i32, i32, i32, P, i32, u64, i32, i32,
-                                         i32, P]
+                                         i32, P, P, i32]
         _lib.orc_parth_build.restype = i64
         _lib.orc_bruteforce_fp64.argtypes = [P, i64, i32, P, P, i64, i32, P, P]
         _lib.orc_bf_hash1.argtypes = [ctypes.c_uint32]
@@ -50,6 +50,7 @@ def lib():
                                           i32, i32]
         _lib.orc_consolidate.argtypes = [P, i64, i32, P, i32, P, f32, i32, i32]
         _lib.orc_consolidate.restype = i64
+        _lib.orc_init_rows.argtypes = [P, i64, i32, i32, i32, P]
         _lib.orc_medoid_live.argtypes = [P, i64, i32, P]
         _lib.orc_medoid_live.restype = i64
         _lib.orc_fresh_out_prune.argtypes = [P, i32, P, i32, i32, f32, i64, i64, P, P, P, i32,
@@ -57,7 +58,7 @@ def lib():
         _lib.orc_fresh_reverse.argtypes = [P, i64, i32, P, i32, i32, f32, i64, i64, P, P, i32,
                                            i32]
         _lib.orc_fresh_build.argtypes = [P, i64, i32, i32, i32, P, i32, u64, i32, i32, i32, i32,
-                                         P]
+                                         P, P, i32]
         _lib.orc_fresh_build.restype = i64
         _lib.orc_num_threads.restype = i32
         _lib.orc_set_num_threads.argtypes = [i32]
@@ -118,15 +119,18 @@ def parth_prune(vecs, adj, R, alpha, cand_ids, cand_dists, dist_mode=0):
 
 
 def parth_build(vecs, R, L, alphas=(1.2, 1.2), row_width=None, seed=1234, visited_cap=128,
-                max_hops=200, dist_mode=0):
-    """Returns (adj [n,row_width] int32, medoid)."""
+                max_hops=200, dist_mode=0, init=None):
+    """Returns (adj [n,row_width] int32, medoid). `init` ([n, w] int32, -1
+    padded) replaces the seeded random initial graph."""
     vecs = _f32(vecs)
     n, d = vecs.shape
     rw = R if row_width is None else row_width
     adj = np.empty((n, rw), dtype=np.int32)
     al = np.asarray(alphas, dtype=np.float32)
+    ini = None if init is None else np.ascontiguousarray(init, dtype=np.int32)
     med = lib().orc_parth_build(_p(vecs), n, d, R, rw, L, _p(al), len(al), seed, visited_cap,
-                                max_hops, dist_mode, _p(adj))
+                                max_hops, dist_mode, _p(adj), _p(ini),
+                                0 if ini is None else ini.shape[1])
     return adj, int(med)
 
 
@@ -159,14 +163,16 @@ def fresh_search(vecs, adj, queries, start, L, out_n=10, P=600, max_hops=1 << 20
 
 
 def fresh_build(vecs, R=64, L=128, alphas=(1.5,), seed=1234, P=600, M=500, max_hops=1 << 20,
-                dist_mode=0):
-    """Returns (adj [n,R] int32, medoid)."""
+                dist_mode=0, init=None):
+    """Returns (adj [n,R] int32, medoid). `init` ([n, w] int32, -1 padded)
+    replaces the seeded random initial graph."""
     vecs = _f32(vecs)
     n, d = vecs.shape
     adj = np.empty((n, R), dtype=np.int32)
     al = np.asarray(alphas, dtype=np.float32)
+    ini = None if init is None else np.ascontiguousarray(init, dtype=np.int32)
     med = lib().orc_fresh_build(_p(vecs), n, d, R, L, _p(al), len(al), seed, P, M, max_hops,
-                                dist_mode, _p(adj))
+                                dist_mode, _p(adj), _p(ini), 0 if ini is None else ini.shape[1])
     return adj, int(med)
 
 

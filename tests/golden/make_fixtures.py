@@ -17,10 +17,18 @@ Outputs (committed):
                data/sift10k/sift10k_randomgraph.bin (R=64, alpha 1.5, medoid 5000)
   toy2d.npz    toy_vecs/toy_adj (20 nodes) and vis_vecs/vis_adj (50 nodes),
                parth/data/{toy,vis}_initial_graph.bin (float 2-D data)
+  randgraph.npz adj (10000x64 i32) and degree of data/sift10k/sift10k_randomgraph.bin,
+               the random initial graph the reference's FreshVamana build starts from
+               (772-byte records; vectors = SIFT10K base)
+  file_digests.json  size and SHA-256 of the reference's graph files, so a
+               writer of those formats can be checked byte for byte without the
+               files themselves
 
 SIFT10K is integer-valued (0..180); storing it as uint8 is lossless, which is
 asserted below.
 """
+import hashlib
+import json
 import os
 import struct
 import sys
@@ -84,6 +92,21 @@ def main(ref):
     nbr = rec[:, 516:].copy().view(np.uint32).astype(np.int64)
     g1 = np.where(np.arange(64)[None, :] < deg[:, None], nbr, -1).astype(np.int32)
     np.savez_compressed(os.path.join(HERE, "g1_graph.npz"), adj=g1, degree=deg)
+    rrec = np.fromfile(os.path.join(ref, "data/sift10k/sift10k_randomgraph.bin"),
+                       dtype=np.uint8).reshape(-1, 772)
+    assert np.array_equal(rrec[:, :512].copy().view(np.float32), base)
+    rdeg = rrec[:, 512:516].copy().view(np.uint32).ravel().astype(np.int32)
+    rnbr = rrec[:, 516:].copy().view(np.uint32).astype(np.int64)
+    radj = np.where(np.arange(64)[None, :] < rdeg[:, None], rnbr, -1).astype(np.int32)
+    np.savez_compressed(os.path.join(HERE, "randgraph.npz"), adj=radj, degree=rdeg)
+    digests = {}
+    for rel in ("parth/data/sift_initial_graph.bin", "parth/data/custom_sift_initial_graph.bin",
+                "parth/data/toy_initial_graph.bin", "parth/data/vis_initial_graph.bin",
+                "vamana/vout/vamana.out", "data/sift10k/sift10k_randomgraph.bin"):
+        raw = open(os.path.join(ref, rel), "rb").read()
+        digests[rel] = {"bytes": len(raw), "sha256": hashlib.sha256(raw).hexdigest()}
+    with open(os.path.join(HERE, "file_digests.json"), "w") as f:
+        json.dump(digests, f, indent=1)
     tv, tadj = read_parth_graph(os.path.join(ref, "parth/data/toy_initial_graph.bin"))
     vv, vadj = read_parth_graph(os.path.join(ref, "parth/data/vis_initial_graph.bin"))
     np.savez_compressed(os.path.join(HERE, "toy2d.npz"),

@@ -51,3 +51,33 @@ def test_null_arguments_are_errors(vmn):
     L = vmn.lib()
     assert L.vmn_build(None, None, 10, 0) == -1
     assert L.vmn_search(None, None, 1, 1, 1, None, None, 0) == -1
+
+
+def test_file_readers_reject_bad_files_without_gpu(vmn, tmp_path):
+    """Every reader checks the file before it touches the device."""
+    err = vmn.VamanaError
+    with pytest.raises(err) as e:
+        vmn.Vamana.load(str(tmp_path / "missing.vmn"))
+    assert e.value.code == -5
+    junk = tmp_path / "junk.vmn"
+    junk.write_bytes(b"not an index file at all")
+    with pytest.raises(err) as e:
+        vmn.Vamana.load(str(junk))
+    assert e.value.code == -5 and "version-1 index file" in str(e.value)
+    # parth graph whose header promises more than the file holds
+    bad = tmp_path / "short_graph.bin"
+    bad.write_bytes(struct_pack(100, 128, 32) + b"\0" * 1000)
+    with pytest.raises(err) as e:
+        vmn.Vamana.load_parth_graph(str(bad))
+    assert e.value.code == -5 and "header" in str(e.value)
+    # FreshVamana records: 772-byte records at dim 128, R 64
+    rec = tmp_path / "records.bin"
+    rec.write_bytes(b"\0" * (772 * 3 + 5))
+    with pytest.raises(err) as e:
+        vmn.Vamana.load_fresh_graph(str(rec), dim=128, R=64)
+    assert e.value.code == -5 and "772-byte records" in str(e.value)
+
+
+def struct_pack(*ints):
+    import struct
+    return struct.pack("<%di" % len(ints), *ints)

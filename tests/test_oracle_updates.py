@@ -87,3 +87,41 @@ def test_medoid_live_excludes_tombstones(vmn, orc):
     dele[m] = 1
     m2 = orc.medoid_live(v, dele)
     assert m2 != m and m2 >= 0
+
+
+def test_consolidation_keeps_live_edges_and_fills_maximally(vmn, orc):
+    """Properties that fix the repair rule: each repaired row starts with the
+    node's live out-neighbours in their old order; every other entry is a live
+    out-neighbour of a tombstoned out-neighbour, not occluded by an entry before
+    it (alpha * d(r, c) <= d(p, c)); and a row left short of R has every unused
+    candidate occluded by some entry of the row."""
+    v = _data(vmn, 3000, seed=12)
+    R, alpha = 24, 1.2
+    adj, med = orc.fresh_build(v, R=R, L=48, alphas=(1.0, 1.2))
+    dele = np.zeros(3000, np.uint8)
+    dele[np.random.default_rng(5).choice(3000, 450, replace=False)] = 1
+    before = adj.copy()
+    orc.consolidate(v, adj, dele, alpha=alpha)
+    # integer data: exact squared distances; the occlusion product is one
+    # float32 rounding, as in the oracle and the kernel
+    d2 = lambda a, b: np.float32(((v[a].astype(np.float64) - v[b]) ** 2).sum())
+    alpha = np.float32(alpha)
+    checked = 0
+    for p in range(3000):
+        row0 = before[p][before[p] >= 0]
+        if dele[p] or not dele[row0].any():
+            continue
+        live0 = [x for x in row0 if not dele[x]]
+        row = list(adj[p][adj[p] >= 0])
+        assert row[:len(live0)] == live0
+        cands = {w for x in row0 if dele[x] for w in before[x][before[x] >= 0]
+                 if not dele[w] and w != p and w not in live0}
+        for j in range(len(live0), len(row)):
+            c = row[j]
+            assert c in cands
+            assert all(alpha * d2(r, c) > d2(p, c) for r in row[:j])
+        if len(row) < R:
+            for c in cands - set(row):
+                assert any(alpha * d2(r, c) <= d2(p, c) for r in row)
+        checked += 1
+    assert checked > 500
