@@ -254,38 +254,56 @@ int fresh_pass(vmn_index* idx, int64_t base, int64_t cnt, float alpha) {
     a.counter = idx->counter.as<unsigned long long>();
     HIPCHK(vmn::launch_fresh_out_prune(a, prune_grid(idx, cnt), idx->stream));
 
-    // reverse pairs -> CSR by destination; per-node arrays are sized for the
-    // index capacity so that growing the index does not reallocate them
     const int64_t npairs = cnt * R;
-    const int64_t NC = std::max<int64_t>(idx->cap, N);
-    HIPCHK(idx->rcount.reserve((size_t)NC * sizeof(int32_t)));
-    HIPCHK(idx->rfill.reserve((size_t)NC * sizeof(int32_t)));
-    HIPCHK(idx->roff.reserve((size_t)(NC + 1) * sizeof(int64_t)));
-    HIPCHK(idx->rscan_tmp.reserve((size_t)((NC + 1023) / 1024 + 2) * sizeof(int64_t)));
-    HIPCHK(idx->rsrc.reserve((size_t)npairs * sizeof(int32_t)));
-    HIPCHK(idx->rd.reserve((size_t)npairs * sizeof(float)));
-    HIPCHK(idx->rlist.reserve((size_t)NC * sizeof(int64_t)));
-    HIPCHK(vmn::launch_fill_i32(idx->rcount.as<int32_t>(), N, 0, idx->stream));
-    HIPCHK(vmn::launch_fill_i32(idx->rfill.as<int32_t>(), N, 0, idx->stream));
-    HIPCHK(vmn::launch_reverse_count(idx->pair_dst.as<int32_t>(), npairs, idx->rcount.as<int32_t>(),
-                                     idx->stream));
-    HIPCHK(vmn::launch_exclusive_scan(idx->rcount.as<int32_t>(), N, idx->roff.as<int64_t>(),
-                                      idx->rscan_tmp.as<int64_t>(), idx->stream));
-    HIPCHK(vmn::launch_reverse_scatter(idx->pair_dst.as<int32_t>(), idx->pair_d.as<float>(), npairs,
-                                       R, base, idx->roff.as<int64_t>(), idx->rfill.as<int32_t>(),
-                                       idx->rsrc.as<int32_t>(), idx->rd.as<float>(), idx->stream));
-    rc = zero_counter(idx);
-    if (rc) return rc;
-    HIPCHK(vmn::launch_compact_nonzero(idx->rcount.as<int32_t>(), N, idx->rlist.as<int64_t>(),
-                                       idx->counter.as<unsigned long long>(), idx->stream));
     unsigned long long nlist = 0;
+    if (npairs * 8 < N) {
+        // a batch much smaller than the index: group the pairs by sorting them
+        const size_t tb = vmn::reverse_group_temp_bytes(npairs);
+        HIPCHK(idx->gtemp.reserve(tb));
+        HIPCHK(idx->rsrc.reserve((size_t)npairs * sizeof(int32_t)));
+        HIPCHK(idx->rd.reserve((size_t)npairs * sizeof(float)));
+        HIPCHK(idx->rlist.reserve((size_t)npairs * sizeof(int64_t)));
+        HIPCHK(idx->roff.reserve((size_t)(npairs + 1) * sizeof(int64_t)));
+        HIPCHK(vmn::launch_reverse_group(idx->pair_dst.as<int32_t>(), idx->pair_d.as<float>(), npairs,
+                                         R, base, idx->gtemp.p, idx->gtemp.bytes,
+                                         idx->rsrc.as<int32_t>(), idx->rd.as<float>(),
+                                         idx->rlist.as<int64_t>(), idx->roff.as<int64_t>(),
+                                         idx->counter.as<unsigned long long>(), idx->stream));
+        a.list_off = idx->roff.as<int64_t>();
+    } else {
+        // reverse pairs -> CSR by destination; per-node arrays are sized for
+        // the index capacity so that growing the index does not reallocate them
+        const int64_t NC = std::max<int64_t>(idx->cap, N);
+        HIPCHK(idx->rcount.reserve((size_t)NC * sizeof(int32_t)));
+        HIPCHK(idx->rfill.reserve((size_t)NC * sizeof(int32_t)));
+        HIPCHK(idx->roff.reserve((size_t)(NC + 1) * sizeof(int64_t)));
+        HIPCHK(idx->rscan_tmp.reserve((size_t)((NC + 1023) / 1024 + 2) * sizeof(int64_t)));
+        HIPCHK(idx->rsrc.reserve((size_t)npairs * sizeof(int32_t)));
+        HIPCHK(idx->rd.reserve((size_t)npairs * sizeof(float)));
+        HIPCHK(idx->rlist.reserve((size_t)NC * sizeof(int64_t)));
+        HIPCHK(vmn::launch_fill_i32(idx->rcount.as<int32_t>(), N, 0, idx->stream));
+        HIPCHK(vmn::launch_fill_i32(idx->rfill.as<int32_t>(), N, 0, idx->stream));
+        HIPCHK(vmn::launch_reverse_count(idx->pair_dst.as<int32_t>(), npairs,
+                                         idx->rcount.as<int32_t>(), idx->stream));
+        HIPCHK(vmn::launch_exclusive_scan(idx->rcount.as<int32_t>(), N, idx->roff.as<int64_t>(),
+                                          idx->rscan_tmp.as<int64_t>(), idx->stream));
+        HIPCHK(vmn::launch_reverse_scatter(idx->pair_dst.as<int32_t>(), idx->pair_d.as<float>(),
+                                           npairs, R, base, idx->roff.as<int64_t>(),
+                                           idx->rfill.as<int32_t>(), idx->rsrc.as<int32_t>(),
+                                           idx->rd.as<float>(), idx->stream));
+        rc = zero_counter(idx);
+        if (rc) return rc;
+        HIPCHK(vmn::launch_compact_nonzero(idx->rcount.as<int32_t>(), N, idx->rlist.as<int64_t>(),
+                                           idx->counter.as<unsigned long long>(), idx->stream));
+        a.rev_off = idx->roff.as<int64_t>();
+        a.list_off = nullptr;
+    }
     HIPCHK(hipMemcpyAsync(&nlist, idx->counter.p, sizeof nlist, hipMemcpyDeviceToHost, idx->stream));
     HIPCHK(hipStreamSynchronize(idx->stream));
     rc = zero_counter(idx);
     if (rc) return rc;
     a.n = (int64_t)nlist;
     a.node_list = idx->rlist.as<int64_t>();
-    a.rev_off = idx->roff.as<int64_t>();
     a.rev_src = idx->rsrc.as<int32_t>();
     a.rev_d = idx->rd.as<float>();
     HIPCHK(vmn::launch_fresh_rev_prune(a, prune_grid(idx, (int64_t)nlist), idx->stream));

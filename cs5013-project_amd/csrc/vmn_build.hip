@@ -313,7 +313,8 @@ __global__ __launch_bounds__(64) void fresh_rev_prune_kernel(PruneArgs a) {
         const int64_t i = next_item(a.counter, &s_q);
         if (i >= a.n) return;
         const int64_t node = a.node_list[i];
-        const int64_t o0 = a.rev_off[node], o1 = a.rev_off[node + 1];
+        const int64_t o0 = a.list_off ? a.list_off[i] : a.rev_off[node];
+        const int64_t o1 = a.list_off ? a.list_off[i + 1] : a.rev_off[node + 1];
         int cur = 0, cnt = 0;
         for (int64_t b = o0; b < o1; b += 64) {
             const int64_t j = b + l;

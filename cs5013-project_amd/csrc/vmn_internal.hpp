@@ -30,6 +30,7 @@ struct DevBuf {
     size_t bytes = 0;
     hipError_t reserve(size_t want) {
         if (want <= bytes) return hipSuccess;
+        const size_t old = bytes;
         if (p) {
             hipError_t e = hipFree(p);
             if (e != hipSuccess) return e;
@@ -39,7 +40,7 @@ struct DevBuf {
         // grow by at least half the old size: buffers sized by batch or node
         // count are not freed and reallocated (a device-wide sync) on every
         // slightly larger call
-        size_t b = std::max<size_t>(std::max<size_t>(want, 256), bytes + bytes / 2);
+        size_t b = std::max<size_t>(std::max<size_t>(want, 256), old + old / 2);
         hipError_t e = hipMalloc(&p, b);
         if (e == hipSuccess) bytes = b;
         return e;
@@ -74,7 +75,7 @@ struct vmn_index {
     // scratch
     vmn_rt::DevBuf qpad, out_ids, out_d, stat_hops, stat_ndist, counter;
     vmn_rt::DevBuf cand_ids, cand_d, vis_ids, vis_d, vis_cnt;
-    vmn_rt::DevBuf pair_dst, pair_d, rcount, rfill, roff, rscan_tmp, rsrc, rd, rlist;
+    vmn_rt::DevBuf pair_dst, pair_d, rcount, rfill, roff, rscan_tmp, rsrc, rd, rlist, gtemp;
     vmn_rt::DevBuf centroid, medoid_key;
     int64_t last_nq = 0;
     int64_t n_deleted = 0;  // tombstones set since the build (upper bound)

@@ -72,7 +72,8 @@ struct PruneArgs {
     int32_t* pair_dst;
     float* pair_d;
     // FRESH reverse prune: CSR of reverse candidates per node
-    const int64_t* rev_off;   // [N+1]
+    const int64_t* rev_off;   // [N+1], indexed by node
+    const int64_t* list_off;  // or [n+1], indexed by node_list position (when not null)
     const int32_t* rev_src;
     const float* rev_d;
     int M;                    // cap on merged candidates (reference 500)
@@ -96,6 +97,15 @@ hipError_t launch_exclusive_scan(const int32_t* counts, int64_t n, int64_t* off,
 hipError_t launch_reverse_scatter(const int32_t* pair_dst, const float* pair_d, int64_t npairs,
                                   int R, int64_t node_base, const int64_t* off, int32_t* fill,
                                   int32_t* rev_src, float* rev_d, hipStream_t s);
+// Reverse pairs grouped by destination with a stable radix sort (work in the
+// number of pairs, for batches much smaller than the node count): writes
+// rev_src / rev_d in destination order, node_list[k] / list_off[k] per
+// destination and *nseg (device). temp is sized by reverse_group_temp_bytes.
+size_t reverse_group_temp_bytes(int64_t npairs);
+hipError_t launch_reverse_group(const int32_t* pair_dst, const float* pair_d, int64_t npairs, int R,
+                                int64_t node_base, void* temp, size_t temp_bytes,
+                                int32_t* rev_src, float* rev_d, int64_t* node_list,
+                                int64_t* list_off, unsigned long long* nseg, hipStream_t s);
 hipError_t launch_compact_nonzero(const int32_t* counts, int64_t n, int64_t* list,
                                   unsigned long long* nlist, hipStream_t s);
 

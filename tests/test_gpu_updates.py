@@ -32,6 +32,30 @@ def test_gpu_insert_matches_oracle(vmn, orc, gpu):
         idx.insertPoints(v[:1])  # capacity exhausted
 
 
+def test_gpu_small_insert_batches_match_oracle(vmn, orc, gpu):
+    """Batches whose reverse pairs are far fewer than the nodes are grouped by a
+    device radix sort instead of a count/scan over every node (batch x R x 8 <
+    nodes); the graph must not depend on which grouping ran. Batches of 1, 3,
+    16 and 31 points go through the sort; 400 points through the node scan;
+    a tombstone makes the searches skip a node in between."""
+    v = _data(vmn, 9000, seed=6)
+    idx = vmn.Vamana(v[:8000], mode="fresh", R=32, L_build=64, alphas=(1.0, 1.2), capacity=9000)
+    oadj, med = orc.fresh_build(v[:8000], R=32, L=64, alphas=(1.0, 1.2))
+    full = np.full((9000, 32), -1, np.int32)
+    full[:8000] = oadj
+    dele = np.zeros(9000, np.uint8)
+    size = 8000
+    for b in (1, 3, 16, 31, 400, 16):
+        if b == 31:
+            idx.deleteIds(np.array([17, 4242], dtype=np.int32))
+            dele[[17, 4242]] = 1
+        idx.insertPoints(v[size:size + b])
+        orc.fresh_insert(v[:size + b], size, full[:size + b], med, L=64, alpha=1.2,
+                         deleted=dele[:size + b])
+        size += b
+        assert np.array_equal(idx.adjacency, full[:size]), b
+
+
 def test_gpu_delete_consolidate_matches_oracle(vmn, orc, gpu):
     v = _data(vmn, 6000, seed=4)
     q = _data(vmn, 80, seed=4, stream=1)
