@@ -62,7 +62,7 @@ def parse():
     ap.add_argument("--L-eval", type=int, default=60)
     ap.add_argument("--serve-n", type=int, default=10_000_000)
     ap.add_argument("--serve-rate", type=float, default=50_000.0, help="offered queries/s")
-    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-L", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
@@ -430,10 +430,24 @@ def serving(args, vmn, torch, dev):
         kinds.append(vmn.REQ_SEARCH if r < 0.9 else (vmn.REQ_INSERT if r < 0.95 else vmn.REQ_DELETE))
         sizes.append(int(np.exp(rng.uniform(0, np.log(256)))))
         t_arr.append(tcur)
-    alive = np.ones(n, dtype=bool)
     srv = vmn.Server(idx, k=10, L=args.serve_L, prec="bf16", rerank=40, max_batch=8192, n_slots=4)
+    # warm-up, untimed: searches, and inserts / deletes at the largest request
+    # size, so every buffer the engine and the update path grow is allocated
+    # before the trace starts
+    warm = [srv.submit(vmn.REQ_SEARCH, pool[:sz]) for sz in (1, 16, 256, 256)]
+    warm += [srv.submit(vmn.REQ_INSERT, extra[len(extra) - 256 * (j + 1):len(extra) - 256 * j])
+             for j in range(2)]
+    dead = rng.choice(n, 512, replace=False).astype(np.int32)
+    warm += [srv.submit(vmn.REQ_DELETE, dead[:256]), srv.submit(vmn.REQ_DELETE, dead[256:])]
+    alive = np.ones(n, dtype=bool)
+    alive[dead] = False
+    for t in warm:
+        while srv.poll(t) is None:
+            pass
+    eng0 = srv.stats()
     ins_off = 0
     outstanding, lat, done_q, results = [], [], 0, 0
+    lat_upd = {vmn.REQ_INSERT: [], vmn.REQ_DELETE: []}
     t_first = None
     start = srv.now() + 0.05
     log(f"serving: {len(t_arr)} requests over {args.serve_seconds} s, offered "
@@ -461,6 +475,8 @@ def serving(args, vmn, torch, dev):
             if k_ == vmn.REQ_SEARCH:
                 lat.append(r["t_done"] - r["t_arrival"])
                 done_q += s_
+            else:
+                lat_upd[k_].append(r["t_done"] - r["t_arrival"])
     t_last_arrival = start + t_arr[-1]
     while outstanding:
         r = srv.poll(outstanding[0][0])
@@ -470,17 +486,38 @@ def serving(args, vmn, torch, dev):
         if k_ == vmn.REQ_SEARCH:
             lat.append(r["t_done"] - r["t_arrival"])
             done_q += s_
+        else:
+            lat_upd[k_].append(r["t_done"] - r["t_arrival"])
     t_end = srv.now()
+    eng1 = srv.stats()
+    eng = {k: eng1[k] - eng0[k] for k in eng1}
+    eng["max_batch_queries"], eng["max_update_s"] = eng1["max_batch_queries"], eng1["max_update_s"]
     srv.close()
     lat_ms = np.array(lat) * 1e3
     thr = done_q / (t_end - start)
     log(f"serving: {done_q} queries, {thr:,.0f} q/s, p50 {np.percentile(lat_ms, 50):.2f} ms, "
         f"p99 {np.percentile(lat_ms, 99):.2f} ms")
+    upd = {name: {"n": len(v), "p50_ms": round(float(np.percentile(v, 50)) * 1e3, 3),
+                  "p99_ms": round(float(np.percentile(v, 99)) * 1e3, 3)}
+           for name, v in (("insert", lat_upd[vmn.REQ_INSERT]), ("delete", lat_upd[vmn.REQ_DELETE]))
+           if v}
+    engine = {"search_batches": eng["batches"],
+              "mean_batch_queries": round(eng["queries"] / max(eng["batches"], 1), 1),
+              "max_batch_queries": eng["max_batch_queries"],
+              "insert_ms_mean": round(eng["insert_s"] / max(eng["inserts"], 1) * 1e3, 3),
+              "delete_ms_mean": round(eng["delete_s"] / max(eng["deletes"], 1) * 1e3, 3),
+              "max_update_ms": round(eng["max_update_s"] * 1e3, 3),
+              "update_busy_frac": round((eng["insert_s"] + eng["delete_s"]) / (t_end - start), 4),
+              "drain_frac": round(eng["drain_s"] / (t_end - start), 4)}
+    log(f"serving: engine {engine}; update latency {upd}")
     # recall on a 10k-query FP64 subset of the final index (live points)
     q_gt = pool[:nq_gt]
     deleted = idx.deleted
     live = np.flatnonzero(~deleted)
-    allv = torch.from_numpy(np.concatenate([base, extra[:ins_off]]))
+    # node ids: base, the two warm-up inserts in submission order, the trace's inserts
+    m = len(extra)
+    allv = torch.from_numpy(np.concatenate([base, extra[m - 256:m], extra[m - 512:m - 256],
+                                            extra[:ins_off]]))
     gt_local, _ = vmn.groundtruth(allv[torch.from_numpy(live)].to(dev).contiguous(),
                                   torch.from_numpy(q_gt).to(dev), 10)
     gt = live[gt_local.cpu().numpy()]
@@ -490,8 +527,15 @@ def serving(args, vmn, torch, dev):
     ids32, _ = idx.searchPoints(torch.from_numpy(q_gt).to(dev), k=10, L=args.serve_L)
     rec32 = vmn.recall_at_k(ids32.cpu().numpy(), gt, 10)
     log(f"serving: recall@10 {rec:.4f} (BF16+rerank), {rec32:.4f} (FP32)")
+    # the same index's recall at wider beams (context for the configured L)
+    curve = {}
+    for Lw in (128, 256):
+        ids_w, _ = idx.searchPoints(torch.from_numpy(q_gt).to(dev), k=10, L=Lw, prec="bf16",
+                                    rerank=40)
+        curve[Lw] = round(vmn.recall_at_k(ids_w.cpu().numpy(), gt, 10), 4)
+    log(f"serving: recall@10 at L=128 / 256: {curve[128]:.4f} / {curve[256]:.4f}")
     out = {"metric": "served index: queries/s with p50/p99 latency", "value": round(thr, 1),
-           "unit": "queries/s", "n_gpus": 1, "steps": len(t_arr), "warmup": 0,
+           "unit": "queries/s", "n_gpus": 1, "steps": len(t_arr), "warmup": len(warm),
            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "bf16", "data": "synthetic: seeded 1,024-component mixture, unit-normalised",
            "config": {"workload": f"served Vamana {n}x{d}, Poisson {args.serve_rate:,.0f} "
@@ -501,7 +545,11 @@ def serving(args, vmn, torch, dev):
                       "deleted": int((~alive).sum()), "L_search": args.serve_L,
                       "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
                       "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
-                      "recall@10_bf16_rerank": round(rec, 4), "recall@10_fp32": round(rec32, 4)}}
+                      "p90_ms": round(float(np.percentile(lat_ms, 90)), 3),
+                      "p999_ms": round(float(np.percentile(lat_ms, 99.9)), 3),
+                      "update_latency": upd, "engine": engine,
+                      "recall@10_bf16_rerank": round(rec, 4), "recall@10_fp32": round(rec32, 4),
+                      "recall@10_bf16_rerank_at_L": curve}}
     print(json.dumps(out), flush=True)
 
 

@@ -84,6 +84,8 @@ struct vmn_server {
     std::atomic<bool> stop{false};
     std::atomic<int64_t> in_flight{0};
     std::string fatal;
+    vmn_server_stats counters{};    // engine counters (engine thread writes, under m)
+    double drain_from = -1.0;       // when the queue head became an update waiting on slots
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
@@ -144,10 +146,16 @@ struct vmn_server {
         s.reqs = batch;
         s.nq = nq;
         s.busy = true;
+        std::lock_guard<std::mutex> g(m);
+        counters.batches++;
+        counters.queries += nq;
+        if (nq > counters.max_batch_queries) counters.max_batch_queries = nq;
     }
 
     void apply_update(Request* r) {
         int rc;
+        const double t = now();
+        bool consolidated = false;
         if (r->kind == VMN_REQ_INSERT) {
             r->ids.resize(r->n);
             rc = vmn_insert(idx, r->f.data(), r->n, r->ids.data(), VMN_MEM_HOST);
@@ -158,8 +166,22 @@ struct vmn_server {
                 if (p.consolidate_every > 0 && deletes_since >= p.consolidate_every) {
                     rc = vmn_consolidate(idx);
                     deletes_since = 0;
+                    consolidated = true;
                 }
             }
+        }
+        {
+            const double dt = now() - t;
+            std::lock_guard<std::mutex> g(m);
+            if (r->kind == VMN_REQ_INSERT) {
+                counters.inserts++;
+                counters.insert_s += dt;
+            } else {
+                counters.deletes++;
+                counters.delete_s += dt;
+            }
+            counters.consolidations += consolidated;
+            if (dt > counters.max_update_s) counters.max_update_s = dt;
         }
         finish(r, rc, rc == VMN_OK ? nullptr : vmn_last_error());
     }
@@ -177,9 +199,14 @@ struct vmn_server {
             Request* head = queue.front();
             if (head->kind != VMN_REQ_SEARCH) {
                 if (any_busy()) {  // ordered write: let launched searches finish
+                    if (drain_from < 0) drain_from = now();
                     lk.unlock();
                     std::this_thread::sleep_for(std::chrono::microseconds(10));
                     continue;
+                }
+                if (drain_from >= 0) {
+                    counters.drain_s += now() - drain_from;
+                    drain_from = -1.0;
                 }
                 queue.pop_front();
                 lk.unlock();
@@ -319,6 +346,14 @@ int vmn_server_poll(vmn_server* s, int64_t ticket, int32_t* ids_out, float* dist
 }
 
 int64_t vmn_server_pending(const vmn_server* s) { return s ? s->in_flight.load() : -1; }
+
+int vmn_server_get_stats(const vmn_server* s, vmn_server_stats* out) {
+    if (!s || !out) return -1;
+    auto* ms = const_cast<vmn_server*>(s);
+    std::lock_guard<std::mutex> g(ms->m);
+    *out = s->counters;
+    return VMN_OK;
+}
 
 int vmn_server_destroy(vmn_server* s) {
     if (!s) return VMN_OK;

@@ -42,6 +42,15 @@ class Params(ctypes.Structure):
                 ("device", ctypes.c_int32), ("low_precision", ctypes.c_int32)]
 
 
+class ServerStats(ctypes.Structure):
+    """Mirror of `vmn_server_stats`."""
+    _fields_ = [("batches", ctypes.c_int64), ("queries", ctypes.c_int64),
+                ("max_batch_queries", ctypes.c_int64), ("inserts", ctypes.c_int64),
+                ("deletes", ctypes.c_int64), ("consolidations", ctypes.c_int64),
+                ("insert_s", ctypes.c_double), ("delete_s", ctypes.c_double),
+                ("max_update_s", ctypes.c_double), ("drain_s", ctypes.c_double)]
+
+
 class ServerParams(ctypes.Structure):
     """Mirror of `vmn_server_params`."""
     _fields_ = [("k", ctypes.c_int32), ("L", ctypes.c_int32), ("prec", ctypes.c_int32),
@@ -113,6 +122,8 @@ def lib():
         L.vmn_server_poll.argtypes = [P, i64, P, P, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double)]
         L.vmn_server_pending.argtypes = [P]
+        L.vmn_server_get_stats.argtypes = [P, ctypes.POINTER(ServerStats)]
+        L.vmn_server_get_stats.restype = ctypes.c_int
         L.vmn_server_pending.restype = i64
         L.vmn_server_destroy.argtypes = [P]
         L.vmn_last_error.argtypes = []
@@ -557,6 +568,12 @@ class Server:
 
     def pending(self):
         return int(lib().vmn_server_pending(self._h))
+
+    def stats(self):
+        """Engine counters (vmn_server_stats) as a dict."""
+        st = ServerStats()
+        _check(lib().vmn_server_get_stats(self._h, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in ServerStats._fields_}
 
     def close(self):
         if self._h:

@@ -258,6 +258,18 @@ int vmn_server_submit(vmn_server* srv, int32_t kind, const void* data, int64_t n
 int vmn_server_poll(vmn_server* srv, int64_t ticket, int32_t* ids_out, float* dists_out,
                     double* t_arrival, double* t_done);
 int64_t vmn_server_pending(const vmn_server* srv);
+
+/* Engine counters since the server was created: where the engine's time goes
+ * (no counterpart in the reference's request loop, engine.cu:407-568). */
+typedef struct vmn_server_stats {
+    int64_t batches, queries, max_batch_queries; /* search launches and their sizes */
+    int64_t inserts, deletes, consolidations;    /* updates applied */
+    double insert_s, delete_s;                   /* engine time applying them (incl. consolidation) */
+    double max_update_s;                         /* longest single update */
+    double drain_s;                              /* time updates waited for launched searches */
+} vmn_server_stats;
+int vmn_server_get_stats(const vmn_server* srv, vmn_server_stats* out);
+
 /* Finishes the queued requests, then stops the engine. */
 int vmn_server_destroy(vmn_server* srv);
 

@@ -45,7 +45,15 @@ def test_server_mixed_equals_alone(vmn, gpu, prec, rr):
                 r = srv.poll(t)
                 if r is not None:
                     out[t] = r
+    st = srv.stats()
     srv.close()
+    # the engine's counters account for every request
+    n_search = sum(len(d) for k, d in reqs if k == vmn.REQ_SEARCH)
+    assert st["queries"] == n_search and 1 <= st["batches"] <= n_search
+    assert st["max_batch_queries"] <= 512
+    assert st["inserts"] == sum(k == vmn.REQ_INSERT for k, _ in reqs)
+    assert st["deletes"] == sum(k == vmn.REQ_DELETE for k, _ in reqs)
+    assert st["insert_s"] > 0 and st["max_update_s"] <= st["insert_s"] + st["delete_s"]
     # replay alone, in arrival order, on an identically built index
     ref = vmn.Vamana(base, **kw)
     for t, (kind, data) in zip(tickets, reqs):
