@@ -97,7 +97,7 @@ int refresh_lowp(vmn_index* idx, int64_t from, int64_t to) {
     const int64_t cap = idx->cap, cnt = to - from;
     if (cnt <= 0) return VMN_OK;
     const float* src = idx->vecs.as<float>() + from * DP;
-    if (idx->p.low_precision & VMN_STORE_BF16) {
+    if ((idx->p.low_precision & VMN_STORE_BF16) || idx->exact_bf16) {
         HIPCHK(idx->bf16_rows.reserve((size_t)cap * DP * 2));
         HIPCHK(idx->bf16_norms.reserve((size_t)cap * sizeof(float)));
         HIPCHK(vmn::launch_to_bf16(src, cnt, DP, static_cast<char*>(idx->bf16_rows.p) + from * DP * 2,
@@ -122,6 +122,24 @@ int refresh_lowp(vmn_index* idx, int64_t from, int64_t to) {
     }
     return VMN_OK;
 }
+
+int update_exact(vmn_index* idx, int64_t from, int64_t to) {
+    if (from == 0) idx->exact_bf16 = true;
+    if (!idx->exact_bf16 || to <= from) return VMN_OK;
+    HIPCHK(idx->exact_flag.reserve(64));
+    HIPCHK(vmn::launch_fill_i32(idx->exact_flag.as<int32_t>(), 1, 1, idx->stream));
+    HIPCHK(vmn::launch_bf16_exact(idx->vecs.as<float>(), from, to, idx->DP,
+                                  idx->exact_flag.as<unsigned int>(), idx->stream));
+    unsigned int ok = 0;
+    HIPCHK(hipMemcpyAsync(&ok, idx->exact_flag.p, sizeof ok, hipMemcpyDeviceToHost, idx->stream));
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    idx->exact_bf16 = ok != 0;
+    return VMN_OK;
+}
+
+// Traversal precision of the build and insert searches: BF16 rows when every
+// row makes it exact (the same graph as FP32, half the row bytes).
+int build_prec(const vmn_index* idx) { return idx->exact_bf16 ? VMN_PREC_BF16 : VMN_PREC_FP32; }
 
 // Search kernel over `nq` padded query rows.
 int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int32_t* out_ids,
@@ -198,7 +216,8 @@ int build_parth(vmn_index* idx, bool random_init) {
     HIPCHK(idx->cand_d.reserve((size_t)n * L * sizeof(float)));
     for (int pass = 0; pass < idx->p.passes; ++pass) {
         int rc = run_search(idx, idx->vecs.as<float>(), n, L, L, idx->cand_ids.as<int32_t>(),
-                            idx->cand_d.as<float>(), true, VMN_MODE_PARTH, false);
+                            idx->cand_d.as<float>(), true, VMN_MODE_PARTH, false, 0,
+                            build_prec(idx));
         if (rc) return rc;
         rc = zero_counter(idx);
         if (rc) return rc;
@@ -227,7 +246,7 @@ int fresh_pass(vmn_index* idx, int64_t base, int64_t cnt, float alpha) {
     const int R = idx->p.R;
     const int64_t N = idx->n;
     int rc = run_search(idx, idx->vecs.as<float>() + base * idx->DP, cnt, idx->p.L_build, 0,
-                        nullptr, nullptr, true, VMN_MODE_FRESH, true, base);
+                        nullptr, nullptr, true, VMN_MODE_FRESH, true, base, build_prec(idx));
     if (rc) return rc;
     HIPCHK(idx->pair_dst.reserve((size_t)cnt * R * sizeof(int32_t)));
     HIPCHK(idx->pair_d.reserve((size_t)cnt * R * sizeof(float)));
@@ -468,10 +487,12 @@ int build_common(vmn_index* idx, const float* vecs, int64_t n, int32_t mem,
                               hipMemcpyHostToDevice, idx->stream));
     rc = compute_medoid(idx);
     if (rc) return rc;
-    rc = idx->p.mode == VMN_MODE_PARTH ? build_parth(idx, !init) : build_fresh(idx, !init);
+    rc = update_exact(idx, 0, n);
     if (rc) return rc;
     idx->fp8_scaled = false;
     rc = refresh_lowp(idx, 0, n);
+    if (rc) return rc;
+    rc = idx->p.mode == VMN_MODE_PARTH ? build_parth(idx, !init) : build_fresh(idx, !init);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(idx->stream));
     idx->built = true;
@@ -556,6 +577,8 @@ int vmn_load_graph(vmn_index* idx, const float* vecs, int64_t n, const int32_t* 
     } else {
         idx->medoid = medoid;
     }
+    rc = update_exact(idx, 0, n);
+    if (rc) return rc;
     idx->fp8_scaled = false;
     rc = refresh_lowp(idx, 0, n);
     if (rc) return rc;
@@ -657,6 +680,9 @@ int vmn_get_adjacency(const vmn_index* idx, int32_t* out, int32_t mem) {
 
 int64_t vmn_size(const vmn_index* idx) { return idx ? idx->n : -1; }
 int64_t vmn_medoid(const vmn_index* idx) { return idx ? idx->medoid : -1; }
+int32_t vmn_build_precision(const vmn_index* idx) {
+    return idx ? build_prec(idx) : VMN_PREC_FP32;
+}
 void* vmn_stream(const vmn_index* idx) { return idx ? (void*)idx->stream : nullptr; }
 
 int vmn_search_stats(const vmn_index* idx, int64_t* expansions, int64_t* dist_evals) {
@@ -762,10 +788,12 @@ int vmn_insert(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out, i
     HIPCHK(vmn::launch_fill_i32(idx->adj.as<int32_t>() + base * idx->p.R, n * idx->p.R, -1,
                                 idx->stream));
     idx->n = base + n;
-    const int last = idx->p.passes > 0 ? idx->p.passes - 1 : 0;
-    rc = fresh_pass(idx, base, n, idx->p.alpha[last]);
+    rc = update_exact(idx, base, base + n);
     if (rc) return rc;
     rc = refresh_lowp(idx, base, base + n);
+    if (rc) return rc;
+    const int last = idx->p.passes > 0 ? idx->p.passes - 1 : 0;
+    rc = fresh_pass(idx, base, n, idx->p.alpha[last]);
     if (rc) return rc;
     std::vector<int32_t> ids(n);
     for (int64_t i = 0; i < n; ++i) ids[i] = (int32_t)(base + i);

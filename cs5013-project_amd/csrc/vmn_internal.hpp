@@ -82,6 +82,10 @@ struct vmn_index {
     vmn_rt::DevBuf staging, find_out, bad;
     vmn_rt::DevBuf bf16_rows, bf16_norms, fp8_rows, fp8_norms, fp8_scales, fp8_absmax;
     bool fp8_scaled = false;
+    // every row so far makes BF16 traversal exact: the build and insert
+    // searches then read BF16 rows (half the bytes, bit-identical result)
+    bool exact_bf16 = false;
+    vmn_rt::DevBuf exact_flag;
     int waves_per_cu = 32;  // search grid = CUs x waves_per_cu (persistent work queue)
     int list_cap = 0;       // minimum beam-list capacity of the search kernel
 };
@@ -108,6 +112,9 @@ int compute_medoid(vmn_index* idx);
 int alloc_graph(vmn_index* idx, int64_t n);
 // (Re)computes the BF16 / FP8 copies of rows [from, to).
 int refresh_lowp(vmn_index* idx, int64_t from, int64_t to);
+// Keeps idx->exact_bf16 current after rows [from, to) were written (from == 0
+// starts over).
+int update_exact(vmn_index* idx, int64_t from, int64_t to);
 // Host copy of an adjacency given by a caller ([n x row_width], -1 padded):
 // each row keeps its entries up to the first -1, in order, without repeated
 // ids (and without i itself when drop_self), -1 padded to width R.

@@ -170,6 +170,8 @@ int vmn_load(const char* path, int32_t device, vmn_index** out) {
                           hipMemcpyHostToDevice, idx->stream));
     idx->medoid = medoid;
     idx->n_deleted = n_deleted;
+    rc = update_exact(idx, 0, n);
+    if (rc) return rc;
     if (fp8) {
         HIPCHK(idx->fp8_scales.reserve((size_t)DP * sizeof(float)));
         HIPCHK(hipMemcpyAsync(idx->fp8_scales.p, scales.data(), scales.size() * sizeof(float),

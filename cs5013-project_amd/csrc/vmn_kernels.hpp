@@ -39,6 +39,9 @@ struct SearchArgs {
 };
 
 hipError_t launch_search(const SearchArgs& a, int grid, hipStream_t s);
+// Clears *flag unless every row of [from, to) makes BF16 traversal exact.
+hipError_t launch_bf16_exact(const float* vecs, int64_t from, int64_t to, int DP, unsigned int* flag,
+                             hipStream_t s);
 hipError_t launch_to_bf16(const float* vecs, int64_t n, int DP, void* rows, float* norms,
                           hipStream_t s);
 hipError_t launch_fp8_scales(const float* vecs, int64_t n, int D, int DP, unsigned int* absmax_bits,

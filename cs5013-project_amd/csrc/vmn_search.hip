@@ -285,6 +285,38 @@ hipError_t launch_search(const SearchArgs& a, int grid, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // Low-precision copies of the rows.
 // ---------------------------------------------------------------------------
+
+// Rows on which BF16 traversal equals FP32 traversal bit for bit: every value
+// an integer with |v| <= 256 (exact in BF16) and ||x||^2 <= 2^22. Between two
+// such rows every product, dot product and norm, ||q||^2 + ||x||^2 - 2 q.x,
+// and every partial sum of sum (x - q)^2 (at most (||x|| + ||q||)^2 <= 2^24)
+// is an integer an FP32 holds exactly, in any summation order. Clears *flag
+// when a row of [from, to) is not such a row. One wavefront per row.
+__global__ void bf16_exact_kernel(const float* __restrict__ vecs, int64_t from, int64_t to, int DP,
+                                  unsigned int* flag) {
+    const int64_t row = from + (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int l = threadIdx.x & 63;
+    if (row >= to) return;
+    bool ok = true;
+    double s = 0.0;
+    for (int k = l; k < DP; k += 64) {
+        const float v = vecs[row * DP + k];
+        ok = ok && v == rintf(v) && fabsf(v) <= 256.0f;
+        s += (double)v * (double)v;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    const uint64_t bad = ballot(!ok);
+    if (l == 0 && (bad || s > 4194304.0)) atomicAnd(flag, 0u);
+}
+hipError_t launch_bf16_exact(const float* vecs, int64_t from, int64_t to, int DP, unsigned int* flag,
+                             hipStream_t s) {
+    if (to <= from) return hipSuccess;
+    const int64_t rows = to - from;
+    hipLaunchKernelGGL(bf16_exact_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, vecs,
+                       from, to, DP, flag);
+    return hipGetLastError();
+}
+
 __global__ void to_bf16_kernel(const float* __restrict__ vecs, int64_t n, int DP, __bf16* rows,
                                float* norms) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -96,6 +96,8 @@ def lib():
         L.vmn_size.restype = i64
         L.vmn_medoid.argtypes = [P]
         L.vmn_medoid.restype = i64
+        L.vmn_build_precision.argtypes = [P]
+        L.vmn_build_precision.restype = i32
         L.vmn_search_stats.argtypes = [P, ctypes.POINTER(i64), ctypes.POINTER(i64)]
         L.vmn_stream.argtypes = [P]
         L.vmn_stream.restype = P
@@ -320,6 +322,13 @@ class Vamana:
     @property
     def stream(self):
         return lib().vmn_stream(self._h)
+
+    @property
+    def build_precision(self):
+        """Rows the build and insert searches read: "bf16" while every row is
+        exact in BF16 (integers, |v| <= 256, squared norm <= 2^22), else "fp32".
+        Both give the same graph."""
+        return {PREC["bf16"]: "bf16", PREC["fp32"]: "fp32"}[lib().vmn_build_precision(self._h)]
 
     def searchPoints(self, queries, k=10, L=None, ids_out=None, dists_out=None, sync=True,
                      prec="fp32", rerank=0):

@@ -193,6 +193,14 @@ int vmn_get_params(const vmn_index* idx, vmn_params* out);
 int64_t vmn_size(const vmn_index* idx);
 int64_t vmn_medoid(const vmn_index* idx);
 
+/* Row precision the build and insert searches traverse: VMN_PREC_BF16 while
+ * every row is an integer vector with |v| <= 256 and squared norm <= 2^22
+ * (BF16 distances then equal FP32 ones bit for bit, so the graph is the same
+ * as an FP32 traversal's at half the row bytes), else VMN_PREC_FP32. The
+ * reference's build and insert searches always read its FP32 rows
+ * (greedySearch calls, vamana_impl.cuh:235, :325). */
+int32_t vmn_build_precision(const vmn_index* idx);
+
 /* Sum over the queries of the last vmn_search: expansions (hops) and distance
  * evaluations. Synchronises the index's stream. */
 int vmn_search_stats(const vmn_index* idx, int64_t* expansions, int64_t* dist_evals);
