@@ -53,11 +53,14 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--rerank", type=int, default=40)
     ap.add_argument("--workload", default="search",
-                    choices=["search", "streaming", "serving", "copy"],
+                    choices=["search", "streaming", "serving", "copy", "engine"],
                     help="search: the headline metric; streaming: 10 rounds of 5%% deletes + "
                          "5%% inserts with consolidation, recall@10 per round; serving: the "
                          "continuous-batching request loop; copy: a 1 GiB device-to-device copy "
-                         "(the calibration kernel for the HBM traffic counters)")
+                         "(the calibration kernel for the HBM traffic counters); engine: the "
+                         "parth engine's replay of its own SIFT workload file (batches of 50 "
+                         "mixed operations), the reference's qps / recall_10 metric")
+    ap.add_argument("--engine-reps", type=int, default=5)
     ap.add_argument("--rounds", type=int, default=10)
     ap.add_argument("--L-eval", type=int, default=60)
     ap.add_argument("--serve-n", type=int, default=10_000_000)
@@ -83,6 +86,8 @@ def main():
         return serving(args, vmn, torch, dev)
     if args.workload == "copy":
         return copy_calibration(args, torch, dev)
+    if args.workload == "engine":
+        return engine_replay(args, vmn, torch)
     n, nq, d = args.n, args.nq, 128
     alphas = tuple(float(x) for x in args.alphas.split(","))
     log(f"generating {n}x{d} base + {nq} queries (256-component integer mixture)")
@@ -397,6 +402,79 @@ def streaming(args, vmn, torch, dev):
                       "R": args.R, "L_build": args.L_build, "alpha": 1.2,
                       "L_eval": [args.L_eval, args.L_build],
                       "recall_round0": [round(r0[0], 4), round(r0[1], 4)], "rounds": rounds}}
+    print(json.dumps(out), flush=True)
+
+
+def engine_replay(args, vmn, torch):
+    """The parth engine's automated sweep (parth/src/engine.cu:578-600) on the
+    reference's own SIFT workload (1,000 operations: 727 searches, 183 inserts,
+    90 deletes, batches of 50) over its initial graph (SIFT10K, R=32), beams
+    10..100, each replayed --engine-reps times from the initial graph. Like the
+    reference, qps is searches over the search launches' device time (events
+    around each batch's search launch); the reference's engine printed 849.68
+    QPS at recall@10 0.9105 for beam 100 (parth/vis_outputs/sift/metrics.json),
+    its only published figure for this metric. The CPU baseline is the oracle's
+    replay of the same operations (search loops timed)."""
+    golden = os.path.join(ROOT, "tests", "golden")
+    w = np.load(os.path.join(golden, "parth_workloads.npz"))
+    ref = json.load(open(os.path.join(golden, "parth_metrics.json")))["sift"]
+    sift = np.load(os.path.join(golden, "sift10k.npz"))
+    base = sift["base"].astype(np.float32)
+    adj = sift["parth_adj"]
+    t, i, ov = w["sift_types"], w["sift_ids"], w["sift_vecs"]
+    sweep = []
+    for L in (10, 20, 32, 64, 100):
+        reps = []
+        for _ in range(args.engine_reps):
+            idx = vmn.Vamana.from_graph(base, adj, medoid=0, mode="parth",
+                                        capacity=ref["max_capacity"])
+            reps.append(idx.run_workload(t, i, ov, ref["batch_size"], L))
+            idx.close()
+        reps.sort(key=lambda r: r["qps"])
+        r = reps[len(reps) // 2]
+        sweep.append({"L": L, "qps": round(r["qps"], 1), "recall@10": round(r["recall"], 4),
+                      "search_ms": round(r["search_ms"], 3),
+                      "replay_ops_per_s": round(len(t) / r["wall_s"], 1),
+                      "hops_per_search": round(r["expansions"] / r["searches"], 1),
+                      "dists_per_search": round(r["dist_evals"] / r["searches"], 1),
+                      "batches": r["batches"]})
+        log(f"engine L={L}: qps {r['qps']:,.0f}, recall@10 {r['recall']:.4f}, "
+            f"search {r['search_ms']:.2f} ms over {r['batches']} batches")
+    top = sweep[-1]
+    # algorithmic bytes of the search launches at beam 100: per search the
+    # query, each expansion's adjacency row and each distance's row
+    d, R = 128, int(adj.shape[1])
+    per_q = 4 * d + top["hops_per_search"] * 4 * R + top["dists_per_search"] * 4 * d + 40
+    n_search = int((t == 2).sum())
+    achieved = per_q * n_search / (top["search_ms"] * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        st = {}
+        orc.engine_run(base, adj, len(base), ref["max_capacity"], t, i, ov, ref["batch_size"],
+                       100, stats=st)
+        cpu = {"value": round(n_search / st["search_s"], 1), "unit": "queries/s",
+               "cores": orc.num_threads(), "kind": "port",
+               "sample": f"the oracle's replay of the same {len(t)} operations at beam 100; "
+                         f"its {n_search} searches timed (OpenMP over each batch's searches)"}
+        log(f"engine cpu baseline {cpu['value']:,.0f} searches/s")
+    out = {"metric": "parth engine qps (searches / search device time) at beam 100",
+           "value": top["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.engine_reps,
+           "warmup": 0, "ms_per_step": top["search_ms"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": round(top["qps"] / ref["qps"], 1), "dtype": "fp32",
+           "data": "the reference's SIFT10K engine workload (parth/data/sift_workload.bin) "
+                   "and initial graph (parth/data/sift_initial_graph.bin)",
+           "config": {"workload": "parth engine replay, SIFT10K R=32, 1000 ops in batches of "
+                                  "50 (727 searches, 183 inserts, 90 deletes), k=10",
+                      "recall@10": top["recall@10"], "reference": {"qps": ref["qps"],
+                      "recall_10": ref["recall_10"]}, "sweep": sweep},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                        "traffic": None,
+                        "note": "about 36 searches per launch: one wavefront each, so the "
+                                "launch is a chain of dependent hops (latency), not bandwidth"},
+           "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 

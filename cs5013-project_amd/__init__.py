@@ -8,10 +8,12 @@ Import with importlib (the directory name is not an identifier):
 """
 from .vamana import (MODE_FRESH, MODE_PARTH, REQ_DELETE, REQ_INSERT, REQ_SEARCH, Params,
                      SearchContext, Server, Vamana, VamanaError, default_params, groundtruth, lib,
-                     recall_at_k)
+                     read_workload, recall_at_k)
+from .engine import VamanaEngine
 from . import datasets, tuning
 
-__all__ = ["Vamana", "VamanaError", "Params", "SearchContext", "Server", "tuning",
+__all__ = ["Vamana", "VamanaEngine", "VamanaError", "Params", "SearchContext", "Server", "tuning",
+           "read_workload",
            "default_params", "groundtruth", "lib",
            "recall_at_k", "datasets", "MODE_PARTH", "MODE_FRESH", "REQ_SEARCH", "REQ_INSERT",
            "REQ_DELETE"]

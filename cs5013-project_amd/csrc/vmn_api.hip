@@ -83,6 +83,8 @@ int alloc_graph(vmn_index* idx, int64_t n) {
     const int64_t cap = std::max<int64_t>(idx->p.capacity, n);
     idx->cap = cap;
     idx->n_deleted = 0;
+    idx->freelist.clear();
+    idx->free_count = 0;
     HIPCHK(idx->vecs.reserve((size_t)cap * idx->DP * sizeof(float)));
     HIPCHK(idx->adj.reserve((size_t)cap * idx->p.R * sizeof(int32_t)));
     HIPCHK(idx->del_bits.reserve((size_t)((cap + 31) / 32) * sizeof(uint32_t)));
@@ -141,11 +143,9 @@ int update_exact(vmn_index* idx, int64_t from, int64_t to) {
 // row makes it exact (the same graph as FP32, half the row bytes).
 int build_prec(const vmn_index* idx) { return idx->exact_bf16 ? VMN_PREC_BF16 : VMN_PREC_FP32; }
 
-// Search kernel over `nq` padded query rows.
 int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int32_t* out_ids,
-               float* out_d, bool queries_are_nodes, int mode, bool want_vis,
-               int64_t node_base = 0, int prec = VMN_PREC_FP32, int krr = 0,
-               vmn_ctx* ctx = nullptr) {
+               float* out_d, bool queries_are_nodes, int mode, bool want_vis, int64_t node_base,
+               int prec, int krr, vmn_ctx* ctx) {
     hipStream_t stream = ctx ? ctx->stream : idx->stream;
     DevBuf& counter = ctx ? ctx->counter : idx->counter;
     DevBuf& stat_hops = ctx ? ctx->stat_hops : idx->stat_hops;

@@ -86,6 +86,12 @@ struct vmn_index {
     // searches then read BF16 rows (half the bytes, bit-identical result)
     bool exact_bf16 = false;
     vmn_rt::DevBuf exact_flag;
+    // parth engine updates: free list of deleted ids and its counter (signed:
+    // the engine's counter can run below zero), reset by alloc_graph
+    std::vector<int32_t> freelist;
+    int64_t free_count = 0;
+    vmn_rt::DevBuf eng_ids, eng_src, eng_q, eng_best, eng_marks, eng_found, eng_gt_i, eng_gt_d,
+        eng_part_d, eng_part_i;
     int waves_per_cu = 32;  // search grid = CUs x waves_per_cu (persistent work queue)
     int list_cap = 0;       // minimum beam-list capacity of the search kernel
 };
@@ -121,5 +127,10 @@ int update_exact(vmn_index* idx, int64_t from, int64_t to);
 int normalize_rows(const int32_t* src, int64_t n, int row_width, int R, bool drop_self,
                    int32_t mem, std::vector<int32_t>& rows);
 int check_params(const vmn_params* p);
+// Search kernel over `nq` padded query rows [nq x DP] from idx->medoid.
+int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int32_t* out_ids,
+               float* out_d, bool queries_are_nodes, int mode, bool want_vis,
+               int64_t node_base = 0, int prec = VMN_PREC_FP32, int krr = 0,
+               vmn_ctx* ctx = nullptr);
 
 }  // namespace vmn_rt

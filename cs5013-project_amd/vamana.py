@@ -51,6 +51,14 @@ class ServerStats(ctypes.Structure):
                 ("max_update_s", ctypes.c_double), ("drain_s", ctypes.c_double)]
 
 
+class EngineResult(ctypes.Structure):
+    _fields_ = [("searches", ctypes.c_int64), ("inserts", ctypes.c_int64),
+                ("deletes", ctypes.c_int64), ("batches", ctypes.c_int64),
+                ("search_ms", ctypes.c_double), ("qps", ctypes.c_double),
+                ("recall", ctypes.c_double), ("wall_s", ctypes.c_double),
+                ("expansions", ctypes.c_int64), ("dist_evals", ctypes.c_int64)]
+
+
 class ServerParams(ctypes.Structure):
     """Mirror of `vmn_server_params`."""
     _fields_ = [("k", ctypes.c_int32), ("L", ctypes.c_int32), ("prec", ctypes.c_int32),
@@ -128,6 +136,11 @@ def lib():
         L.vmn_server_get_stats.restype = ctypes.c_int
         L.vmn_server_pending.restype = i64
         L.vmn_server_destroy.argtypes = [P]
+        L.vmn_read_workload.argtypes = [ctypes.c_char_p, i32, P, P, P, i64, ctypes.POINTER(i64)]
+        L.vmn_engine_delete.argtypes = [P, P, i64]
+        L.vmn_engine_insert.argtypes = [P, P, i64, i32, P, i32]
+        L.vmn_engine_run.argtypes = [P, P, P, P, i64, i32, i32, i32, ctypes.POINTER(EngineResult),
+                                     P, P]
         L.vmn_last_error.argtypes = []
         L.vmn_last_error.restype = ctypes.c_char_p
         for name in ("vmn_create", "vmn_destroy", "vmn_build", "vmn_build_from", "vmn_load_graph",
@@ -139,7 +152,8 @@ def lib():
                      "vmn_get_deleted", "vmn_search_ex", "vmn_get_lowp_rows", "vmn_ctx_create",
                      "vmn_ctx_destroy", "vmn_ctx_search", "vmn_ctx_capture", "vmn_ctx_replay",
                      "vmn_set_launch", "vmn_server_create", "vmn_server_submit",
-                     "vmn_server_poll", "vmn_server_destroy"):
+                     "vmn_server_poll", "vmn_server_destroy", "vmn_read_workload",
+                     "vmn_engine_delete", "vmn_engine_insert", "vmn_engine_run"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -420,6 +434,43 @@ class Vamana:
         _check(lib().vmn_get_adjacency(self._h, out.ctypes.data_as(ctypes.c_void_p), MEM_HOST))
         return out
 
+    def engine_delete(self, ids):
+        """parth engine delete (engine.cu:131-147): mark and push on the free list."""
+        a = np.ascontiguousarray(ids, dtype=np.int32)
+        _check(lib().vmn_engine_delete(self._h, a.ctypes.data_as(ctypes.c_void_p), len(a)))
+
+    def engine_insert(self, vectors, link_L=128):
+        """parth engine insert (engine.cu:161-239): free-list or new ids, each new
+        node linked to its nearest node both ways; returns the ids."""
+        vp, mem, keep = _arg(vectors, np.float32)
+        n = int(vectors.shape[0])
+        ids = np.empty(max(n, 1), np.int32)
+        _check(lib().vmn_engine_insert(self._h, vp, n, int(link_L),
+                                       ids.ctypes.data_as(ctypes.c_void_p), mem))
+        return ids[:n]
+
+    def run_workload(self, types, ids, vectors, batch_size, L, link_L=128, results=False):
+        """VamanaEngine::process_workload (engine.cu:407-568): returns a dict with
+        qps, recall, counts and times; with results=True also the found and
+        ground-truth ids [searches, 10]."""
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        i = np.ascontiguousarray(ids, dtype=np.int32)
+        v = np.ascontiguousarray(vectors, dtype=np.float32)
+        ns = int(np.count_nonzero((t != 0) & (t != 1)))
+        found = np.full((max(ns, 1), 10), -1, np.int32) if results else None
+        gt = np.full((max(ns, 1), 10), -1, np.int32) if results else None
+        r = EngineResult()
+        _check(lib().vmn_engine_run(self._h, t.ctypes.data_as(ctypes.c_void_p),
+                                    i.ctypes.data_as(ctypes.c_void_p),
+                                    v.ctypes.data_as(ctypes.c_void_p), len(t), int(batch_size),
+                                    int(L), int(link_L), ctypes.byref(r),
+                                    found.ctypes.data_as(ctypes.c_void_p) if results else None,
+                                    gt.ctypes.data_as(ctypes.c_void_p) if results else None))
+        out = {f: getattr(r, f) for f, _ in EngineResult._fields_}
+        if results:
+            out["found"], out["gt"] = found[:ns], gt[:ns]
+        return out
+
     def set_launch(self, waves_per_cu=32, list_cap=0):
         _check(lib().vmn_set_launch(self._h, waves_per_cu, list_cap))
 
@@ -445,6 +496,21 @@ class Vamana:
         e, d = ctypes.c_int64(), ctypes.c_int64()
         _check(lib().vmn_search_stats(self._h, ctypes.byref(e), ctypes.byref(d)))
         return int(e.value), int(d.value)
+
+
+def read_workload(path, dim):
+    """The parth engine's workload file (records {int32 type; int32 id; float
+    vec[dim]}, parth/src/common.h:63-67) -> (types, ids, vecs [n, dim])."""
+    n = ctypes.c_int64()
+    _check(lib().vmn_read_workload(os.fsencode(path), int(dim), None, None, None, 0,
+                                   ctypes.byref(n)))
+    t = np.empty(max(n.value, 1), np.int32)
+    i = np.empty(max(n.value, 1), np.int32)
+    v = np.empty((max(n.value, 1), dim), np.float32)
+    _check(lib().vmn_read_workload(os.fsencode(path), int(dim), t.ctypes.data_as(ctypes.c_void_p),
+                                   i.ctypes.data_as(ctypes.c_void_p),
+                                   v.ctypes.data_as(ctypes.c_void_p), n.value, ctypes.byref(n)))
+    return t[:n.value], i[:n.value], v[:n.value]
 
 
 def groundtruth(base, queries, k, device=0):

@@ -294,6 +294,72 @@ int vmn_ctx_replay(vmn_ctx* ctx);
  * beam-list capacity template (0: the smallest of 64/128/256/512 >= L). */
 int vmn_set_launch(vmn_index* idx, int32_t waves_per_cu, int32_t list_cap);
 
+/* ---- parth engine: workload files, free-list updates, workload replay ----
+ *
+ * The parth engine (parth/src/engine.cu:304-568, VamanaEngine) replays a
+ * workload file of operations in batches: per batch the deletes, then the
+ * inserts, then the searches (from node 0, top 10), each search scored
+ * against brute force over the live nodes. Its updates differ from the
+ * FreshVamana ones: a delete also pushes the id on a free list, and an
+ * insert takes an id from that list (else the next new id), links the new
+ * node to its nearest node both ways and prunes nothing. Load the graph with
+ * vmn_load_parth_graph(path, &p, 0, &idx) (medoid 0 = the engine's start
+ * node) and p.capacity = the engine's MAX_N (20000 for dim 128, engine.cu:721).
+ * The free list starts empty when the index is created or loaded. */
+#define VMN_OP_DELETE 0 /* parth/src/common.h:51-55 OpType */
+#define VMN_OP_INSERT 1
+#define VMN_OP_SEARCH 2
+
+/* Reads a workload file: records { int32 type; int32 id; float vec[dim] }
+ * (common.h:63-67; written by parth/tools/gen_sift_workload.py:80-86, read by
+ * engine.cu:359-372) up to the first incomplete record. With types == NULL
+ * only *count is set; otherwise at most `capacity` records are copied into
+ * types [n], ids [n] and vecs [n x dim]. */
+int vmn_read_workload(const char* path, int32_t dim, int32_t* types, int32_t* ids, float* vecs,
+                      int64_t capacity, int64_t* count);
+
+/* deleteKernel (engine.cu:131-147): each id in [0, capacity) is marked
+ * deleted and pushed on the free list at the list's counter (a push at a
+ * negative counter is dropped, as the reference's write lands outside its
+ * list); other ids are ignored. Ids in order, host memory. */
+int vmn_engine_delete(vmn_index* idx, const int32_t* ids, int64_t n);
+
+/* insertPrepareKernel + insertLinkKernel (engine.cu:161-239) as one batch, in
+ * this order: ids assigned (every insert sees the free-list count of the
+ * batch start; when it is positive each insert decrements it and takes the
+ * slot below the old value if that is positive, otherwise the next new id);
+ * vectors written (a reused id is unmarked), rows reset to -1; the nearest
+ * node of every new vector searched on the graph before any new link (beam
+ * link_L, the engine's BEAM_WIDTH = 128); slot 0 of every new row set to it;
+ * then, in input order, the new id written into the first -1 slot of its
+ * nearest node's row. ids_out [n] gets the assigned ids. vecs [n x dim] in
+ * `mem`; ids_out in host memory (or NULL). */
+int vmn_engine_insert(vmn_index* idx, const float* vecs, int64_t n, int32_t link_L,
+                      int32_t* ids_out, int32_t mem);
+
+typedef struct vmn_engine_result {
+    int64_t searches;      /* search operations run */
+    int64_t inserts, deletes;
+    int64_t batches;
+    double search_ms;      /* sum of the search launches' device time */
+    double qps;            /* searches / search time (the reference's `qps`) */
+    double recall;         /* mean recall@10 (the reference's `recall_10`) */
+    double wall_s;         /* host wall time of the whole replay */
+    int64_t expansions;    /* searches' node expansions (hops), summed */
+    int64_t dist_evals;    /* searches' distance evaluations, summed */
+} vmn_engine_result;
+
+/* VamanaEngine::process_workload (engine.cu:407-568): ops [n_ops] (types,
+ * ids, vecs [n_ops x dim], host memory) in batches of batch_size; searches
+ * of beam L from the entry point return 10 ids (-1 padded); ground truth is
+ * the first 10 live ids below the node count by (FP64 distance, id)
+ * (bruteForceKernel, engine.cu:254-279); recall of a search = found ids in
+ * its ground truth / 10. found_out and gt_out ([searches x 10], host, in op
+ * order) may be NULL. */
+int vmn_engine_run(vmn_index* idx, const int32_t* types, const int32_t* ids, const float* vecs,
+                   int64_t n_ops, int32_t batch_size, int32_t L, int32_t link_L,
+                   vmn_engine_result* result, int32_t* found_out, int32_t* gt_out);
+
 /* Message for the last failing call on this thread ("" if none). */
 const char* vmn_last_error(void);
 

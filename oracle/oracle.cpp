@@ -30,6 +30,7 @@
 // explicit fmaf() separately rounded.
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -756,6 +757,119 @@ void orc_bruteforce_fp64(const float* vecs, int64_t n, int d, const uint8_t* del
             if (out_dists) out_dists[qi * k + j] = j < (int)best.size() ? best[j].first : INFINITY;
         }
     }
+}
+
+// parth/src/engine.cu:407-568 (`VamanaEngine::process_workload`) with the
+// kernels it launches, run in a fixed sequential order. The reference runs
+// each kernel's threads concurrently; this restatement fixes the order a
+// single converged wavefront would give:
+//  * a batch is ops [b, b + batch): its deletes, then its inserts, then its
+//    searches, each in op order; a type other than 0 / 1 is a search;
+//  * deleteKernel (:131-147): an id in [0, cap) is marked deleted and stored
+//    at free-list slot c1++ (a slot outside [0, cap) is dropped: the
+//    reference writes outside its list there);
+//  * insertPrepareKernel (:161-199): every insert of the batch reads the
+//    count c1 of the batch start; when it is positive each insert decrements
+//    c1 and, if the old value was positive, takes free-list slot old - 1 (and
+//    unmarks it); otherwise it takes id c0++. Then each writes its vector and
+//    a -1 row (op order: a repeated id keeps the last vector);
+//  * insertLinkKernel (:213-239): the nearest node of each new node's stored
+//    vector by greedy_search (beam link_L) on the graph before any link of the
+//    batch; slot 0 of every new row is set to it; then, in op order, the new id
+//    goes into the first -1 slot of that node's row;
+//  * searchKernel (:101-120): greedy_search top 10 with beam L from `start`;
+//  * bruteForceKernel (:254-279): top 10 live ids below c0, CandidateList order.
+// vecs [cap x d], adj [cap x W] and del [cap] are updated in place, *n_io is c0
+// and *c1_io the free-list counter (start 0), freelist [cap]. found / gt
+// [searches x 10], assigned [inserts], *search_s the wall time of the search
+// loops (ground truth excluded). Returns the number of searches, or -1 when an
+// insert would pass cap.
+int64_t orc_engine_run(float* vecs, int32_t* adj, uint8_t* del, int64_t cap, int d, int W,
+                       int64_t* n_io, int64_t* c1_io, int32_t* freelist, const int32_t* types,
+                       const int32_t* ids, const float* opvecs, int64_t n_ops, int batch, int L,
+                       int link_L, int visited_cap, int max_hops, int dist_mode, int32_t start,
+                       int32_t* found, int32_t* gt, int32_t* assigned, double* search_s) {
+    int64_t c0 = *n_io, c1 = *c1_io, n_search = 0, n_assigned = 0;
+    double t_search = 0.0;
+    for (int64_t b = 0; b < n_ops; b += batch) {
+        const int64_t e = std::min<int64_t>(b + batch, n_ops);
+        std::vector<int64_t> dels, ins, srch;
+        for (int64_t i = b; i < e; ++i) {
+            if (types[i] == 0) dels.push_back(i);
+            else if (types[i] == 1) ins.push_back(i);
+            else srch.push_back(i);
+        }
+        for (int64_t i : dels) {
+            const int32_t t = ids[i];
+            if (t < 0 || t >= cap) continue;
+            del[t] = 1;
+            const int64_t slot = c1++;
+            if (slot >= 0 && slot < cap) freelist[slot] = t;
+        }
+        if (!ins.empty()) {
+            const int64_t m = (int64_t)ins.size();
+            const int64_t fc = c1;
+            const int64_t popped = fc > 0 ? std::min<int64_t>(m, fc) : 0;
+            if (c0 + (m - popped) > cap) return -1;
+            std::vector<int32_t> my(m);
+            for (int64_t j = 0; j < m; ++j) {
+                int32_t id = -1;
+                if (fc > 0) {
+                    const int64_t old = c1--;
+                    if (old > 0) {
+                        id = freelist[old - 1];
+                        del[id] = 0;
+                    }
+                }
+                if (id == -1) id = (int32_t)c0++;
+                my[j] = id;
+                if (assigned) assigned[n_assigned] = id;
+                ++n_assigned;
+                std::memcpy(vecs + (int64_t)id * d, opvecs + ins[j] * d, d * sizeof(float));
+                for (int k = 0; k < W; ++k) adj[(int64_t)id * W + k] = -1;
+            }
+            std::vector<int32_t> best(m);
+#pragma omp parallel for schedule(dynamic, 4)
+            for (int64_t j = 0; j < m; ++j)
+                parth_search_one(vecs, d, adj, W, del, vecs + (int64_t)my[j] * d, start, link_L,
+                                 visited_cap, max_hops, dist_mode, 1, &best[j], nullptr, nullptr,
+                                 nullptr);
+            for (int64_t j = 0; j < m; ++j)
+                if (best[j] != -1) adj[(int64_t)my[j] * W] = best[j];
+            for (int64_t j = 0; j < m; ++j) {
+                if (best[j] == -1) continue;
+                int32_t* row = adj + (int64_t)best[j] * W;
+                for (int k = 0; k < W; ++k)
+                    if (row[k] == -1) {
+                        row[k] = my[j];
+                        break;
+                    }
+            }
+        }
+        const int64_t ns = (int64_t)srch.size();
+        const auto ts = std::chrono::steady_clock::now();
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int64_t j = 0; j < ns; ++j)
+            parth_search_one(vecs, d, adj, W, del, opvecs + srch[j] * d, start, L, visited_cap,
+                             max_hops, dist_mode, 10, found + (n_search + j) * 10, nullptr,
+                             nullptr, nullptr);
+        t_search += std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int64_t j = 0; j < ns; ++j) {
+            const float* q = opvecs + srch[j] * d;
+            CandList top(10);
+            for (int64_t i = 0; i < c0; ++i) {
+                if (del[i]) continue;
+                top.insert((int32_t)i, l2sq_m(q, vecs + i * d, d, dist_mode), 10);
+            }
+            for (int k = 0; k < 10; ++k) gt[(n_search + j) * 10 + k] = k < top.size ? top.ids[k] : -1;
+        }
+        n_search += ns;
+    }
+    *n_io = c0;
+    *c1_io = c1;
+    if (search_s) *search_s = t_search;
+    return n_search;
 }
 
 int orc_num_threads(void) {

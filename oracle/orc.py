@@ -51,6 +51,9 @@ def lib():
         _lib.orc_consolidate.argtypes = [P, i64, i32, P, i32, P, f32, i32, i32]
         _lib.orc_consolidate.restype = i64
         _lib.orc_init_rows.argtypes = [P, i64, i32, i32, i32, P]
+        _lib.orc_engine_run.argtypes = [P, P, P, i64, i32, i32, P, P, P, P, P, P, i64, i32, i32,
+                                        i32, i32, i32, i32, i32, P, P, P, P]
+        _lib.orc_engine_run.restype = i64
         _lib.orc_medoid_live.argtypes = [P, i64, i32, P]
         _lib.orc_medoid_live.restype = i64
         _lib.orc_fresh_out_prune.argtypes = [P, i32, P, i32, i32, f32, i64, i64, P, P, P, i32,
@@ -212,6 +215,44 @@ def bruteforce_fp64(vecs, queries, k, deleted=None):
     lib().orc_bruteforce_fp64(_p(vecs), vecs.shape[0], vecs.shape[1], _p(dm), _p(queries), nq,
                               k, _p(ids), _p(dists))
     return ids, dists
+
+
+def engine_run(vecs, adj, n, cap, types, ids, opvecs, batch, L, link_L=128, start=0,
+               visited_cap=128, max_hops=200, dist_mode=0, stats=None):
+    """parth engine workload replay (engine.cu:407-568). vecs [n, d] and adj
+    [n, W] are the initial graph; returns (found, gt, vecs [cap, d], adj [cap, W],
+    deleted [cap], c0, assigned) with found / gt [searches, 10]. A dict `stats`
+    receives search_s, the wall time of the search loops."""
+    vecs = _f32(vecs)
+    d = vecs.shape[1]
+    W = adj.shape[1]
+    V = np.zeros((cap, d), np.float32)
+    V[:n] = vecs[:n]
+    A = np.full((cap, W), -1, np.int32)
+    A[:n] = adj[:n]
+    dm = np.zeros(cap, np.uint8)
+    fl = np.zeros(cap, np.int32)
+    types = np.ascontiguousarray(types, dtype=np.int32)
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    opvecs = _f32(opvecs)
+    n_ops = len(types)
+    nsr = int((types != 0).sum() - (types == 1).sum())
+    found = np.full((max(nsr, 1), 10), -1, np.int32)
+    gt = np.full((max(nsr, 1), 10), -1, np.int32)
+    assigned = np.full(max(int((types == 1).sum()), 1), -1, np.int32)
+    n_io = ctypes.c_int64(n)
+    c1 = ctypes.c_int64(0)
+    ts = ctypes.c_double(0.0)
+    r = lib().orc_engine_run(_p(V), _p(A), _p(dm), cap, d, W, ctypes.byref(n_io), ctypes.byref(c1),
+                             _p(fl), _p(types), _p(ids), _p(opvecs), n_ops, batch, L, link_L,
+                             visited_cap, max_hops, dist_mode, start, _p(found), _p(gt),
+                             _p(assigned), ctypes.byref(ts))
+    if stats is not None:
+        stats["search_s"] = ts.value
+    if r < 0:
+        raise RuntimeError("engine replay exceeds capacity")
+    return (found[:r], gt[:r], V, A, dm, int(n_io.value),
+            assigned[:int((types == 1).sum())])
 
 
 def num_threads():

@@ -23,6 +23,13 @@ Outputs (committed):
   file_digests.json  size and SHA-256 of the reference's graph files, so a
                writer of those formats can be checked byte for byte without the
                files themselves
+  parth_workloads.npz  the parth engine's workload files parth/data/
+               {sift,custom_sift,toy,vis}_workload.bin as <name>_types, <name>_ids
+               (int32) and <name>_vecs (float32 [ops x dim])
+  parth_metrics.json   per workload: the reference engine's own result
+               (parth/vis_outputs/<dir>/metrics.json: qps, recall_10), the batch
+               size and MAX_N of its config (parth/config_<name>.json) and the
+               beam its file was written at (the last of engine.cu:584's sweep)
 
 SIFT10K is integer-valued (0..180); storing it as uint8 is lossless, which is
 asserted below.
@@ -58,6 +65,20 @@ def read_parth_graph(path):
         vecs = np.frombuffer(f.read(n * d * 4), dtype=np.float32).reshape(n, d).copy()
         adj = np.frombuffer(f.read(n * md * 4), dtype=np.int32).reshape(n, md).copy()
     return vecs, adj
+
+
+def read_workload(path, dim):
+    """parth workload file: records { int32 type; int32 id; float vec[dim] }
+    (written by parth/tools/gen_sift_workload.py:80-86, read by
+    parth/src/engine.cu:359-372 up to the first incomplete record)."""
+    raw = np.fromfile(path, dtype=np.uint8)
+    rec = 8 + 4 * dim
+    m = len(raw) // rec
+    arr = raw[: m * rec].reshape(m, rec)
+    types = arr[:, :4].copy().view(np.int32).ravel()
+    ids = arr[:, 4:8].copy().view(np.int32).ravel()
+    vecs = arr[:, 8:].copy().view(np.float32)
+    return types, ids, vecs
 
 
 def to_u8(a):
@@ -111,6 +132,22 @@ def main(ref):
     vv, vadj = read_parth_graph(os.path.join(ref, "parth/data/vis_initial_graph.bin"))
     np.savez_compressed(os.path.join(HERE, "toy2d.npz"),
                         toy_vecs=tv, toy_adj=tadj, vis_vecs=vv, vis_adj=vadj)
+    wl, metrics = {}, {}
+    for name, dim, vis_dir in (("sift", 128, "sift"), ("custom_sift", 128, "custom_sift"),
+                               ("toy", 2, "toy"), ("vis", 2, "step_by_step")):
+        t, i, v = read_workload(os.path.join(ref, f"parth/data/{name}_workload.bin"), dim)
+        wl[f"{name}_types"], wl[f"{name}_ids"], wl[f"{name}_vecs"] = t, i, v
+        with open(os.path.join(ref, f"parth/config_{name}.json")) as f:
+            cfg = json.load(f)
+        with open(os.path.join(ref, f"parth/vis_outputs/{vis_dir}/metrics.json")) as f:
+            m = json.load(f)
+        metrics[name] = {"qps": m["qps"], "recall_10": m["recall_10"],
+                         "batch_size": cfg["execution"]["batch_size"],
+                         "max_capacity": cfg["max_capacity"], "dim": dim, "L": 100,
+                         "metrics_file": f"parth/vis_outputs/{vis_dir}/metrics.json"}
+    np.savez_compressed(os.path.join(HERE, "parth_workloads.npz"), **wl)
+    with open(os.path.join(HERE, "parth_metrics.json"), "w") as f:
+        json.dump(metrics, f, indent=1)
     print("wrote", os.listdir(HERE))
 
 

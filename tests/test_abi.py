@@ -81,3 +81,23 @@ def test_file_readers_reject_bad_files_without_gpu(vmn, tmp_path):
 def struct_pack(*ints):
     import struct
     return struct.pack("<%di" % len(ints), *ints)
+
+
+def test_read_workload_without_gpu(vmn, tmp_path):
+    """vmn_read_workload against the reference's sift workload (records of
+    {int32 type; int32 id; float vec[128]}, parth/src/common.h:63-67), a
+    trailing partial record (ignored, as engine.cu:364-370 stops there) and a
+    missing file."""
+    import numpy as np
+    w = np.load(os.path.join(ROOT, "tests", "golden", "parth_workloads.npz"))
+    t, i, v = w["sift_types"], w["sift_ids"], w["sift_vecs"]
+    rec = np.zeros(len(t), dtype=[("t", "<i4"), ("i", "<i4"), ("v", "<f4", (128,))])
+    rec["t"], rec["i"], rec["v"] = t, i, v
+    path = tmp_path / "sift_workload.bin"
+    path.write_bytes(rec.tobytes() + b"\x02\x00\x00\x00\xff")
+    assert os.path.getsize(path) == 520000 + 5
+    rt, ri, rv = vmn.read_workload(str(path), 128)
+    assert np.array_equal(rt, t) and np.array_equal(ri, i) and np.array_equal(rv, v)
+    assert np.bincount(rt).tolist() == [90, 183, 727]
+    with pytest.raises(vmn.VamanaError):
+        vmn.read_workload(str(tmp_path / "missing.bin"), 128)
