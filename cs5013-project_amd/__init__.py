@@ -8,12 +8,13 @@ Import with importlib (the directory name is not an identifier):
 """
 from .vamana import (MODE_FRESH, MODE_PARTH, REQ_DELETE, REQ_INSERT, REQ_SEARCH, Params,
                      SearchContext, Server, Vamana, VamanaError, default_params, groundtruth, lib,
-                     read_workload, recall_at_k)
+                     read_truthset, read_vectors, read_workload, recall_at_k,
+                     write_truthset, write_vectors)
 from .engine import VamanaEngine
 from . import datasets, tuning
 
 __all__ = ["Vamana", "VamanaEngine", "VamanaError", "Params", "SearchContext", "Server", "tuning",
-           "read_workload",
+           "read_workload", "read_vectors", "write_vectors", "read_truthset", "write_truthset",
            "default_params", "groundtruth", "lib",
            "recall_at_k", "datasets", "MODE_PARTH", "MODE_FRESH", "REQ_SEARCH", "REQ_INSERT",
            "REQ_DELETE"]

@@ -18,6 +18,10 @@
 // * parth graph file (written by build_graph.cu:361-369, read by
 //   engine.cu:338-357): i32 n, dim, max_degree; f32 vectors[n x dim];
 //   i32 adjacency[n x max_degree] (-1 padded).
+// * vector files (fvecs / ivecs / bvecs, and the .bin form with a u32 n, dim
+//   header), DiskANN ground-truth files (u32 n, k; ids; optional distances)
+//   and the BANG / DiskANN disk index (772-byte records with ascending
+//   neighbour ids plus a 32-byte metadata file).
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -303,4 +307,200 @@ int vmn_load_parth_graph(const char* path, const vmn_params* p, int64_t medoid, 
     }
     *out = idx;
     return VMN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Vector files, ground-truth files and the BANG / DiskANN disk index.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+int elem_bytes(int32_t format) {
+    switch (format) {
+        case VMN_FILE_FVECS: return 4;
+        case VMN_FILE_IVECS: return 4;
+        case VMN_FILE_BVECS: return 1;
+        case VMN_FILE_BIN: return 4;
+        default: return 0;
+    }
+}
+
+}  // namespace
+
+int vmn_read_vectors(const char* path, int32_t format, void* out, int64_t capacity, int64_t* n,
+                     int32_t* dim) {
+    if (!path || !n || !dim) return fail(VMN_ERR_ARG, "NULL argument");
+    const int eb = elem_bytes(format);
+    if (!eb) return fail(VMN_ERR_ARG, "unknown vector file format %d", format);
+    File f(path, "rb");
+    if (!f.f) return fail(VMN_ERR_IO, "cannot open %s", path);
+    const long size = file_size(f.f);
+    int64_t count = 0;
+    int32_t d = 0;
+    if (format == VMN_FILE_BIN) {
+        uint32_t hdr[2];
+        if (!f.get(hdr, sizeof hdr)) return fail(VMN_ERR_IO, "%s: no header", path);
+        count = hdr[0];
+        d = (int32_t)hdr[1];
+        // a header may claim more vectors than the file holds (the reference's
+        // data/sift10k/siftsmall_query.bin says 10,000 for its 100): the
+        // vectors present are read
+        const int64_t held = d >= 1 ? (size - 8) / ((int64_t)d * 4) : 0;
+        if (d < 1 || (size - 8) % ((int64_t)d * 4) != 0 || held > count)
+            return fail(VMN_ERR_IO, "%s: %ld bytes do not hold whole %d-float vectors (header %lld)",
+                        path, size, d, (long long)count);
+        count = held;
+    } else {
+        if (!f.get(d) || d < 1) return fail(VMN_ERR_IO, "%s: bad first record", path);
+        const int64_t rec = 4 + (int64_t)d * eb;
+        if (size % rec != 0)
+            return fail(VMN_ERR_IO, "%s: %ld bytes is not a whole number of %lld-byte records", path,
+                        size, (long long)rec);
+        count = size / rec;
+        std::rewind(f.f);
+    }
+    *n = count;
+    *dim = d;
+    if (!out) return VMN_OK;
+    const int64_t m = count < capacity ? count : capacity;
+    char* o = static_cast<char*>(out);
+    if (format == VMN_FILE_BIN) {
+        if (!f.get(o, (size_t)(m * d) * 4)) return fail(VMN_ERR_IO, "short read from %s", path);
+    } else {
+        for (int64_t i = 0; i < m; ++i) {
+            int32_t di = 0;
+            if (!f.get(di) || !f.get(o + (size_t)(i * d) * eb, (size_t)d * eb))
+                return fail(VMN_ERR_IO, "short read from %s", path);
+            if (di != d)
+                return fail(VMN_ERR_IO, "%s: record %lld has dimension %d, not %d", path,
+                            (long long)i, di, d);
+        }
+    }
+    *n = m;
+    return VMN_OK;
+}
+
+int vmn_write_vectors(const char* path, int32_t format, const void* vecs, int64_t n, int32_t dim) {
+    if (!path || (!vecs && n)) return fail(VMN_ERR_ARG, "NULL argument");
+    const int eb = elem_bytes(format);
+    if (!eb) return fail(VMN_ERR_ARG, "unknown vector file format %d", format);
+    if (n < 0 || dim < 1 || (format == VMN_FILE_BIN && n > (int64_t)UINT32_MAX))
+        return fail(VMN_ERR_ARG, "bad shape");
+    File f(path, "wb");
+    if (!f.f) return fail(VMN_ERR_IO, "cannot open %s for writing", path);
+    const char* v = static_cast<const char*>(vecs);
+    bool ok = true;
+    if (format == VMN_FILE_BIN) {
+        const uint32_t hdr[2] = {(uint32_t)n, (uint32_t)dim};
+        ok = f.put(hdr, sizeof hdr) && f.put(v, (size_t)(n * dim) * eb);
+    } else {
+        for (int64_t i = 0; ok && i < n; ++i)
+            ok = f.put(dim) && f.put(v + (size_t)(i * dim) * eb, (size_t)dim * eb);
+    }
+    if (!ok) return fail(VMN_ERR_IO, "short write to %s", path);
+    return VMN_OK;
+}
+
+int vmn_read_truthset(const char* path, int32_t* ids, float* dists, int64_t capacity, int64_t* n,
+                      int32_t* k, int32_t* has_dists) {
+    if (!path || !n || !k) return fail(VMN_ERR_ARG, "NULL argument");
+    File f(path, "rb");
+    if (!f.f) return fail(VMN_ERR_IO, "cannot open %s", path);
+    const long size = file_size(f.f);
+    uint32_t hdr[2];
+    if (!f.get(hdr, sizeof hdr)) return fail(VMN_ERR_IO, "%s: no header", path);
+    const int64_t cnt = hdr[0], kk = hdr[1];
+    const int64_t cell = cnt * kk * 4;
+    int with = 0;
+    if (size == 8 + 2 * cell) with = 1;
+    else if (size != 8 + cell)
+        return fail(VMN_ERR_IO, "%s: %ld bytes fit neither %lld x %lld ids nor ids + distances",
+                    path, size, (long long)cnt, (long long)kk);
+    *n = cnt;
+    *k = (int32_t)kk;
+    if (has_dists) *has_dists = with;
+    if (!ids) return VMN_OK;
+    const int64_t m = cnt < capacity ? cnt : capacity;
+    if (!f.get(ids, (size_t)(m * kk) * 4)) return fail(VMN_ERR_IO, "short read from %s", path);
+    if (dists && with) {
+        if (std::fseek(f.f, (long)(8 + cell), SEEK_SET) != 0 ||
+            !f.get(dists, (size_t)(m * kk) * 4))
+            return fail(VMN_ERR_IO, "short read from %s", path);
+    }
+    *n = m;
+    return VMN_OK;
+}
+
+int vmn_write_truthset(const char* path, const int32_t* ids, const float* dists, int64_t n,
+                       int32_t k) {
+    if (!path || (!ids && n)) return fail(VMN_ERR_ARG, "NULL argument");
+    if (n < 0 || k < 1 || n > (int64_t)UINT32_MAX) return fail(VMN_ERR_ARG, "bad shape");
+    File f(path, "wb");
+    if (!f.f) return fail(VMN_ERR_IO, "cannot open %s for writing", path);
+    const uint32_t hdr[2] = {(uint32_t)n, (uint32_t)k};
+    bool ok = f.put(hdr, sizeof hdr) && f.put(ids, (size_t)(n * k) * 4);
+    if (ok && dists) ok = f.put(dists, (size_t)(n * k) * 4);
+    if (!ok) return fail(VMN_ERR_IO, "short write to %s", path);
+    return VMN_OK;
+}
+
+int vmn_save_bang_index(const vmn_index* idx, const char* index_path, const char* metadata_path) {
+    if (!idx || !index_path || !metadata_path) return fail(VMN_ERR_ARG, "NULL argument");
+    std::vector<float> vecs;
+    std::vector<int32_t> adj;
+    int rc = fetch(idx, vecs, adj, nullptr);
+    if (rc) return rc;
+    const int D = idx->p.dim, R = idx->p.R;
+    {
+        File f(index_path, "wb");
+        if (!f.f) return fail(VMN_ERR_IO, "cannot open %s for writing", index_path);
+        std::vector<uint32_t> nbr(R);
+        for (int64_t i = 0; i < idx->n; ++i) {
+            uint32_t deg = 0;
+            for (int j = 0; j < R; ++j)
+                if (adj[(size_t)i * R + j] >= 0) nbr[deg++] = (uint32_t)adj[(size_t)i * R + j];
+            std::sort(nbr.begin(), nbr.begin() + deg);
+            for (int j = (int)deg; j < R; ++j) nbr[j] = 0xffffffffu;
+            if (!f.put(&vecs[(size_t)i * D], (size_t)D * sizeof(float)) || !f.put(deg) ||
+                !f.put(nbr.data(), (size_t)R * sizeof(uint32_t)))
+                return fail(VMN_ERR_IO, "short write to %s", index_path);
+        }
+    }
+    File m(metadata_path, "wb");
+    if (!m.f) return fail(VMN_ERR_IO, "cannot open %s for writing", metadata_path);
+    const uint64_t med = (uint64_t)idx->medoid, rec = (uint64_t)D * 4 + 4 + (uint64_t)R * 4;
+    const uint32_t tail[4] = {2u, (uint32_t)D, (uint32_t)R, (uint32_t)idx->n};
+    if (!m.put(med) || !m.put(rec) || !m.put(tail, sizeof tail))
+        return fail(VMN_ERR_IO, "short write to %s", metadata_path);
+    return VMN_OK;
+}
+
+int vmn_load_bang_index(const char* index_path, const char* metadata_path, const vmn_params* p,
+                        vmn_index** out) {
+    if (!index_path || !metadata_path || !p || !out) return fail(VMN_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    uint64_t med = 0, rec = 0;
+    uint32_t tail[4];
+    {
+        File m(metadata_path, "rb");
+        if (!m.f) return fail(VMN_ERR_IO, "cannot open %s", metadata_path);
+        if (file_size(m.f) != 32 || !m.get(med) || !m.get(rec) || !m.get(tail, sizeof tail))
+            return fail(VMN_ERR_IO, "%s is not a 32-byte index metadata file", metadata_path);
+    }
+    if (tail[0] != 2u)
+        return fail(VMN_ERR_IO, "%s: data type %u (only 2 = float is read)", metadata_path, tail[0]);
+    vmn_params q = *p;
+    q.dim = (int32_t)tail[1];
+    q.R = (int32_t)tail[2];
+    const int64_t n = tail[3];
+    if (rec != (uint64_t)q.dim * 4 + 4 + (uint64_t)q.R * 4 || med >= (uint64_t)n)
+        return fail(VMN_ERR_IO, "%s: record size or medoid does not fit its shape", metadata_path);
+    {
+        File f(index_path, "rb");
+        if (!f.f) return fail(VMN_ERR_IO, "cannot open %s", index_path);
+        if (file_size(f.f) != (long)(n * (int64_t)rec))
+            return fail(VMN_ERR_IO, "%s: size is not %lld records of %llu bytes", index_path,
+                        (long long)n, (unsigned long long)rec);
+    }
+    return vmn_load_fresh_graph(index_path, &q, 0, (int64_t)med, out);
 }

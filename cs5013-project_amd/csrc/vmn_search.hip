@@ -131,27 +131,42 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
                     a.vis_d[qi * a.P + nexp - 1] = nd;
                 }
             }
+            // PARTH: the list's ids, LCAP / 64 per lane, for the membership test
+            // below (read before the adjacency row arrives)
+            constexpr int EPL = MODE == VMN_MODE_PARTH ? LCAP / 64 : 1;
+            uint32_t ent[EPL];
+            if (MODE == VMN_MODE_PARTH) {
+#pragma unroll
+                for (int k = 0; k < EPL; ++k) {
+                    const int i = l + 64 * k;
+                    ent[k] = i < cnt ? (s_id[cur][i] & VMN_ID_MASK) : 0xffffffffu;
+                }
+            }
             // neighbours up to the first -1
             int32_t nb = l < W ? a.adj[(int64_t)node * W + l] : -1;
             const uint64_t endm = ballot(l < W && nb == -1);
             const int deg = endm ? __ffsll((unsigned long long)endm) - 1 : W;
             bool valid = l < deg;
-            if (valid && a.del_bits) valid = !bit_test(a.del_bits, (uint32_t)nb);
+            // the tombstone word is consumed after the membership test, so its
+            // load overlaps that work
+            const bool gone = valid && a.del_bits && bit_test(a.del_bits, (uint32_t)nb);
             if (MODE == VMN_MODE_FRESH && valid && nb == self) valid = false;
-            __syncthreads();
             if (MODE == VMN_MODE_PARTH) {
-                // skip ids already in the list
-                bool dup = false;
-                const uint32_t key = (uint32_t)nb;
-                int i = 0;
-                for (; i + 4 <= cnt; i += 4) {
-                    const uint4 v = *reinterpret_cast<const uint4*>(&s_id[cur][i]);
-                    dup |= ((v.x & VMN_ID_MASK) == key) | ((v.y & VMN_ID_MASK) == key) |
-                           ((v.z & VMN_ID_MASK) == key) | ((v.w & VMN_ID_MASK) == key);
+                // skip ids already in the list: neighbour j is broadcast and
+                // matched against every lane's list entries with one ballot
+                uint64_t dupm = 0;
+                for (int j = 0; j < deg; ++j) {
+                    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane(nb, j);
+                    bool hit = false;
+#pragma unroll
+                    for (int k = 0; k < EPL; ++k) hit |= ent[k] == c;
+                    if (ballot(hit)) dupm |= 1ull << j;
                 }
-                for (; i < cnt; ++i) dup |= (s_id[cur][i] & VMN_ID_MASK) == key;
-                valid = valid && !dup;
+                valid = valid && !((dupm >> l) & 1ull) && !gone;
+                __syncthreads();
             } else {
+                valid = valid && !gone;
+                __syncthreads();
                 uint32_t h1 = 0, h2 = 0;
                 bool seen = true;
                 if (valid) {

@@ -137,6 +137,15 @@ def lib():
         L.vmn_server_pending.restype = i64
         L.vmn_server_destroy.argtypes = [P]
         L.vmn_read_workload.argtypes = [ctypes.c_char_p, i32, P, P, P, i64, ctypes.POINTER(i64)]
+        L.vmn_read_vectors.argtypes = [ctypes.c_char_p, i32, P, i64, ctypes.POINTER(i64),
+                                       ctypes.POINTER(i32)]
+        L.vmn_write_vectors.argtypes = [ctypes.c_char_p, i32, P, i64, i32]
+        L.vmn_read_truthset.argtypes = [ctypes.c_char_p, P, P, i64, ctypes.POINTER(i64),
+                                        ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.vmn_write_truthset.argtypes = [ctypes.c_char_p, P, P, i64, i32]
+        L.vmn_save_bang_index.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p]
+        L.vmn_load_bang_index.argtypes = [ctypes.c_char_p, ctypes.c_char_p,
+                                          ctypes.POINTER(Params), ctypes.POINTER(P)]
         L.vmn_engine_delete.argtypes = [P, P, i64]
         L.vmn_engine_insert.argtypes = [P, P, i64, i32, P, i32]
         L.vmn_engine_run.argtypes = [P, P, P, P, i64, i32, i32, i32, ctypes.POINTER(EngineResult),
@@ -153,7 +162,9 @@ def lib():
                      "vmn_ctx_destroy", "vmn_ctx_search", "vmn_ctx_capture", "vmn_ctx_replay",
                      "vmn_set_launch", "vmn_server_create", "vmn_server_submit",
                      "vmn_server_poll", "vmn_server_destroy", "vmn_read_workload",
-                     "vmn_engine_delete", "vmn_engine_insert", "vmn_engine_run"):
+                     "vmn_engine_delete", "vmn_engine_insert", "vmn_engine_run",
+                     "vmn_read_vectors", "vmn_write_vectors", "vmn_read_truthset",
+                     "vmn_write_truthset", "vmn_save_bang_index", "vmn_load_bang_index"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -279,6 +290,19 @@ class Vamana:
 
     def save_parth_graph(self, path):
         _check(lib().vmn_save_parth_graph(self._h, os.fsencode(path)))
+
+    @classmethod
+    def load_bang_index(cls, index_path, metadata_path, mode="fresh", **kw):
+        """BANG / DiskANN disk index (records + 32-byte metadata with the medoid)."""
+        p = cls._params(mode, **kw)
+        h = ctypes.c_void_p()
+        _check(lib().vmn_load_bang_index(os.fsencode(index_path), os.fsencode(metadata_path),
+                                         ctypes.byref(p), ctypes.byref(h)))
+        return cls._wrap(h)
+
+    def save_bang_index(self, index_path, metadata_path):
+        _check(lib().vmn_save_bang_index(self._h, os.fsencode(index_path),
+                                         os.fsencode(metadata_path)))
 
     @staticmethod
     def _params(mode, dim=None, R=None, L_build=None, alphas=None, capacity=0, device=0,
@@ -496,6 +520,50 @@ class Vamana:
         e, d = ctypes.c_int64(), ctypes.c_int64()
         _check(lib().vmn_search_stats(self._h, ctypes.byref(e), ctypes.byref(d)))
         return int(e.value), int(d.value)
+
+
+VEC_FORMATS = {"fvecs": (0, np.float32), "ivecs": (1, np.int32), "bvecs": (2, np.uint8),
+               "bin": (3, np.float32)}
+
+
+def read_vectors(path, fmt):
+    """fvecs / ivecs / bvecs / bin (u32 n, dim header + floats) -> [n, dim]."""
+    code, dt = VEC_FORMATS[fmt]
+    n, d = ctypes.c_int64(), ctypes.c_int32()
+    _check(lib().vmn_read_vectors(os.fsencode(path), code, None, 0, ctypes.byref(n),
+                                  ctypes.byref(d)))
+    out = np.empty((max(n.value, 1), d.value), dt)
+    _check(lib().vmn_read_vectors(os.fsencode(path), code, out.ctypes.data_as(ctypes.c_void_p),
+                                  n.value, ctypes.byref(n), ctypes.byref(d)))
+    return out[:n.value]
+
+
+def write_vectors(path, vecs, fmt):
+    code, dt = VEC_FORMATS[fmt]
+    a = np.ascontiguousarray(vecs, dtype=dt)
+    _check(lib().vmn_write_vectors(os.fsencode(path), code, a.ctypes.data_as(ctypes.c_void_p),
+                                   a.shape[0], a.shape[1]))
+
+
+def read_truthset(path):
+    """DiskANN ground-truth file -> (ids [n, k] int32, dists [n, k] float32 or None)."""
+    n, k, hd = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+    _check(lib().vmn_read_truthset(os.fsencode(path), None, None, 0, ctypes.byref(n),
+                                   ctypes.byref(k), ctypes.byref(hd)))
+    ids = np.empty((max(n.value, 1), k.value), np.int32)
+    dists = np.empty((max(n.value, 1), k.value), np.float32) if hd.value else None
+    _check(lib().vmn_read_truthset(os.fsencode(path), ids.ctypes.data_as(ctypes.c_void_p),
+                                   dists.ctypes.data_as(ctypes.c_void_p) if hd.value else None,
+                                   n.value, ctypes.byref(n), ctypes.byref(k), ctypes.byref(hd)))
+    return ids[:n.value], (dists[:n.value] if hd.value else None)
+
+
+def write_truthset(path, ids, dists=None):
+    i = np.ascontiguousarray(ids, dtype=np.int32)
+    d = None if dists is None else np.ascontiguousarray(dists, dtype=np.float32)
+    _check(lib().vmn_write_truthset(os.fsencode(path), i.ctypes.data_as(ctypes.c_void_p),
+                                    None if d is None else d.ctypes.data_as(ctypes.c_void_p),
+                                    i.shape[0], i.shape[1]))
 
 
 def read_workload(path, dim):

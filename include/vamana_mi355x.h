@@ -131,6 +131,48 @@ int vmn_load_fresh_graph(const char* path, const vmn_params* p, int32_t build, i
 int vmn_save_parth_graph(const vmn_index* idx, const char* path);
 int vmn_load_parth_graph(const char* path, const vmn_params* p, int64_t medoid, vmn_index** out);
 
+/* Vector files. Element type per format: float (FVECS, BIN), int32 (IVECS),
+ * uint8 (BVECS).
+ *   FVECS / IVECS / BVECS: per vector int32 dim, then dim elements
+ *     (siftsmall/*.fvecs, *.ivecs; FVecs::load_from_fvecs_file, vamana.cu:108-153);
+ *   BIN: uint32 n, uint32 dim, then n x dim floats (FVecs::save_to_binary /
+ *     load_from_binary, vamana.cu:156-195; scripts/vecs_to_binary.py:21-39;
+ *     data/sift10k/siftsmall_query.bin).
+ * Reading: with out == NULL only *n and *dim are set; otherwise at most
+ * `capacity` vectors are copied. A BIN header that claims more vectors than
+ * the file holds (the reference's siftsmall_query.bin says 10,000 for its 100)
+ * reads the vectors present. */
+#define VMN_FILE_FVECS 0
+#define VMN_FILE_IVECS 1
+#define VMN_FILE_BVECS 2
+#define VMN_FILE_BIN 3
+int vmn_read_vectors(const char* path, int32_t format, void* out, int64_t capacity, int64_t* n,
+                     int32_t* dim);
+int vmn_write_vectors(const char* path, int32_t format, const void* vecs, int64_t n, int32_t dim);
+
+/* DiskANN ground-truth ("truthset") file: uint32 n, uint32 k, uint32 ids[n x
+ * k], then float dists[n x k] when the file size has room for them (the
+ * reference's data/sift10k/sift10k_groundtruth.bin has them,
+ * siftsmall_groundtruth.bin does not; read by load_truthset,
+ * bang_exact/src/parANN.cuh:371; written by DiskANN's compute_groundtruth,
+ * vamana.cu:1644). *has_dists (may be NULL) says which. dists == NULL on
+ * writing leaves them out. */
+int vmn_read_truthset(const char* path, int32_t* ids, float* dists, int64_t capacity, int64_t* n,
+                      int32_t* k, int32_t* has_dists);
+int vmn_write_truthset(const char* path, const int32_t* ids, const float* dists, int64_t n,
+                       int32_t k);
+
+/* BANG / DiskANN disk index (scripts/bang-preprocess.py:37-77;
+ * data/sift10k/sift10k_index_disk.bin + _metadata.bin): n records { float
+ * vec[dim]; uint32 degree; uint32 nbr[R] } with the neighbours in ascending id
+ * order (0xffffffff past the degree when saving), and a 32-byte metadata file
+ * { uint64 medoid; uint64 record bytes; uint32 data type (2 = float); uint32
+ * dim; uint32 R; uint32 n }. Loading takes dim, R, n and the entry point from
+ * the metadata and the other parameters from *p. */
+int vmn_save_bang_index(const vmn_index* idx, const char* index_path, const char* metadata_path);
+int vmn_load_bang_index(const char* index_path, const char* metadata_path, const vmn_params* p,
+                        vmn_index** out);
+
 /* Batched greedy search of nq queries [nq x dim] from the medoid with beam L;
  * writes the k best ids (-1 padded) and squared L2 distances (may be NULL) per
  * query. Replaces `searchPoints` (vamana.cuh:96, vamana_impl.cuh:55-96) and

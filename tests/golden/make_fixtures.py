@@ -23,6 +23,9 @@ Outputs (committed):
   file_digests.json  size and SHA-256 of the reference's graph files, so a
                writer of those formats can be checked byte for byte without the
                files themselves
+  diskann_graph.npz    adj (10000x64 i32) of data/sift10k/sift10k_index_disk.bin, the
+               DiskANN-built SIFT10K graph in BANG's disk layout (medoid 3732 in its
+               _metadata.bin; every node has degree 64, neighbours in ascending order)
   parth_workloads.npz  the parth engine's workload files parth/data/
                {sift,custom_sift,toy,vis}_workload.bin as <name>_types, <name>_ids
                (int32) and <name>_vecs (float32 [ops x dim])
@@ -123,7 +126,9 @@ def main(ref):
     digests = {}
     for rel in ("parth/data/sift_initial_graph.bin", "parth/data/custom_sift_initial_graph.bin",
                 "parth/data/toy_initial_graph.bin", "parth/data/vis_initial_graph.bin",
-                "vamana/vout/vamana.out", "data/sift10k/sift10k_randomgraph.bin"):
+                "vamana/vout/vamana.out", "data/sift10k/sift10k_randomgraph.bin",
+                "data/sift10k/sift10k_index_disk.bin", "data/sift10k/sift10k_index_disk_metadata.bin",
+                "data/sift10k/siftsmall_query.bin", "data/sift10k/siftsmall_groundtruth.bin"):
         raw = open(os.path.join(ref, rel), "rb").read()
         digests[rel] = {"bytes": len(raw), "sha256": hashlib.sha256(raw).hexdigest()}
     with open(os.path.join(HERE, "file_digests.json"), "w") as f:
@@ -132,6 +137,13 @@ def main(ref):
     vv, vadj = read_parth_graph(os.path.join(ref, "parth/data/vis_initial_graph.bin"))
     np.savez_compressed(os.path.join(HERE, "toy2d.npz"),
                         toy_vecs=tv, toy_adj=tadj, vis_vecs=vv, vis_adj=vadj)
+    drec = np.fromfile(os.path.join(ref, "data/sift10k/sift10k_index_disk.bin"),
+                       dtype=np.uint8).reshape(-1, 772)
+    assert np.array_equal(drec[:, :512].copy().view(np.float32), base)
+    ddeg = drec[:, 512:516].copy().view(np.uint32).ravel()
+    assert (ddeg == 64).all()
+    np.savez_compressed(os.path.join(HERE, "diskann_graph.npz"),
+                        adj=drec[:, 516:].copy().view(np.int32))
     wl, metrics = {}, {}
     for name, dim, vis_dir in (("sift", 128, "sift"), ("custom_sift", 128, "custom_sift"),
                                ("toy", 2, "toy"), ("vis", 2, "step_by_step")):

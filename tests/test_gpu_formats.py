@@ -12,7 +12,11 @@
   and through the record-file loader equal to the in-memory call.
 * Index file: a FRESH index after inserts and deletes, saved and loaded,
   answers every precision's searches identically and keeps updating
-  identically (the FP8 scales of the build survive the round trip)."""
+  identically (the FP8 scales of the build survive the round trip).
+* BANG / DiskANN disk index: the DiskANN-built SIFT10K graph installed with
+  its medoid 3732 and written with vmn_save_bang_index is the reference's
+  sift10k_index_disk.bin and _metadata.bin byte for byte; loaded back it
+  searches exactly like the oracle on that graph."""
 import hashlib
 import json
 import os
@@ -131,3 +135,21 @@ def test_index_file_round_trip(vmn, gpu, tmp_path):
         ix.consolidate()
     assert np.array_equal(back.adjacency, idx.adjacency)
     assert np.array_equal(back.lowp_rows("fp8"), idx.lowp_rows("fp8"))
+
+
+def test_bang_index_is_the_reference_file(vmn, orc, sift, gpu, tmp_path, digests):
+    base = sift["base"].astype(np.float32)
+    adj = np.load(os.path.join(GOLD, "diskann_graph.npz"))["adj"]
+    idx = vmn.Vamana.from_graph(base, adj, medoid=3732, mode="fresh", R=64)
+    ip, mp = str(tmp_path / "sift10k_index_disk.bin"), str(tmp_path / "sift10k_index_disk_metadata.bin")
+    idx.save_bang_index(ip, mp)
+    assert digest(ip) == digests["data/sift10k/sift10k_index_disk.bin"]
+    assert digest(mp) == digests["data/sift10k/sift10k_index_disk_metadata.bin"]
+    back = vmn.Vamana.load_bang_index(ip, mp, mode="fresh")
+    assert back.medoid == 3732 and np.array_equal(back.adjacency, adj)
+    q = sift["query"].astype(np.float32)
+    for L in (10, 40):
+        ids, d = back.searchPoints(q, k=10, L=L)
+        oi, od, _, _ = orc.fresh_search(base, adj, q, 3732, L=L)
+        assert np.array_equal(ids, oi) and np.array_equal(d, od)
+    assert vmn.recall_at_k(ids, sift["gt100"][:, :10], 10) > 0.95
