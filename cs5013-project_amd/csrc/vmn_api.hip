@@ -763,6 +763,34 @@ int vmn_groundtruth(const float* base, int64_t n, int32_t dim, const float* quer
     return VMN_OK;
 }
 
+double vmn_recall(const int32_t* found, int32_t k_found, const int32_t* gt, const float* gt_dists,
+                  int32_t k_gt, int64_t nq, int32_t recall_at) {
+    if (!found || !gt || nq <= 0 || recall_at < 1 || k_found < recall_at || k_gt < recall_at)
+        return fail(VMN_ERR_ARG, "bad recall arguments"), -1.0;
+    double total = 0.0;
+    std::vector<int32_t> res;
+    for (int64_t i = 0; i < nq; ++i) {
+        const int32_t* g = gt + i * k_gt;
+        int64_t m = recall_at;
+        if (gt_dists) {
+            const float* gd = gt_dists + i * k_gt;
+            m = recall_at - 1;
+            while (m < k_gt && gd[m] == gd[recall_at - 1]) ++m;
+        }
+        res.assign(found + i * k_found, found + i * k_found + recall_at);
+        std::sort(res.begin(), res.end());
+        res.erase(std::unique(res.begin(), res.end()), res.end());
+        std::vector<int32_t> gs(g, g + m);
+        std::sort(gs.begin(), gs.end());
+        gs.erase(std::unique(gs.begin(), gs.end()), gs.end());
+        int hits = 0;
+        for (int32_t v : gs)
+            if (v >= 0 && std::binary_search(res.begin(), res.end(), v)) ++hits;
+        total += hits;
+    }
+    return total / ((double)nq * recall_at);
+}
+
 int vmn_insert(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out, int32_t mem) {
     if (!idx || (!vecs && n)) return fail(VMN_ERR_ARG, "NULL argument");
     if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");

@@ -136,6 +136,8 @@ def lib():
         L.vmn_server_get_stats.restype = ctypes.c_int
         L.vmn_server_pending.restype = i64
         L.vmn_server_destroy.argtypes = [P]
+        L.vmn_recall.argtypes = [P, i32, P, P, i32, i64, i32]
+        L.vmn_recall.restype = ctypes.c_double
         L.vmn_read_workload.argtypes = [ctypes.c_char_p, i32, P, P, P, i64, ctypes.POINTER(i64)]
         L.vmn_read_vectors.argtypes = [ctypes.c_char_p, i32, P, i64, ctypes.POINTER(i64),
                                        ctypes.POINTER(i32)]
@@ -603,14 +605,25 @@ def groundtruth(base, queries, k, device=0):
     return ids, d
 
 
-def recall_at_k(found, gt_ids, k=10):
-    """Mean |found[:, :k] ∩ gt[:, :k]| / k (engine.cu:506-528 counts the same)."""
-    found = np.asarray(found)[:, :k]
-    gt = np.asarray(gt_ids)[:, :k]
-    hits = 0
-    for f, g in zip(found, gt):
-        hits += len(set(int(x) for x in f if x >= 0) & set(int(x) for x in g))
-    return hits / (k * found.shape[0])
+def recall_at_k(found, gt_ids, k=10, gt_dists=None):
+    """Mean |found[:, :k] ∩ gt[:, :k]| / k (engine.cu:506-528 counts the same);
+    with gt_dists the ground-truth set also takes the ids tied with the k-th
+    (calculate_recall, bang_exact/src/main.cu:87-137). Computed by vmn_recall."""
+    if _is_torch(found):
+        found = found.cpu().numpy()
+    if _is_torch(gt_ids):
+        gt_ids = gt_ids.cpu().numpy()
+    f = np.ascontiguousarray(found, dtype=np.int32)
+    g = np.ascontiguousarray(gt_ids, dtype=np.int32)
+    d = None if gt_dists is None else np.ascontiguousarray(
+        gt_dists.cpu().numpy() if _is_torch(gt_dists) else gt_dists, dtype=np.float32)
+    r = lib().vmn_recall(f.ctypes.data_as(ctypes.c_void_p), f.shape[1],
+                         g.ctypes.data_as(ctypes.c_void_p),
+                         None if d is None else d.ctypes.data_as(ctypes.c_void_p), g.shape[1],
+                         f.shape[0], int(k))
+    if r < 0:
+        raise VamanaError(-1, lib().vmn_last_error().decode())
+    return r
 
 
 class SearchContext:

@@ -257,6 +257,16 @@ void* vmn_stream(const vmn_index* idx);
 int vmn_groundtruth(const float* base, int64_t n, int32_t dim, const float* queries, int64_t nq,
                     int32_t k, int32_t* ids_out, double* dists_out, int32_t mem, int32_t device);
 
+/* Mean recall@recall_at of nq result rows (found [nq x k_found], -1 padded)
+ * against ground truth rows gt [nq x k_gt] (k_gt >= recall_at): the share of
+ * the first recall_at ground-truth ids found among the first recall_at
+ * results. With gt_dists [nq x k_gt] (may be NULL) the ground-truth set also
+ * takes every further id whose distance equals the recall_at-th one (ties),
+ * as calculate_recall (bang_exact/src/main.cu:87-137) does; the result is a
+ * fraction, not a percentage. Host memory. */
+double vmn_recall(const int32_t* found, int32_t k_found, const int32_t* gt, const float* gt_dists,
+                  int32_t k_gt, int64_t nq, int32_t recall_at);
+
 /* Search contexts. Each has its own HIP stream and scratch, so searches on
  * different contexts run concurrently on the device. Searches only read the
  * index: the caller must not run an update (build, insert, delete,

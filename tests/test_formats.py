@@ -77,3 +77,20 @@ def test_bad_vector_files(vmn, tmp_path):
     odd.write_bytes(np.array([2, 2], np.uint32).tobytes() + bytes(12))
     with pytest.raises(vmn.VamanaError):
         vmn.read_vectors(str(odd), "bin")
+
+
+def test_recall_with_and_without_ties(vmn, sift):
+    """vmn_recall: found [1, 2, 9] against ground truth [1, 2, 3, 9, 8] whose
+    third and fourth ids are tied: 2/3 plain; 3/3 when the tie extends the set
+    (calculate_recall, bang_exact/src/main.cu:87-137)."""
+    found = np.array([[1, 2, 9, -1]], np.int32)
+    gt = np.array([[1, 2, 3, 9, 8]], np.int32)
+    d = np.array([[0.0, 1.0, 2.0, 2.0, 3.0]], np.float32)
+    assert vmn.recall_at_k(found, gt, 3) == pytest.approx(2 / 3)
+    assert vmn.recall_at_k(found, gt, 3, gt_dists=d) == pytest.approx(1.0)
+    # -1 results never count; repeated results count once
+    assert vmn.recall_at_k(np.array([[-1, 1, 1]], np.int32), gt, 3) == pytest.approx(1 / 3)
+    # the SIFT10K top-10 sets of siftsmall_groundtruth.ivecs and of the DiskANN
+    # truthset (with its distances) agree for all 100 queries
+    assert vmn.recall_at_k(sift["gt100"][:, :10], sift["diskann_gt_ids"], 10,
+                           gt_dists=sift["diskann_gt_dists"]) == 1.0
