@@ -732,7 +732,7 @@ int vmn_groundtruth(const float* base, int64_t n, int32_t dim, const float* quer
     if (rc) return rc;
     rc = stage_rows(idx, queries, nq, mem, idx->qpad, &q);
     if (rc) return rc;
-    int splits = (int)std::max<int64_t>(1, std::min<int64_t>(64, (int64_t)idx->cus * 256 / std::max<int64_t>(nq, 1)));
+    int splits = (int)std::max<int64_t>(1, std::min<int64_t>(256, (int64_t)idx->cus * 256 / std::max<int64_t>(nq, 1)));
     splits = (int)std::min<int64_t>(splits, n);
     const int KP = k <= 16 ? 16 : 128;
     DevBuf pd, pi, oi, od;

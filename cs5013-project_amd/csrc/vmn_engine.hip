@@ -292,7 +292,7 @@ int vmn_engine_run(vmn_index* idx, const int32_t* types, const int32_t* ids, con
             HIPCHK(hipEventRecord(ev1, idx->stream));
             const int64_t n = idx->n;
             int splits = (int)std::max<int64_t>(
-                1, std::min<int64_t>(64, (int64_t)idx->cus * 256 / std::max<int64_t>(nq, 1)));
+                1, std::min<int64_t>(256, (int64_t)idx->cus * 256 / std::max<int64_t>(nq, 1)));
             splits = (int)std::min<int64_t>(splits, n);
             HIPCHK(idx->eng_part_d.reserve((size_t)nq * splits * 16 * sizeof(double)));
             HIPCHK(idx->eng_part_i.reserve((size_t)nq * splits * 16 * sizeof(int32_t)));

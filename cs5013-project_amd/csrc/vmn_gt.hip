@@ -5,8 +5,8 @@
 // double(x_k))^2; equal distances are ordered by id.
 //
 // Pass 1: one lane per (query, base split) keeps a sorted top-KP list in
-// registers while base rows stream through LDS 32 at a time. Pass 2: one lane
-// per query merges its splits' lists.
+// registers while base rows stream through LDS 32 at a time. Pass 2: one
+// wavefront per query merges its splits' lists.
 #include "vmn_kernels.hpp"
 
 namespace vmn {
@@ -87,35 +87,49 @@ __global__ __launch_bounds__(64) void gt_partial_kernel(const float* __restrict_
     }
 }
 
+// One wavefront per query: k rounds, each taking the smallest (dist, id)
+// after the previous pick; every lane scans a strided share of the query's
+// split lists and a butterfly over the wave keeps the minimum.
 template <int KP>
-__global__ void gt_merge_kernel(const double* part_d, const int32_t* part_i, int64_t nq, int splits,
-                                int k, int32_t* out_ids, double* out_d) {
-    const int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void gt_merge_kernel(const double* part_d, const int32_t* part_i,
+                                                      int64_t nq, int splits, int k,
+                                                      int32_t* out_ids, double* out_d) {
+    const int64_t qi = blockIdx.x;
     if (qi >= nq) return;
+    const int l = threadIdx.x;
     const int64_t o = qi * splits * KP;
     const int total = splits * KP;
-    // k rounds of selecting the smallest (dist, id) greater than the last pick
     double last_d = -INFINITY;
     int32_t last_i = -1;
     for (int j = 0; j < k; ++j) {
-        double bdist = INFINITY;
-        int32_t bid = -1;
-        for (int c = 0; c < total; ++c) {
+        double bd = INFINITY;
+        int32_t bi = -1;
+        for (int c = l; c < total; c += 64) {
             const double d = part_d[o + c];
             const int32_t id = part_i[o + c];
             if (id < 0) continue;
             const bool after_last = d > last_d || (d == last_d && id > last_i);
-            const bool better = d < bdist || (d == bdist && (bid < 0 || id < bid));
+            const bool better = bi < 0 || d < bd || (d == bd && id < bi);
             if (after_last && better) {
-                bdist = d;
-                bid = id;
+                bd = d;
+                bi = id;
             }
         }
-        out_ids[qi * k + j] = bid;
-        if (out_d) out_d[qi * k + j] = bdist;
-        if (bid >= 0) {
-            last_d = bdist;
-            last_i = bid;
+        for (int off = 32; off > 0; off >>= 1) {
+            const double od = __shfl_xor(bd, off);
+            const int32_t oi = __shfl_xor(bi, off);
+            if (oi >= 0 && (bi < 0 || od < bd || (od == bd && oi < bi))) {
+                bd = od;
+                bi = oi;
+            }
+        }
+        if (l == 0) {
+            out_ids[qi * k + j] = bi;
+            if (out_d) out_d[qi * k + j] = bd;
+        }
+        if (bi >= 0) {
+            last_d = bd;
+            last_i = bi;
         }
     }
 }
@@ -127,8 +141,8 @@ static void gt_launch(const float* base, int64_t n, const uint32_t* del_bits, co
     dim3 g((unsigned)((nq + 63) / 64), (unsigned)splits);
     hipLaunchKernelGGL((gt_partial_kernel<DP, KP>), g, dim3(64), 0, s, base, n, del_bits, queries,
                        nq, splits, part_d, part_i);
-    hipLaunchKernelGGL((gt_merge_kernel<KP>), dim3((unsigned)((nq + 63) / 64)), dim3(64), 0, s,
-                       part_d, part_i, nq, splits, k, out_ids, out_d);
+    hipLaunchKernelGGL((gt_merge_kernel<KP>), dim3((unsigned)nq), dim3(64), 0, s, part_d, part_i,
+                       nq, splits, k, out_ids, out_d);
 }
 
 hipError_t launch_gt(const float* base, int64_t n, int DP, const uint32_t* del_bits,

@@ -152,17 +152,32 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
             const bool gone = valid && a.del_bits && bit_test(a.del_bits, (uint32_t)nb);
             if (MODE == VMN_MODE_FRESH && valid && nb == self) valid = false;
             if (MODE == VMN_MODE_PARTH) {
-                // skip ids already in the list: neighbour j is broadcast and
-                // matched against every lane's list entries with one ballot
-                uint64_t dupm = 0;
-                for (int j = 0; j < deg; ++j) {
-                    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane(nb, j);
-                    bool hit = false;
+                // skip ids already in the list. A short list is scanned from
+                // LDS by every lane; a long one is matched the other way round:
+                // neighbour j is broadcast and compared with every lane's list
+                // entries in one ballot (deg steps instead of cnt / 4 loads).
+                bool dup = false;
+                if (cnt <= 32) {
+                    const uint32_t key = (uint32_t)nb;
+                    int i = 0;
+                    for (; i + 4 <= cnt; i += 4) {
+                        const uint4 v = *reinterpret_cast<const uint4*>(&s_id[cur][i]);
+                        dup |= ((v.x & VMN_ID_MASK) == key) | ((v.y & VMN_ID_MASK) == key) |
+                               ((v.z & VMN_ID_MASK) == key) | ((v.w & VMN_ID_MASK) == key);
+                    }
+                    for (; i < cnt; ++i) dup |= (s_id[cur][i] & VMN_ID_MASK) == key;
+                } else {
+                    uint64_t dupm = 0;
+                    for (int j = 0; j < deg; ++j) {
+                        const uint32_t c = (uint32_t)__builtin_amdgcn_readlane(nb, j);
+                        bool hit = false;
 #pragma unroll
-                    for (int k = 0; k < EPL; ++k) hit |= ent[k] == c;
-                    if (ballot(hit)) dupm |= 1ull << j;
+                        for (int k = 0; k < EPL; ++k) hit |= ent[k] == c;
+                        if (ballot(hit)) dupm |= 1ull << j;
+                    }
+                    dup = (dupm >> l) & 1ull;
                 }
-                valid = valid && !((dupm >> l) & 1ull) && !gone;
+                valid = valid && !dup && !gone;
                 __syncthreads();
             } else {
                 valid = valid && !gone;
