@@ -468,4 +468,59 @@ __device__ __forceinline__ int merge_sorted(const uint32_t* src_id, const float*
     return nc < limit ? nc : limit;
 }
 
+// merge_sorted<CAP, false> for new candidates left in place: the lanes set
+// in `km` hold them (my_id / my_d), in lane order for ties. Same merged order
+// as compacting them to lanes 0.. and calling merge_sorted, without the
+// compaction's LDS round trip.
+template <int CAP>
+__device__ __forceinline__ int merge_masked(const uint32_t* src_id, const float* src_d, int cnt,
+                                            uint32_t my_id, float my_d, uint64_t km,
+                                            uint32_t* dst_id, float* dst_d, int limit) {
+    constexpr int PER = (CAP + 63) / 64;
+    const int l = lane();
+    float bdv[PER];
+    uint32_t bidv[PER];
+    int less[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int bi = l + 64 * j;
+        bdv[j] = bi < cnt ? src_d[bi] : 0.0f;
+        bidv[j] = bi < cnt ? src_id[bi] : 0u;
+        less[j] = 0;
+    }
+    int rank = 0, ub = 0;
+    for (uint64_t m = km; m; m &= m - 1) {
+        const int t = __ffsll((unsigned long long)m) - 1;
+        const float dt = readlane_f(my_d, t);
+        int c = 0;
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            c += __popcll((unsigned long long)ballot((l + 64 * j) < cnt && bdv[j] <= dt));
+        if (l == t) ub = c;
+        rank += (dt < my_d) || (dt == my_d && t < l);
+#pragma unroll
+        for (int j = 0; j < PER; ++j) less[j] += dt < bdv[j];
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int bi = l + 64 * j;
+        if (bi < cnt) {
+            const int pos = bi + less[j];
+            if (pos < limit) {
+                dst_id[pos] = bidv[j];
+                dst_d[pos] = bdv[j];
+            }
+        }
+    }
+    if ((km >> l) & 1ull) {
+        const int pos = ub + rank;
+        if (pos < limit) {
+            dst_id[pos] = my_id;
+            dst_d[pos] = my_d;
+        }
+    }
+    const int nc = cnt + __popcll((unsigned long long)km);
+    return nc < limit ? nc : limit;
+}
+
 }  // namespace vmn

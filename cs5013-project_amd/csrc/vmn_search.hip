@@ -106,23 +106,28 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
         int cur = 0, cnt = 1, nexp = 0, hops = 0, ndist = 1;
         while (hops < a.max_hops) {
             ++hops;
-            // closest unexpanded entry
+            // closest unexpanded entry (its id and distance taken from the
+            // lane that read it)
             int p = -1;
+            uint32_t e = 0u;
+            float nd = 0.0f;
             for (int b = 0; b < cnt; b += 64) {
                 const int i = b + l;
-                const bool f = i < cnt && !(s_id[cur][i] & VMN_VIS_BIT);
-                const uint64_t m = ballot(f);
+                const uint32_t v = i < cnt ? s_id[cur][i] : VMN_VIS_BIT;
+                const float dv = i < cnt ? s_d[cur][i] : 0.0f;
+                const uint64_t m = ballot(!(v & VMN_VIS_BIT));
                 if (m) {
-                    p = b + __ffsll((unsigned long long)m) - 1;
+                    const int t = __ffsll((unsigned long long)m) - 1;
+                    p = b + t;
+                    e = (uint32_t)readlane_i((int)v, t);
+                    nd = readlane_f(dv, t);
                     break;
                 }
             }
             if (p < 0) break;
             if (MODE == VMN_MODE_PARTH && nexp >= a.visited_cap) break;
             ++nexp;
-            const uint32_t e = s_id[cur][p];
             const int32_t node = (int32_t)(e & VMN_ID_MASK);
-            const float nd = s_d[cur][p];
             __syncthreads();
             if (l == 0) {
                 s_id[cur][p] = e | VMN_VIS_BIT;
@@ -210,24 +215,11 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
             // A full list keeps its entries on ties, so a candidate at or beyond
             // its last distance can never enter: drop those before ranking.
             const float thr = cnt == a.L ? s_d[cur][cnt - 1] : INFINITY;
-            uint32_t my_id = l < nnew ? (uint32_t)s_sid[l] : 0u;
-            float my_d = l < nnew ? s_sd[l] : 0.0f;
-            const bool keep = l < nnew && my_d < thr;
-            const uint64_t km = ballot(keep);
-            const int nkeep = __popcll((unsigned long long)km);
-            if (nkeep != nnew) {
-                __syncthreads();
-                if (keep) {
-                    const int k = __popcll((unsigned long long)(km & lanemask_lt()));
-                    s_sid[k] = (int32_t)my_id;
-                    s_sd[k] = my_d;
-                }
-                __syncthreads();
-                my_id = l < nkeep ? (uint32_t)s_sid[l] : 0u;
-                my_d = l < nkeep ? s_sd[l] : 0.0f;
-            }
-            cnt = merge_sorted<LCAP, false>(s_id[cur], s_d[cur], cnt, my_id, my_d, nkeep,
-                                            s_id[cur ^ 1], s_d[cur ^ 1], a.L);
+            const uint32_t my_id = l < nnew ? (uint32_t)s_sid[l] : 0u;
+            const float my_d = l < nnew ? s_sd[l] : 0.0f;
+            const uint64_t km = ballot(l < nnew && my_d < thr);
+            cnt = merge_masked<LCAP>(s_id[cur], s_d[cur], cnt, my_id, my_d, km, s_id[cur ^ 1],
+                                     s_d[cur ^ 1], a.L);
             cur ^= 1;
             __syncthreads();
         }
