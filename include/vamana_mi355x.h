@@ -134,7 +134,8 @@ int vmn_load_parth_graph(const char* path, const vmn_params* p, int64_t medoid, 
 /* Vector files. Element type per format: float (FVECS, BIN), int32 (IVECS),
  * uint8 (BVECS).
  *   FVECS / IVECS / BVECS: per vector int32 dim, then dim elements
- *     (siftsmall/*.fvecs, *.ivecs; FVecs::load_from_fvecs_file, vamana.cu:108-153);
+ *     (the siftsmall .fvecs / .ivecs files; FVecs::load_from_fvecs_file,
+ *     vamana.cu:108-153);
  *   BIN: uint32 n, uint32 dim, then n x dim floats (FVecs::save_to_binary /
  *     load_from_binary, vamana.cu:156-195; scripts/vecs_to_binary.py:21-39;
  *     data/sift10k/siftsmall_query.bin).

@@ -101,3 +101,23 @@ def test_read_workload_without_gpu(vmn, tmp_path):
     assert np.bincount(rt).tolist() == [90, 183, 727]
     with pytest.raises(vmn.VamanaError):
         vmn.read_workload(str(tmp_path / "missing.bin"), 128)
+
+
+def _compile_parth_engine(out):
+    """The C++ driver in cs5013-project_amd/examples/parth_engine.cpp, built
+    against include/vamana_mi355x.h and libvmn.so with the host compiler."""
+    import subprocess
+    pkg = os.path.join(ROOT, "cs5013-project_amd")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I",
+           os.path.join(ROOT, "include"), os.path.join(pkg, "examples", "parth_engine.cpp"),
+           "-L", pkg, "-lvmn", "-Wl,-rpath," + pkg, "-Wl,-rpath-link,/opt/rocm/lib", "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return out
+
+
+def test_cpp_driver_builds_against_the_c_abi(tmp_path):
+    import subprocess
+    exe = _compile_parth_engine(str(tmp_path / "parth_engine"))
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage:" in r.stdout

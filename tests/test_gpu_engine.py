@@ -128,3 +128,31 @@ def test_gpu_engine_python_mirror(vmn, sift, gpu, tmp_path):
     assert [r["L"] for r in rows] == [32, 100]
     out = json.load(open(tmp_path / "metrics.json"))
     assert abs(out["recall_10"] - m["recall_10"]) <= 0.002 and out["qps"] > m["qps"]
+
+
+def test_gpu_cpp_driver_replays_reference_workload(sift, gpu, tmp_path):
+    """The C++ driver (examples/parth_engine.cpp) run like the reference's
+    engine binary on the reference's SIFT graph and workload files: five sweep
+    rows and metrics.json at beam 100 with the reference's recall."""
+    import subprocess
+    from test_abi import _compile_parth_engine
+    w = np.load(os.path.join(GOLDEN, "parth_workloads.npz"))
+    m = json.load(open(os.path.join(GOLDEN, "parth_metrics.json")))["sift"]
+    g = tmp_path / "sift_initial_graph.bin"
+    g.write_bytes(np.array([10000, 128, 32], np.int32).tobytes()
+                  + sift["base"].astype(np.float32).tobytes()
+                  + sift["parth_adj"].astype(np.int32).tobytes())
+    rec = np.zeros(len(w["sift_types"]), dtype=[("t", "<i4"), ("i", "<i4"), ("v", "<f4", (128,))])
+    rec["t"], rec["i"], rec["v"] = w["sift_types"], w["sift_ids"], w["sift_vecs"]
+    wf = tmp_path / "sift_workload.bin"
+    wf.write_bytes(rec.tobytes())
+    exe = _compile_parth_engine(str(tmp_path / "parth_engine"))
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([exe, str(g), str(wf), "50", str(out)], capture_output=True, text=True,
+                       timeout=100)
+    assert r.returncode == 0, r.stderr
+    rows = [ln.split() for ln in r.stdout.splitlines() if ln[:1].isdigit()]
+    assert [int(x[0]) for x in rows] == [10, 20, 32, 64, 100]
+    res = json.load(open(out / "metrics.json"))
+    assert abs(res["recall_10"] - m["recall_10"]) <= 0.002 and res["qps"] > m["qps"]
