@@ -634,6 +634,38 @@ int vmn_search(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int3
     return vmn_search_ex(idx, queries, nq, k, L, VMN_PREC_FP32, 0, ids_out, dists_out, mem);
 }
 
+int vmn_search_vectors(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int32_t L,
+                       float* vecs_out, int32_t mem) {
+    if (!idx || !queries || !vecs_out) return fail(VMN_ERR_ARG, "NULL argument");
+    if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");
+    if (nq < 0 || k < 1 || k > L || L < 1 || L > 512)
+        return fail(VMN_ERR_ARG, "need 1 <= k <= L <= 512 (k=%d, L=%d)", k, L);
+    if (nq == 0) return VMN_OK;
+    int rc = set_device(idx);
+    if (rc) return rc;
+    const float* q = nullptr;
+    rc = stage_rows(idx, queries, nq, mem, idx->qpad, &q);
+    if (rc) return rc;
+    HIPCHK(idx->out_ids.reserve((size_t)nq * k * sizeof(int32_t)));
+    rc = run_search(idx, q, nq, L, k, idx->out_ids.as<int32_t>(), nullptr, false, idx->p.mode,
+                    false);
+    if (rc) return rc;
+    const int D = idx->p.dim;
+    float* dst = vecs_out;
+    if (mem == VMN_MEM_HOST) {
+        HIPCHK(idx->staging.reserve((size_t)nq * k * D * sizeof(float)));
+        dst = idx->staging.as<float>();
+    }
+    HIPCHK(vmn::launch_rows_by_id(idx->out_ids.as<int32_t>(), nq * k, idx->vecs.as<float>(),
+                                  idx->DP, D, dst, idx->stream));
+    if (mem == VMN_MEM_HOST) {
+        HIPCHK(hipMemcpyAsync(vecs_out, dst, (size_t)nq * k * D * sizeof(float),
+                              hipMemcpyDeviceToHost, idx->stream));
+        HIPCHK(hipStreamSynchronize(idx->stream));
+    }
+    return VMN_OK;
+}
+
 int vmn_get_lowp_rows(const vmn_index* idx, int32_t prec, float* out) {
     if (!idx || !out) return fail(VMN_ERR_ARG, "NULL argument");
     const int DP = idx->DP, D = idx->p.dim;

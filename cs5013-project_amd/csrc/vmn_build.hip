@@ -43,6 +43,25 @@ hipError_t launch_pad_rows(const float* src, int64_t n, int D, float* dst, int D
     return hipGetLastError();
 }
 
+// dst[i] = row ids[i] of vecs ([cap x DP]) cut to D columns, zeros for id -1.
+__global__ void rows_by_id_kernel(const int32_t* ids, int64_t m, const float* vecs, int DP, int D,
+                                  float* dst) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = m * D;
+    for (; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i / D;
+        const int c = (int)(i - r * D);
+        const int32_t id = ids[r];
+        dst[i] = id >= 0 ? vecs[(int64_t)id * DP + c] : 0.0f;
+    }
+}
+hipError_t launch_rows_by_id(const int32_t* ids, int64_t m, const float* vecs, int DP, int D,
+                             float* dst, hipStream_t s) {
+    hipLaunchKernelGGL(rows_by_id_kernel, dim3(fill_grid(m * D)), dim3(256), 0, s, ids, m, vecs, DP,
+                       D, dst);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Random initial graph: parth/src/build_graph.cu:28-56. Per node a 64-bit LCG
 // (multiplier 6364136223846793005, increment 1442695040888963407) seeded with

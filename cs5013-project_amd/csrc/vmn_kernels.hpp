@@ -91,6 +91,8 @@ hipError_t launch_random_init(int32_t* adj, int64_t n, int R, int W, uint64_t se
 hipError_t launch_medoid(const float* vecs, int64_t n, int D, int DP, const uint32_t* del_bits,
                          float* centroid, unsigned long long* best, hipStream_t s);
 hipError_t launch_pad_rows(const float* src, int64_t n, int D, float* dst, int DP, hipStream_t s);
+hipError_t launch_rows_by_id(const int32_t* ids, int64_t m, const float* vecs, int DP, int D,
+                             float* dst, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 hipError_t launch_reverse_count(const int32_t* pair_dst, int64_t npairs, int32_t* counts,

@@ -98,6 +98,7 @@ def lib():
         L.vmn_get_params.argtypes = [P, ctypes.POINTER(Params)]
         L.vmn_search.argtypes = [P, P, i64, i32, i32, P, P, i32]
         L.vmn_search_ex.argtypes = [P, P, i64, i32, i32, i32, i32, P, P, i32]
+        L.vmn_search_vectors.argtypes = [P, P, i64, i32, i32, P, i32]
         L.vmn_get_lowp_rows.argtypes = [P, i32, P]
         L.vmn_get_adjacency.argtypes = [P, P, i32]
         L.vmn_size.argtypes = [P]
@@ -166,7 +167,8 @@ def lib():
                      "vmn_server_poll", "vmn_server_destroy", "vmn_read_workload",
                      "vmn_engine_delete", "vmn_engine_insert", "vmn_engine_run",
                      "vmn_read_vectors", "vmn_write_vectors", "vmn_read_truthset",
-                     "vmn_write_truthset", "vmn_save_bang_index", "vmn_load_bang_index"):
+                     "vmn_write_truthset", "vmn_save_bang_index", "vmn_load_bang_index",
+                     "vmn_search_vectors"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -402,6 +404,23 @@ class Vamana:
         return ids, dists
 
     search = searchPoints
+
+    def searchPoints2(self, queries, k=10, L=None):
+        """The k nearest rows of each query, [nq, k, dim] (vamana.cuh:127)."""
+        L = k if L is None else L
+        nq = int(queries.shape[0])
+        qp, mem, keep = _arg(queries, np.float32)
+        if mem == MEM_DEVICE:
+            import torch
+            out = torch.empty((nq, k, self.params.dim), dtype=torch.float32, device=queries.device)
+            _check(lib().vmn_search_vectors(self._h, qp, nq, k, L,
+                                            ctypes.c_void_p(out.data_ptr()), mem))
+            self.synchronize()
+            return out
+        out = np.empty((nq, k, self.params.dim), np.float32)
+        _check(lib().vmn_search_vectors(self._h, qp, nq, k, L,
+                                        out.ctypes.data_as(ctypes.c_void_p), mem))
+        return out
 
     def synchronize(self):
         import torch

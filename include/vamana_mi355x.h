@@ -190,6 +190,12 @@ int vmn_search_ex(vmn_index* idx, const float* queries, int64_t nq, int32_t k, i
                   int32_t prec, int32_t k_rerank, int32_t* ids_out, float* dists_out,
                   int32_t mem);
 
+/* vmn_search returning the k nearest rows themselves, vecs_out [nq x k x dim]
+ * (zeros past the results found), like `searchPoints2` (vamana.cuh:127,
+ * vamana_impl.cuh:99-180), which returns the worklist's vectors. */
+int vmn_search_vectors(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int32_t L,
+                       float* vecs_out, int32_t mem);
+
 /* The rows as the low-precision traversal sees them, decoded to float
  * ([size x dim], host): BF16 values, or FP8 values times their scales. */
 int vmn_get_lowp_rows(const vmn_index* idx, int32_t prec, float* out);

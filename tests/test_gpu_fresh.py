@@ -1,6 +1,8 @@
 """GPU (HIP) FreshVamana-mode path (bloom-filter search, <= prune, reverse
 edges) against the CPU oracle's restatement of vamana/include/{search,kernels}.cuh.
 Bit-exact on integer-valued data."""
+import os
+
 import numpy as np
 import pytest
 
@@ -56,3 +58,18 @@ def test_gpu_fresh_graph_invariants(vmn, gpu):
         assert i not in r and len(set(r.tolist())) == len(r)
         # -1 only as a tail
         assert (row[len(r):] == -1).all()
+
+
+def test_gpu_search_vectors(vmn, sift, gpu):
+    """vmn_search_vectors (searchPoints2, vamana.cuh:127): the rows of the ids
+    vmn_search returns, host and device buffers alike."""
+    import torch
+    base = sift["base"].astype(np.float32)
+    g1 = np.load(os.path.join(os.path.dirname(__file__), "golden", "g1_graph.npz"))
+    idx = vmn.Vamana.from_graph(base, g1["adj"], medoid=5000, mode="fresh")
+    q = sift["query"].astype(np.float32)
+    ids, _ = idx.searchPoints(q, k=10, L=40)
+    v = idx.searchPoints2(q, k=10, L=40)
+    assert v.shape == (100, 10, 128) and np.array_equal(v, base[ids])
+    vd = idx.searchPoints2(torch.from_numpy(q).to(gpu), k=10, L=40)
+    assert np.array_equal(vd.cpu().numpy(), v)
