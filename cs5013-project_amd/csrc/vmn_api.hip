@@ -195,6 +195,10 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
         a.fp8_scales = idx->fp8_scales.as<float>();
     }
     a.lcap = idx->list_cap;
+    // a batch that leaves most of the GPU idle is latency-bound: there the
+    // BF16 row norms are read from the table (an extra 4-byte load per
+    // candidate) instead of being summed from the row registers
+    a.table_norms = nq <= (int64_t)idx->cus * 4;
     const int64_t want = std::min<int64_t>(nq, (int64_t)idx->cus * idx->waves_per_cu);
     HIPCHK(vmn::launch_search(a, (int)std::max<int64_t>(want, 1), stream));
     if (ctx) ctx->last_nq = nq;

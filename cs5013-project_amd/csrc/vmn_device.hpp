@@ -139,6 +139,8 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 #define VMN_PREC_FP32 0
 #define VMN_PREC_BF16 1
 #define VMN_PREC_FP8 2
+// kernel-internal: BF16 traversal with row norms read from the norms table
+#define VMN_PREC_BF16_TABLE_NORMS 3
 
 template <int DP>
 struct QLow {
@@ -231,15 +233,19 @@ __device__ __forceinline__ uint32_t bperm(int src_byte, uint32_t v) {
 // Distances from the prepared query to `cnt` candidates whose ids sit in LDS
 // ids[0..cnt); dist_out[s] (LDS) receives the distance of ids[s]. Two tiles
 // of 16 candidates are loaded before either is multiplied, so one memory round
-// trip serves 32 candidates. BF16 row norms come from the loaded registers
-// (v_dot2_f32_bf16); FP8 norms (per-dimension scales) from `norms`.
+// trip serves 32 candidates. FP8 row norms (per-dimension scales) come from
+// `norms`. BF16 row norms come from the loaded registers (v_dot2_f32_bf16),
+// or, with TN, from `norms` as well: one 4-byte load per tile column issued
+// with the row loads. That saves the dot products and two cross-lane sums on a
+// latency-bound batch, but costs a cache line per candidate, which a
+// bandwidth-bound batch cannot afford.
 //
 // Loads are coalesced per row: in load step s, lane l reads 16 bytes of tile
 // row l/4 at byte 64*s + 16*(l%4), so each 4-lane quad reads 64 contiguous
 // bytes of one row (one L1 access, against four when lanes are laid out as
 // the MFMA operand wants). ds_bpermute then hands every dword to the lane
 // that holds it as the B operand (lane L: column L%16, k-slice L/16).
-template <int DP, int PREC, int NG = 2>
+template <int DP, int PREC, int NG = 2, bool TN = false>
 __device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* __restrict__ rows,
                                                  const float* __restrict__ norms,
                                                  const int32_t* ids, float* dist_out, int cnt) {
@@ -266,6 +272,8 @@ __device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* 
                     const uint4* r = reinterpret_cast<const uint4*>(reinterpret_cast<const __bf16*>(rows) + (int64_t)id[t] * DP);
 #pragma unroll
                     for (int kb = 0; kb < KB; ++kb) v[t][kb] = r[4 * kb + sub];
+                    // tile column L%16's id sits in lane 4*(L%16)
+                    if constexpr (TN) x[t] = norms[__builtin_amdgcn_ds_bpermute((4 * (l & 15)) << 2, id[t])];
                 }
             }
 #pragma unroll
@@ -277,11 +285,15 @@ __device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* 
                                                bperm(src, v[t][kb].z), bperm(src, v[t][kb].w)};
                         const uint4 u = {w[0], w[1], w[2], w[3]};
                         acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Q.a[kb], __builtin_bit_cast(bf16x8_t, u), acc[t], 0, 0, 0);
+                        if constexpr (!TN) {
 #pragma unroll
-                        for (int e = 0; e < 4; ++e) { const bf16x2_t p = __builtin_bit_cast(bf16x2_t, w[e]); x[t] = __builtin_amdgcn_fdot2_f32_bf16(p, p, x[t], false); }
+                            for (int e = 0; e < 4; ++e) { const bf16x2_t p = __builtin_bit_cast(bf16x2_t, w[e]); x[t] = __builtin_amdgcn_fdot2_f32_bf16(p, p, x[t], false); }
+                        }
                     }
-                    x[t] += __shfl_xor(x[t], 16);
-                    x[t] += __shfl_xor(x[t], 32);
+                    if constexpr (!TN) {
+                        x[t] += __shfl_xor(x[t], 16);
+                        x[t] += __shfl_xor(x[t], 32);
+                    }
                 }
             }
         } else if constexpr (DP % 64 == 0) {

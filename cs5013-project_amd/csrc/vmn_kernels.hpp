@@ -33,6 +33,7 @@ struct SearchArgs {
     const float* fp8_scales;  // [DP] per-dimension FP8 scales
     int krr;                  // > 0: exact FP32 re-rank of the first krr list entries (<= 64)
     int lcap;                 // minimum list capacity template (0: smallest >= L)
+    int table_norms;          // BF16, FRESH: row norms from low_norms (latency-bound batches)
     int32_t* stat_hops;       // [nq] expansions
     int32_t* stat_ndist;      // [nq] distance evaluations
     unsigned long long* counter;  // work queue head, zero before launch

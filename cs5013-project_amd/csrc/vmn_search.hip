@@ -48,8 +48,12 @@ __global__ void fill_zero_u32(unsigned int* p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
-template <int DP, int LCAP, int MODE, int PREC>
+// PREC_ARG is a VMN_PREC_* value, or VMN_PREC_BF16_TABLE_NORMS: BF16 rows with
+// the row norms read from the norms table (small batches, FRESH mode).
+template <int DP, int LCAP, int MODE, int PREC_ARG>
 __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
+    constexpr int PREC = PREC_ARG == VMN_PREC_BF16_TABLE_NORMS ? VMN_PREC_BF16 : PREC_ARG;
+    constexpr bool TN = PREC_ARG == VMN_PREC_BF16_TABLE_NORMS;
     __shared__ __align__(16) uint32_t s_id[2][LCAP];
     __shared__ __align__(16) float s_d[2][LCAP];
     __shared__ int32_t s_sid[64];
@@ -86,7 +90,7 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
         } else {
             if (l == 0) s_sid[0] = st;
             __syncthreads();
-            gather_dists_low<DP, PREC>(ql, a.low_rows, a.low_norms, s_sid, s_sd, 1);
+            gather_dists_low<DP, PREC, 2, TN>(ql, a.low_rows, a.low_norms, s_sid, s_sd, 1);
             __syncthreads();
             sd = s_sd[0];
         }
@@ -209,7 +213,8 @@ __global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
             if (PREC == VMN_PREC_FP32)
                 gather_dists<DP>(q, vecs, s_sid, s_sd, nnew);
             else
-                gather_dists_low<DP, PREC, VMN_GATHER_TILES>(ql, a.low_rows, a.low_norms, s_sid, s_sd, nnew);
+                gather_dists_low<DP, PREC, VMN_GATHER_TILES, TN>(ql, a.low_rows, a.low_norms, s_sid,
+                                                                 s_sd, nnew);
             __syncthreads();
             ndist += nnew;
             // A full list keeps its entries on ties, so a candidate at or beyond
@@ -266,7 +271,11 @@ static hipError_t launch_search_p(const SearchArgs& a, int grid, hipStream_t s) 
             hipLaunchKernelGGL((search_kernel<DP, LCAP, MODE, VMN_PREC_FP32>), dim3(grid), dim3(64), 0, s, a);
             break;
         case VMN_PREC_BF16:
-            hipLaunchKernelGGL((search_kernel<DP, LCAP, MODE, VMN_PREC_BF16>), dim3(grid), dim3(64), 0, s, a);
+            if (MODE == VMN_MODE_FRESH && a.table_norms)
+                hipLaunchKernelGGL((search_kernel<DP, LCAP, MODE, MODE == VMN_MODE_FRESH ? VMN_PREC_BF16_TABLE_NORMS : VMN_PREC_BF16>),
+                                   dim3(grid), dim3(64), 0, s, a);
+            else
+                hipLaunchKernelGGL((search_kernel<DP, LCAP, MODE, VMN_PREC_BF16>), dim3(grid), dim3(64), 0, s, a);
             break;
         case VMN_PREC_FP8:
             hipLaunchKernelGGL((search_kernel<DP, LCAP, MODE, VMN_PREC_FP8>), dim3(grid), dim3(64), 0, s, a);
