@@ -522,9 +522,14 @@ def serving(args, vmn, torch, dev):
     for t in warm:
         while srv.poll(t) is None:
             pass
+    srv.reset_stats()
     eng0 = srv.stats()
     ins_off = 0
-    outstanding, lat, done_q, results = [], [], 0, 0
+    # deletes draw live base ids uniformly without replacement; the order is
+    # drawn once here so the request loop keeps its arrival schedule
+    del_order = rng.permutation(np.flatnonzero(alive[:n])).astype(np.int32)
+    del_off = 0
+    outstanding, lat, done_q, results, lags = [], [], 0, 0, []
     lat_upd = {vmn.REQ_INSERT: [], vmn.REQ_DELETE: []}
     t_first = None
     start = srv.now() + 0.05
@@ -540,10 +545,11 @@ def serving(args, vmn, torch, dev):
             data = extra[ins_off:ins_off + sz]
             ins_off += sz
         else:
-            ids = rng.choice(np.flatnonzero(alive[:n]), sz, replace=False).astype(np.int32)
-            alive[ids] = False
-            data = ids
+            data = del_order[del_off:del_off + sz]
+            del_off += sz
+            alive[data] = False
         outstanding.append((srv.submit(kind, data, t_arrival=target), kind, sz))
+        lags.append(srv.now() - target)
         # reap finished requests from the front
         while outstanding:
             r = srv.poll(outstanding[0][0])
@@ -626,6 +632,9 @@ def serving(args, vmn, torch, dev):
                       "p90_ms": round(float(np.percentile(lat_ms, 90)), 3),
                       "p999_ms": round(float(np.percentile(lat_ms, 99.9)), 3),
                       "update_latency": upd, "engine": engine,
+                      "client_submit_lag_ms": {
+                          "p99": round(float(np.percentile(lags, 99)) * 1e3, 3),
+                          "max": round(float(np.max(lags)) * 1e3, 3)},
                       "recall@10_bf16_rerank": round(rec, 4), "recall@10_fp32": round(rec32, 4),
                       "recall@10_bf16_rerank_at_L": curve}}
     print(json.dumps(out), flush=True)

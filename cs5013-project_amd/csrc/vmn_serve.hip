@@ -355,6 +355,13 @@ int vmn_server_get_stats(const vmn_server* s, vmn_server_stats* out) {
     return VMN_OK;
 }
 
+int vmn_server_reset_stats(vmn_server* s) {
+    if (!s) return -1;
+    std::lock_guard<std::mutex> g(s->m);
+    s->counters = vmn_server_stats{};
+    return VMN_OK;
+}
+
 int vmn_server_destroy(vmn_server* s) {
     if (!s) return VMN_OK;
     s->stop = true;

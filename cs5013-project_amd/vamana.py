@@ -135,6 +135,8 @@ def lib():
         L.vmn_server_pending.argtypes = [P]
         L.vmn_server_get_stats.argtypes = [P, ctypes.POINTER(ServerStats)]
         L.vmn_server_get_stats.restype = ctypes.c_int
+        L.vmn_server_reset_stats.argtypes = [P]
+        L.vmn_server_reset_stats.restype = ctypes.c_int
         L.vmn_server_pending.restype = i64
         L.vmn_server_destroy.argtypes = [P]
         L.vmn_recall.argtypes = [P, i32, P, P, i32, i64, i32]
@@ -750,6 +752,9 @@ class Server:
         _check(lib().vmn_server_get_stats(self._h, ctypes.byref(st)))
         return {name: getattr(st, name) for name, _ in ServerStats._fields_}
 
+    def reset_stats(self):
+        """Zero the engine counters (after a warm-up)."""
+        _check(lib().vmn_server_reset_stats(self._h))
     def close(self):
         if self._h:
             lib().vmn_server_destroy(self._h)

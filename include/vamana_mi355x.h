@@ -336,6 +336,8 @@ typedef struct vmn_server_stats {
     double drain_s;                              /* time updates waited for launched searches */
 } vmn_server_stats;
 int vmn_server_get_stats(const vmn_server* srv, vmn_server_stats* out);
+/* Zeroes the counters (e.g. after a warm-up), maxima included. */
+int vmn_server_reset_stats(vmn_server* srv);
 
 /* Finishes the queued requests, then stops the engine. */
 int vmn_server_destroy(vmn_server* srv);

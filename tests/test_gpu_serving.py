@@ -54,6 +54,8 @@ def test_server_mixed_equals_alone(vmn, gpu, prec, rr):
     assert st["inserts"] == sum(k == vmn.REQ_INSERT for k, _ in reqs)
     assert st["deletes"] == sum(k == vmn.REQ_DELETE for k, _ in reqs)
     assert st["insert_s"] > 0 and st["max_update_s"] <= st["insert_s"] + st["delete_s"]
+    srv2_stats = srv_reset_check(vmn, base, kw)
+    assert all(v == 0 for v in srv2_stats.values())
     # replay alone, in arrival order, on an identically built index
     ref = vmn.Vamana(base, **kw)
     for t, (kind, data) in zip(tickets, reqs):
@@ -84,3 +86,17 @@ def test_server_admission_by_memory(vmn, gpu):
         pass
     assert rejected > 0
     srv.close()
+
+
+def srv_reset_check(vmn, base, kw):
+    """vmn_server_reset_stats zeroes every counter, maxima included."""
+    idx = vmn.Vamana(base[:2000], **kw)
+    srv = vmn.Server(idx, k=10, L=32)
+    t = srv.submit(vmn.REQ_SEARCH, base[:5])
+    while srv.poll(t) is None:
+        pass
+    assert srv.stats()["queries"] == 5
+    srv.reset_stats()
+    st = srv.stats()
+    srv.close()
+    return st
