@@ -454,17 +454,8 @@ int vmn_destroy(vmn_index* idx) {
     if (!idx) return VMN_OK;
     (void)hipSetDevice(idx->p.device);
     (void)hipStreamSynchronize(idx->stream);
-    DevBuf* bufs[] = {&idx->vecs, &idx->adj, &idx->del_bits, &idx->qpad, &idx->out_ids,
-                      &idx->out_d, &idx->stat_hops, &idx->stat_ndist, &idx->counter,
-                      &idx->cand_ids, &idx->cand_d, &idx->vis_ids, &idx->vis_d, &idx->vis_cnt,
-                      &idx->pair_dst, &idx->pair_d, &idx->rcount, &idx->rfill, &idx->roff,
-                      &idx->rscan_tmp, &idx->rsrc, &idx->rd, &idx->rlist, &idx->centroid,
-                      &idx->medoid_key, &idx->staging, &idx->find_out, &idx->bad,
-                      &idx->bf16_rows, &idx->bf16_norms, &idx->fp8_rows, &idx->fp8_norms,
-                      &idx->fp8_scales, &idx->fp8_absmax};
-    for (DevBuf* b : bufs) b->release();
     (void)hipStreamDestroy(idx->stream);
-    delete idx;
+    delete idx;  // every device buffer is freed by its destructor
     return VMN_OK;
 }
 
@@ -1013,11 +1004,8 @@ int vmn_ctx_destroy(vmn_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     if (c->exec) (void)hipGraphExecDestroy(c->exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
-    c->counter.release();
-    c->stat_hops.release();
-    c->stat_ndist.release();
     (void)hipStreamDestroy(c->stream);
-    delete c;
+    delete c;  // frees the context's device buffers
     return VMN_OK;
 }
 

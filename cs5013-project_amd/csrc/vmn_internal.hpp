@@ -24,10 +24,15 @@ int fail(int code, const char* fmt, ...);
                         __FILE__, __LINE__);                                                \
     } while (0)
 
-// Device allocation that grows on demand and is kept across calls.
+// Device allocation that grows on demand and is kept across calls; freed
+// with its owner (the owner's device must be current then).
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
     hipError_t reserve(size_t want) {
         if (want <= bytes) return hipSuccess;
         const size_t old = bytes;
