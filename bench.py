@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--R", type=int, default=64)
     ap.add_argument("--L-build", type=int, default=128)
     ap.add_argument("--alphas", default="1.0,1.2")
-    ap.add_argument("--sweep", default="10,20,40,60,80,100,120,150,200")
+    ap.add_argument("--sweep", default="10,20,40,50,60,70,80,100,120,150,200")
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--rerank", type=int, default=40)
