@@ -103,21 +103,22 @@ def test_read_workload_without_gpu(vmn, tmp_path):
         vmn.read_workload(str(tmp_path / "missing.bin"), 128)
 
 
-def _compile_parth_engine(out):
-    """The C++ driver in cs5013-project_amd/examples/parth_engine.cpp, built
-    against include/vamana_mi355x.h and libvmn.so with the host compiler."""
+def compile_example(name, out):
+    """A C++ driver in cs5013-project_amd/examples/, built against
+    include/vamana_mi355x.h and libvmn.so with the host compiler."""
     import subprocess
     pkg = os.path.join(ROOT, "cs5013-project_amd")
     cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I",
-           os.path.join(ROOT, "include"), os.path.join(pkg, "examples", "parth_engine.cpp"),
+           os.path.join(ROOT, "include"), os.path.join(pkg, "examples", name + ".cpp"),
            "-L", pkg, "-lvmn", "-Wl,-rpath," + pkg, "-Wl,-rpath-link,/opt/rocm/lib", "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     return out
 
 
-def test_cpp_driver_builds_against_the_c_abi(tmp_path):
+@pytest.mark.parametrize("name", ["parth_engine", "fresh_vamana"])
+def test_cpp_drivers_build_against_the_c_abi(tmp_path, name):
     import subprocess
-    exe = _compile_parth_engine(str(tmp_path / "parth_engine"))
+    exe = compile_example(name, str(tmp_path / name))
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 1 and "Usage:" in r.stdout

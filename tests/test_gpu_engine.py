@@ -135,7 +135,7 @@ def test_gpu_cpp_driver_replays_reference_workload(sift, gpu, tmp_path):
     engine binary on the reference's SIFT graph and workload files: five sweep
     rows and metrics.json at beam 100 with the reference's recall."""
     import subprocess
-    from test_abi import _compile_parth_engine
+    from test_abi import compile_example
     w = np.load(os.path.join(GOLDEN, "parth_workloads.npz"))
     m = json.load(open(os.path.join(GOLDEN, "parth_metrics.json")))["sift"]
     g = tmp_path / "sift_initial_graph.bin"
@@ -146,7 +146,7 @@ def test_gpu_cpp_driver_replays_reference_workload(sift, gpu, tmp_path):
     rec["t"], rec["i"], rec["v"] = w["sift_types"], w["sift_ids"], w["sift_vecs"]
     wf = tmp_path / "sift_workload.bin"
     wf.write_bytes(rec.tobytes())
-    exe = _compile_parth_engine(str(tmp_path / "parth_engine"))
+    exe = compile_example("parth_engine", str(tmp_path / "parth_engine"))
     out = tmp_path / "out"
     out.mkdir()
     r = subprocess.run([exe, str(g), str(wf), "50", str(out)], capture_output=True, text=True,

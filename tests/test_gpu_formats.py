@@ -153,3 +153,32 @@ def test_bang_index_is_the_reference_file(vmn, orc, sift, gpu, tmp_path, digests
         oi, od, _, _ = orc.fresh_search(base, adj, q, 3732, L=L)
         assert np.array_equal(ids, oi) and np.array_equal(d, od)
     assert vmn.recall_at_k(ids, sift["gt100"][:, :10], 10) > 0.95
+
+
+def test_cpp_fresh_vamana_driver(sift, gpu, tmp_path):
+    """examples/fresh_vamana.cpp, the reference's FreshVamana driver over the
+    C ABI, on the reference's random-graph file: builds, finds node 0, finds
+    the inserted copy of it (new id 10000, distance 1), writes 10,001 records."""
+    import subprocess
+    from test_abi import compile_example
+    base = sift["base"].astype(np.float32)
+    rg = np.load(os.path.join(GOLD, "randgraph.npz"))
+    rec = np.zeros(10000, dtype=[("v", "<f4", (128,)), ("deg", "<u4"), ("nbr", "<u4", (64,))])
+    rec["v"], rec["deg"] = base, rg["degree"]
+    rec["nbr"] = np.where(rg["adj"] >= 0, rg["adj"], 0).astype(np.uint32)
+    g = tmp_path / "sift10k_randomgraph.bin"
+    g.write_bytes(rec.tobytes())
+    assert digest(str(g))["bytes"] == 7720000
+    exe = compile_example("fresh_vamana", str(tmp_path / "fresh_vamana"))
+    out = tmp_path / "out.bin"
+    r = subprocess.run([exe, str(g), "unused", str(out)], capture_output=True, text=True,
+                       timeout=100)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    first = [ln for ln in lines if ln.strip().startswith("id ")]
+    assert first[0].split()[1] == "0" and first[0].split()[3] == "0.0"
+    assert "inserted id 10000" in r.stdout
+    # the inserted vector is node 0 plus 1 in its first coordinate
+    assert first[5].split()[1] == "10000" and first[5].split()[3] == "0.0"
+    assert first[6].split()[1] == "0" and first[6].split()[3] == "1.0"
+    assert os.path.getsize(out) == 10001 * 772
