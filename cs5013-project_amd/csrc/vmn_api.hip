@@ -83,6 +83,7 @@ int alloc_graph(vmn_index* idx, int64_t n) {
     const int64_t cap = std::max<int64_t>(idx->p.capacity, n);
     idx->cap = cap;
     idx->n_deleted = 0;
+    idx->n_pend = 0;
     idx->freelist.clear();
     idx->free_count = 0;
     HIPCHK(idx->vecs.reserve((size_t)cap * idx->DP * sizeof(float)));
@@ -610,8 +611,7 @@ int vmn_search_ex(vmn_index* idx, const float* queries, int64_t nq, int32_t k, i
         oi = idx->out_ids.as<int32_t>();
         od = idx->out_d.as<float>();
     }
-    rc = run_search(idx, q, nq, L, k, oi, od, false, idx->p.mode, false, 0, prec,
-                    prec == VMN_PREC_FP32 ? 0 : k_rerank);
+    rc = search_into(idx, q, nq, k, L, prec, prec == VMN_PREC_FP32 ? 0 : k_rerank, oi, od);
     if (rc) return rc;
     if (mem == VMN_MEM_HOST) {
         HIPCHK(hipMemcpyAsync(ids_out, oi, (size_t)nq * k * sizeof(int32_t), hipMemcpyDeviceToHost,
@@ -642,8 +642,7 @@ int vmn_search_vectors(vmn_index* idx, const float* queries, int64_t nq, int32_t
     rc = stage_rows(idx, queries, nq, mem, idx->qpad, &q);
     if (rc) return rc;
     HIPCHK(idx->out_ids.reserve((size_t)nq * k * sizeof(int32_t)));
-    rc = run_search(idx, q, nq, L, k, idx->out_ids.as<int32_t>(), nullptr, false, idx->p.mode,
-                    false);
+    rc = search_into(idx, q, nq, k, L, VMN_PREC_FP32, 0, idx->out_ids.as<int32_t>(), nullptr);
     if (rc) return rc;
     const int D = idx->p.dim;
     float* dst = vecs_out;
@@ -652,7 +651,7 @@ int vmn_search_vectors(vmn_index* idx, const float* queries, int64_t nq, int32_t
         dst = idx->staging.as<float>();
     }
     HIPCHK(vmn::launch_rows_by_id(idx->out_ids.as<int32_t>(), nq * k, idx->vecs.as<float>(),
-                                  idx->DP, D, dst, idx->stream));
+                                  idx->DP, D, idx->pend.as<float>(), idx->n, dst, idx->stream));
     if (mem == VMN_MEM_HOST) {
         HIPCHK(hipMemcpyAsync(vecs_out, dst, (size_t)nq * k * D * sizeof(float),
                               hipMemcpyDeviceToHost, idx->stream));
@@ -818,20 +817,23 @@ double vmn_recall(const int32_t* found, int32_t k_found, const int32_t* gt, cons
     return total / ((double)nq * recall_at);
 }
 
-int vmn_insert(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out, int32_t mem) {
-    if (!idx || (!vecs && n)) return fail(VMN_ERR_ARG, "NULL argument");
-    if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");
-    if (n < 0) return fail(VMN_ERR_ARG, "negative count");
-    if (n == 0) return VMN_OK;
-    if (idx->n + n > idx->cap)
-        return fail(VMN_ERR_CAPACITY, "insert of %lld exceeds capacity %lld (size %lld)",
-                    (long long)n, (long long)idx->cap, (long long)idx->n);
+}  // extern "C"
+
+namespace vmn_rt {
+
+// FreshVamana insert of n rows: host or device rows of width dim (`mem`), or,
+// with padded, device rows of stride DP.
+int insert_rows(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out, int32_t mem,
+                bool padded) {
     int rc = set_device(idx);
     if (rc) return rc;
     const int64_t base = idx->n;
     const int DP = idx->DP;
     float* dst = idx->vecs.as<float>() + base * DP;
-    if (mem == VMN_MEM_DEVICE) {
+    if (padded) {
+        HIPCHK(hipMemcpyAsync(dst, vecs, (size_t)n * DP * sizeof(float), hipMemcpyDeviceToDevice,
+                              idx->stream));
+    } else if (mem == VMN_MEM_DEVICE) {
         HIPCHK(vmn::launch_pad_rows(vecs, n, idx->p.dim, dst, DP, idx->stream));
     } else {
         const float* staged = nullptr;
@@ -859,6 +861,111 @@ int vmn_insert(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out, i
     HIPCHK(hipStreamSynchronize(idx->stream));
     return VMN_OK;
 }
+
+int apply_staged(vmn_index* idx) {
+    const int64_t m = idx->n_pend;
+    if (!m) return VMN_OK;
+    idx->n_pend = 0;
+    return insert_rows(idx, idx->pend.as<float>(), m, nullptr, VMN_MEM_DEVICE, true);
+}
+
+// Grows `b` to `want` bytes keeping its first `used` bytes.
+int grow_keep(DevBuf& b, size_t want, size_t used, hipStream_t s) {
+    if (want <= b.bytes) return VMN_OK;
+    DevBuf nb;
+    HIPCHK(nb.reserve(std::max(want, b.bytes * 2)));
+    if (used) HIPCHK(hipMemcpyAsync(nb.p, b.p, used, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    b.release();
+    b.p = nb.p;
+    b.bytes = nb.bytes;
+    nb.p = nullptr;
+    nb.bytes = 0;
+    return VMN_OK;
+}
+
+// Search of nq padded query rows into device buffers ri / rd (rd may be
+// NULL); with staged rows present, their exact k nearest (FP64 brute force)
+// are merged into the graph's k by (distance, id), as merge_and_rerank_kernel
+// (utils.cuh:199-285) merges the insert list into the worklist.
+int search_into(vmn_index* idx, const float* q, int64_t nq, int k, int L, int prec, int krr,
+                int32_t* ri, float* rd) {
+    if (!idx->n_pend)
+        return run_search(idx, q, nq, L, k, ri, rd, false, idx->p.mode, false, 0, prec, krr);
+    if (k > 128) return fail(VMN_ERR_ARG, "searches over staged inserts take k <= 128");
+    HIPCHK(idx->pend_i.reserve((size_t)nq * k * sizeof(int32_t)));
+    HIPCHK(idx->pend_d.reserve((size_t)nq * k * sizeof(float)));
+    int rc = run_search(idx, q, nq, L, k, idx->pend_i.as<int32_t>(), idx->pend_d.as<float>(), false,
+                        idx->p.mode, false, 0, prec, krr);
+    if (rc) return rc;
+    const int64_t m = idx->n_pend;
+    int splits = (int)std::max<int64_t>(
+        1, std::min<int64_t>(256, (int64_t)idx->cus * 256 / std::max<int64_t>(nq, 1)));
+    splits = (int)std::min<int64_t>(splits, m);
+    const int KP = k <= 16 ? 16 : 128;
+    HIPCHK(idx->pend_part_d.reserve((size_t)nq * splits * KP * sizeof(double)));
+    HIPCHK(idx->pend_part_i.reserve((size_t)nq * splits * KP * sizeof(int32_t)));
+    HIPCHK(idx->pend_gi.reserve((size_t)nq * k * sizeof(int32_t)));
+    HIPCHK(idx->pend_gd.reserve((size_t)nq * k * sizeof(double)));
+    HIPCHK(vmn::launch_gt(idx->pend.as<float>(), m, idx->DP, nullptr, q, nq, k, splits,
+                          idx->pend_part_d.as<double>(), idx->pend_part_i.as<int32_t>(),
+                          idx->pend_gi.as<int32_t>(), idx->pend_gd.as<double>(), idx->stream));
+    HIPCHK(vmn::launch_merge_staged(idx->pend_i.as<int32_t>(), idx->pend_d.as<float>(),
+                                    idx->pend_gi.as<int32_t>(), idx->pend_gd.as<double>(), nq, k,
+                                    idx->n, ri, rd, idx->stream));
+    return VMN_OK;
+}
+
+}  // namespace vmn_rt
+
+extern "C" {
+
+int vmn_insert(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out, int32_t mem) {
+    if (!idx || (!vecs && n)) return fail(VMN_ERR_ARG, "NULL argument");
+    if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");
+    if (n < 0) return fail(VMN_ERR_ARG, "negative count");
+    if (n == 0) return VMN_OK;
+    if (idx->n + idx->n_pend + n > idx->cap)
+        return fail(VMN_ERR_CAPACITY, "insert of %lld exceeds capacity %lld (size %lld, staged %lld)",
+                    (long long)n, (long long)idx->cap, (long long)idx->n, (long long)idx->n_pend);
+    // staged rows were promised the next ids: they go in first
+    int rc = apply_staged(idx);
+    if (rc) return rc;
+    return insert_rows(idx, vecs, n, ids_out, mem, false);
+}
+
+int vmn_insert_staged(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out, int32_t mem) {
+    if (!idx || (!vecs && n)) return fail(VMN_ERR_ARG, "NULL argument");
+    if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");
+    if (n < 0) return fail(VMN_ERR_ARG, "negative count");
+    if (n == 0) return VMN_OK;
+    if (idx->n + idx->n_pend + n > idx->cap)
+        return fail(VMN_ERR_CAPACITY, "insert of %lld exceeds capacity %lld (size %lld, staged %lld)",
+                    (long long)n, (long long)idx->cap, (long long)idx->n, (long long)idx->n_pend);
+    int rc = set_device(idx);
+    if (rc) return rc;
+    const int DP = idx->DP;
+    const float* staged = nullptr;
+    rc = stage_rows(idx, vecs, n, mem, idx->staging, &staged);
+    if (rc) return rc;
+    const size_t row = (size_t)DP * sizeof(float);
+    rc = grow_keep(idx->pend, (size_t)(idx->n_pend + n) * row, (size_t)idx->n_pend * row,
+                   idx->stream);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(static_cast<char*>(idx->pend.p) + (size_t)idx->n_pend * row, staged,
+                          (size_t)n * row, hipMemcpyDeviceToDevice, idx->stream));
+    std::vector<int32_t> ids(n);
+    for (int64_t i = 0; i < n; ++i) ids[i] = (int32_t)(idx->n + idx->n_pend + i);
+    idx->n_pend += n;
+    if (ids_out && mem == VMN_MEM_HOST) std::memcpy(ids_out, ids.data(), n * sizeof(int32_t));
+    if (ids_out && mem == VMN_MEM_DEVICE)
+        HIPCHK(hipMemcpyAsync(ids_out, ids.data(), n * sizeof(int32_t), hipMemcpyHostToDevice,
+                              idx->stream));
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    return VMN_OK;
+}
+
+int64_t vmn_staged_count(const vmn_index* idx) { return idx ? idx->n_pend : -1; }
 
 int vmn_delete_ids(vmn_index* idx, const int32_t* ids, int64_t n, int32_t mem) {
     if (!idx || (!ids && n)) return fail(VMN_ERR_ARG, "NULL argument");
@@ -919,8 +1026,11 @@ int vmn_delete_vectors(vmn_index* idx, const float* vecs, int64_t n, int32_t* id
 int vmn_consolidate(vmn_index* idx) {
     if (!idx) return fail(VMN_ERR_ARG, "NULL index");
     if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");
+    // patchGraph: the staged inserts join the graph first
+    int rc = apply_staged(idx);
+    if (rc) return rc;
     if (idx->n_deleted == 0) return VMN_OK;
-    int rc = set_device(idx);
+    rc = set_device(idx);
     if (rc) return rc;
     const int64_t N = idx->n;
     const int R = idx->p.R;

@@ -43,22 +43,72 @@ hipError_t launch_pad_rows(const float* src, int64_t n, int D, float* dst, int D
     return hipGetLastError();
 }
 
-// dst[i] = row ids[i] of vecs ([cap x DP]) cut to D columns, zeros for id -1.
+// dst[i] = row ids[i] cut to D columns, zeros for id -1: ids below id_base
+// from vecs, the rest from the staged rows pend (both [.. x DP]).
 __global__ void rows_by_id_kernel(const int32_t* ids, int64_t m, const float* vecs, int DP, int D,
-                                  float* dst) {
+                                  const float* pend, int64_t id_base, float* dst) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t total = m * D;
     for (; i < total; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = i / D;
         const int c = (int)(i - r * D);
-        const int32_t id = ids[r];
-        dst[i] = id >= 0 ? vecs[(int64_t)id * DP + c] : 0.0f;
+        const int64_t id = ids[r];
+        dst[i] = id < 0 ? 0.0f
+                        : (id < id_base ? vecs[id * DP + c] : pend[(id - id_base) * DP + c]);
     }
 }
 hipError_t launch_rows_by_id(const int32_t* ids, int64_t m, const float* vecs, int DP, int D,
-                             float* dst, hipStream_t s) {
+                             const float* pend, int64_t id_base, float* dst, hipStream_t s) {
     hipLaunchKernelGGL(rows_by_id_kernel, dim3(fill_grid(m * D)), dim3(256), 0, s, ids, m, vecs, DP,
-                       D, dst);
+                       D, pend, id_base, dst);
+    return hipGetLastError();
+}
+
+// Per query, the first k of two lists sorted by (distance, id): the graph's
+// (gi, gd) and the staged rows' exact nearest (pi as staged index, FP64 pd),
+// which take ids id_base + index. Equal distances keep the lower id first.
+__global__ void merge_staged_kernel(const int32_t* gi, const float* gd, const int32_t* pi,
+                                    const double* pd, int64_t nq, int k, int64_t id_base,
+                                    int32_t* ri, float* rd) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int64_t o = q * k;
+    int a = 0, b = 0;
+    for (int j = 0; j < k; ++j) {
+        const bool ha = a < k && gi[o + a] >= 0;
+        const bool hb = b < k && pi[o + b] >= 0;
+        int32_t id = -1;
+        float d = 1e30f;
+        if (ha && hb) {
+            const float db = (float)pd[o + b];
+            const int64_t ib = id_base + pi[o + b];
+            if (gd[o + a] < db || (gd[o + a] == db && gi[o + a] < ib)) {
+                id = gi[o + a];
+                d = gd[o + a];
+                ++a;
+            } else {
+                id = (int32_t)ib;
+                d = db;
+                ++b;
+            }
+        } else if (ha) {
+            id = gi[o + a];
+            d = gd[o + a];
+            ++a;
+        } else if (hb) {
+            id = (int32_t)(id_base + pi[o + b]);
+            d = (float)pd[o + b];
+            ++b;
+        }
+        ri[o + j] = id;
+        if (rd) rd[o + j] = d;
+    }
+}
+hipError_t launch_merge_staged(const int32_t* gi, const float* gd, const int32_t* pi,
+                               const double* pd, int64_t nq, int k, int64_t id_base, int32_t* ri,
+                               float* rd, hipStream_t s) {
+    hipLaunchKernelGGL(merge_staged_kernel, dim3((unsigned)((nq + 63) / 64)), dim3(64), 0, s, gi, gd,
+                       pi, pd, nq, k, id_base, ri, rd);
     return hipGetLastError();
 }
 

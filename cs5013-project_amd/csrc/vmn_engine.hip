@@ -80,6 +80,8 @@ int check_engine(const vmn_index* idx) {
     if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");
     if (idx->p.mode != VMN_MODE_PARTH)
         return fail(VMN_ERR_STATE, "parth engine operations need a PARTH-mode index");
+    if (idx->n_pend)
+        return fail(VMN_ERR_STATE, "staged inserts pending: vmn_consolidate adds them first");
     return VMN_OK;
 }
 

@@ -97,6 +97,10 @@ struct vmn_index {
     int64_t free_count = 0;
     vmn_rt::DevBuf eng_ids, eng_src, eng_q, eng_best, eng_marks, eng_found, eng_gt_i, eng_gt_d,
         eng_part_d, eng_part_i;
+    // staged inserts (insertPoints before patchGraph): [n_pend x DP] rows that
+    // take ids n .. n + n_pend - 1 when vmn_consolidate adds them
+    vmn_rt::DevBuf pend, pend_part_d, pend_part_i, pend_i, pend_d, pend_gi, pend_gd;
+    int64_t n_pend = 0;
     int waves_per_cu = 32;  // search grid = CUs x waves_per_cu (persistent work queue)
     int list_cap = 0;       // minimum beam-list capacity of the search kernel
 };
@@ -132,6 +136,12 @@ int update_exact(vmn_index* idx, int64_t from, int64_t to);
 int normalize_rows(const int32_t* src, int64_t n, int row_width, int R, bool drop_self,
                    int32_t mem, std::vector<int32_t>& rows);
 int check_params(const vmn_params* p);
+// Adds the staged rows to the graph as one insert batch.
+int apply_staged(vmn_index* idx);
+// Search of nq padded query rows into device ids / dists (dists may be NULL),
+// with the staged rows merged in.
+int search_into(vmn_index* idx, const float* q, int64_t nq, int k, int L, int prec, int krr,
+                int32_t* ri, float* rd);
 // Search kernel over `nq` padded query rows [nq x DP] from idx->medoid.
 int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int32_t* out_ids,
                float* out_d, bool queries_are_nodes, int mode, bool want_vis,

@@ -53,6 +53,9 @@ const char kMagic[8] = {'V', 'M', 'N', 'I', 'D', 'X', '0', '1'};
 int fetch(const vmn_index* idx, std::vector<float>& vecs, std::vector<int32_t>& adj,
           std::vector<uint32_t>* del) {
     if (!idx->built) return fail(VMN_ERR_STATE, "index not built");
+    if (idx->n_pend)
+        return fail(VMN_ERR_STATE, "%lld staged inserts pending: vmn_consolidate adds them first",
+                    (long long)idx->n_pend);
     int rc = set_device(idx);
     if (rc) return rc;
     const int64_t n = idx->n;

@@ -93,7 +93,10 @@ hipError_t launch_medoid(const float* vecs, int64_t n, int D, int DP, const uint
                          float* centroid, unsigned long long* best, hipStream_t s);
 hipError_t launch_pad_rows(const float* src, int64_t n, int D, float* dst, int DP, hipStream_t s);
 hipError_t launch_rows_by_id(const int32_t* ids, int64_t m, const float* vecs, int DP, int D,
-                             float* dst, hipStream_t s);
+                             const float* pend, int64_t id_base, float* dst, hipStream_t s);
+hipError_t launch_merge_staged(const int32_t* gi, const float* gd, const int32_t* pi,
+                               const double* pd, int64_t nq, int k, int64_t id_base, int32_t* ri,
+                               float* rd, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 hipError_t launch_reverse_count(const int32_t* pair_dst, int64_t npairs, int32_t* counts,

@@ -4,9 +4,9 @@
 //     utils.cuh:113-148, then the Vamana<T> constructor, vamana_impl.cuh:10-20);
 //   * search for the first base vector and print its 5 nearest rows
 //     (searchPoints2 + printFinalVectors);
-//   * insert that vector with 1 added to its first coordinate (insertPoints),
-//     search for the shifted vector, patch (patchGraph; inserts are already in
-//     the graph here) and search again;
+//   * stage that vector with 1 added to its first coordinate (insertPoints),
+//     search for the shifted vector (the staged row is merged in by exact
+//     distance), patch (patchGraph adds it to the graph) and search again;
 //   * write the graph as records (saveGraph, vamana.cu:1346-1362).
 //
 // Build and run from the repository root:
@@ -72,7 +72,7 @@ int main(int argc, char** argv) {
 
     q[0] += 1.0f;
     int32_t new_id = -1;
-    if (vmn_insert(idx, q.data(), 1, &new_id, VMN_MEM_HOST)) return fail("insert");
+    if (vmn_insert_staged(idx, q.data(), 1, &new_id, VMN_MEM_HOST)) return fail("insert");
     std::printf("inserted id %d\nsearch the inserted vector:\n", new_id);
     if (show(idx, q.data(), dim)) return 1;
     if (vmn_consolidate(idx)) return fail("patch");

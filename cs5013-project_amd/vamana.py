@@ -151,6 +151,9 @@ def lib():
         L.vmn_save_bang_index.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p]
         L.vmn_load_bang_index.argtypes = [ctypes.c_char_p, ctypes.c_char_p,
                                           ctypes.POINTER(Params), ctypes.POINTER(P)]
+        L.vmn_insert_staged.argtypes = [P, P, i64, P, i32]
+        L.vmn_staged_count.argtypes = [P]
+        L.vmn_staged_count.restype = i64
         L.vmn_engine_delete.argtypes = [P, P, i64]
         L.vmn_engine_insert.argtypes = [P, P, i64, i32, P, i32]
         L.vmn_engine_run.argtypes = [P, P, P, P, i64, i32, i32, i32, ctypes.POINTER(EngineResult),
@@ -170,7 +173,7 @@ def lib():
                      "vmn_engine_delete", "vmn_engine_insert", "vmn_engine_run",
                      "vmn_read_vectors", "vmn_write_vectors", "vmn_read_truthset",
                      "vmn_write_truthset", "vmn_save_bang_index", "vmn_load_bang_index",
-                     "vmn_search_vectors"):
+                     "vmn_search_vectors", "vmn_insert_staged"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -480,6 +483,31 @@ class Vamana:
         out = np.empty((self.size, self.params.R), dtype=np.int32)
         _check(lib().vmn_get_adjacency(self._h, out.ctypes.data_as(ctypes.c_void_p), MEM_HOST))
         return out
+
+    def insert_staged(self, vectors):
+        """The reference's insertPoints (vamana_impl.cuh:22-33): held aside, merged
+        into searches by exact distance, added to the graph by patchGraph();
+        returns the ids they will take."""
+        vp, mem, keep = _arg(vectors, np.float32)
+        n = int(vectors.shape[0])
+        ids = np.empty(max(n, 1), np.int32)
+        if mem == MEM_DEVICE:
+            import torch
+            t = torch.empty(max(n, 1), dtype=torch.int32, device=vectors.device)
+            _check(lib().vmn_insert_staged(self._h, vp, n, ctypes.c_void_p(t.data_ptr()), mem))
+            self.synchronize()
+            return t[:n].cpu().numpy()
+        _check(lib().vmn_insert_staged(self._h, vp, n, ids.ctypes.data_as(ctypes.c_void_p), mem))
+        return ids[:n]
+
+    @property
+    def staged_count(self):
+        return int(lib().vmn_staged_count(self._h))
+
+    def patchGraph(self):
+        """vamana_impl.cuh:182-265: staged inserts join the graph (then
+        tombstones are consolidated)."""
+        self.consolidate()
 
     def engine_delete(self, ids):
         """parth engine delete (engine.cu:131-147): mark and push on the free list."""

@@ -208,6 +208,19 @@ int vmn_get_lowp_rows(const vmn_index* idx, int32_t prec, float* out);
  * 182-265 with insertPoints, vamana.cuh:42,130). Needs capacity. */
 int vmn_insert(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out, int32_t mem);
 
+/* Staged insert, the reference's insertPoints (vamana_impl.cuh:22-33, InsertList
+ * insert_list.cuh:12-87): the vectors are held aside and are promised ids
+ * size() + staged + i (ids_out). vmn_search, vmn_search_ex and
+ * vmn_search_vectors merge their exact k nearest (FP64 brute force over the
+ * staged rows) into the graph's results by (distance, id), as searchPoints2's
+ * merge_and_rerank_kernel (utils.cuh:199-285) does; k <= 128 then. Search
+ * contexts and the server see the graph only. vmn_consolidate (patchGraph,
+ * vamana_impl.cuh:182-265) adds them to the graph as one batch, and so does
+ * vmn_insert before its own rows. The savers refuse while rows are staged. */
+int vmn_insert_staged(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out,
+                      int32_t mem);
+int64_t vmn_staged_count(const vmn_index* idx);
+
 /* Tombstone ids: searches no longer expand or return them. Ids outside
  * [0, size) give VMN_ERR_ARG (the valid ones are still applied). Replaces the
  * delete list (delete_list.cuh:9-115) and parth deleteKernel (engine.cu:131-147). */
@@ -219,11 +232,15 @@ int vmn_delete_ids(vmn_index* idx, const int32_t* ids, int64_t n, int32_t mem);
 int vmn_delete_vectors(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out,
                        int32_t mem);
 
-/* Delete consolidation (FreshDiskANN Algorithm 4; the reference has none):
- * every live node with a tombstoned out-neighbour is re-pruned from its live
- * neighbours plus the live neighbours of its tombstoned neighbours; rows of
- * tombstoned nodes are emptied; a tombstoned entry point moves to the medoid
- * of the live nodes. Ids stay tombstoned (not reused). */
+/* patchGraph (vamana_impl.cuh:182-265) plus delete consolidation. First the
+ * staged inserts join the graph (one insert batch). Then (FreshDiskANN
+ * Algorithm 4 in spirit; the reference consolidates nothing) every live node
+ * with a tombstoned out-neighbour keeps its live out-neighbours in order and
+ * fills the freed slots from the live out-neighbours of its tombstoned ones,
+ * nearest first, skipping a candidate that a kept neighbour r occludes
+ * (alpha * d(r, c) <= d(p, c)); rows of tombstoned nodes are emptied; a
+ * tombstoned entry point moves to the medoid of the live nodes. Ids stay
+ * tombstoned (not reused). */
 int vmn_consolidate(vmn_index* idx);
 
 /* Tombstone mask [size] (1 = deleted) into host memory. */

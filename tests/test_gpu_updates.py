@@ -137,3 +137,41 @@ def test_gpu_parth_delete_search_matches_oracle(vmn, orc, sift, gpu):
     assert np.array_equal(gi, oi) and np.array_equal(gd, od)
     with pytest.raises(vmn.VamanaError):
         idx.deleteIds(np.array([10000], np.int32))
+
+
+def test_gpu_staged_inserts_merge_then_patch(vmn, orc, gpu, tmp_path):
+    """insertPoints / searchPoints2 / patchGraph semantics (vamana_impl.cuh:22-33,
+    :99-180, :182-265): staged rows take the next ids, every search merges their
+    exact distances into the graph's results by (distance, id), the savers
+    refuse while rows are staged, and patchGraph adds them as one insert batch,
+    which must equal the oracle's insert of that batch."""
+    v = _data(vmn, 3300, seed=8)
+    idx = vmn.Vamana(v[:3000], mode="fresh", R=32, L_build=64, alphas=(1.2,), capacity=3300)
+    oadj, med = orc.fresh_build(v[:3000], R=32, L=64, alphas=(1.2,))
+    ids = idx.insert_staged(v[3000:3150])
+    assert ids.tolist() == list(range(3000, 3150))
+    assert idx.staged_count == 150 and idx.size == 3000
+    q = _data(vmn, 40, seed=8, stream=1)
+    gi, gd = idx.searchPoints(q, k=10, L=48)
+    oi, od, _, _ = orc.fresh_search(v[:3000], oadj, q, med, L=48)
+    pdist = ((v[3000:3150][None].astype(np.float64) - q[:, None].astype(np.float64)) ** 2).sum(-1)
+    for i in range(len(q)):
+        cand = [(float(od[i, j]), int(oi[i, j])) for j in range(10) if oi[i, j] >= 0]
+        cand += [(float(np.float32(pdist[i, j])), 3000 + j) for j in range(150)]
+        cand.sort()
+        assert [c[1] for c in cand[:10]] == gi[i].tolist(), i
+        assert np.array_equal(np.float32([c[0] for c in cand[:10]]), gd[i]), i
+    rows = idx.searchPoints2(q, k=10, L=48)
+    assert np.array_equal(rows, v[:3150][gi])
+    with pytest.raises(vmn.VamanaError):
+        idx.save(str(tmp_path / "pending.vmn"))
+    idx.patchGraph()
+    assert idx.staged_count == 0 and idx.size == 3150
+    full = np.full((3150, 32), -1, np.int32)
+    full[:3000] = oadj
+    orc.fresh_insert(v[:3150], 3000, full, med, L=64, alpha=1.2)
+    assert np.array_equal(idx.adjacency, full)
+    # immediate inserts after staged ones: the staged rows go in first
+    idx.insert_staged(v[3150:3160])
+    assert idx.insertPoints(v[3160:3170]).tolist() == list(range(3160, 3170))
+    assert idx.size == 3170 and idx.staged_count == 0
