@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--mode", default="fresh", choices=["parth", "fresh"])
+    ap.add_argument("--components", type=int, default=256,
+                    help="mixture components of the synthetic data (the 10k x 128 config uses 64)")
     ap.add_argument("--R", type=int, default=64)
     ap.add_argument("--L-build", type=int, default=128)
     ap.add_argument("--alphas", default="1.0,1.2")
@@ -90,10 +92,11 @@ def main():
         return engine_replay(args, vmn, torch)
     n, nq, d = args.n, args.nq, 128
     alphas = tuple(float(x) for x in args.alphas.split(","))
-    log(f"generating {n}x{d} base + {nq} queries (256-component integer mixture)")
-    base = vmn.datasets.integer_mixture(n, d, k=256, sigma=20.0, seed=1234, stream=0)
-    qs = vmn.datasets.integer_mixture(nq, d, k=256, sigma=20.0, seed=1234, stream=1)
-    qs2 = vmn.datasets.integer_mixture(nq, d, k=256, sigma=20.0, seed=1234, stream=2)
+    kc = args.components
+    log(f"generating {n}x{d} base + {nq} queries ({kc}-component integer mixture)")
+    base = vmn.datasets.integer_mixture(n, d, k=kc, sigma=20.0, seed=1234, stream=0)
+    qs = vmn.datasets.integer_mixture(nq, d, k=kc, sigma=20.0, seed=1234, stream=1)
+    qs2 = vmn.datasets.integer_mixture(nq, d, k=kc, sigma=20.0, seed=1234, stream=2)
     base_t = torch.from_numpy(base).to(dev)
     q_t = torch.from_numpy(qs).to(dev)
     q2_t = torch.from_numpy(qs2).to(dev)
@@ -245,7 +248,7 @@ def main():
         "metric": METRIC, "value": head["qps"], "unit": "queries/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(head["ms_per_step"], 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
-        "data": "synthetic: seeded 256-component Gaussian mixture, integer-valued in [0,255]",
+        "data": f"synthetic: seeded {kc}-component Gaussian mixture, integer-valued in [0,255]",
         "config": {
             "workload": f"Vamana search {n}x{d}, BF16 rows on MFMA + FP32 re-rank of "
                         f"{args.rerank}, {nq} queries per batch, k=10",
@@ -260,7 +263,7 @@ def main():
             "bf16_vs_fp32_qps_serial": round(head["qps_serial"] / timed["fp32"]["qps_serial"], 3),
             "qps_serial": head["qps_serial"], "recall@10_set2": head["recall@10_set2"],
             "qps_host_buffers": round(nq / host_ms * 1e3, 1),
-            "batches": "K batches of 10,000 queries alternating over 2 search contexts "
+            "batches": f"K batches of {nq:,} queries alternating over 2 search contexts "
                        "(query sets: mixture streams 1 and 2)",
             "distance_error_vs_fp64": err, "sweep": sweeps},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_GBs"], 1),
