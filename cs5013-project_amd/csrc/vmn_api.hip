@@ -144,6 +144,12 @@ int update_exact(vmn_index* idx, int64_t from, int64_t to) {
 // row makes it exact (the same graph as FP32, half the row bytes).
 int build_prec(const vmn_index* idx) { return idx->exact_bf16 ? VMN_PREC_BF16 : VMN_PREC_FP32; }
 
+// The prunes read the same BF16 rows under the same condition.
+void set_prune_rows(const vmn_index* idx, vmn::PruneArgs& a) {
+    a.bf16_rows = idx->exact_bf16 ? idx->bf16_rows.p : nullptr;
+    a.bf16_norms = idx->exact_bf16 ? idx->bf16_norms.as<float>() : nullptr;
+}
+
 int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int32_t* out_ids,
                float* out_d, bool queries_are_nodes, int mode, bool want_vis, int64_t node_base,
                int prec, int krr, vmn_ctx* ctx) {
@@ -230,6 +236,7 @@ int build_parth(vmn_index* idx, bool random_init) {
         a.mode = VMN_MODE_PARTH;
         a.DP = idx->DP;
         a.vecs = idx->vecs.as<float>();
+        set_prune_rows(idx, a);
         a.adj = idx->adj.as<int32_t>();
         a.W = idx->p.R;
         a.R = idx->p.R;
@@ -261,6 +268,7 @@ int fresh_pass(vmn_index* idx, int64_t base, int64_t cnt, float alpha) {
     a.mode = VMN_MODE_FRESH;
     a.DP = idx->DP;
     a.vecs = idx->vecs.as<float>();
+    set_prune_rows(idx, a);
     a.adj = idx->adj.as<int32_t>();
     a.W = R;
     a.R = R;

@@ -221,7 +221,7 @@ hipError_t launch_medoid(const float* vecs, int64_t n, int D, int DP, const uint
 // list (fresh distances); a stable sort by distance (the reference's bubble
 // sort) equals a stable merge of the two with list entries first on ties.
 // ---------------------------------------------------------------------------
-template <int DP, int CCAP>
+template <int DP, int CCAP, bool BF>
 __global__ __launch_bounds__(64) void parth_prune_kernel(PruneArgs a) {
     __shared__ __align__(16) uint32_t c_id[2][CCAP];
     __shared__ __align__(16) float c_d[2][CCAP];
@@ -233,8 +233,9 @@ __global__ __launch_bounds__(64) void parth_prune_kernel(PruneArgs a) {
     __shared__ float act_d[CCAP];
     __shared__ int32_t newrow[64];
     __shared__ int64_t s_q;
-    const int l = threadIdx.x, sub = l & 7;
+    const int l = threadIdx.x;
     const int W = a.W, Lc = a.Lc;
+    const PruneRows rows{a.vecs, a.bf16_rows, a.bf16_norms};
     for (;;) {
         const int64_t i = next_item(a.counter, &s_q);
         if (i >= a.n) return;
@@ -261,15 +262,15 @@ __global__ __launch_bounds__(64) void parth_prune_kernel(PruneArgs a) {
             for (int t = 0; t < m; ++t) dup |= c_id[0][t] == (uint32_t)nb;
             pass = pass && !dup;
         }
-        QReg<DP> q;
-        q.load(a.vecs + node * DP, sub);
-        const int ne = row_extras<DP>(a.vecs, a.adj, W, node, q, pass, nb, e_id, e_d);
+        PruneQ<DP, BF> q;
+        q.load(rows, node);
+        const int ne = row_extras<DP, BF>(rows, q, pass, nb, e_id, e_d);
         const uint32_t my_id = l < ne ? (uint32_t)e_id[l] : 0u;
         const float my_d = l < ne ? e_d[l] : 0.0f;
         const int cnt = merge_sorted<CCAP, false>(c_id[0], c_d[0], m, my_id, my_d, ne, c_id[1],
                                                   c_d[1], CCAP);
         __syncthreads();
-        const int nk = robust_prune<DP, true>(a.vecs, c_id[1], c_d[1], cnt, a.alpha, a.R, kept,
+        const int nk = robust_prune<DP, true, BF>(rows, c_id[1], c_d[1], cnt, a.alpha, a.R, kept,
                                               act_id, act_pos, act_d, newrow, nullptr);
         for (int j = l; j < W; j += 64) a.adj[node * W + j] = j < nk ? newrow[j] : -1;
         __syncthreads();
@@ -286,7 +287,7 @@ __global__ __launch_bounds__(64) void parth_prune_kernel(PruneArgs a) {
 // nodes (a tombstoned start node can be expanded). Prune with <=. Each
 // kept p* emits the reverse pair (p* <- node) with d(node, p*) (:292-301).
 // ---------------------------------------------------------------------------
-template <int DP, int CCAP>
+template <int DP, int CCAP, bool BF>
 __global__ __launch_bounds__(64) void fresh_out_prune_kernel(PruneArgs a) {
     __shared__ __align__(16) uint32_t c_id[2][CCAP];
     __shared__ __align__(16) float c_d[2][CCAP];
@@ -299,8 +300,9 @@ __global__ __launch_bounds__(64) void fresh_out_prune_kernel(PruneArgs a) {
     __shared__ int32_t newrow[64];
     __shared__ float newd[64];
     __shared__ int64_t s_q;
-    const int l = threadIdx.x, sub = l & 7;
+    const int l = threadIdx.x;
     const int W = a.W, P = a.P;
+    const PruneRows rows{a.vecs, a.bf16_rows, a.bf16_norms};
     for (;;) {
         const int64_t i = next_item(a.counter, &s_q);
         if (i >= a.n) return;
@@ -332,16 +334,16 @@ __global__ __launch_bounds__(64) void fresh_out_prune_kernel(PruneArgs a) {
         const int32_t nb = l < W ? a.adj[node * W + l] : -1;
         const bool pass = nb != -1 && nb != (int32_t)node &&
                           !(a.del_bits && bit_test(a.del_bits, (uint32_t)nb));
-        QReg<DP> q;
-        q.load(a.vecs + node * DP, sub);
-        const int ne = row_extras<DP>(a.vecs, a.adj, W, node, q, pass, nb, e_id, e_d);
+        PruneQ<DP, BF> q;
+        q.load(rows, node);
+        const int ne = row_extras<DP, BF>(rows, q, pass, nb, e_id, e_d);
         const uint32_t my_id = l < ne ? (uint32_t)e_id[l] : 0u;
         const float my_d = l < ne ? e_d[l] : 0.0f;
         cnt = merge_sorted<CCAP, false>(c_id[cur], c_d[cur], cnt, my_id, my_d, ne, c_id[cur ^ 1],
                                         c_d[cur ^ 1], P);
         cur ^= 1;
         __syncthreads();
-        const int nk = robust_prune<DP, false>(a.vecs, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
+        const int nk = robust_prune<DP, false, BF>(rows, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
                                                kept, act_id, act_pos, act_d, newrow, newd);
         for (int j = l; j < W; j += 64) a.adj[node * W + j] = j < nk ? newrow[j] : -1;
         for (int j = l; j < a.R; j += 64) {
@@ -364,7 +366,7 @@ __global__ __launch_bounds__(64) void fresh_out_prune_kernel(PruneArgs a) {
 // :539-564); re-pruning an unchanged row with the alpha it was pruned with
 // returns it unchanged.
 // ---------------------------------------------------------------------------
-template <int DP, int CCAP>
+template <int DP, int CCAP, bool BF>
 __global__ __launch_bounds__(64) void fresh_rev_prune_kernel(PruneArgs a) {
     __shared__ __align__(16) uint32_t c_id[2][CCAP];
     __shared__ __align__(16) float c_d[2][CCAP];
@@ -376,8 +378,9 @@ __global__ __launch_bounds__(64) void fresh_rev_prune_kernel(PruneArgs a) {
     __shared__ float act_d[CCAP];
     __shared__ int32_t newrow[64];
     __shared__ int64_t s_q;
-    const int l = threadIdx.x, sub = l & 7;
+    const int l = threadIdx.x;
     const int W = a.W, M = a.M;
+    const PruneRows rows{a.vecs, a.bf16_rows, a.bf16_norms};
     for (;;) {
         const int64_t i = next_item(a.counter, &s_q);
         if (i >= a.n) return;
@@ -398,16 +401,16 @@ __global__ __launch_bounds__(64) void fresh_rev_prune_kernel(PruneArgs a) {
         }
         const int32_t nb = l < W ? a.adj[node * W + l] : -1;
         const bool pass = nb != -1 && nb != (int32_t)node;
-        QReg<DP> q;
-        q.load(a.vecs + node * DP, sub);
-        const int ne = row_extras<DP>(a.vecs, a.adj, W, node, q, pass, nb, e_id, e_d);
+        PruneQ<DP, BF> q;
+        q.load(rows, node);
+        const int ne = row_extras<DP, BF>(rows, q, pass, nb, e_id, e_d);
         const uint32_t my_id = l < ne ? (uint32_t)e_id[l] : 0u;
         const float my_d = l < ne ? e_d[l] : 0.0f;
         cnt = merge_sorted<CCAP, false>(c_id[cur], c_d[cur], cnt, my_id, my_d, ne, c_id[cur ^ 1],
                                         c_d[cur ^ 1], M);
         cur ^= 1;
         __syncthreads();
-        const int nk = robust_prune<DP, false>(a.vecs, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
+        const int nk = robust_prune<DP, false, BF>(rows, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
                                                kept, act_id, act_pos, act_d, newrow, nullptr);
         for (int j = l; j < W; j += 64) a.adj[node * W + j] = j < nk ? newrow[j] : -1;
         __syncthreads();
@@ -426,14 +429,26 @@ __global__ __launch_bounds__(64) void fresh_rev_prune_kernel(PruneArgs a) {
         default: return hipErrorInvalidValue;             \
     }
 
+// BF16 rows when the index supplies them (exact distances), else FP32 rows
+#define VMN_PRUNE_LAUNCH(KERNEL, CAP)                                                        \
+    do {                                                                                    \
+        if (a.bf16_rows) {                                                                  \
+            VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((KERNEL<DPc, CAP, true>), dim3(grid),   \
+                                                     dim3(64), 0, s, a));                   \
+        } else {                                                                            \
+            VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((KERNEL<DPc, CAP, false>), dim3(grid),  \
+                                                     dim3(64), 0, s, a));                   \
+        }                                                                                   \
+    } while (0)
+
 hipError_t launch_parth_prune(const PruneArgs& a, int grid, hipStream_t s) {
     const int need = a.Lc + a.W;
     if (need <= 128) {
-        VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((parth_prune_kernel<DPc, 128>), dim3(grid), dim3(64), 0, s, a));
+        VMN_PRUNE_LAUNCH(parth_prune_kernel, 128);
     } else if (need <= 256) {
-        VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((parth_prune_kernel<DPc, 256>), dim3(grid), dim3(64), 0, s, a));
+        VMN_PRUNE_LAUNCH(parth_prune_kernel, 256);
     } else if (need <= 576) {
-        VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((parth_prune_kernel<DPc, 576>), dim3(grid), dim3(64), 0, s, a));
+        VMN_PRUNE_LAUNCH(parth_prune_kernel, 576);
     } else {
         return hipErrorInvalidValue;
     }
@@ -444,9 +459,9 @@ hipError_t launch_fresh_out_prune(const PruneArgs& a, int grid, hipStream_t s) {
     // list capacity must hold min(P, visited + W) entries
     const int need = a.P;
     if (need <= 256) {
-        VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((fresh_out_prune_kernel<DPc, 256>), dim3(grid), dim3(64), 0, s, a));
+        VMN_PRUNE_LAUNCH(fresh_out_prune_kernel, 256);
     } else if (need <= 640) {
-        VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((fresh_out_prune_kernel<DPc, 640>), dim3(grid), dim3(64), 0, s, a));
+        VMN_PRUNE_LAUNCH(fresh_out_prune_kernel, 640);
     } else {
         return hipErrorInvalidValue;
     }
@@ -456,9 +471,9 @@ hipError_t launch_fresh_out_prune(const PruneArgs& a, int grid, hipStream_t s) {
 hipError_t launch_fresh_rev_prune(const PruneArgs& a, int grid, hipStream_t s) {
     const int need = a.M;
     if (need <= 256) {
-        VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((fresh_rev_prune_kernel<DPc, 256>), dim3(grid), dim3(64), 0, s, a));
+        VMN_PRUNE_LAUNCH(fresh_rev_prune_kernel, 256);
     } else if (need <= 512) {
-        VMN_DISPATCH_DP(a.DP, hipLaunchKernelGGL((fresh_rev_prune_kernel<DPc, 512>), dim3(grid), dim3(64), 0, s, a));
+        VMN_PRUNE_LAUNCH(fresh_rev_prune_kernel, 512);
     } else {
         return hipErrorInvalidValue;
     }

@@ -56,6 +56,8 @@ struct PruneArgs {
     int mode;
     int DP;
     const float* vecs;
+    const void* bf16_rows;    // BF16 copies of vecs when every row is exact in BF16, else null
+    const float* bf16_norms;
     int32_t* adj;             // rows rewritten in place
     const uint32_t* del_bits; // FRESH out-prune: tombstoned ids are not candidates (or null)
     int W, R;

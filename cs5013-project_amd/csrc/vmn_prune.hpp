@@ -4,6 +4,48 @@
 
 namespace vmn {
 
+// Where a prune reads node rows: the FP32 rows, and the BF16 copies with
+// their squared norms (null unless every row is exact in BF16).
+struct PruneRows {
+    const float* f32;
+    const void* bf16;
+    const float* bf16_norms;
+};
+
+// One node's row as the "query" of a prune's distances. BF = false: FP32 rows,
+// 8 lanes per row. BF = true: BF16 rows on MFMA, 16 rows per tile. BF is only
+// taken when every row of the index holds integers of magnitude <= 256 with
+// squared norm <= 2^22: BF16 then holds the values exactly and every partial
+// sum of ||a||^2 + ||b||^2 - 2 a.b is an integer below 2^24, so the distance
+// equals the FP32 sum of squared differences bit for bit, from half the bytes.
+template <int DP, bool BF>
+struct PruneQ;
+
+template <int DP>
+struct PruneQ<DP, false> {
+    QReg<DP> q;
+    __device__ __forceinline__ void load(const PruneRows& r, int64_t id) {
+        q.load(r.f32 + id * DP, lane() & 7);
+    }
+    __device__ __forceinline__ void dists(const PruneRows& r, const int32_t* ids, float* out,
+                                          int cnt) const {
+        gather_dists<DP>(q, r.f32, ids, out, cnt);
+    }
+};
+
+template <int DP>
+struct PruneQ<DP, true> {
+    QLow<DP> q;
+    __device__ __forceinline__ void load(const PruneRows& r, int64_t id) {
+        prep_query<DP, VMN_PREC_BF16>(q, r.f32 + id * DP, nullptr);
+    }
+    __device__ __forceinline__ void dists(const PruneRows& r, const int32_t* ids, float* out,
+                                          int cnt) const {
+        gather_dists_low<DP, VMN_PREC_BF16, VMN_GATHER_TILES>(q, r.bf16, r.bf16_norms, ids, out,
+                                                              cnt);
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Robust prune of a sorted candidate list held in LDS.
 //
@@ -14,12 +56,12 @@ namespace vmn {
 // one float rounding, compared without fusion. Stops once R are kept. The
 // d(p*, p') of one round are computed by the whole wave (8 rows at a time).
 // ---------------------------------------------------------------------------
-template <int DP, bool STRICT>
-__device__ int robust_prune(const float* __restrict__ vecs, const uint32_t* c_id,
+template <int DP, bool STRICT, bool BF = false>
+__device__ int robust_prune(const PruneRows& rows, const uint32_t* c_id,
                             const float* c_d, int cnt, float alpha, int R, uint8_t* s_kept,
                             int32_t* s_act_id, int32_t* s_act_pos, float* s_act_d,
                             int32_t* out_ids, float* out_d) {
-    const int l = lane(), sub = l & 7;
+    const int l = lane();
     for (int i = l; i < cnt; i += 64) s_kept[i] = 1;
     __syncthreads();
     int nk = 0, from = 0;
@@ -41,8 +83,8 @@ __device__ int robust_prune(const float* __restrict__ vecs, const uint32_t* c_id
         }
         ++nk;
         if (nk >= R) break;
-        QReg<DP> pq;
-        pq.load(vecs + (int64_t)ps * DP, sub);
+        PruneQ<DP, BF> pq;
+        pq.load(rows, ps);
         int nact = 0;
         for (int b = p + 1; b < cnt; b += 64) {
             const int j = b + l;
@@ -56,7 +98,7 @@ __device__ int robust_prune(const float* __restrict__ vecs, const uint32_t* c_id
             nact += __popcll((unsigned long long)m);
         }
         __syncthreads();
-        gather_dists<DP>(pq, vecs, s_act_id, s_act_d, nact);
+        pq.dists(rows, s_act_id, s_act_d, nact);
         __syncthreads();
         for (int k = l; k < nact; k += 64) {
             const int j = s_act_pos[k];
@@ -73,17 +115,15 @@ __device__ int robust_prune(const float* __restrict__ vecs, const uint32_t* c_id
 
 // Distances from `node` to its current row entries that pass `keep`, appended
 // (compacted, adjacency order) to e_id/e_d. Returns the count.
-template <int DP>
-__device__ __forceinline__ int row_extras(const float* __restrict__ vecs, const int32_t* adj, int W,
-                                          int64_t node, const QReg<DP>& q, bool pass,
-                                          int32_t nb, int32_t* e_id, float* e_d) {
+template <int DP, bool BF>
+__device__ __forceinline__ int row_extras(const PruneRows& rows, const PruneQ<DP, BF>& q,
+                                          bool pass, int32_t nb, int32_t* e_id, float* e_d) {
     const uint64_t m = ballot(pass);
     const int ne = __popcll((unsigned long long)m);
     if (pass) e_id[__popcll((unsigned long long)(m & lanemask_lt()))] = nb;
     __syncthreads();
-    gather_dists<DP>(q, vecs, e_id, e_d, ne);
+    q.dists(rows, e_id, e_d, ne);
     __syncthreads();
-    (void)adj; (void)W; (void)node;
     return ne;
 }
 

@@ -106,8 +106,9 @@ __global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
         const int64_t i = next_item(a.counter, &s_q);
         if (i >= a.nlist) return;
         const int64_t p = a.list[i];
-        QReg<DP> q;
-        q.load(a.vecs + p * DP, sub);
+        const PruneRows rows{a.vecs, nullptr, nullptr};
+        PruneQ<DP, false> q;
+        q.load(rows, p);
         const int32_t own = l < W ? a.adj[p * W + l] : -1;
         const bool own_dead = own >= 0 && bit_test(a.del_bits, (uint32_t)own);
         const uint64_t dmask = ballot(own_dead);
@@ -126,7 +127,7 @@ __global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
                 for (int t = 0; t < cnt; ++t) dup |= c_id[cur][t] == (uint32_t)v;
                 pass = !dup;
             }
-            const int ne = row_extras<DP>(a.vecs, a.adj, W, p, q, pass, v, e_id, e_d);
+            const int ne = row_extras<DP, false>(rows, q, pass, v, e_id, e_d);
             const uint32_t my_id = l < ne ? (uint32_t)e_id[l] : 0u;
             const float my_d = l < ne ? e_d[l] : 0.0f;
             cnt = merge_sorted<CCAP, true>(c_id[cur], c_d[cur], cnt, my_id, my_d, ne,
