@@ -270,7 +270,7 @@ __global__ __launch_bounds__(64) void parth_prune_kernel(PruneArgs a) {
         const int cnt = merge_sorted<CCAP, false>(c_id[0], c_d[0], m, my_id, my_d, ne, c_id[1],
                                                   c_d[1], CCAP);
         __syncthreads();
-        const int nk = robust_prune<DP, true, BF>(rows, c_id[1], c_d[1], cnt, a.alpha, a.R, kept,
+        const int nk = prune_list<DP, true, BF, CCAP>(rows, c_id[1], c_d[1], cnt, a.alpha, a.R, kept,
                                               act_id, act_pos, act_d, newrow, nullptr);
         for (int j = l; j < W; j += 64) a.adj[node * W + j] = j < nk ? newrow[j] : -1;
         __syncthreads();
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(64) void fresh_out_prune_kernel(PruneArgs a) {
                                         c_d[cur ^ 1], P);
         cur ^= 1;
         __syncthreads();
-        const int nk = robust_prune<DP, false, BF>(rows, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
+        const int nk = prune_list<DP, false, BF, CCAP>(rows, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
                                                kept, act_id, act_pos, act_d, newrow, newd);
         for (int j = l; j < W; j += 64) a.adj[node * W + j] = j < nk ? newrow[j] : -1;
         for (int j = l; j < a.R; j += 64) {
@@ -410,7 +410,7 @@ __global__ __launch_bounds__(64) void fresh_rev_prune_kernel(PruneArgs a) {
                                         c_d[cur ^ 1], M);
         cur ^= 1;
         __syncthreads();
-        const int nk = robust_prune<DP, false, BF>(rows, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
+        const int nk = prune_list<DP, false, BF, CCAP>(rows, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
                                                kept, act_id, act_pos, act_d, newrow, nullptr);
         for (int j = l; j < W; j += 64) a.adj[node * W + j] = j < nk ? newrow[j] : -1;
         __syncthreads();

@@ -56,7 +56,7 @@ struct PruneQ<DP, true> {
 // one float rounding, compared without fusion. Stops once R are kept. The
 // d(p*, p') of one round are computed by the whole wave (8 rows at a time).
 // ---------------------------------------------------------------------------
-template <int DP, bool STRICT, bool BF = false>
+template <int DP, bool STRICT>
 __device__ int robust_prune(const PruneRows& rows, const uint32_t* c_id,
                             const float* c_d, int cnt, float alpha, int R, uint8_t* s_kept,
                             int32_t* s_act_id, int32_t* s_act_pos, float* s_act_d,
@@ -83,7 +83,7 @@ __device__ int robust_prune(const PruneRows& rows, const uint32_t* c_id,
         }
         ++nk;
         if (nk >= R) break;
-        PruneQ<DP, BF> pq;
+        PruneQ<DP, false> pq;
         pq.load(rows, ps);
         int nact = 0;
         for (int b = p + 1; b < cnt; b += 64) {
@@ -111,6 +111,173 @@ __device__ int robust_prune(const PruneRows& rows, const uint32_t* c_id,
     }
     __syncthreads();
     return nk;
+}
+
+// ---------------------------------------------------------------------------
+// The same prune for rows exact in BF16, with the p*-to-candidate distances
+// as MFMA tiles. The next 16 kept candidates (p0, then the first 15 of the
+// active list behind it) are the 16 rows of A; one pass over the active
+// candidates' BF16 rows gives the distances from all 16 of them to every
+// active candidate, reduced at once to a 16-bit mask per candidate ("row j
+// occludes this candidate and it lies after row j"). The 16 are then taken in
+// list order as the sequential loop takes them: one still kept at its turn is
+// a p*, is output, and drops the later candidates its mask bit marks; one an
+// earlier p* dropped is skipped. Every candidate between two of the 16 was
+// already dropped, so the next scan, from just after the 16th, finds the
+// next p*, and the output equals robust_prune's (the distances are exact
+// integers either way). One gather of the active rows serves up to 16 rounds.
+// ---------------------------------------------------------------------------
+template <int DP, bool STRICT, int CCAP>
+__device__ int robust_prune_mfma(const PruneRows& rows, const uint32_t* c_id, const float* c_d,
+                                 int cnt, float alpha, int R, uint8_t* s_kept,
+                                 int32_t* s_act_id, int32_t* s_act_pos, uint32_t* s_pm,
+                                 int32_t* out_ids, float* out_d) {
+    constexpr int KB = DP / 32;
+    constexpr int NCH = (CCAP + 63) / 64;
+    const int l = lane(), g = l >> 4, rr = l >> 2, sub = l & 3, col = l & 15;
+    const __bf16* bro = reinterpret_cast<const __bf16*>(rows.bf16);
+    const float* nrm = rows.bf16_norms;
+    for (int i = l; i < cnt; i += 64) s_kept[i] = 1;
+    __syncthreads();
+    int nk = 0, from = 0;
+    while (nk < R) {
+        // kept candidates from `from` on: the first is p0, the others the active list
+        int nall = 0, p0 = -1;
+        for (int b = from; b < cnt; b += 64) {
+            const int j = b + l;
+            const bool f = j < cnt && s_kept[j];
+            const uint64_t m = ballot(f);
+            const int r = nall + __popcll((unsigned long long)(m & lanemask_lt()));
+            if (p0 < 0 && m) p0 = b + __ffsll((unsigned long long)m) - 1;
+            if (f && r > 0) {
+                s_act_id[r - 1] = (int32_t)(c_id[j] & VMN_ID_MASK);
+                s_act_pos[r - 1] = j;
+            }
+            nall += __popcll((unsigned long long)m);
+        }
+        if (nall == 0) break;
+        const int nact = nall - 1;
+        const int ns = nall < 16 ? nall : 16;
+        const int32_t id0 = (int32_t)(c_id[p0] & VMN_ID_MASK);
+        __syncthreads();
+        if (nact > 0) {
+            // A row `col`: p0 (row 0) or active entry col - 1 (rows past the
+            // list repeat p0 and are never read)
+            const int32_t sid = (col > 0 && col - 1 < nact) ? s_act_id[col - 1] : id0;
+            bf16x8_t A[KB];
+#pragma unroll
+            for (int kb = 0; kb < KB; ++kb)
+                A[kb] = *reinterpret_cast<const bf16x8_t*>(bro + (int64_t)sid * DP + 32 * kb + 8 * g);
+            // squared norms of rows 4g .. 4g+3, the rows this lane's accumulators hold
+            float an[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int r = 4 * g + i;
+                an[i] = nrm[(r > 0 && r - 1 < nact) ? s_act_id[r - 1] : id0];
+            }
+            const int src = (4 * col + g) << 2;
+            for (int base = 0; base < nact; base += 32) {
+                // two tiles of 16 active candidates per round trip; lane l
+                // loads 16 bytes of tile row l/4 per k-block (coalesced), and
+                // ds_bpermute hands each dword to the lane that holds it as B
+                int32_t cid[2];
+                uint4 v[2][KB];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const int c = base + 16 * t + rr;
+                    cid[t] = c < nact ? s_act_id[c] : s_act_id[0];
+                    if (t == 0 || base + 16 * t < nact) {
+                        const uint4* r = reinterpret_cast<const uint4*>(bro + (int64_t)cid[t] * DP);
+#pragma unroll
+                        for (int kb = 0; kb < KB; ++kb) v[t][kb] = r[4 * kb + sub];
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    if (t == 0 || base + 16 * t < nact) {
+                        f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                        for (int kb = 0; kb < KB; ++kb) {
+                            const uint4 u = {bperm(src, v[t][kb].x), bperm(src, v[t][kb].y),
+                                             bperm(src, v[t][kb].z), bperm(src, v[t][kb].w)};
+                            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[kb], __builtin_bit_cast(bf16x8_t, u), acc, 0, 0, 0);
+                        }
+                        // this lane: candidate k (column col), rows 4g .. 4g+3;
+                        // row j prunes only candidates after it (k >= j)
+                        const int k = base + 16 * t + col;
+                        uint32_t bits = 0;
+                        if (k < nact) {
+                            const float xn = nrm[s_act_id[k]];
+                            const float dn = c_d[s_act_pos[k]];
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                const int j = 4 * g + i;
+                                const float lhs = alpha * ((an[i] + xn) - 2.0f * acc[i]);
+                                const bool pr = STRICT ? (lhs < dn) : (lhs <= dn);
+                                bits |= (uint32_t)(pr && j < ns && k >= j) << j;
+                            }
+                        }
+                        bits |= __shfl_xor(bits, 16);
+                        bits |= __shfl_xor(bits, 32);
+                        if (g == 0 && k < nact) s_pm[k] = bits;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        uint32_t pm[NCH];
+        uint64_t keep[NCH];
+#pragma unroll
+        for (int m = 0; m < NCH; ++m) {
+            const int k = 64 * m + l;
+            pm[m] = k < nact ? s_pm[k] : 0u;
+            keep[m] = ballot(k < nact);
+        }
+        bool done = false;
+        for (int j = 0; j < ns; ++j) {
+            // row j >= 1 is active entry j - 1: still kept?
+            if (j > 0 && !((keep[0] >> (j - 1)) & 1ull)) continue;
+            if (l == 0) {
+                out_ids[nk] = j == 0 ? id0 : s_act_id[j - 1];
+                if (out_d) out_d[nk] = c_d[j == 0 ? p0 : s_act_pos[j - 1]];
+            }
+            ++nk;
+            if (nk >= R) {
+                done = true;
+                break;
+            }
+#pragma unroll
+            for (int m = 0; m < NCH; ++m) keep[m] &= ~ballot((pm[m] >> j) & 1u);
+        }
+        if (done) break;
+        // drops back to the list; the next scan starts after the last of the 16
+#pragma unroll
+        for (int m = 0; m < NCH; ++m) {
+            const int k = 64 * m + l;
+            if (k < nact && !((keep[m] >> l) & 1ull)) s_kept[s_act_pos[k]] = 0;
+        }
+        from = (ns == 1 ? p0 : s_act_pos[ns - 2]) + 1;
+        __syncthreads();
+    }
+    __syncthreads();
+    return nk;
+}
+
+// robust_prune over FP32 rows, or robust_prune_mfma when the rows are exact in
+// BF16 (same output).
+template <int DP, bool STRICT, bool BF, int CCAP>
+__device__ __forceinline__ int prune_list(const PruneRows& rows, const uint32_t* c_id,
+                                          const float* c_d, int cnt, float alpha, int R,
+                                          uint8_t* s_kept, int32_t* s_act_id, int32_t* s_act_pos,
+                                          float* s_act_d, int32_t* out_ids, float* out_d) {
+    if constexpr (BF)
+        return robust_prune_mfma<DP, STRICT, CCAP>(rows, c_id, c_d, cnt, alpha, R, s_kept,
+                                                   s_act_id, s_act_pos,
+                                                   reinterpret_cast<uint32_t*>(s_act_d), out_ids,
+                                                   out_d);
+    else
+        return robust_prune<DP, STRICT>(rows, c_id, c_d, cnt, alpha, R, s_kept, s_act_id,
+                                        s_act_pos, s_act_d, out_ids, out_d);
 }
 
 // Distances from `node` to its current row entries that pass `keep`, appended
