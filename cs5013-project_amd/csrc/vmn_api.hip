@@ -1059,6 +1059,8 @@ int vmn_consolidate(vmn_index* idx) {
         vmn::ConsArgs a{};
         a.DP = idx->DP;
         a.vecs = idx->vecs.as<float>();
+        a.bf16_rows = idx->exact_bf16 ? idx->bf16_rows.p : nullptr;
+        a.bf16_norms = idx->exact_bf16 ? idx->bf16_norms.as<float>() : nullptr;
         a.adj = idx->adj.as<int32_t>();
         a.del_bits = idx->del_bits.as<uint32_t>();
         a.W = R;

@@ -114,32 +114,110 @@ __device__ int robust_prune(const PruneRows& rows, const uint32_t* c_id,
 }
 
 // ---------------------------------------------------------------------------
+// Occlusion masks on MFMA for rows exact in BF16. Up to 16 "row" nodes (this
+// lane's row is `sid`, the node of A row l % 16; rows at or past `nrows` are
+// ignored) against the active candidates s_act_id[0..nact): s_pm[k] gets bit
+// j when row j occludes candidate k, alpha * d(row j, k) < d(node, k)
+// (STRICT) or <= (c_d[s_act_pos[k]] is d(node, k)); ORDERED keeps only
+// candidates after row j (k >= j, row j >= 1 being active entry j - 1).
+// Distances are ||a||^2 + ||b||^2 - 2 a.b from the BF16 rows and their norms:
+// exact integers, equal to the FP32 sums of squared differences. Lane l loads
+// 16 bytes of tile row l/4 per k-block (coalesced); ds_bpermute hands each
+// dword to the lane that holds it as the B operand. Two tiles of 16
+// candidates per round trip.
+// ---------------------------------------------------------------------------
+template <int DP, bool STRICT, bool ORDERED>
+__device__ __forceinline__ void gram_masks(const PruneRows& rows, int32_t sid, int nrows,
+                                           int nact, const int32_t* s_act_id,
+                                           const int32_t* s_act_pos, const float* c_d,
+                                           float alpha, uint32_t* s_pm) {
+    constexpr int KB = DP / 32;
+    const int l = lane(), g = l >> 4, rr = l >> 2, sub = l & 3, col = l & 15;
+    const __bf16* bro = reinterpret_cast<const __bf16*>(rows.bf16);
+    const float* nrm = rows.bf16_norms;
+    bf16x8_t A[KB];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+        A[kb] = *reinterpret_cast<const bf16x8_t*>(bro + (int64_t)sid * DP + 32 * kb + 8 * g);
+    // squared norms of rows 4g .. 4g+3 (lane r < 16 holds row r's node)
+    float an[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) an[i] = nrm[__builtin_amdgcn_ds_bpermute((4 * g + i) << 2, sid)];
+    const int src = (4 * col + g) << 2;
+    for (int base = 0; base < nact; base += 32) {
+        int32_t cid[2];
+        uint4 v[2][KB];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int c = base + 16 * t + rr;
+            cid[t] = c < nact ? s_act_id[c] : s_act_id[0];
+            if (t == 0 || base + 16 * t < nact) {
+                const uint4* r = reinterpret_cast<const uint4*>(bro + (int64_t)cid[t] * DP);
+#pragma unroll
+                for (int kb = 0; kb < KB; ++kb) v[t][kb] = r[4 * kb + sub];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            if (t == 0 || base + 16 * t < nact) {
+                f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int kb = 0; kb < KB; ++kb) {
+                    const uint4 u = {bperm(src, v[t][kb].x), bperm(src, v[t][kb].y),
+                                     bperm(src, v[t][kb].z), bperm(src, v[t][kb].w)};
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[kb], __builtin_bit_cast(bf16x8_t, u), acc, 0, 0, 0);
+                }
+                // this lane: candidate k (column col) against rows 4g .. 4g+3
+                const int k = base + 16 * t + col;
+                uint32_t bits = 0;
+                if (k < nact) {
+                    const float xn = nrm[s_act_id[k]];
+                    const float dn = c_d[s_act_pos[k]];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int j = 4 * g + i;
+                        const float lhs = alpha * ((an[i] + xn) - 2.0f * acc[i]);
+                        const bool pr = STRICT ? (lhs < dn) : (lhs <= dn);
+                        bits |= (uint32_t)(pr && j < nrows && (!ORDERED || k >= j)) << j;
+                    }
+                }
+                bits |= __shfl_xor(bits, 16);
+                bits |= __shfl_xor(bits, 32);
+                if (g == 0 && k < nact) s_pm[k] = bits;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // The same prune for rows exact in BF16, with the p*-to-candidate distances
 // as MFMA tiles. The next 16 kept candidates (p0, then the first 15 of the
 // active list behind it) are the 16 rows of A; one pass over the active
 // candidates' BF16 rows gives the distances from all 16 of them to every
-// active candidate, reduced at once to a 16-bit mask per candidate ("row j
-// occludes this candidate and it lies after row j"). The 16 are then taken in
-// list order as the sequential loop takes them: one still kept at its turn is
-// a p*, is output, and drops the later candidates its mask bit marks; one an
-// earlier p* dropped is skipped. Every candidate between two of the 16 was
-// already dropped, so the next scan, from just after the 16th, finds the
-// next p*, and the output equals robust_prune's (the distances are exact
-// integers either way). One gather of the active rows serves up to 16 rounds.
+// active candidate, reduced at once to a 16-bit mask per candidate
+// (gram_masks, ORDERED). The 16 are then taken in list order as the
+// sequential loop takes them: one still kept at its turn is a p*, is output,
+// and drops the later candidates its mask bit marks; one an earlier p*
+// dropped is skipped. Every candidate between two of the 16 was already
+// dropped, so the next scan, from just after the 16th, finds the next p*, and
+// the output equals robust_prune's (the distances are exact integers either
+// way). One gather of the active rows serves up to 16 rounds.
+// nk0 entries of out_ids are already taken (output continues after them);
+// with init_kept false, s_kept[0..cnt) holds the caller's kept flags.
 // ---------------------------------------------------------------------------
 template <int DP, bool STRICT, int CCAP>
 __device__ int robust_prune_mfma(const PruneRows& rows, const uint32_t* c_id, const float* c_d,
                                  int cnt, float alpha, int R, uint8_t* s_kept,
                                  int32_t* s_act_id, int32_t* s_act_pos, uint32_t* s_pm,
-                                 int32_t* out_ids, float* out_d) {
-    constexpr int KB = DP / 32;
+                                 int32_t* out_ids, float* out_d, int nk0 = 0,
+                                 bool init_kept = true) {
     constexpr int NCH = (CCAP + 63) / 64;
-    const int l = lane(), g = l >> 4, rr = l >> 2, sub = l & 3, col = l & 15;
-    const __bf16* bro = reinterpret_cast<const __bf16*>(rows.bf16);
-    const float* nrm = rows.bf16_norms;
-    for (int i = l; i < cnt; i += 64) s_kept[i] = 1;
+    const int l = lane(), col = l & 15;
+    if (init_kept) {
+        for (int i = l; i < cnt; i += 64) s_kept[i] = 1;
+    }
     __syncthreads();
-    int nk = 0, from = 0;
+    int nk = nk0, from = 0;
     while (nk < R) {
         // kept candidates from `from` on: the first is p0, the others the active list
         int nall = 0, p0 = -1;
@@ -162,67 +240,10 @@ __device__ int robust_prune_mfma(const PruneRows& rows, const uint32_t* c_id, co
         __syncthreads();
         if (nact > 0) {
             // A row `col`: p0 (row 0) or active entry col - 1 (rows past the
-            // list repeat p0 and are never read)
+            // list repeat p0 and are ignored)
             const int32_t sid = (col > 0 && col - 1 < nact) ? s_act_id[col - 1] : id0;
-            bf16x8_t A[KB];
-#pragma unroll
-            for (int kb = 0; kb < KB; ++kb)
-                A[kb] = *reinterpret_cast<const bf16x8_t*>(bro + (int64_t)sid * DP + 32 * kb + 8 * g);
-            // squared norms of rows 4g .. 4g+3, the rows this lane's accumulators hold
-            float an[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int r = 4 * g + i;
-                an[i] = nrm[(r > 0 && r - 1 < nact) ? s_act_id[r - 1] : id0];
-            }
-            const int src = (4 * col + g) << 2;
-            for (int base = 0; base < nact; base += 32) {
-                // two tiles of 16 active candidates per round trip; lane l
-                // loads 16 bytes of tile row l/4 per k-block (coalesced), and
-                // ds_bpermute hands each dword to the lane that holds it as B
-                int32_t cid[2];
-                uint4 v[2][KB];
-#pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    const int c = base + 16 * t + rr;
-                    cid[t] = c < nact ? s_act_id[c] : s_act_id[0];
-                    if (t == 0 || base + 16 * t < nact) {
-                        const uint4* r = reinterpret_cast<const uint4*>(bro + (int64_t)cid[t] * DP);
-#pragma unroll
-                        for (int kb = 0; kb < KB; ++kb) v[t][kb] = r[4 * kb + sub];
-                    }
-                }
-#pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    if (t == 0 || base + 16 * t < nact) {
-                        f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-                        for (int kb = 0; kb < KB; ++kb) {
-                            const uint4 u = {bperm(src, v[t][kb].x), bperm(src, v[t][kb].y),
-                                             bperm(src, v[t][kb].z), bperm(src, v[t][kb].w)};
-                            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[kb], __builtin_bit_cast(bf16x8_t, u), acc, 0, 0, 0);
-                        }
-                        // this lane: candidate k (column col), rows 4g .. 4g+3;
-                        // row j prunes only candidates after it (k >= j)
-                        const int k = base + 16 * t + col;
-                        uint32_t bits = 0;
-                        if (k < nact) {
-                            const float xn = nrm[s_act_id[k]];
-                            const float dn = c_d[s_act_pos[k]];
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) {
-                                const int j = 4 * g + i;
-                                const float lhs = alpha * ((an[i] + xn) - 2.0f * acc[i]);
-                                const bool pr = STRICT ? (lhs < dn) : (lhs <= dn);
-                                bits |= (uint32_t)(pr && j < ns && k >= j) << j;
-                            }
-                        }
-                        bits |= __shfl_xor(bits, 16);
-                        bits |= __shfl_xor(bits, 32);
-                        if (g == 0 && k < nact) s_pm[k] = bits;
-                    }
-                }
-            }
+            gram_masks<DP, STRICT, true>(rows, sid, ns, nact, s_act_id, s_act_pos, c_d, alpha,
+                                         s_pm);
         }
         __syncthreads();
         uint32_t pm[NCH];
@@ -261,6 +282,42 @@ __device__ int robust_prune_mfma(const PruneRows& rows, const uint32_t* c_id, co
     }
     __syncthreads();
     return nk;
+}
+
+// Every node of members[0..nm) drops the kept candidates it occludes
+// (alpha * d(member, c) <= d(node, c)), 16 members per pass over the kept
+// candidates' BF16 rows. The members all apply, in any order, so a candidate
+// stays exactly when no member occludes it.
+template <int DP>
+__device__ void occlude_mfma(const PruneRows& rows, const int32_t* members, int nm,
+                             const uint32_t* c_id, const float* c_d, int cnt, float alpha,
+                             uint8_t* s_kept, int32_t* s_act_id, int32_t* s_act_pos,
+                             uint32_t* s_pm) {
+    const int l = lane(), col = l & 15;
+    for (int g0 = 0; g0 < nm; g0 += 16) {
+        int nact = 0;
+        for (int b = 0; b < cnt; b += 64) {
+            const int j = b + l;
+            const bool f = j < cnt && s_kept[j];
+            const uint64_t m = ballot(f);
+            const int k = nact + __popcll((unsigned long long)(m & lanemask_lt()));
+            if (f) {
+                s_act_id[k] = (int32_t)(c_id[j] & VMN_ID_MASK);
+                s_act_pos[k] = j;
+            }
+            nact += __popcll((unsigned long long)m);
+        }
+        if (nact == 0) break;
+        __syncthreads();
+        const int nrows = nm - g0 < 16 ? nm - g0 : 16;
+        const int32_t sid = members[g0 + (col < nrows ? col : 0)];
+        gram_masks<DP, false, false>(rows, sid, nrows, nact, s_act_id, s_act_pos, c_d, alpha,
+                                     s_pm);
+        __syncthreads();
+        for (int k = l; k < nact; k += 64)
+            if (s_pm[k]) s_kept[s_act_pos[k]] = 0;
+        __syncthreads();
+    }
 }
 
 // robust_prune over FP32 rows, or robust_prune_mfma when the rows are exact in

@@ -87,7 +87,7 @@ hipError_t launch_cons_flags(const int32_t* adj, int64_t n, int W, const uint32_
 // holds R. Re-pruning the whole pool instead lets the 2-hop candidates
 // occlude long edges the build chose from a full search, and recall decays
 // with every round of deletes.
-template <int DP, int CCAP>
+template <int DP, int CCAP, bool BF>
 __global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
     __shared__ __align__(16) uint32_t c_id[2][CCAP];
     __shared__ __align__(16) float c_d[2][CCAP];
@@ -106,8 +106,8 @@ __global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
         const int64_t i = next_item(a.counter, &s_q);
         if (i >= a.nlist) return;
         const int64_t p = a.list[i];
-        const PruneRows rows{a.vecs, nullptr, nullptr};
-        PruneQ<DP, false> q;
+        const PruneRows rows{a.vecs, a.bf16_rows, a.bf16_norms};
+        PruneQ<DP, BF> q;
         q.load(rows, p);
         const int32_t own = l < W ? a.adj[p * W + l] : -1;
         const bool own_dead = own >= 0 && bit_test(a.del_bits, (uint32_t)own);
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
                 for (int t = 0; t < cnt; ++t) dup |= c_id[cur][t] == (uint32_t)v;
                 pass = !dup;
             }
-            const int ne = row_extras<DP, false>(rows, q, pass, v, e_id, e_d);
+            const int ne = row_extras<DP, BF>(rows, q, pass, v, e_id, e_d);
             const uint32_t my_id = l < ne ? (uint32_t)e_id[l] : 0u;
             const float my_d = l < ne ? e_d[l] : 0.0f;
             cnt = merge_sorted<CCAP, true>(c_id[cur], c_d[cur], cnt, my_id, my_d, ne,
@@ -139,71 +139,82 @@ __global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
         for (int t = l; t < cnt; t += 64) kept[t] = 1;
         if (l < nK) newrow[l] = k_id[l];
         __syncthreads();
-        for (int t = 0; t < nK && cnt > 0; ++t) {
-            int nact = 0;
-            for (int b = 0; b < cnt; b += 64) {
-                const int j = b + l;
-                const bool f = j < cnt && kept[j];
-                const uint64_t m = ballot(f);
-                const int k = nact + __popcll((unsigned long long)(m & lanemask_lt()));
-                if (f) {
-                    act_id[k] = (int32_t)(c_id[cur][j] & VMN_ID_MASK);
-                    act_pos[k] = j;
+        int nk = nK;
+        if constexpr (BF) {
+            occlude_mfma<DP>(rows, k_id, nK, c_id[cur], c_d[cur], cnt, a.alpha, kept, act_id,
+                             act_pos, reinterpret_cast<uint32_t*>(act_d));
+            // then the remaining candidates in order, as robust prune takes them
+            nk = robust_prune_mfma<DP, false, CCAP>(rows, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
+                                                    kept, act_id, act_pos,
+                                                    reinterpret_cast<uint32_t*>(act_d), newrow,
+                                                    nullptr, nK, false);
+        } else {
+            for (int t = 0; t < nK && cnt > 0; ++t) {
+                int nact = 0;
+                for (int b = 0; b < cnt; b += 64) {
+                    const int j = b + l;
+                    const bool f = j < cnt && kept[j];
+                    const uint64_t m = ballot(f);
+                    const int k = nact + __popcll((unsigned long long)(m & lanemask_lt()));
+                    if (f) {
+                        act_id[k] = (int32_t)(c_id[cur][j] & VMN_ID_MASK);
+                        act_pos[k] = j;
+                    }
+                    nact += __popcll((unsigned long long)m);
                 }
-                nact += __popcll((unsigned long long)m);
-            }
-            if (nact == 0) break;
-            QReg<DP> kq;
-            kq.load(a.vecs + (int64_t)k_id[t] * DP, sub);
-            __syncthreads();
-            gather_dists<DP>(kq, a.vecs, act_id, act_d, nact);
-            __syncthreads();
-            for (int k = l; k < nact; k += 64) {
-                const int j = act_pos[k];
-                if (a.alpha * act_d[k] <= c_d[cur][j]) kept[j] = 0;
-            }
-            __syncthreads();
-        }
-        // then the remaining candidates in order, as robust prune takes them
-        int nk = nK, from = 0;
-        while (nk < a.R) {
-            int pi = -1;
-            for (int b = from; b < cnt; b += 64) {
-                const int j = b + l;
-                const uint64_t m = ballot(j < cnt && kept[j]);
-                if (m) {
-                    pi = b + __ffsll((unsigned long long)m) - 1;
-                    break;
+                if (nact == 0) break;
+                QReg<DP> kq;
+                kq.load(a.vecs + (int64_t)k_id[t] * DP, sub);
+                __syncthreads();
+                gather_dists<DP>(kq, a.vecs, act_id, act_d, nact);
+                __syncthreads();
+                for (int k = l; k < nact; k += 64) {
+                    const int j = act_pos[k];
+                    if (a.alpha * act_d[k] <= c_d[cur][j]) kept[j] = 0;
                 }
+                __syncthreads();
             }
-            if (pi < 0) break;
-            const int32_t ps = (int32_t)(c_id[cur][pi] & VMN_ID_MASK);
-            if (l == 0) newrow[nk] = ps;
-            ++nk;
-            if (nk >= a.R) break;
-            int nact = 0;
-            for (int b = pi + 1; b < cnt; b += 64) {
-                const int j = b + l;
-                const bool f = j < cnt && kept[j];
-                const uint64_t m = ballot(f);
-                const int k = nact + __popcll((unsigned long long)(m & lanemask_lt()));
-                if (f) {
-                    act_id[k] = (int32_t)(c_id[cur][j] & VMN_ID_MASK);
-                    act_pos[k] = j;
+            // then the remaining candidates in order, as robust prune takes them
+            int from = 0;
+            while (nk < a.R) {
+                int pi = -1;
+                for (int b = from; b < cnt; b += 64) {
+                    const int j = b + l;
+                    const uint64_t m = ballot(j < cnt && kept[j]);
+                    if (m) {
+                        pi = b + __ffsll((unsigned long long)m) - 1;
+                        break;
+                    }
                 }
-                nact += __popcll((unsigned long long)m);
+                if (pi < 0) break;
+                const int32_t ps = (int32_t)(c_id[cur][pi] & VMN_ID_MASK);
+                if (l == 0) newrow[nk] = ps;
+                ++nk;
+                if (nk >= a.R) break;
+                int nact = 0;
+                for (int b = pi + 1; b < cnt; b += 64) {
+                    const int j = b + l;
+                    const bool f = j < cnt && kept[j];
+                    const uint64_t m = ballot(f);
+                    const int k = nact + __popcll((unsigned long long)(m & lanemask_lt()));
+                    if (f) {
+                        act_id[k] = (int32_t)(c_id[cur][j] & VMN_ID_MASK);
+                        act_pos[k] = j;
+                    }
+                    nact += __popcll((unsigned long long)m);
+                }
+                QReg<DP> pq;
+                pq.load(a.vecs + (int64_t)ps * DP, sub);
+                __syncthreads();
+                gather_dists<DP>(pq, a.vecs, act_id, act_d, nact);
+                __syncthreads();
+                for (int k = l; k < nact; k += 64) {
+                    const int j = act_pos[k];
+                    if (a.alpha * act_d[k] <= c_d[cur][j]) kept[j] = 0;
+                }
+                __syncthreads();
+                from = pi + 1;
             }
-            QReg<DP> pq;
-            pq.load(a.vecs + (int64_t)ps * DP, sub);
-            __syncthreads();
-            gather_dists<DP>(pq, a.vecs, act_id, act_d, nact);
-            __syncthreads();
-            for (int k = l; k < nact; k += 64) {
-                const int j = act_pos[k];
-                if (a.alpha * act_d[k] <= c_d[cur][j]) kept[j] = 0;
-            }
-            __syncthreads();
-            from = pi + 1;
         }
         __syncthreads();
         for (int j = l; j < W; j += 64) a.adj[p * W + j] = j < nk ? newrow[j] : -1;
@@ -216,7 +227,10 @@ hipError_t launch_consolidate(const ConsArgs& a, int grid, hipStream_t s) {
     switch (a.DP) {
 #define VMN_CONS_CASE(DPV)                                                                   \
     case DPV:                                                                                \
-        hipLaunchKernelGGL((consolidate_kernel<DPV, 512>), dim3(grid), dim3(64), 0, s, a); \
+        if (a.bf16_rows)                                                                     \
+            hipLaunchKernelGGL((consolidate_kernel<DPV, 512, true>), dim3(grid), dim3(64), 0, s, a); \
+        else                                                                                 \
+            hipLaunchKernelGGL((consolidate_kernel<DPV, 512, false>), dim3(grid), dim3(64), 0, s, a); \
         break;
         VMN_CONS_CASE(32)
         VMN_CONS_CASE(64)

@@ -7,6 +7,8 @@ namespace vmn {
 struct ConsArgs {
     int DP;
     const float* vecs;
+    const void* bf16_rows;    // BF16 copies of vecs when every row is exact in BF16, else null
+    const float* bf16_norms;
     int32_t* adj;
     const uint32_t* del_bits;
     int W, R, M;
