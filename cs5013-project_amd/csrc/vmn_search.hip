@@ -50,8 +50,12 @@ __global__ void fill_zero_u32(unsigned int* p, int n) {
 
 // PREC_ARG is a VMN_PREC_* value, or VMN_PREC_BF16_TABLE_NORMS: BF16 rows with
 // the row norms read from the norms table (small batches, FRESH mode).
+// FP8 with beam lists up to 128 entries is held to 5 waves per SIMD (96
+// VGPRs, no spills; the compiler's own choice takes 4): the 10k-query batch
+// runs 7% faster, single queries 3% slower.
 template <int DP, int LCAP, int MODE, int PREC_ARG>
-__global__ __launch_bounds__(64) void search_kernel(SearchArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
+    (PREC_ARG == VMN_PREC_FP8 && LCAP <= 128) ? 5 : 1, 8))) void search_kernel(SearchArgs a) {
     constexpr int PREC = PREC_ARG == VMN_PREC_BF16_TABLE_NORMS ? VMN_PREC_BF16 : PREC_ARG;
     constexpr bool TN = PREC_ARG == VMN_PREC_BF16_TABLE_NORMS;
     __shared__ __align__(16) uint32_t s_id[2][LCAP];

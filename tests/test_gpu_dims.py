@@ -91,3 +91,23 @@ def test_lowp_search_at_width(vmn, gpu, d):
     gt = np.argsort(((base[None, :, :].astype(np.float64) - q[:, None, :]) ** 2).sum(-1),
                     axis=1, kind="stable")[:, :10]
     assert vmn.recall_at_k(xi, gt) >= vmn.recall_at_k(fi, gt) - 0.05
+
+
+@pytest.mark.parametrize("d", DIMS)
+@pytest.mark.parametrize("mode", ["fresh", "parth"])
+def test_exact_bf16_build_at_width(vmn, orc, gpu, d, mode):
+    """On rows exact in BF16 the build's searches and prunes (the 16-row MFMA
+    Gram) read BF16 rows; the graph must be the oracle's FP32 graph at every
+    padded width (one to eight 64-byte k-blocks per row). R = 12 stops most
+    prunes at R, with candidates left over."""
+    base = half_scale(vmn, 1500, d, 0)
+    if mode == "fresh":
+        idx = vmn.Vamana(base, mode="fresh", R=12, L_build=40, alphas=(1.0, 1.2))
+        oadj, med = orc.fresh_build(base, R=12, L=40, alphas=(1.0, 1.2))
+    else:
+        idx = vmn.Vamana(base, mode="parth", R=12, L_build=40, alphas=(1.2, 1.2))
+        oadj, med = orc.parth_build(base, R=12, L=40, alphas=(1.2, 1.2))
+    assert idx.build_precision == "bf16"
+    assert idx.medoid == med
+    assert np.array_equal(idx.adjacency, oadj)
+    idx.close()
