@@ -93,8 +93,9 @@ int alloc_graph(vmn_index* idx, int64_t n) {
     return VMN_OK;
 }
 
-// (Re)computes the low-precision copies of rows [from, to). FP8 scales are
-// fixed from the rows present at the first call (later rows are clamped).
+// (Re)computes the low-precision copies of rows [from, to). The FP8 centre
+// and scales are fixed from the rows present at the first call (later rows
+// are coded with them and clamped).
 int refresh_lowp(vmn_index* idx, int64_t from, int64_t to) {
     const int DP = idx->DP;
     const int64_t cap = idx->cap, cnt = to - from;
@@ -110,16 +111,20 @@ int refresh_lowp(vmn_index* idx, int64_t from, int64_t to) {
         HIPCHK(idx->fp8_rows.reserve((size_t)cap * DP));
         HIPCHK(idx->fp8_norms.reserve((size_t)cap * sizeof(float)));
         HIPCHK(idx->fp8_scales.reserve((size_t)DP * sizeof(float)));
+        HIPCHK(idx->fp8_centre.reserve((size_t)DP * sizeof(float)));
         HIPCHK(idx->fp8_absmax.reserve((size_t)DP * sizeof(unsigned int)));
         if (!idx->fp8_scaled) {
+            HIPCHK(idx->fp8_temp.reserve(vmn::fp8_scales_temp_bytes(to, DP)));
             HIPCHK(vmn::launch_fp8_scales(idx->vecs.as<float>(), to, idx->p.dim, DP,
                                           idx->fp8_absmax.as<unsigned int>(),
+                                          idx->fp8_temp.as<double>(), idx->fp8_centre.as<float>(),
                                           idx->fp8_scales.as<float>(), idx->stream));
             idx->fp8_scaled = true;
             from = 0;
             src = idx->vecs.as<float>();
         }
         HIPCHK(vmn::launch_to_fp8(src, to - from, DP, idx->fp8_scales.as<float>(),
+                                  idx->fp8_centre.as<float>(),
                                   static_cast<char*>(idx->fp8_rows.p) + from * DP,
                                   idx->fp8_norms.as<float>() + from, idx->stream));
     }
@@ -200,6 +205,7 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
         a.low_rows = idx->fp8_rows.p;
         a.low_norms = idx->fp8_norms.as<float>();
         a.fp8_scales = idx->fp8_scales.as<float>();
+        a.fp8_centre = idx->fp8_centre.as<float>();
     }
     a.lcap = idx->list_cap;
     // a batch that leaves most of the GPU idle is latency-bound: there the
@@ -688,7 +694,7 @@ int vmn_get_lowp_rows(const vmn_index* idx, int32_t prec, float* out) {
         DevBuf tmp;
         HIPCHK(tmp.reserve(rows.size() * sizeof(float)));
         HIPCHK(vmn::launch_fp8_dequant(idx->fp8_rows.p, n, DP, idx->fp8_scales.as<float>(),
-                                       tmp.as<float>(), idx->stream));
+                                       idx->fp8_centre.as<float>(), tmp.as<float>(), idx->stream));
         HIPCHK(hipMemcpyAsync(rows.data(), tmp.p, rows.size() * sizeof(float), hipMemcpyDeviceToHost,
                               idx->stream));
         HIPCHK(hipStreamSynchronize(idx->stream));

@@ -168,10 +168,16 @@ __device__ __forceinline__ float fp8_to_f32(uint32_t word, int byte) {
 }
 
 // Query preparation (whole wave). qf: padded FP32 query row. BF16: q_hat =
-// bf16(q), qn = ||q_hat||^2. FP8: q'_j = q_j * s_j (per-dimension scales of
-// the stored rows), t = max|q'| / 448, A = e4m3(q' / t), qn = ||q||^2.
+// bf16(q), qn = ||q_hat||^2. FP8 (rows stored as e4m3((x_j - c_j) / s_j)
+// about the per-dimension centre c): q'_j = (q_j - c_j) * s_j,
+// t = max|q'| / 448, A = e4m3(q' / t), qn = ||q - c||^2. The FP8 codes are
+// made one element per lane per 64 (j = l + 64 r) and handed to the operand
+// layout (8 consecutive codes per lane and k-block) through `stage`, DP bytes
+// of LDS, so only a few values per lane are live at once.
 template <int DP, int PREC>
-__device__ __forceinline__ void prep_query(QLow<DP>& Q, const float* qf, const float* scales) {
+__device__ __forceinline__ void prep_query(QLow<DP>& Q, const float* qf, const float* scales,
+                                           const float* centre = nullptr,
+                                           uint8_t* stage = nullptr) {
     const int l = lane(), g = l >> 4;
     if (PREC == VMN_PREC_BF16) {
         float part = 0.0f;
@@ -191,34 +197,36 @@ __device__ __forceinline__ void prep_query(QLow<DP>& Q, const float* qf, const f
         Q.qn = part;
         Q.t = 1.0f;
     } else {
+        constexpr int NR = (DP + 63) / 64;
+        float v[NR];
         float mx = 0.0f, part = 0.0f;
 #pragma unroll
-        for (int kb = 0; kb < QLow<DP>::KB; ++kb)
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int j = 32 * kb + 8 * g + e;
-                const float v = qf[j] * scales[j];
-                mx = fmaxf(mx, fabsf(v));
-                part += qf[j] * qf[j];
+        for (int r = 0; r < NR; ++r) {
+            const int j = l + 64 * r;
+            v[r] = 0.0f;
+            if (j < DP) {
+                const float u = qf[j] - centre[j];
+                v[r] = u * scales[j];
+                mx = fmaxf(mx, fabsf(v[r]));
+                part += u * u;
             }
-        mx = fmaxf(mx, __shfl_xor(mx, 16));
-        mx = fmaxf(mx, __shfl_xor(mx, 32));
-        part += __shfl_xor(part, 16);
-        part += __shfl_xor(part, 32);
+        }
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            mx = fmaxf(mx, __shfl_xor(mx, o));
+            part += __shfl_xor(part, o);
+        }
         const float t = mx > 0.0f ? mx / 448.0f : 1.0f;
 #pragma unroll
-        for (int kb = 0; kb < QLow<DP>::KB; ++kb) {
-            uint32_t w[4];
-#pragma unroll
-            for (int e = 0; e < 8; e += 2) {
-                const int j = 32 * kb + 8 * g + e;
-                const float v0 = fminf(fmaxf(qf[j] * scales[j] / t, -448.0f), 448.0f);
-                const float v1 = fminf(fmaxf(qf[j + 1] * scales[j + 1] / t, -448.0f), 448.0f);
-                w[e / 2] = pk_fp8(v0, v1);
-            }
-            const uint32_t lo = w[0] | (w[1] << 16), hi = w[2] | (w[3] << 16);
-            Q.a8[kb] = (long)(((unsigned long)hi << 32) | lo);
+        for (int r = 0; r < NR; ++r) {
+            const int j = l + 64 * r;
+            if (j < DP) stage[j] = (uint8_t)(pk_fp8(fminf(fmaxf(v[r] / t, -448.0f), 448.0f), 0.0f) & 0xff);
         }
+        __syncthreads();
+#pragma unroll
+        for (int kb = 0; kb < QLow<DP>::KB; ++kb)
+            Q.a8[kb] = *reinterpret_cast<const long*>(stage + 32 * kb + 8 * g);
+        __syncthreads();
         Q.qn = part;
         Q.t = t;
     }

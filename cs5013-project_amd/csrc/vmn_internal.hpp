@@ -85,7 +85,8 @@ struct vmn_index {
     int64_t last_nq = 0;
     int64_t n_deleted = 0;  // tombstones set since the build (upper bound)
     vmn_rt::DevBuf staging, find_out, bad;
-    vmn_rt::DevBuf bf16_rows, bf16_norms, fp8_rows, fp8_norms, fp8_scales, fp8_absmax;
+    vmn_rt::DevBuf bf16_rows, bf16_norms, fp8_rows, fp8_norms, fp8_scales, fp8_centre, fp8_absmax,
+        fp8_temp;
     bool fp8_scaled = false;
     // every row so far makes BF16 traversal exact: the build and insert
     // searches then read BF16 rows (half the bytes, bit-identical result)

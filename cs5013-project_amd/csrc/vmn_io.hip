@@ -4,14 +4,17 @@
 // * Index file (vmn_save / vmn_load): everything a later search or update
 //   needs, so a loaded index behaves exactly like the saved one: parameters,
 //   vectors, adjacency, tombstones, entry point and the FP8 per-dimension
-//   scales (fixed at the build; re-deriving them from all rows after inserts
-//   would change the FP8 rows). Little-endian:
-//     char magic[8] = "VMNIDX01"; u32 version = 1; u32 DP;
+//   scales and centre (fixed at the build; re-deriving them from all rows
+//   after inserts would change the FP8 rows). Little-endian:
+//     char magic[8] = "VMNIDX01"; u32 version = 2; u32 DP;
 //     i32 dim, R, L_build, passes; f32 alpha[4]; u64 seed;
 //     i32 mode, visited_cap, max_hops, P, M; i64 capacity;
 //     i32 low_precision, fp8_scaled; i64 n, medoid, n_deleted;
 //     f32 vectors[n x dim]; i32 adjacency[n x R] (-1 padded);
-//     u32 tombstone words[(n + 31) / 32]; f32 fp8_scales[DP] if fp8_scaled.
+//     u32 tombstone words[(n + 31) / 32];
+//     if fp8_scaled: f32 fp8_scales[DP], f32 fp8_centre[DP].
+//   Version-1 files (no centre: their FP8 rows were coded about 0) load with
+//   a zero centre.
 // * FreshVamana record file (saveGraph vamana.cu:1346-1362, read by initGraph
 //   utils.cuh:113-148): n records of { f32 vec[dim]; u32 degree; u32
 //   nbr[R] }, no header (772 bytes at dim 128, R 64).
@@ -92,16 +95,19 @@ int vmn_save(const vmn_index* idx, const char* path) {
     int rc = fetch(idx, vecs, adj, &del);
     if (rc) return rc;
     const bool fp8 = (idx->p.low_precision & VMN_STORE_FP8) && idx->fp8_scaled;
-    std::vector<float> scales;
+    std::vector<float> scales, centre;
     if (fp8) {
         scales.resize(idx->DP);
+        centre.resize(idx->DP);
         HIPCHK(hipMemcpy(scales.data(), idx->fp8_scales.p, scales.size() * sizeof(float),
+                         hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(centre.data(), idx->fp8_centre.p, centre.size() * sizeof(float),
                          hipMemcpyDeviceToHost));
     }
     File f(path, "wb");
     if (!f.f) return fail(VMN_ERR_IO, "cannot open %s for writing", path);
     const vmn_params& p = idx->p;
-    bool ok = f.put(kMagic, 8) && f.put((uint32_t)1) && f.put((uint32_t)idx->DP) &&
+    bool ok = f.put(kMagic, 8) && f.put((uint32_t)2) && f.put((uint32_t)idx->DP) &&
               f.put(p.dim) && f.put(p.R) && f.put(p.L_build) && f.put(p.passes) &&
               f.put(p.alpha, sizeof p.alpha) && f.put(p.seed) && f.put(p.mode) &&
               f.put(p.visited_cap) && f.put(p.max_hops) && f.put(p.P) && f.put(p.M) &&
@@ -110,7 +116,8 @@ int vmn_save(const vmn_index* idx, const char* path) {
               f.put(vecs.data(), vecs.size() * sizeof(float)) &&
               f.put(adj.data(), adj.size() * sizeof(int32_t)) &&
               f.put(del.data(), del.size() * sizeof(uint32_t)) &&
-              (!fp8 || f.put(scales.data(), scales.size() * sizeof(float)));
+              (!fp8 || (f.put(scales.data(), scales.size() * sizeof(float)) &&
+                        f.put(centre.data(), centre.size() * sizeof(float))));
     if (!ok) return fail(VMN_ERR_IO, "short write to %s", path);
     return VMN_OK;
 }
@@ -122,9 +129,9 @@ int vmn_load(const char* path, int32_t device, vmn_index** out) {
     if (!f.f) return fail(VMN_ERR_IO, "cannot open %s", path);
     char magic[8];
     uint32_t version = 0, DP = 0;
-    if (!f.get(magic, 8) || std::memcmp(magic, kMagic, 8) != 0 || !f.get(version) || version != 1 ||
-        !f.get(DP))
-        return fail(VMN_ERR_IO, "%s is not a version-1 index file", path);
+    if (!f.get(magic, 8) || std::memcmp(magic, kMagic, 8) != 0 || !f.get(version) ||
+        (version != 1 && version != 2) || !f.get(DP))
+        return fail(VMN_ERR_IO, "%s is not a version-1 or version-2 index file", path);
     vmn_params p{};
     int32_t fp8 = 0;
     int64_t n = 0, medoid = 0, n_deleted = 0;
@@ -142,11 +149,12 @@ int vmn_load(const char* path, int32_t device, vmn_index** out) {
     std::vector<float> vecs((size_t)n * p.dim);
     std::vector<int32_t> adj((size_t)n * p.R);
     std::vector<uint32_t> del((size_t)(n + 31) / 32);
-    std::vector<float> scales(fp8 ? DP : 0);
+    std::vector<float> scales(fp8 ? DP : 0), centre(fp8 ? DP : 0, 0.0f);
     ok = f.get(vecs.data(), vecs.size() * sizeof(float)) &&
          f.get(adj.data(), adj.size() * sizeof(int32_t)) &&
          f.get(del.data(), del.size() * sizeof(uint32_t)) &&
-         (!fp8 || f.get(scales.data(), scales.size() * sizeof(float)));
+         (!fp8 || f.get(scales.data(), scales.size() * sizeof(float))) &&
+         (!fp8 || version < 2 || f.get(centre.data(), centre.size() * sizeof(float)));
     if (!ok) return fail(VMN_ERR_IO, "truncated body in %s", path);
     for (int32_t v : adj)
         if (v < -1 || v >= n) return fail(VMN_ERR_IO, "adjacency id %d out of range in %s", v, path);
@@ -181,7 +189,10 @@ int vmn_load(const char* path, int32_t device, vmn_index** out) {
     if (rc) return rc;
     if (fp8) {
         HIPCHK(idx->fp8_scales.reserve((size_t)DP * sizeof(float)));
+        HIPCHK(idx->fp8_centre.reserve((size_t)DP * sizeof(float)));
         HIPCHK(hipMemcpyAsync(idx->fp8_scales.p, scales.data(), scales.size() * sizeof(float),
+                              hipMemcpyHostToDevice, idx->stream));
+        HIPCHK(hipMemcpyAsync(idx->fp8_centre.p, centre.data(), centre.size() * sizeof(float),
                               hipMemcpyHostToDevice, idx->stream));
         idx->fp8_scaled = true;
     } else {

@@ -31,6 +31,7 @@ struct SearchArgs {
     const void* low_rows;     // [cap x DP] BF16 or FP8 rows
     const float* low_norms;   // [cap] ||x_hat||^2 of the low-precision rows
     const float* fp8_scales;  // [DP] per-dimension FP8 scales
+    const float* fp8_centre;  // [DP] per-dimension centre the FP8 rows are coded about
     int krr;                  // > 0: exact FP32 re-rank of the first krr list entries (<= 64)
     int lcap;                 // minimum list capacity template (0: smallest >= L)
     int table_norms;          // BF16, FRESH: row norms from low_norms (latency-bound batches)
@@ -45,12 +46,15 @@ hipError_t launch_bf16_exact(const float* vecs, int64_t from, int64_t to, int DP
                              hipStream_t s);
 hipError_t launch_to_bf16(const float* vecs, int64_t n, int DP, void* rows, float* norms,
                           hipStream_t s);
+// Per-dimension centre (mean, FP64 in fixed chunks) and scales (absmax of the
+// centred values / 448) of rows [0, n); temp holds fp8_scales_temp_bytes(n, DP).
+size_t fp8_scales_temp_bytes(int64_t n, int DP);
 hipError_t launch_fp8_scales(const float* vecs, int64_t n, int D, int DP, unsigned int* absmax_bits,
-                             float* scales, hipStream_t s);
-hipError_t launch_to_fp8(const float* vecs, int64_t n, int DP, const float* scales, void* rows,
-                         float* norms, hipStream_t s);
-hipError_t launch_fp8_dequant(const void* rows, int64_t n, int DP, const float* scales, float* out,
-                              hipStream_t s);
+                             double* temp, float* centre, float* scales, hipStream_t s);
+hipError_t launch_to_fp8(const float* vecs, int64_t n, int DP, const float* scales,
+                         const float* centre, void* rows, float* norms, hipStream_t s);
+hipError_t launch_fp8_dequant(const void* rows, int64_t n, int DP, const float* scales,
+                              const float* centre, float* out, hipStream_t s);
 
 struct PruneArgs {
     int mode;

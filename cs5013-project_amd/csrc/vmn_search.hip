@@ -50,17 +50,17 @@ __global__ void fill_zero_u32(unsigned int* p, int n) {
 
 // PREC_ARG is a VMN_PREC_* value, or VMN_PREC_BF16_TABLE_NORMS: BF16 rows with
 // the row norms read from the norms table (small batches, FRESH mode).
-// FP8 with beam lists up to 128 entries is held to 5 waves per SIMD (96
-// VGPRs, no spills; the compiler's own choice takes 4): the 10k-query batch
-// runs 7% faster, single queries 3% slower.
+// FP8 at row widths up to 128 with beam lists up to 128 entries is held to 5
+// waves per SIMD (96 VGPRs, no spills; the compiler's own choice takes 4):
+// the 10k-query batch runs 7% faster, single queries 3% slower.
 template <int DP, int LCAP, int MODE, int PREC_ARG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
-    (PREC_ARG == VMN_PREC_FP8 && LCAP <= 128) ? 5 : 1, 8))) void search_kernel(SearchArgs a) {
+    (PREC_ARG == VMN_PREC_FP8 && LCAP <= 128 && DP <= 128) ? 5 : 1, 8))) void search_kernel(SearchArgs a) {
     constexpr int PREC = PREC_ARG == VMN_PREC_BF16_TABLE_NORMS ? VMN_PREC_BF16 : PREC_ARG;
     constexpr bool TN = PREC_ARG == VMN_PREC_BF16_TABLE_NORMS;
     __shared__ __align__(16) uint32_t s_id[2][LCAP];
     __shared__ __align__(16) float s_d[2][LCAP];
-    __shared__ int32_t s_sid[64];
+    __shared__ __align__(16) int32_t s_sid[64];
     __shared__ float s_sd[64];
     __shared__ uint32_t s_bloom[MODE == VMN_MODE_FRESH ? VMN_BF_WORDS : 1];
     __shared__ int64_t s_q;
@@ -81,7 +81,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
         QReg<DP> q;
         if (PREC == VMN_PREC_FP32) q.load(a.queries + qi * DP, sub);
         QLow<DP> ql;
-        if (PREC != VMN_PREC_FP32) prep_query<DP, PREC>(ql, a.queries + qi * DP, a.fp8_scales);
+        if (PREC != VMN_PREC_FP32) prep_query<DP, PREC>(ql, a.queries + qi * DP, a.fp8_scales, a.fp8_centre,
+                                 reinterpret_cast<uint8_t*>(s_sid));
         const int32_t self = a.query_is_node ? (int32_t)(a.node_base + qi) : -1;
 
         if (MODE == VMN_MODE_FRESH) {
@@ -372,13 +373,37 @@ hipError_t launch_to_bf16(const float* vecs, int64_t n, int DP, void* rows, floa
     return hipGetLastError();
 }
 
-// Per-dimension absmax over the rows (non-negative float bits order as uints).
-__global__ void absmax_kernel(const float* __restrict__ vecs, int64_t n, int DP,
-                              unsigned int* absmax_bits) {
+// FP8 rows are coded about the per-dimension mean c of the build's rows.
+// Squared L2 is unchanged by the shift, the centred values are smaller, and
+// e4m3's rounding error is relative to the value coded: on 100k rows of the
+// 256-component 128-d integer mixture the relative distance error over each
+// query's 1000 nearest rows falls from 3.5% to 1.0% rms. The mean is summed in FP64 over fixed row
+// chunks (colsum_kernel) and the chunks in order (centre_kernel), so it does
+// not depend on scheduling.
+__global__ void colsum_kernel(const float* __restrict__ vecs, int64_t n, int DP, int64_t chunk,
+                              double* part) {
     const int d = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * chunk;
+    const int64_t r1 = r0 + chunk < n ? r0 + chunk : n;
+    double s = 0.0;
+    for (int64_t i = r0; i < r1; ++i) s += (double)vecs[i * DP + d];
+    part[(int64_t)blockIdx.x * DP + d] = s;
+}
+__global__ void centre_kernel(const double* part, int nb, int64_t n, int DP, float* centre) {
+    const int d = threadIdx.x;
+    if (d >= DP) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[(int64_t)b * DP + d];
+    centre[d] = (float)(s / (double)n);
+}
+// Per-dimension absmax of the centred rows (non-negative float bits order as uints).
+__global__ void absmax_kernel(const float* __restrict__ vecs, int64_t n, int DP,
+                              const float* __restrict__ centre, unsigned int* absmax_bits) {
+    const int d = threadIdx.x;
+    const float c = centre[d];
     unsigned int m = 0;
     for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
-        const unsigned int b = __float_as_uint(fabsf(vecs[i * DP + d]));
+        const unsigned int b = __float_as_uint(fabsf(vecs[i * DP + d] - c));
         m = b > m ? b : m;
     }
     atomicMax(&absmax_bits[d], m);
@@ -389,25 +414,38 @@ __global__ void scales_kernel(const unsigned int* absmax_bits, int DP, float* sc
     const float m = __uint_as_float(absmax_bits[d]);
     scales[d] = m > 0.0f ? m / 448.0f : 1.0f;
 }
+size_t fp8_scales_temp_bytes(int64_t n, int DP) {
+    const int64_t nb = n < 1024 ? (n < 1 ? 1 : n) : 1024;
+    return (size_t)nb * DP * sizeof(double);
+}
 hipError_t launch_fp8_scales(const float* vecs, int64_t n, int D, int DP, unsigned int* absmax_bits,
-                             float* scales, hipStream_t s) {
+                             double* temp, float* centre, float* scales, hipStream_t s) {
     (void)D;
+    if (n < 1) return hipErrorInvalidValue;
+    int64_t nb = n < 1024 ? n : 1024;
+    const int64_t chunk = (n + nb - 1) / nb;
+    nb = (n + chunk - 1) / chunk;
+    hipLaunchKernelGGL(colsum_kernel, dim3((unsigned)nb), dim3(DP), 0, s, vecs, n, DP, chunk, temp);
+    hipLaunchKernelGGL(centre_kernel, dim3(1), dim3(256), 0, s, temp, (int)nb, n, DP, centre);
     hipLaunchKernelGGL(fill_zero_u32, dim3(1), dim3(256), 0, s, absmax_bits, DP);
-    const int64_t g = n < 2048 ? (n < 1 ? 1 : n) : 2048;
-    hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)g), dim3(DP), 0, s, vecs, n, DP, absmax_bits);
+    const int64_t g = n < 2048 ? n : 2048;
+    hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)g), dim3(DP), 0, s, vecs, n, DP, centre,
+                       absmax_bits);
     hipLaunchKernelGGL(scales_kernel, dim3(1), dim3(256), 0, s, absmax_bits, DP, scales);
     return hipGetLastError();
 }
 
-// x~ = e4m3(clamp(x / s)), norms = ||s * x~||^2 with the hardware decode.
+// x~ = e4m3(clamp((x - c) / s)), norms = ||s * x~||^2 with the hardware decode.
 __global__ void to_fp8_kernel(const float* __restrict__ vecs, int64_t n, int DP,
-                              const float* __restrict__ scales, uint8_t* rows, float* norms) {
+                              const float* __restrict__ scales, const float* __restrict__ centre,
+                              uint8_t* rows, float* norms) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float s = 0.0f;
     for (int k = 0; k < DP; k += 2) {
-        const float v0 = fminf(fmaxf(vecs[i * DP + k] / scales[k], -448.0f), 448.0f);
-        const float v1 = fminf(fmaxf(vecs[i * DP + k + 1] / scales[k + 1], -448.0f), 448.0f);
+        const float v0 = fminf(fmaxf((vecs[i * DP + k] - centre[k]) / scales[k], -448.0f), 448.0f);
+        const float v1 =
+            fminf(fmaxf((vecs[i * DP + k + 1] - centre[k + 1]) / scales[k + 1], -448.0f), 448.0f);
         const uint32_t w = pk_fp8(v0, v1);
         rows[i * DP + k] = (uint8_t)(w & 0xff);
         rows[i * DP + k + 1] = (uint8_t)(w >> 8);
@@ -418,24 +456,24 @@ __global__ void to_fp8_kernel(const float* __restrict__ vecs, int64_t n, int DP,
     }
     norms[i] = s;
 }
-hipError_t launch_to_fp8(const float* vecs, int64_t n, int DP, const float* scales, void* rows,
-                         float* norms, hipStream_t s) {
+hipError_t launch_to_fp8(const float* vecs, int64_t n, int DP, const float* scales,
+                         const float* centre, void* rows, float* norms, hipStream_t s) {
     hipLaunchKernelGGL(to_fp8_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, vecs, n,
-                       DP, scales, reinterpret_cast<uint8_t*>(rows), norms);
+                       DP, scales, centre, reinterpret_cast<uint8_t*>(rows), norms);
     return hipGetLastError();
 }
 
-// Dequantised FP8 rows (s * x~), for error reports and tests.
+// Dequantised FP8 rows (s * x~ + c), for error reports and tests.
 __global__ void fp8_dequant_kernel(const uint8_t* rows, int64_t n, int DP, const float* scales,
-                                   float* out) {
+                                   const float* centre, float* out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * DP) return;
-    out[i] = fp8_to_f32(rows[i], 0) * scales[i % DP];
+    out[i] = fp8_to_f32(rows[i], 0) * scales[i % DP] + centre[i % DP];
 }
-hipError_t launch_fp8_dequant(const void* rows, int64_t n, int DP, const float* scales, float* out,
-                              hipStream_t s) {
+hipError_t launch_fp8_dequant(const void* rows, int64_t n, int DP, const float* scales,
+                              const float* centre, float* out, hipStream_t s) {
     hipLaunchKernelGGL(fp8_dequant_kernel, dim3((unsigned)((n * DP + 255) / 256)), dim3(256), 0, s,
-                       reinterpret_cast<const uint8_t*>(rows), n, DP, scales, out);
+                       reinterpret_cast<const uint8_t*>(rows), n, DP, scales, centre, out);
     return hipGetLastError();
 }
 

@@ -47,7 +47,8 @@ extern "C" {
 /* Distance precision during traversal (vmn_search_ex). */
 #define VMN_PREC_FP32 0 /* FP32 rows, lane-tree sums (the reference's arithmetic) */
 #define VMN_PREC_BF16 1 /* BF16 rows on MFMA, FP32 accumulation */
-#define VMN_PREC_FP8 2  /* FP8 E4M3 rows with per-dimension absmax scales on MFMA */
+#define VMN_PREC_FP8 2  /* FP8 E4M3 rows, centred on the per-dimension mean and scaled by the
+                          centred absmax, on MFMA */
 
 /* vmn_params.low_precision bits: which low-precision copies the index keeps. */
 #define VMN_STORE_BF16 1
@@ -103,7 +104,7 @@ int vmn_load_graph(vmn_index* idx, const float* vecs, int64_t n, const int32_t* 
                    int32_t row_width, int64_t medoid, int32_t mem);
 
 /* Index file: parameters, vectors, adjacency, tombstones, entry point and the
- * FP8 scales, so the loaded index searches and updates exactly as the saved
+ * FP8 scales and centre, so the loaded index searches and updates exactly as the saved
  * one. vmn_load creates the index on `device`. Replaces `saveGraph`
  * (vamana.cu:1346-1362), which writes only vectors and adjacency. */
 int vmn_save(const vmn_index* idx, const char* path);
@@ -197,7 +198,7 @@ int vmn_search_vectors(vmn_index* idx, const float* queries, int64_t nq, int32_t
                        float* vecs_out, int32_t mem);
 
 /* The rows as the low-precision traversal sees them, decoded to float
- * ([size x dim], host): BF16 values, or FP8 values times their scales. */
+ * ([size x dim], host): BF16 values, or FP8 values times their scales plus the centre. */
 int vmn_get_lowp_rows(const vmn_index* idx, int32_t prec, float* out);
 
 /* Batched insert of n vectors [n x dim]: they become nodes size .. size+n-1

@@ -63,7 +63,7 @@ def test_file_readers_reject_bad_files_without_gpu(vmn, tmp_path):
     junk.write_bytes(b"not an index file at all")
     with pytest.raises(err) as e:
         vmn.Vamana.load(str(junk))
-    assert e.value.code == -5 and "version-1 index file" in str(e.value)
+    assert e.value.code == -5 and "not a version-1 or version-2 index file" in str(e.value)
     # parth graph whose header promises more than the file holds
     bad = tmp_path / "short_graph.bin"
     bad.write_bytes(struct_pack(100, 128, 32) + b"\0" * 1000)

@@ -9,12 +9,13 @@ here.
   distance (||q||^2 + ||x||^2 - 2 q.x, each at most 2 * 256 * 127^2) stays
   below 2^24 and BF16 must equal FP32 bit for bit.
 * FP8: the returned (not re-ranked) distances must equal an independent model
-  of the FP8 distance: rows coded as E4M3(x_j / s_j) with s_j = absmax_j / 448,
-  the query as E4M3(q_j s_j / t) with t = max_j |q_j s_j| / 448 (nearest code,
-  ties to the even code), distance ||q||^2 + ||x_hat||^2 - 2 t sum(codes
-  products). Only the FP32 accumulation order differs, so the tolerance is
-  relative 3e-5 of the terms' magnitude; a misplaced byte moves the distance
-  by far more.
+  of the FP8 distance: c_j the mean of column j, rows coded as
+  E4M3((x_j - c_j) / s_j) with s_j = max_i |x_ij - c_j| / 448, the query as
+  E4M3((q_j - c_j) s_j / t) with t = max_j |(q_j - c_j) s_j| / 448 (nearest
+  code, ties to the even code), distance ||q - c||^2 + ||x_hat||^2 - 2 t
+  sum(codes products). Only the FP32 accumulation order differs, so the
+  tolerance is relative 3e-5 of the terms' magnitude; a misplaced byte or a
+  missing centre moves the distance by far more.
 * Re-ranked FP8 results carry exact FP32 distances."""
 import numpy as np
 import pytest
@@ -55,15 +56,18 @@ def e4m3(x):
 
 def fp8_model(base, q, ids):
     f32 = np.float32
-    absmax = np.abs(base).max(0).astype(f32)
+    c = (base.astype(np.float64).sum(0) / len(base)).astype(f32)       # column means
+    xb = (base - c).astype(f32)
+    absmax = np.abs(xb).max(0).astype(f32)
     s = np.where(absmax > 0, absmax / f32(448.0), f32(1.0)).astype(f32)
-    xc = e4m3(np.clip((base / s).astype(f32), -448, 448))            # row codes
+    xc = e4m3(np.clip((xb / s).astype(f32), -448, 448))              # row codes
     xn = ((xc * s.astype(np.float64)) ** 2).sum(1)                      # ||x_hat||^2
-    qs = (q * s).astype(f32)
+    qq = (q - c).astype(f32)
+    qs = (qq * s).astype(f32)
     mx = np.abs(qs).max(1)
     t = np.where(mx > 0, mx / f32(448.0), f32(1.0)).astype(f32)
     qc = e4m3(np.clip((qs / t[:, None]).astype(f32), -448, 448))       # query codes
-    qn = (q.astype(np.float64) ** 2).sum(1)
+    qn = (qq.astype(np.float64) ** 2).sum(1)
     dot = np.einsum("qd,qkd->qk", qc, xc[ids]) * t[:, None]
     mag = qn[:, None] + xn[ids] + 2 * np.abs(dot)
     return qn[:, None] + xn[ids] - 2 * dot, mag
