@@ -137,6 +137,32 @@ def test_index_file_round_trip(vmn, gpu, tmp_path):
     assert np.array_equal(back.lowp_rows("fp8"), idx.lowp_rows("fp8"))
 
 
+def test_version1_index_file_loads_with_zero_centre(vmn, gpu, tmp_path):
+    """A version-1 index file (FP8 scales, no centre: its rows were coded about
+    0) loads with a zero centre: the decoded FP8 rows are e4m3(x / s) * s with
+    the file's scales. The v1 file is the v2 file with the version field set
+    to 1 and the trailing centre (DP floats) removed."""
+    v = vmn.datasets.integer_mixture(2000, 128, k=16, seed=5, stream=0)
+    idx = vmn.Vamana(v, mode="fresh", R=24, L_build=48, alphas=(1.2,), low_precision="fp8")
+    p2 = tmp_path / "v2.vmn"
+    idx.save(str(p2))
+    raw = bytearray(p2.read_bytes())
+    assert raw[:8] == b"VMNIDX01" and int.from_bytes(raw[8:12], "little") == 2
+    dp = int.from_bytes(raw[12:16], "little")
+    scales = np.frombuffer(bytes(raw[-8 * dp:-4 * dp]), dtype=np.float32)
+    centre = np.frombuffer(bytes(raw[-4 * dp:]), dtype=np.float32)
+    assert np.allclose(centre[:128], v.astype(np.float64).mean(0), atol=1e-3)
+    raw[8:12] = (1).to_bytes(4, "little")
+    p1 = tmp_path / "v1.vmn"
+    p1.write_bytes(bytes(raw[:-4 * dp]))
+    back = vmn.Vamana.load(str(p1))
+    from test_gpu_dims import e4m3
+    want = (e4m3(np.clip((v / scales[:128]).astype(np.float32), -448, 448)) * scales[:128]).astype(np.float32)
+    assert np.allclose(back.lowp_rows("fp8"), want, rtol=0, atol=1e-4 * scales.max())
+    ids, _ = back.searchPoints(v[:32], k=10, L=40, prec="fp8", rerank=20)
+    assert (ids[:, 0] == np.arange(32)).all()
+
+
 def test_bang_index_is_the_reference_file(vmn, orc, sift, gpu, tmp_path, digests):
     base = sift["base"].astype(np.float32)
     adj = np.load(os.path.join(GOLD, "diskann_graph.npz"))["adj"]
