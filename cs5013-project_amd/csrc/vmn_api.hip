@@ -133,6 +133,10 @@ int refresh_lowp(vmn_index* idx, int64_t from, int64_t to) {
 
 int update_exact(vmn_index* idx, int64_t from, int64_t to) {
     if (from == 0) idx->exact_bf16 = true;
+    return check_exact(idx, from, to);
+}
+
+int check_exact(vmn_index* idx, int64_t from, int64_t to) {
     if (!idx->exact_bf16 || to <= from) return VMN_OK;
     HIPCHK(idx->exact_flag.reserve(64));
     HIPCHK(vmn::launch_fill_i32(idx->exact_flag.as<int32_t>(), 1, 1, idx->stream));

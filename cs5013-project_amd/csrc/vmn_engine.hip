@@ -214,13 +214,14 @@ int vmn_engine_insert(vmn_index* idx, const float* vecs, int64_t n, int32_t link
     rc = mark(idx, reused, 0);
     if (rc) return rc;
     idx->n = c0;
-    // the new rows are not checked for exact BF16 traversal: the build and
-    // FreshVamana insert searches read FP32 rows from here on
-    idx->exact_bf16 = false;
+    // the traversal stays on exact BF16 rows only if the new rows qualify too
+    // (rows [lo, hi] hold them; the others there were checked before)
+    rc = check_exact(idx, lo, (int64_t)hi + 1);
+    if (rc) return rc;
     rc = refresh_lowp(idx, lo, (int64_t)hi + 1);
     if (rc) return rc;
     rc = run_search(idx, idx->eng_q.as<float>(), n, link_L, 1, idx->eng_best.as<int32_t>(),
-                    nullptr, false, VMN_MODE_PARTH, false);
+                    nullptr, false, VMN_MODE_PARTH, false, 0, build_prec(idx));
     if (rc) return rc;
     hipLaunchKernelGGL(engine_link_kernel, dim3(1), dim3(64), 0, idx->stream,
                        idx->adj.as<int32_t>(), W, idx->eng_ids.as<int32_t>(),
@@ -289,7 +290,7 @@ int vmn_engine_run(vmn_index* idx, const int32_t* types, const int32_t* ids, con
             HIPCHK(idx->eng_found.reserve((size_t)nq * K * sizeof(int32_t)));
             HIPCHK(hipEventRecord(ev0, idx->stream));
             rc = run_search(idx, q, nq, L, K, idx->eng_found.as<int32_t>(), nullptr, false,
-                            VMN_MODE_PARTH, false);
+                            VMN_MODE_PARTH, false, 0, build_prec(idx));
             if (rc) return rc;
             HIPCHK(hipEventRecord(ev1, idx->stream));
             const int64_t n = idx->n;

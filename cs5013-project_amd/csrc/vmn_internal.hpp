@@ -131,6 +131,11 @@ int refresh_lowp(vmn_index* idx, int64_t from, int64_t to);
 // Keeps idx->exact_bf16 current after rows [from, to) were written (from == 0
 // starts over).
 int update_exact(vmn_index* idx, int64_t from, int64_t to);
+// Clears idx->exact_bf16 unless rows [from, to) qualify too (never sets it).
+int check_exact(vmn_index* idx, int64_t from, int64_t to);
+// Traversal precision of the build, insert and engine searches: BF16 rows
+// while every row makes it exact, else FP32.
+int build_prec(const vmn_index* idx);
 // Host copy of an adjacency given by a caller ([n x row_width], -1 padded):
 // each row keeps its entries up to the first -1, in order, without repeated
 // ids (and without i itself when drop_self), -1 padded to width R.

@@ -83,6 +83,8 @@ def test_gpu_engine_integer_workload_bit_exact(vmn, orc, sift, toy2d, gpu, graph
     assert np.array_equal(idx.vectors, V[:c0])
     assert np.array_equal(idx.deleted.astype(np.uint8), dl[:c0])
     assert abs(r["recall"] - _recall(of, ogt)) < 1e-9
+    # the inserted rows are integers too: the searches stayed on exact BF16 rows
+    assert idx.build_precision == "bf16"
 
 
 def test_gpu_engine_direct_calls_match_replay(vmn, orc, sift, toy2d, gpu):
