@@ -115,3 +115,22 @@ def test_exact_bf16_build_at_width(vmn, orc, gpu, d, mode):
     assert idx.medoid == med
     assert np.array_equal(idx.adjacency, oadj)
     idx.close()
+
+
+@pytest.mark.parametrize("d", [32, 96, 200])
+def test_exact_bf16_consolidation_at_width(vmn, orc, gpu, d):
+    """Delete consolidation on exact BF16 rows (occlusion checks as 16-row
+    MFMA Gram masks, then the MFMA prune from the kept members on) equals the
+    oracle's FP32 consolidation at other padded widths than 128."""
+    base = half_scale(vmn, 1500, d, 0)
+    idx = vmn.Vamana(base, mode="fresh", R=16, L_build=40, alphas=(1.0, 1.2))
+    adj, med = orc.fresh_build(base, R=16, L=40, alphas=(1.0, 1.2))
+    assert idx.build_precision == "bf16" and np.array_equal(idx.adjacency, adj)
+    kill = np.random.default_rng(d).choice(1500, 300, replace=False).astype(np.int32)
+    dele = np.zeros(1500, np.uint8)
+    dele[kill] = 1
+    idx.deleteIds(kill)
+    idx.consolidate()
+    orc.consolidate(base, adj, dele, alpha=1.2)
+    assert np.array_equal(idx.adjacency, adj)
+    idx.close()
