@@ -77,7 +77,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded "
-              "design is described in DESIGN.md and is not run here.", file=sys.stderr)
+              "form (queries split over replicated indexes, or per-shard graphs merged by one "
+              "all-gather of k pairs per query) is not run here.", file=sys.stderr)
         sys.exit(2)
     import torch
     vmn = importlib.import_module("cs5013-project_amd")
