@@ -146,35 +146,76 @@ hipError_t launch_random_init(int32_t* adj, int64_t n, int R, int W, uint64_t se
 // ---------------------------------------------------------------------------
 // Medoid: parth/src/build_graph.cu:326-343. The reference sums the centroid on
 // the host node by node in float; float addition is not associative, so the
-// same sequential order is kept here: one lane per dimension walks the rows in
-// order (loads are issued UNR rows ahead). Then every node's distance to the
-// centroid (separately rounded multiply and add, index order) and the lowest
-// id with the smallest distance.
+// same sequential order is kept here: lane c of wave 0 adds column c of every
+// row in row order. The chain is fed from LDS. A block owns CCOLS columns;
+// waves 1-3 stage tiles of CROWS rows (double-buffered), issuing all of a
+// tile's loads before storing any, so about a tile of rows is in flight per
+// column. Tombstoned rows and rows past n are staged as +0.0: a sum that
+// starts at +0.0 never becomes -0.0, and adding +0.0 to it is exact, so this
+// equals skipping them. Then every node's distance to the centroid
+// (separately rounded multiply and add, index order) and the lowest id with
+// the smallest distance.
 // ---------------------------------------------------------------------------
-__global__ void centroid_kernel(const float* __restrict__ vecs, int64_t n, int D, int DP,
-                                const uint32_t* del_bits, float* centroid) {
+constexpr int CCOLS = 16;   // columns per block (one 64-byte segment of a row)
+constexpr int CROWS = 1024; // rows per staged tile
+constexpr int CSTEP = 48;   // rows one staging round covers (3 waves x 16 rows)
+constexpr int CLOADS = (CROWS + CSTEP - 1) / CSTEP;
+
+__global__ __launch_bounds__(256) void centroid_kernel(const float* __restrict__ vecs, int64_t n,
+                                                       int D, int DP, const uint32_t* del_bits,
+                                                       float* centroid) {
 #pragma clang fp contract(off)
-    const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= D) return;
-    constexpr int UNR = 32;
-    float acc = 0.0f;
-    int64_t i = 0, live = 0;
-    if (!del_bits) {
-        for (; i + UNR <= n; i += UNR) {
-            float v[UNR];
+    __shared__ __align__(16) float tile[2][CROWS][CCOLS];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int c0 = blockIdx.x * CCOLS;
+    const int64_t ntiles = (n + CROWS - 1) / CROWS;
+    // staging lane (waves 1-3): row (lane / 4) of its wave's 16, columns
+    // c0 + 4 * (lane % 4) .. + 3 (one float4; DP is a multiple of 32, so the
+    // columns past D that the float4 reads are zero padding)
+    const int sr = (w - 1) * 16 + (l >> 2), sc = 4 * (l & 3);
+    auto stage = [&](int b, int64_t tt) {
+        float4 v[CLOADS];
 #pragma unroll
-            for (int u = 0; u < UNR; ++u) v[u] = vecs[(i + u) * DP + d];
-#pragma unroll
-            for (int u = 0; u < UNR; ++u) acc = acc + v[u];
+        for (int k = 0; k < CLOADS; ++k) {
+            const int r = sr + CSTEP * k;
+            const int64_t i = tt * CROWS + r;
+            v[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (r < CROWS && i < n && !(del_bits && bit_test(del_bits, (uint32_t)i)))
+                v[k] = *reinterpret_cast<const float4*>(vecs + i * DP + c0 + sc);
         }
-        live = i;
+#pragma unroll
+        for (int k = 0; k < CLOADS; ++k) {
+            const int r = sr + CSTEP * k;
+            if (r < CROWS) *reinterpret_cast<float4*>(&tile[b][r][sc]) = v[k];
+        }
+    };
+    if (w > 0) stage(0, 0);
+    __syncthreads();
+    float acc = 0.0f;
+    int64_t live = 0;
+    for (int64_t tt = 0; tt < ntiles; ++tt) {
+        const int b = (int)(tt & 1);
+        if (w > 0) {
+            if (tt + 1 < ntiles) stage(b ^ 1, tt + 1);
+        } else {
+            if (l < CCOLS) {
+#pragma unroll 32
+                for (int r = 0; r < CROWS; ++r) acc = acc + tile[b][r][l];
+            }
+            // live rows of this tile: lane l looks at rows l, l + 64, ...
+            int cnt = 0;
+            for (int r = l; r < CROWS; r += 64) {
+                const int64_t i = tt * CROWS + r;
+                cnt += i < n && !(del_bits && bit_test(del_bits, (uint32_t)i));
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+            live += cnt;
+        }
+        __syncthreads();
     }
-    for (; i < n; ++i) {
-        if (del_bits && bit_test(del_bits, (uint32_t)i)) continue;
-        acc = acc + vecs[i * DP + d];
-        ++live;
-    }
-    centroid[d] = live ? acc / (float)live : 0.0f;
+    const int d = c0 + l;
+    if (w == 0 && l < CCOLS && d < D) centroid[d] = live ? acc / (float)live : 0.0f;
 }
 
 __global__ void centroid_dist_kernel(const float* __restrict__ vecs, int64_t n, int D, int DP,
@@ -203,8 +244,8 @@ __global__ void centroid_dist_kernel(const float* __restrict__ vecs, int64_t n, 
 
 hipError_t launch_medoid(const float* vecs, int64_t n, int D, int DP, const uint32_t* del_bits,
                          float* centroid, unsigned long long* best, hipStream_t s) {
-    hipLaunchKernelGGL(centroid_kernel, dim3((D + 63) / 64), dim3(64), 0, s, vecs, n, D, DP,
-                       del_bits, centroid);
+    hipLaunchKernelGGL(centroid_kernel, dim3((D + CCOLS - 1) / CCOLS), dim3(256), 0, s, vecs, n,
+                       D, DP, del_bits, centroid);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = launch_fill_u64(best, 1, ~0ull, s);
