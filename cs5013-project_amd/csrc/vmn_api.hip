@@ -262,6 +262,47 @@ int build_parth(vmn_index* idx, bool random_init) {
     return VMN_OK;
 }
 
+// A FRESH prune over `nitems` items (rev: the reverse prune) in two tiers
+// when the full candidate list is longer than 256: a 256-entry launch (twice
+// the resident waves) lists the items whose candidates could exceed it, and a
+// full-list launch takes those. `a` is the caller's, by value.
+int tiered_prune(vmn_index* idx, vmn::PruneArgs a, int64_t nitems, bool rev) {
+    if (nitems <= 0) return VMN_OK;
+    auto launch = [&](const vmn::PruneArgs& x, int64_t n) {
+        return rev ? vmn::launch_fresh_rev_prune(x, prune_grid(idx, n), idx->stream)
+                   : vmn::launch_fresh_out_prune(x, prune_grid(idx, n), idx->stream);
+    };
+    a.n = nitems;
+    a.items = nullptr;
+    a.overflow = nullptr;
+    a.overflow_n = nullptr;
+    const int full = rev ? a.M : a.P;
+    int rc = zero_counter(idx);
+    if (rc) return rc;
+    if (full <= 256 || a.W >= 256) {
+        HIPCHK(launch(a, nitems));
+        return VMN_OK;
+    }
+    HIPCHK(idx->prune_ovf.reserve((size_t)nitems * sizeof(int64_t)));
+    unsigned long long* cnts = idx->counter.as<unsigned long long>();
+    HIPCHK(vmn::launch_fill_u64(cnts, 2, 0ull, idx->stream));
+    a.overflow = idx->prune_ovf.as<int64_t>();
+    a.overflow_n = cnts + 1;
+    HIPCHK(launch(a, nitems));
+    unsigned long long novf = 0;
+    HIPCHK(hipMemcpyAsync(&novf, cnts + 1, sizeof novf, hipMemcpyDeviceToHost, idx->stream));
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    if (!novf) return VMN_OK;
+    rc = zero_counter(idx);
+    if (rc) return rc;
+    a.overflow = nullptr;
+    a.overflow_n = nullptr;
+    a.items = idx->prune_ovf.as<int64_t>();
+    a.n = (int64_t)novf;
+    HIPCHK(launch(a, (int64_t)novf));
+    return VMN_OK;
+}
+
 // One FRESH pass over nodes [base, base + cnt): search, out-prune emitting
 // reverse pairs, group pairs by destination, reverse prune.
 int fresh_pass(vmn_index* idx, int64_t base, int64_t cnt, float alpha) {
@@ -294,7 +335,8 @@ int fresh_pass(vmn_index* idx, int64_t base, int64_t cnt, float alpha) {
     a.del_bits = idx->n_deleted ? idx->del_bits.as<uint32_t>() : nullptr;
     a.M = idx->p.M;
     a.counter = idx->counter.as<unsigned long long>();
-    HIPCHK(vmn::launch_fresh_out_prune(a, prune_grid(idx, cnt), idx->stream));
+    rc = tiered_prune(idx, a, cnt, false);
+    if (rc) return rc;
 
     const int64_t npairs = cnt * R;
     unsigned long long nlist = 0;
@@ -344,12 +386,10 @@ int fresh_pass(vmn_index* idx, int64_t base, int64_t cnt, float alpha) {
     HIPCHK(hipStreamSynchronize(idx->stream));
     rc = zero_counter(idx);
     if (rc) return rc;
-    a.n = (int64_t)nlist;
     a.node_list = idx->rlist.as<int64_t>();
     a.rev_src = idx->rsrc.as<int32_t>();
     a.rev_d = idx->rd.as<float>();
-    HIPCHK(vmn::launch_fresh_rev_prune(a, prune_grid(idx, (int64_t)nlist), idx->stream));
-    return VMN_OK;
+    return tiered_prune(idx, a, (int64_t)nlist, true);
 }
 
 int build_fresh(vmn_index* idx, bool random_init) {

@@ -345,10 +345,17 @@ __global__ __launch_bounds__(64) void fresh_out_prune_kernel(PruneArgs a) {
     const int W = a.W, P = a.P;
     const PruneRows rows{a.vecs, a.bf16_rows, a.bf16_norms};
     for (;;) {
-        const int64_t i = next_item(a.counter, &s_q);
-        if (i >= a.n) return;
+        const int64_t it = next_item(a.counter, &s_q);
+        if (it >= a.n) return;
+        const int64_t i = a.items ? a.items[it] : it;
         const int64_t node = a.node_base + i;
         const int vc = a.vis_cnt[i];
+        // at most vc + W candidates: beyond this list's capacity, leave the
+        // node to the full-capacity launch
+        if (a.overflow && vc + W > CCAP) {
+            if (l == 0) a.overflow[atomicAdd(a.overflow_n, 1ull)] = i;
+            continue;
+        }
         int cur = 0, cnt = 0;
         for (int b = 0; b < vc; b += 64) {
             const int j = b + l;
@@ -423,11 +430,17 @@ __global__ __launch_bounds__(64) void fresh_rev_prune_kernel(PruneArgs a) {
     const int W = a.W, M = a.M;
     const PruneRows rows{a.vecs, a.bf16_rows, a.bf16_norms};
     for (;;) {
-        const int64_t i = next_item(a.counter, &s_q);
-        if (i >= a.n) return;
+        const int64_t it = next_item(a.counter, &s_q);
+        if (it >= a.n) return;
+        const int64_t i = a.items ? a.items[it] : it;
         const int64_t node = a.node_list[i];
         const int64_t o0 = a.list_off ? a.list_off[i] : a.rev_off[node];
         const int64_t o1 = a.list_off ? a.list_off[i + 1] : a.rev_off[node + 1];
+        // at most (o1 - o0) + W candidates: see the out-prune
+        if (a.overflow && (o1 - o0) + W > CCAP) {
+            if (l == 0) a.overflow[atomicAdd(a.overflow_n, 1ull)] = i;
+            continue;
+        }
         int cur = 0, cnt = 0;
         for (int64_t b = o0; b < o1; b += 64) {
             const int64_t j = b + l;
@@ -497,8 +510,9 @@ hipError_t launch_parth_prune(const PruneArgs& a, int grid, hipStream_t s) {
 }
 
 hipError_t launch_fresh_out_prune(const PruneArgs& a, int grid, hipStream_t s) {
-    // list capacity must hold min(P, visited + W) entries
-    const int need = a.P;
+    // list capacity must hold min(P, visited + W) entries (first tier: 256,
+    // the larger ones are listed)
+    const int need = a.overflow ? 256 : a.P;
     if (need <= 256) {
         VMN_PRUNE_LAUNCH(fresh_out_prune_kernel, 256);
     } else if (need <= 640) {
@@ -510,7 +524,7 @@ hipError_t launch_fresh_out_prune(const PruneArgs& a, int grid, hipStream_t s) {
 }
 
 hipError_t launch_fresh_rev_prune(const PruneArgs& a, int grid, hipStream_t s) {
-    const int need = a.M;
+    const int need = a.overflow ? 256 : a.M;
     if (need <= 256) {
         VMN_PRUNE_LAUNCH(fresh_rev_prune_kernel, 256);
     } else if (need <= 512) {
