@@ -67,3 +67,30 @@ def test_gpu_insert_switches_traversal_once(vmn, orc, gpu):
     gi, gd = idx.searchPoints(q, k=10, L=48)
     oi, od, _, _ = orc.fresh_search(v, full, q, med, L=48)
     assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+
+
+def test_gpu_prune_tiers_match_oracle(vmn, orc, gpu):
+    """The FRESH prunes run in two tiers: a 256-entry candidate list first,
+    then the full list for the nodes listed as not fitting. L_build = 300
+    makes every visited set overflow the first tier, and a hub point (the
+    centre of the data, kept by every other node) gets ~1,500 reverse
+    candidates, so both prunes take the second tier; the graph must still be
+    the oracle's."""
+    rng = np.random.default_rng(3)
+    c = rng.integers(60, 120, 128)
+    v = np.clip(c + rng.integers(-6, 7, (1500, 128)), 0, 255).astype(np.float32)
+    v[0] = c
+    idx = vmn.Vamana(v, mode="fresh", R=24, L_build=300, alphas=(1.0, 1.2))
+    adj, med = orc.fresh_build(v, R=24, L=300, alphas=(1.0, 1.2))
+    assert idx.medoid == med
+    assert (adj == 0).sum() > 256  # the hub's reverse candidates overflow the first tier
+    assert np.array_equal(idx.adjacency, adj)
+    # the same through an insert batch (fresh_pass over new rows only)
+    w = np.clip(c + rng.integers(-6, 7, (300, 128)), 0, 255).astype(np.float32)
+    full = np.full((1800, 24), -1, np.int32)
+    full[:1500] = adj
+    vv = np.concatenate([v, w])
+    idx2 = vmn.Vamana(v, mode="fresh", R=24, L_build=300, alphas=(1.0, 1.2), capacity=1800)
+    idx2.insertPoints(w)
+    orc.fresh_insert(vv, 1500, full, med, L=300, alpha=1.2)
+    assert np.array_equal(idx2.adjacency, full)
