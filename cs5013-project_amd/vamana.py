@@ -194,12 +194,22 @@ def _is_torch(x):
     return type(x).__module__.startswith("torch")
 
 
+def _torch_ready(x):
+    """The library reads device tensors on its own streams: wait for the work
+    torch has queued on its current stream (e.g. the kernel that produced the
+    tensor) before handing the pointer over."""
+    if _is_torch(x) and x.is_cuda:
+        import torch
+        torch.cuda.current_stream(x.device).synchronize()
+
+
 def _arg(x, dtype):
     """(pointer, mem kind, keepalive) for a numpy array or a torch tensor."""
     if _is_torch(x):
         import torch
         tdt = {np.float32: torch.float32, np.int32: torch.int32}[dtype]
         assert x.dtype == tdt and x.is_contiguous(), "tensor must be contiguous " + str(tdt)
+        _torch_ready(x)
         mem = MEM_DEVICE if x.is_cuda else MEM_HOST
         return ctypes.c_void_p(x.data_ptr()), mem, x
     a = np.ascontiguousarray(x, dtype=dtype)
@@ -690,6 +700,7 @@ class SearchContext:
         return lib().vmn_ctx_stream(self._h)
 
     def _args(self, q, k, L, prec, rerank, ids, dists):
+        _torch_ready(q)
         pc = PREC[prec] if isinstance(prec, str) else prec
         return (self._h, ctypes.c_void_p(q.data_ptr()), int(q.shape[0]), k, L, pc, rerank,
                 ctypes.c_void_p(ids.data_ptr()), ctypes.c_void_p(dists.data_ptr()))

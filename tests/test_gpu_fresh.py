@@ -73,3 +73,37 @@ def test_gpu_search_vectors(vmn, sift, gpu):
     assert v.shape == (100, 10, 128) and np.array_equal(v, base[ids])
     vd = idx.searchPoints2(torch.from_numpy(q).to(gpu), k=10, L=40)
     assert np.array_equal(vd.cpu().numpy(), v)
+
+
+def _np(x):
+    return x.cpu().numpy() if hasattr(x, "cpu") else np.asarray(x)
+
+
+def test_gpu_device_inputs_wait_for_pending_torch_work(vmn, gpu):
+    """The library reads device tensors on its own streams. A tensor made by
+    torch work still queued on torch's stream (here behind ~10 ms of
+    matrix products) must be read only once that work is done: search and
+    ground truth on such inputs equal the host-array results."""
+    import torch
+    base = vmn.datasets.integer_mixture(3000, 128, k=16, seed=8, stream=0)
+    q = vmn.datasets.integer_mixture(64, 128, k=16, seed=8, stream=1)
+    idx = vmn.Vamana(base, mode="fresh", R=32, L_build=64, alphas=(1.2,))
+    want_i, want_d = idx.searchPoints(q, k=10, L=40)
+    gt_i, gt_d = vmn.groundtruth(base, q, 10)
+    src_q = torch.from_numpy(q).cuda()
+    src_b = torch.from_numpy(base).cuda()
+    torch.cuda.synchronize()
+    for _ in range(2):
+        m = torch.ones((4096, 4096), device="cuda") / 4096.0
+        for _ in range(8):
+            m = m @ m
+        qt = src_q.index_select(0, torch.arange(64, device="cuda")) + m[0, 0] * 0.0
+        got_i, got_d = idx.searchPoints(qt, k=10, L=40)
+        assert np.array_equal(_np(got_i), want_i) and np.array_equal(_np(got_d), want_d)
+        m = torch.ones((4096, 4096), device="cuda") / 4096.0
+        for _ in range(8):
+            m = m @ m
+        bt = src_b.index_select(0, torch.arange(3000, device="cuda")) + m[0, 0] * 0.0
+        ti, td = vmn.groundtruth(bt, src_q, 10)
+        assert np.array_equal(_np(ti), gt_i) and np.array_equal(_np(td), gt_d)
+    idx.close()
