@@ -1120,7 +1120,29 @@ int vmn_consolidate(vmn_index* idx) {
         a.list = idx->rlist.as<int64_t>();
         a.nlist = (int64_t)nlist;
         a.counter = idx->counter.as<unsigned long long>();
-        HIPCHK(vmn::launch_consolidate(a, prune_grid(idx, (int64_t)nlist), idx->stream));
+        if (R >= 256) {
+            HIPCHK(vmn::launch_consolidate(a, prune_grid(idx, (int64_t)nlist), idx->stream));
+        } else {
+            // two tiers: pools of at most 256 first, the listed rest with the full pool
+            HIPCHK(idx->prune_ovf.reserve((size_t)nlist * sizeof(int64_t)));
+            unsigned long long* cnts = idx->counter.as<unsigned long long>();
+            HIPCHK(vmn::launch_fill_u64(cnts, 2, 0ull, idx->stream));
+            a.overflow = idx->prune_ovf.as<int64_t>();
+            a.overflow_n = cnts + 1;
+            HIPCHK(vmn::launch_consolidate(a, prune_grid(idx, (int64_t)nlist), idx->stream));
+            unsigned long long novf = 0;
+            HIPCHK(hipMemcpyAsync(&novf, cnts + 1, sizeof novf, hipMemcpyDeviceToHost, idx->stream));
+            HIPCHK(hipStreamSynchronize(idx->stream));
+            if (novf) {
+                rc = zero_counter(idx);
+                if (rc) return rc;
+                a.overflow = nullptr;
+                a.overflow_n = nullptr;
+                a.items = idx->prune_ovf.as<int64_t>();
+                a.nlist = (int64_t)novf;
+                HIPCHK(vmn::launch_consolidate(a, prune_grid(idx, (int64_t)novf), idx->stream));
+            }
+        }
     }
     HIPCHK(vmn::launch_clear_deleted_rows(idx->adj.as<int32_t>(), N, R, idx->del_bits.as<uint32_t>(),
                                           idx->stream));

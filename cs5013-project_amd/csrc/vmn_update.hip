@@ -103,8 +103,9 @@ __global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
     const int l = threadIdx.x, sub = l & 7;
     const int W = a.W;
     for (;;) {
-        const int64_t i = next_item(a.counter, &s_q);
-        if (i >= a.nlist) return;
+        const int64_t it = next_item(a.counter, &s_q);
+        if (it >= a.nlist) return;
+        const int64_t i = a.items ? a.items[it] : it;
         const int64_t p = a.list[i];
         const PruneRows rows{a.vecs, a.bf16_rows, a.bf16_norms};
         PruneQ<DP, BF> q;
@@ -114,6 +115,11 @@ __global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
         const uint64_t dmask = ballot(own_dead);
         const uint64_t kmask = ballot(own >= 0 && !own_dead);
         const int nK = __popcll((unsigned long long)kmask);
+        // the pool takes at most W entries per tombstoned neighbour
+        if (a.overflow && __popcll((unsigned long long)dmask) * W > CCAP) {
+            if (l == 0) a.overflow[atomicAdd(a.overflow_n, 1ull)] = i;
+            continue;
+        }
         if (own >= 0 && !own_dead) k_id[__popcll((unsigned long long)(kmask & lanemask_lt()))] = own;
         __syncthreads();
         int cur = 0, cnt = 0;
@@ -225,11 +231,15 @@ __global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
 hipError_t launch_consolidate(const ConsArgs& a, int grid, hipStream_t s) {
     if (a.M > 512) return hipErrorInvalidValue;
     switch (a.DP) {
-#define VMN_CONS_CASE(DPV)                                                                   \
-    case DPV:                                                                                \
-        if (a.bf16_rows)                                                                     \
+#define VMN_CONS_CASE(DPV)                                                                        \
+    case DPV:                                                                                     \
+        if (a.overflow && a.bf16_rows)                                                            \
+            hipLaunchKernelGGL((consolidate_kernel<DPV, 256, true>), dim3(grid), dim3(64), 0, s, a); \
+        else if (a.overflow)                                                                      \
+            hipLaunchKernelGGL((consolidate_kernel<DPV, 256, false>), dim3(grid), dim3(64), 0, s, a); \
+        else if (a.bf16_rows)                                                                     \
             hipLaunchKernelGGL((consolidate_kernel<DPV, 512, true>), dim3(grid), dim3(64), 0, s, a); \
-        else                                                                                 \
+        else                                                                                      \
             hipLaunchKernelGGL((consolidate_kernel<DPV, 512, false>), dim3(grid), dim3(64), 0, s, a); \
         break;
         VMN_CONS_CASE(32)

@@ -16,6 +16,13 @@ struct ConsArgs {
     const int64_t* list;   // nodes to re-prune
     int64_t nlist;
     unsigned long long* counter;
+    // two tiers, as the FRESH prunes: with `overflow` set the launch uses a
+    // 256-entry pool and lists the list positions whose pool could exceed it
+    // (tombstoned neighbours x W > 256); `items` then names the positions a
+    // full-pool launch takes (nlist = their number)
+    int64_t* overflow;
+    unsigned long long* overflow_n;
+    const int64_t* items;
 };
 
 hipError_t launch_tombstone(const int32_t* ids, int64_t m, int64_t n, uint32_t* del_bits,

@@ -175,3 +175,24 @@ def test_gpu_staged_inserts_merge_then_patch(vmn, orc, gpu, tmp_path):
     idx.insert_staged(v[3150:3160])
     assert idx.insertPoints(v[3160:3170]).tolist() == list(range(3160, 3170))
     assert idx.size == 3170 and idx.staged_count == 0
+
+
+def test_gpu_heavy_delete_consolidation_tiers(vmn, orc, gpu):
+    """Consolidation runs pools of at most 256 candidates first and lists rows
+    whose pool could be larger (tombstoned neighbours x R > 256) for a
+    full-pool launch. With 60% of the nodes deleted most rows at R = 32 have
+    more than 8 tombstoned neighbours, so both tiers run; the graph must be
+    the oracle's."""
+    v = _data(vmn, 4000, seed=9)
+    idx = vmn.Vamana(v, mode="fresh", R=32, L_build=64, alphas=(1.0, 1.2))
+    adj, med = orc.fresh_build(v, R=32, L=64, alphas=(1.0, 1.2))
+    kill = np.random.default_rng(9).choice(4000, 2400, replace=False).astype(np.int32)
+    dele = np.zeros(4000, np.uint8)
+    dele[kill] = 1
+    dead_nbrs = (np.where(adj >= 0, dele[np.maximum(adj, 0)], 0)).sum(1)
+    live = dele == 0
+    assert (dead_nbrs[live] * 32 > 256).any() and (dead_nbrs[live] * 32 <= 256).any()
+    idx.deleteIds(kill)
+    idx.consolidate()
+    orc.consolidate(v, adj, dele, alpha=1.2)
+    assert np.array_equal(idx.adjacency, adj)
