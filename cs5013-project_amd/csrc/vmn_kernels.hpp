@@ -88,12 +88,13 @@ struct PruneArgs {
     const float* rev_d;
     int M;                    // cap on merged candidates (reference 500)
     unsigned long long* counter;
-    // FRESH prunes, two tiers: with `overflow` set the launch uses a 256-entry
-    // candidate list (twice the resident waves of the full one) and lists the
-    // items whose candidates could exceed it in overflow[0..*overflow_n); a
-    // second launch with `items` = that list and a.n = its length handles them
-    // with the full list. Each item's prune touches only its own row, so the
-    // split does not change the graph.
+    // FRESH prunes, two tiers (the host uses them for the out-prune): with
+    // `overflow` set the launch uses a 256-entry candidate list (twice the
+    // resident waves of the full one) and lists the items whose candidates
+    // could exceed it in overflow[0..*overflow_n); a second launch with
+    // `items` = that list and a.n = its length handles them with the full
+    // list. Each item's prune touches only its own row, so the split does not
+    // change the graph.
     int64_t* overflow;
     unsigned long long* overflow_n;
     const int64_t* items;     // optional: work item k is item items[k]
