@@ -266,9 +266,10 @@ int build_parth(vmn_index* idx, bool random_init) {
 // runs in two tiers when its full candidate list is longer than 256: a
 // 256-entry launch (twice the resident waves) lists the items whose
 // candidates could exceed it, and a full-list launch takes those. The reverse
-// prune stays one launch: its overflow items are the hubs with hundreds of
+// prune stays one launch: its large items are the hubs with hundreds of
 // reverse candidates, and run on their own they form a tail longer than the
-// time the first tier saves (2 x 28 ms -> 2 x 31.5 ms per 1M build).
+// time a first tier saves (2 x 28 ms -> 2 x 31.5 ms per 1M build). They are
+// handed out first instead, so the light items fill in behind them.
 // `a` is the caller's, by value.
 int tiered_prune(vmn_index* idx, vmn::PruneArgs a, int64_t nitems, bool rev) {
     if (nitems <= 0) return VMN_OK;
@@ -283,7 +284,18 @@ int tiered_prune(vmn_index* idx, vmn::PruneArgs a, int64_t nitems, bool rev) {
     const int full = rev ? a.M : a.P;
     int rc = zero_counter(idx);
     if (rc) return rc;
-    if (rev || full <= 256 || a.W >= 256) {
+    if (rev) {
+        // one launch, hubs first
+        HIPCHK(idx->prune_ovf.reserve((size_t)nitems * sizeof(int64_t)));
+        unsigned long long* cnts = idx->counter.as<unsigned long long>();
+        HIPCHK(vmn::launch_fill_u64(cnts + 1, 2, 0ull, idx->stream));
+        HIPCHK(vmn::launch_heavy_first(a.node_list, a.list_off, a.rev_off, nitems, a.W, 256,
+                                       idx->prune_ovf.as<int64_t>(), cnts + 1, idx->stream));
+        a.items = idx->prune_ovf.as<int64_t>();
+        HIPCHK(launch(a, nitems));
+        return VMN_OK;
+    }
+    if (full <= 256 || a.W >= 256) {
         HIPCHK(launch(a, nitems));
         return VMN_OK;
     }

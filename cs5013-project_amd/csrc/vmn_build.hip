@@ -656,4 +656,28 @@ hipError_t launch_compact_nonzero(const int32_t* counts, int64_t n, int64_t* lis
     return hipGetLastError();
 }
 
+// Work order of the reverse prune: items whose candidates (reverse ones plus
+// the row) exceed `heavy` go first, the rest after them, so the hubs start at
+// the beginning of the launch instead of forming its tail. The order inside
+// each class follows atomic arrival, which does not matter: each item's prune
+// reads and writes only its own row.
+__global__ void heavy_first_kernel(const int64_t* node_list, const int64_t* list_off,
+                                   const int64_t* rev_off, int64_t n, int W, int heavy,
+                                   int64_t* order, unsigned long long* ctr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t node = node_list[i];
+    const int64_t c = list_off ? list_off[i + 1] - list_off[i] : rev_off[node + 1] - rev_off[node];
+    if (c + W > heavy) order[atomicAdd(&ctr[0], 1ull)] = i;
+    else order[n - 1 - (int64_t)atomicAdd(&ctr[1], 1ull)] = i;
+}
+hipError_t launch_heavy_first(const int64_t* node_list, const int64_t* list_off,
+                              const int64_t* rev_off, int64_t n, int W, int heavy, int64_t* order,
+                              unsigned long long* ctr, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(heavy_first_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       node_list, list_off, rev_off, n, W, heavy, order, ctr);
+    return hipGetLastError();
+}
+
 }  // namespace vmn

@@ -131,6 +131,11 @@ hipError_t launch_reverse_group(const int32_t* pair_dst, const float* pair_d, in
                                 int64_t node_base, void* temp, size_t temp_bytes,
                                 int32_t* rev_src, float* rev_d, int64_t* node_list,
                                 int64_t* list_off, unsigned long long* nseg, hipStream_t s);
+// Reverse-prune work order with the items of more than `heavy` candidates
+// first (ctr: two zeroed counters).
+hipError_t launch_heavy_first(const int64_t* node_list, const int64_t* list_off,
+                              const int64_t* rev_off, int64_t n, int W, int heavy, int64_t* order,
+                              unsigned long long* ctr, hipStream_t s);
 hipError_t launch_compact_nonzero(const int32_t* counts, int64_t n, int64_t* list,
                                   unsigned long long* nlist, hipStream_t s);
 
