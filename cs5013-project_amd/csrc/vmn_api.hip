@@ -295,7 +295,9 @@ int tiered_prune(vmn_index* idx, vmn::PruneArgs a, int64_t nitems, bool rev) {
         HIPCHK(launch(a, nitems));
         return VMN_OK;
     }
-    if (full <= 256 || a.W >= 256) {
+    // a batch too small to fill the GPU gains nothing from the first tier's
+    // occupancy and would pay its host round trip (small inserts)
+    if (full <= 256 || a.W >= 256 || nitems < (int64_t)idx->cus * 16) {
         HIPCHK(launch(a, nitems));
         return VMN_OK;
     }

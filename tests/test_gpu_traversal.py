@@ -70,27 +70,25 @@ def test_gpu_insert_switches_traversal_once(vmn, orc, gpu):
 
 
 def test_gpu_prune_tiers_match_oracle(vmn, orc, gpu):
-    """The FRESH out-prune runs in two tiers: a 256-entry candidate list
-    first, then the full list for the nodes listed as not fitting. L_build =
-    300 makes every visited set overflow the first tier. A hub point (the
-    centre of the data, kept by every other node) gets ~1,500 reverse
-    candidates, more than the reverse prune's cap M = 500 keeps. The graph
-    must still be the oracle's."""
+    """The FRESH out-prune of a batch that fills the GPU (at least 16 items
+    per CU) runs in two tiers: a 256-entry candidate list first, then the full
+    list for the nodes listed as not fitting. L_build = 300 makes every
+    visited set overflow the first tier. A hub point (the centre of the data,
+    kept by every other node) gets thousands of reverse candidates, more than
+    the reverse prune's cap M = 500 keeps, and goes first in its work order.
+    The graph must still be the oracle's, in a build and an insert batch."""
     rng = np.random.default_rng(3)
     c = rng.integers(60, 120, 128)
-    v = np.clip(c + rng.integers(-6, 7, (1500, 128)), 0, 255).astype(np.float32)
+    v = np.clip(c + rng.integers(-6, 7, (5000, 128)), 0, 255).astype(np.float32)
     v[0] = c
-    idx = vmn.Vamana(v, mode="fresh", R=24, L_build=300, alphas=(1.0, 1.2))
+    idx = vmn.Vamana(v, mode="fresh", R=24, L_build=300, alphas=(1.0, 1.2), capacity=9500)
     adj, med = orc.fresh_build(v, R=24, L=300, alphas=(1.0, 1.2))
     assert idx.medoid == med
     assert (adj == 0).sum() > 500  # the hub's reverse candidates exceed M
     assert np.array_equal(idx.adjacency, adj)
-    # the same through an insert batch (fresh_pass over new rows only)
-    w = np.clip(c + rng.integers(-6, 7, (300, 128)), 0, 255).astype(np.float32)
-    full = np.full((1800, 24), -1, np.int32)
-    full[:1500] = adj
-    vv = np.concatenate([v, w])
-    idx2 = vmn.Vamana(v, mode="fresh", R=24, L_build=300, alphas=(1.0, 1.2), capacity=1800)
-    idx2.insertPoints(w)
-    orc.fresh_insert(vv, 1500, full, med, L=300, alpha=1.2)
-    assert np.array_equal(idx2.adjacency, full)
+    w = np.clip(c + rng.integers(-6, 7, (4500, 128)), 0, 255).astype(np.float32)
+    full = np.full((9500, 24), -1, np.int32)
+    full[:5000] = adj
+    idx.insertPoints(w)
+    orc.fresh_insert(np.concatenate([v, w]), 5000, full, med, L=300, alpha=1.2)
+    assert np.array_equal(idx.adjacency, full)
