@@ -104,3 +104,17 @@ def test_gpu_edge_cases(vmn, orc, gpu):
     idx5 = vmn.Vamana(v5, mode="parth", R=6, L_build=12)
     a5, m5 = orc.parth_build(v5, R=6, L=12, alphas=(1.2, 1.2))
     assert idx5.medoid == m5 and np.array_equal(idx5.adjacency, a5)
+
+
+@pytest.mark.parametrize("name", ["toy", "vis"])
+def test_gpu_build_matches_reference_2d_graphs(vmn, toy2d, gpu, name):
+    """The reference's 2-D float graphs (parth/data/{toy,vis}_initial_graph.bin,
+    gen_toy_data.py: R = 6, L = 12, alpha 1.2, rows padded to 32): rebuilt on
+    the GPU from their vectors, row for row. Real-valued data, two dimensions:
+    each distance is one FMA chain, the same in the kernel and the reference."""
+    v = toy2d[name + "_vecs"].astype(np.float32)
+    ref = toy2d[name + "_adj"]
+    idx = vmn.Vamana(v, mode="parth", R=6, L_build=12, alphas=(1.2, 1.2))
+    assert idx.build_precision == "fp32"
+    got = idx.adjacency
+    assert np.array_equal(got, ref[:, :6]) and (ref[:, 6:] == -1).all()
