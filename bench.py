@@ -459,7 +459,7 @@ def engine_replay(args, vmn, torch):
         orc.engine_run(base, adj, len(base), ref["max_capacity"], t, i, ov, ref["batch_size"],
                        100, stats=st)
         cpu = {"value": round(n_search / st["search_s"], 1), "unit": "queries/s",
-               "cores": orc.num_threads(), "kind": "port",
+               "cores": orc.num_threads(), "kind": "port", "cpu_model": cpu_model(),
                "sample": f"the oracle's replay of the same {len(t)} operations at beam 100; "
                          f"its {n_search} searches timed (OpenMP over each batch's searches)"}
         log(f"engine cpu baseline {cpu['value']:,.0f} searches/s")
@@ -664,6 +664,18 @@ def measured_traffic(form, n, nq, L, kernel):
     return int(e["traffic_bytes"])
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(idx, base, qs, gt, L, seconds, mode):
     """The CPU oracle's search (same graph, same L) on a bounded query sample:
     the query set is cycled until the sample holds about `seconds` of work."""
@@ -686,6 +698,7 @@ def cpu_baseline(idx, base, qs, gt, L, seconds, mode):
     el = time.time() - t
     log(f"cpu baseline: {ns} queries in {el:.1f} s on {threads} threads")
     return {"value": round(ns / el, 1), "unit": "queries/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "sample": f"{ns} searches ({el:.1f} s) cycling the {qs.shape[0]} queries, FP32 on the "
                       f"same graph at L={L}, OpenMP over queries",
             "recall@10_sample": round(float(np.mean(
