@@ -322,17 +322,28 @@ int tiered_prune(vmn_index* idx, vmn::PruneArgs a, int64_t nitems, bool rev) {
 }
 
 // One FRESH pass over nodes [base, base + cnt): search, out-prune emitting
-// reverse pairs, group pairs by destination, reverse prune.
+// reverse pairs, group pairs by destination, reverse prune. The search and
+// out-prune run in chunks when the visited sets of the whole range would not
+// fit the build's scratch budget: a chunk's rows go to a shadow adjacency, and
+// the shadow replaces the range's rows after the last chunk, so every search
+// of the pass sees the graph from before it (one batch, as the reference's
+// pass) and the result is the one-chunk result.
+constexpr int64_t kVisitedBudget = (int64_t)24 << 30;  // bytes of visited sets per chunk
+
+int64_t pass_chunk(const vmn_index* idx, int64_t cnt) {
+    if (idx->p.build_chunk > 0) return std::min<int64_t>(idx->p.build_chunk, cnt);
+    const int64_t per = (int64_t)idx->p.P * (int64_t)(sizeof(int32_t) + sizeof(float));
+    return std::min<int64_t>(cnt, std::max<int64_t>(1 << 20, kVisitedBudget / per));
+}
+
 int fresh_pass(vmn_index* idx, int64_t base, int64_t cnt, float alpha) {
     const int R = idx->p.R;
     const int64_t N = idx->n;
-    int rc = run_search(idx, idx->vecs.as<float>() + base * idx->DP, cnt, idx->p.L_build, 0,
-                        nullptr, nullptr, true, VMN_MODE_FRESH, true, base, build_prec(idx));
-    if (rc) return rc;
+    const int64_t chunk = pass_chunk(idx, cnt);
+    const bool shadow = chunk < cnt;
     HIPCHK(idx->pair_dst.reserve((size_t)cnt * R * sizeof(int32_t)));
     HIPCHK(idx->pair_d.reserve((size_t)cnt * R * sizeof(float)));
-    rc = zero_counter(idx);
-    if (rc) return rc;
+    if (shadow) HIPCHK(idx->adj_shadow.reserve((size_t)cnt * R * sizeof(int32_t)));
     vmn::PruneArgs a{};
     a.mode = VMN_MODE_FRESH;
     a.DP = idx->DP;
@@ -342,19 +353,37 @@ int fresh_pass(vmn_index* idx, int64_t base, int64_t cnt, float alpha) {
     a.W = R;
     a.R = R;
     a.alpha = alpha;
-    a.n = cnt;
-    a.node_base = base;
-    a.vis_ids = idx->vis_ids.as<int32_t>();
-    a.vis_d = idx->vis_d.as<float>();
-    a.vis_cnt = idx->vis_cnt.as<int32_t>();
     a.P = idx->p.P;
-    a.pair_dst = idx->pair_dst.as<int32_t>();
-    a.pair_d = idx->pair_d.as<float>();
     a.del_bits = idx->n_deleted ? idx->del_bits.as<uint32_t>() : nullptr;
     a.M = idx->p.M;
+    int rc = VMN_OK;
+    for (int64_t c0 = 0; c0 < cnt; c0 += chunk) {
+        const int64_t cs = std::min(chunk, cnt - c0);
+        rc = run_search(idx, idx->vecs.as<float>() + (base + c0) * idx->DP, cs, idx->p.L_build, 0,
+                        nullptr, nullptr, true, VMN_MODE_FRESH, true, base + c0, build_prec(idx));
+        if (rc) return rc;
+        vmn::PruneArgs c = a;
+        // the work counter exists once a search has run (run_search allocates it)
+        c.counter = idx->counter.as<unsigned long long>();
+        c.node_base = base + c0;
+        c.vis_ids = idx->vis_ids.as<int32_t>();
+        c.vis_d = idx->vis_d.as<float>();
+        c.vis_cnt = idx->vis_cnt.as<int32_t>();
+        c.pair_dst = idx->pair_dst.as<int32_t>() + c0 * R;
+        c.pair_d = idx->pair_d.as<float>() + c0 * R;
+        c.adj_out = shadow ? idx->adj_shadow.as<int32_t>() + c0 * R : nullptr;
+        rc = tiered_prune(idx, c, cs, false);
+        if (rc) return rc;
+    }
+    if (shadow)
+        HIPCHK(hipMemcpyAsync(idx->adj.as<int32_t>() + base * R, idx->adj_shadow.p,
+                              (size_t)cnt * R * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                              idx->stream));
+    a.n = cnt;
+    a.node_base = base;
+    a.pair_dst = idx->pair_dst.as<int32_t>();
+    a.pair_d = idx->pair_d.as<float>();
     a.counter = idx->counter.as<unsigned long long>();
-    rc = tiered_prune(idx, a, cnt, false);
-    if (rc) return rc;
 
     const int64_t npairs = cnt * R;
     unsigned long long nlist = 0;
@@ -470,6 +499,7 @@ int check_params(const vmn_params* p) {
     if (p->M < 1 || p->M > 512) return fail(VMN_ERR_ARG, "M %d not in [1, 512]", p->M);
     if (p->mode == VMN_MODE_PARTH && p->L_build + p->R > 576)
         return fail(VMN_ERR_ARG, "L_build + R must be <= 576");
+    if (p->build_chunk < 0) return fail(VMN_ERR_ARG, "build_chunk must be >= 0");
     return VMN_OK;
 }
 

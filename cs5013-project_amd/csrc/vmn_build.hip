@@ -393,7 +393,8 @@ __global__ __launch_bounds__(64) void fresh_out_prune_kernel(PruneArgs a) {
         __syncthreads();
         const int nk = prune_list<DP, false, BF, CCAP>(rows, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
                                                kept, act_id, act_pos, act_d, newrow, newd);
-        for (int j = l; j < W; j += 64) a.adj[node * W + j] = j < nk ? newrow[j] : -1;
+        int32_t* row = a.adj_out ? a.adj_out + i * W : a.adj + node * W;
+        for (int j = l; j < W; j += 64) row[j] = j < nk ? newrow[j] : -1;
         for (int j = l; j < a.R; j += 64) {
             a.pair_dst[i * a.R + j] = j < nk ? newrow[j] : -1;
             a.pair_d[i * a.R + j] = j < nk ? newd[j] : 0.0f;

@@ -82,6 +82,7 @@ struct vmn_index {
     vmn_rt::DevBuf cand_ids, cand_d, vis_ids, vis_d, vis_cnt;
     vmn_rt::DevBuf pair_dst, pair_d, rcount, rfill, roff, rscan_tmp, rsrc, rd, rlist, gtemp;
     vmn_rt::DevBuf prune_ovf;  // items left to the full-capacity prune launch
+    vmn_rt::DevBuf adj_shadow;  // out-pruned rows of a chunked FRESH pass
     vmn_rt::DevBuf centroid, medoid_key;
     int64_t last_nq = 0;
     int64_t n_deleted = 0;  // tombstones set since the build (upper bound)

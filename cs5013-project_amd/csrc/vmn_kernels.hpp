@@ -63,6 +63,7 @@ struct PruneArgs {
     const void* bf16_rows;    // BF16 copies of vecs when every row is exact in BF16, else null
     const float* bf16_norms;
     int32_t* adj;             // rows rewritten in place
+    int32_t* adj_out;         // FRESH out-prune: rows written here instead ([n x W], by item)
     const uint32_t* del_bits; // FRESH out-prune: tombstoned ids are not candidates (or null)
     int W, R;
     float alpha;

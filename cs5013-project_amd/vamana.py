@@ -39,7 +39,8 @@ class Params(ctypes.Structure):
                 ("seed", ctypes.c_uint64), ("mode", ctypes.c_int32),
                 ("visited_cap", ctypes.c_int32), ("max_hops", ctypes.c_int32),
                 ("P", ctypes.c_int32), ("M", ctypes.c_int32), ("capacity", ctypes.c_int64),
-                ("device", ctypes.c_int32), ("low_precision", ctypes.c_int32)]
+                ("device", ctypes.c_int32), ("low_precision", ctypes.c_int32),
+                ("build_chunk", ctypes.c_int64)]
 
 
 class ServerStats(ctypes.Structure):
@@ -228,7 +229,8 @@ class Vamana:
 
     def __init__(self, vectors=None, mode="parth", *, dim=None, R=None, L_build=None,
                  alphas=None, passes=None, seed=None, visited_cap=None, max_hops=None, P=None,
-                 M=None, capacity=0, device=0, low_precision=0, init_adj=None, _build=True):
+                 M=None, capacity=0, device=0, low_precision=0, init_adj=None, build_chunk=None,
+                 _build=True):
         m = {"parth": MODE_PARTH, "fresh": MODE_FRESH}[mode] if isinstance(mode, str) else mode
         p = default_params(m)
         if vectors is not None:
@@ -236,7 +238,8 @@ class Vamana:
         if dim is not None:
             p.dim = dim
         for name, v in (("R", R), ("L_build", L_build), ("passes", passes), ("seed", seed),
-                        ("visited_cap", visited_cap), ("max_hops", max_hops), ("P", P), ("M", M)):
+                        ("visited_cap", visited_cap), ("max_hops", max_hops), ("P", P), ("M", M),
+                        ("build_chunk", build_chunk)):
             if v is not None:
                 setattr(p, name, v)
         if alphas is not None:

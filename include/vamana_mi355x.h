@@ -73,6 +73,10 @@ typedef struct vmn_params {
     int64_t capacity;    /* node capacity for inserts (0: the build size) */
     int32_t device;      /* HIP device ordinal */
     int32_t low_precision; /* VMN_STORE_* bits: BF16 / FP8 row copies kept beside FP32 */
+    int64_t build_chunk; /* FRESH passes: nodes searched and out-pruned per chunk (their rows go
+                            to a shadow adjacency until the pass's last chunk, so every search
+                            still sees the graph from before the pass); 0: as many as 24 GiB
+                            of visited sets hold */
 } vmn_params;
 
 /* Reference defaults for `mode` (parth: R=32, L=64, alpha 1.2 x 2 passes). */

@@ -92,3 +92,25 @@ def test_gpu_prune_tiers_match_oracle(vmn, orc, gpu):
     idx.insertPoints(w)
     orc.fresh_insert(np.concatenate([v, w]), 5000, full, med, L=300, alpha=1.2)
     assert np.array_equal(idx.adjacency, full)
+
+
+def test_gpu_chunked_pass_equals_one_batch(vmn, orc, gpu):
+    """A FRESH pass searched and out-pruned in chunks (build_chunk), with the
+    chunks' rows held in a shadow adjacency until the pass ends, gives the
+    one-batch graph: every search still sees the graph from before the pass.
+    Checked for a two-pass build (4 chunks) and an insert batch (3 chunks)."""
+    v = _mix(vmn, 4000, seed=31)
+    w = _mix(vmn, 2500, seed=31, stream=2)
+    one = vmn.Vamana(v, mode="fresh", R=32, L_build=64, alphas=(1.0, 1.2), capacity=6500)
+    chk = vmn.Vamana(v, mode="fresh", R=32, L_build=64, alphas=(1.0, 1.2), capacity=6500,
+                     build_chunk=1000)
+    oadj, med = orc.fresh_build(v, R=32, L=64, alphas=(1.0, 1.2))
+    assert chk.medoid == one.medoid == med
+    assert np.array_equal(chk.adjacency, oadj) and np.array_equal(one.adjacency, oadj)
+    one.insertPoints(w)
+    chk.insertPoints(w)
+    assert np.array_equal(chk.adjacency, one.adjacency)
+    full = np.full((6500, 32), -1, np.int32)
+    full[:4000] = oadj
+    orc.fresh_insert(np.concatenate([v, w]), 4000, full, med, L=64, alpha=1.2)
+    assert np.array_equal(chk.adjacency, full)
