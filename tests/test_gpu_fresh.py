@@ -48,6 +48,22 @@ def test_gpu_fresh_two_pass_schedule_matches_oracle(vmn, orc, gpu):
     assert np.array_equal(idx.adjacency, oadj)
 
 
+@pytest.mark.parametrize("alphas", [(1.0, 1.2, 1.2), (1.0, 1.0, 1.2, 1.2)])
+def test_gpu_fresh_later_passes_match_oracle(vmn, orc, gpu, alphas):
+    """Passes over an already good graph: two wide components, so the
+    searches find their own node and hubs receive hundreds of reverse pairs
+    (the reverse prune's large lists)."""
+    base = vmn.datasets.integer_mixture(6000, 128, k=2, seed=77, stream=0)
+    idx = vmn.Vamana(base, mode="fresh", R=32, L_build=64, alphas=alphas)
+    oadj, omed = orc.fresh_build(base, R=32, L=64, alphas=alphas)
+    assert idx.medoid == omed
+    indeg = np.bincount(oadj[oadj >= 0], minlength=len(base))
+    assert indeg.max() > 256
+    adj = idx.adjacency
+    same = (adj == oadj).all(axis=1)
+    assert same.all(), f"{(~same).sum()} rows differ, first {np.flatnonzero(~same)[:5]}"
+
+
 def test_gpu_fresh_graph_invariants(vmn, gpu):
     ds = vmn.datasets
     base = ds.integer_mixture(6000, 128, k=32, seed=5, stream=0)
