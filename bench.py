@@ -8,9 +8,10 @@ at the smallest beam width whose recall@10 reaches the target. The K steps are
 K batches issued alternately on two search contexts (streams), each replaying
 its own captured graph over its own 10,000-query set, so the drain of one
 batch overlaps the start of the next; `value` is the queries of the K batches
-over the wall time of the K steps. The same form timed one batch at a time on
-one context (no overlap) is reported as `qps_serial`, and the roofline uses
-that per-launch kernel time.
+over the wall time of the K steps. Each form's launch shape (waves per CU,
+beam-list capacity) is tuned on that same alternating stream of batches. The
+same form timed one batch at a time on one context (no overlap) is reported as
+`qps_serial`, and the roofline uses that per-launch kernel time.
 
 Prints one JSON line (rank 0). Progress goes to stderr.
 """
@@ -166,7 +167,8 @@ def main():
         L = chosen[form][0]["L"]
         prec, rr = forms[form]
         rr = min(rr, L)
-        tune = vmn.tuning.launch_shape(idx, q_t, k=10, L=L, prec=prec, rerank=rr, reps=3)
+        tune = vmn.tuning.launch_shape(idx, q_t, k=10, L=L, prec=prec, rerank=rr, reps=12,
+                                       overlap=True)
         ctx[0].capture(q_t, ids, dists, k=10, L=L, prec=prec, rerank=rr)
         ctx[1].capture(q2_t, ids2, dists2, k=10, L=L, prec=prec, rerank=rr)
         # one batch at a time on one context: per-launch kernel time
@@ -271,7 +273,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(head["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": head["traffic"],
                      "kernel_ms": round(head["kernel_ms"], 4),
-                     "bytes_per_launch": head["bytes_per_launch"]},
+                     "bytes_per_launch": head["bytes_per_launch"], "kernel": head["kernel"],
+                     "launch": head["launch"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -3,9 +3,14 @@
 beam_width(): the smallest beam width L whose recall@k against exact ground
 truth reaches a target (the reference hard-codes L_g = 199,
 vamana/include/constants.cuh:10). launch_shape(): times the batched search
-under each persistent-grid size (waves per CU) and beam-list capacity template
-on one search context, keeps the fastest, and installs it on the index.
+under each persistent-grid size (waves per CU) and beam-list capacity template,
+keeps the fastest, and installs it on the index. It times either one batch at
+a time on one search context, or batches alternating over two contexts, so
+that each batch's drain overlaps the next batch's start (the throughput of a
+stream of batches).
 """
+import time
+
 import numpy as np
 
 from .vamana import SearchContext, recall_at_k
@@ -25,6 +30,23 @@ def _time(ctx, q, ids, d, k, L, prec, rerank, reps):
     return e0.elapsed_time(e1) / reps
 
 
+def _time_overlapped(ctxs, q, outs, k, L, prec, rerank, reps):
+    """Milliseconds per batch over `reps` batches alternating between the
+    contexts, each replaying its captured search (host wall clock between
+    device-wide synchronisations)."""
+    import torch
+    for c, (ids, d) in zip(ctxs, outs):
+        c.capture(q, ids, d, k, L, prec, rerank)
+    for r in range(2):
+        ctxs[r].replay()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for r in range(reps):
+        ctxs[r % 2].replay()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) * 1e3 / reps
+
+
 def beam_width(index, queries, gt_ids, target, widths, k=10, prec="fp32", rerank=0):
     """(L, recall) of the smallest width reaching `target`, else the best seen."""
     best = None
@@ -39,14 +61,18 @@ def beam_width(index, queries, gt_ids, target, widths, k=10, prec="fp32", rerank
 
 
 def launch_shape(index, queries, k=10, L=64, prec="fp32", rerank=0,
-                 waves=(12, 16, 20, 24, 32), caps=(0, 128, 256), reps=3):
+                 waves=(12, 16, 20, 24, 32), caps=(0, 128, 256), reps=3, overlap=False):
     """Fastest (waves_per_cu, list_cap) for this batch; installs it and
-    returns {"waves_per_cu", "list_cap", "ms", "table"}."""
+    returns {"waves_per_cu", "list_cap", "ms", "table"}. overlap=True times
+    batches alternating over two search contexts (ms per batch of the
+    stream) instead of one batch at a time."""
     import torch
-    ctx = SearchContext(index)
+    nctx = 2 if overlap else 1
+    ctxs = [SearchContext(index) for _ in range(nctx)]
     nq = int(queries.shape[0])
-    ids = torch.empty((nq, k), dtype=torch.int32, device=queries.device)
-    d = torch.empty((nq, k), dtype=torch.float32, device=queries.device)
+    outs = [(torch.empty((nq, k), dtype=torch.int32, device=queries.device),
+             torch.empty((nq, k), dtype=torch.float32, device=queries.device))
+            for _ in range(nctx)]
     rr = min(rerank, L)
     table = []
     for cap in caps:
@@ -54,9 +80,13 @@ def launch_shape(index, queries, k=10, L=64, prec="fp32", rerank=0,
             continue
         for w in waves:
             index.set_launch(w, cap)
-            table.append({"waves_per_cu": w, "list_cap": cap,
-                          "ms": round(_time(ctx, queries, ids, d, k, L, prec, rr, reps), 4)})
+            if overlap:
+                ms = _time_overlapped(ctxs, queries, outs, k, L, prec, rr, reps)
+            else:
+                ms = _time(ctxs[0], queries, outs[0][0], outs[0][1], k, L, prec, rr, reps)
+            table.append({"waves_per_cu": w, "list_cap": cap, "ms": round(ms, 4)})
     best = min(table, key=lambda t: t["ms"])
     index.set_launch(best["waves_per_cu"], best["list_cap"])
-    ctx.close()
+    for c in ctxs:
+        c.close()
     return dict(best, table=table)

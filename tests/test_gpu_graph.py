@@ -59,5 +59,12 @@ def test_contexts_concurrent_and_launch_shapes(vmn, gpu):
         assert np.array_equal(x, ref) and np.array_equal(xd, refd)
     t = vmn.tuning.launch_shape(idx, q, k=10, L=64, waves=(16, 32), caps=(0, 128), reps=1)
     assert t["waves_per_cu"] in (16, 32)
+    t = vmn.tuning.launch_shape(idx, q, k=10, L=64, waves=(12, 24), caps=(0, 128), reps=4,
+                                overlap=True)
+    assert (t["waves_per_cu"], t["list_cap"]) in ((12, 0), (24, 0), (12, 128), (24, 128))
+    assert len(t["table"]) == 4
+    # the installed shape gives the same results
+    x, xd = idx.searchPoints(qs, k=10, L=64)
+    assert np.array_equal(x, ref) and np.array_equal(xd, refd)
     for c in ctxs:
         c.close()
