@@ -54,7 +54,10 @@ def parse():
     ap.add_argument("--sweep", default="10,20,40,50,60,70,80,100,120,150,200")
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--rerank", type=int, default=40)
+    ap.add_argument("--rerank", type=int, default=20,
+                    help="FP32 re-rank depth of the BF16 form (2k; BF16 rounding is small)")
+    ap.add_argument("--rerank-fp8", type=int, default=40,
+                    help="FP32 re-rank depth of the FP8 form (deeper: its distance error is larger)")
     ap.add_argument("--workload", default="search",
                     choices=["search", "streaming", "serving", "copy", "engine"],
                     help="search: the headline metric; streaming: 10 rounds of 5%% deletes + "
@@ -123,7 +126,7 @@ def main():
     ids = torch.empty((nq, 10), dtype=torch.int32, device=dev)
     dists = torch.empty((nq, 10), dtype=torch.float32, device=dev)
     forms = {"fp32": ("fp32", 0), "bf16+rerank": ("bf16", args.rerank),
-             "fp8+rerank": ("fp8", args.rerank)}
+             "fp8+rerank": ("fp8", args.rerank_fp8)}
 
     def run(form, L, reps, sync_first=True):
         prec, rr = forms[form]
@@ -209,7 +212,8 @@ def main():
                        "kernel_ms": kern_ms, "bytes_per_launch": int(bytes_launch),
                        "achieved_GBs": bytes_launch / (kern_ms / 1e3) / 1e9,
                        "launch": {"waves_per_cu": tune["waves_per_cu"], "list_cap": tune["list_cap"]},
-                       "kernel": kernel, "traffic": measured_traffic(form, n, nq, L, kernel),
+                       "kernel": kernel, "rerank": rr,
+                       "traffic": measured_traffic(form, n, nq, L, kernel, rr),
                        "hops_per_query": round(hops / nq, 1), "dists_per_query": round(nd / nq, 1)}
         log(f"timed {form:12s} L={L} {timed[form]['qps']:,.0f} QPS (serial {timed[form]['qps_serial']:,.0f})"
             f" recall {rec:.4f}/{rec2:.4f} kernel {kern_ms:.3f} ms {timed[form]['achieved_GBs']:.0f} GB/s"
@@ -647,12 +651,12 @@ def serving(args, vmn, torch, dev):
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(form, n, nq, L, kernel):
+def measured_traffic(form, n, nq, L, kernel, rerank):
     """HBM bytes per launch of the search kernel for `form`, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (each scaled by what the counter
     reads for a 1 GiB copy), when that profile covers this exact workload
-    (same n, nq, precision and L; the beam-list capacity of the instance
-    does not change the traversal); otherwise None."""
+    (same n, nq, precision, L and re-rank depth; the beam-list capacity of the
+    instance does not change the traversal); otherwise None."""
     path = os.path.join(ROOT, "profiles", "search_1m_hbm_traffic.json")
     try:
         with open(path) as f:
@@ -662,7 +666,7 @@ def measured_traffic(form, n, nq, L, kernel):
     e = prof.get("forms", {}).get(form)
     same = lambda a, b: a.split(",")[0] == b.split(",")[0] and a.split(",")[2:] == b.split(",")[2:]
     if not e or prof.get("n") != n or prof.get("nq") != nq or e.get("L") != L \
-            or not same(e.get("kernel", ""), kernel):
+            or e.get("rerank") != rerank or not same(e.get("kernel", ""), kernel):
         return None
     return int(e["traffic_bytes"])
 
