@@ -105,3 +105,27 @@ def test_fp8_rerank_recall_and_exact_distances(vmn, orc, gpu):
     exact = ((base[ri].astype(np.float64) - q[:, None, :]) ** 2).sum(-1).astype(np.float32)
     assert np.array_equal(rd, exact)
     assert (np.diff(rd, axis=1) >= 0).all()
+
+
+def test_lowp_rerank_recall_on_real_valued_data(vmn, orc, gpu):
+    """Unit-normalised real-valued rows, where BF16 rounds: BF16 re-ranking 2k
+    rows and FP8 re-ranking 40 keep recall@10 within 0.005 of plain FP32 (the
+    stated tolerance), and the re-ranked distances are FP32 distances."""
+    ds = vmn.datasets
+    base = ds.unit_mixture(30_000, 96, k=64, sigma=0.1, seed=7, stream=0)
+    q = ds.unit_mixture(500, 96, k=64, sigma=0.1, seed=7, stream=1)
+    idx = vmn.Vamana(base, mode="fresh", R=32, L_build=64, alphas=(1.0, 1.2),
+                     low_precision="both")
+    assert idx.build_precision == "fp32"
+    gt, _ = orc.bruteforce_fp64(base, q, 10)
+    for L in (40, 64):
+        fi, _ = idx.searchPoints(q, k=10, L=L)
+        rf = vmn.recall_at_k(fi, gt)
+        for prec, rr in (("bf16", 20), ("fp8", 40)):
+            ri, rd = idx.searchPoints(q, k=10, L=L, prec=prec, rerank=rr)
+            r = vmn.recall_at_k(ri, gt)
+            assert r >= rf - 0.005, (prec, L, r, rf)
+            exact = ((base[ri].astype(np.float64) - q[:, None, :]) ** 2).sum(-1)
+            # FP32 sums of 96 terms: ~1e-6 rounding; BF16 distances are off by ~1e-3
+            assert np.allclose(rd, exact, rtol=1e-4, atol=1e-6), np.abs(rd - exact).max()
+            assert (np.diff(rd, axis=1) >= 0).all()
