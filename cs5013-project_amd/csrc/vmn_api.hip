@@ -80,6 +80,7 @@ int compute_medoid(vmn_index* idx) {
 }
 
 int alloc_graph(vmn_index* idx, int64_t n) {
+    ++idx->epoch;
     const int64_t cap = std::max<int64_t>(idx->p.capacity, n);
     idx->cap = cap;
     idx->n_deleted = 0;
@@ -152,6 +153,23 @@ int check_exact(vmn_index* idx, int64_t from, int64_t to) {
 // Traversal precision of the build and insert searches: BF16 rows when every
 // row makes it exact (the same graph as FP32, half the row bytes).
 int build_prec(const vmn_index* idx) { return idx->exact_bf16 ? VMN_PREC_BF16 : VMN_PREC_FP32; }
+
+// Traversal precision for nq padded query rows q (device) that are not index
+// rows: BF16 only when the index rows and the queries all qualify, since a
+// query rounded to BF16 would give other distances than the FP32 ones.
+int query_prec(vmn_index* idx, const float* q, int64_t nq, int* prec) {
+    *prec = VMN_PREC_FP32;
+    if (!idx->exact_bf16 || nq <= 0) return VMN_OK;
+    HIPCHK(idx->exact_flag.reserve(64));
+    unsigned int* flag = idx->exact_flag.as<unsigned int>() + 1;
+    HIPCHK(vmn::launch_fill_i32(reinterpret_cast<int32_t*>(flag), 1, 1, idx->stream));
+    HIPCHK(vmn::launch_bf16_exact(q, 0, nq, idx->DP, flag, idx->stream));
+    unsigned int ok = 0;
+    HIPCHK(hipMemcpyAsync(&ok, flag, sizeof ok, hipMemcpyDeviceToHost, idx->stream));
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    if (ok) *prec = VMN_PREC_BF16;
+    return VMN_OK;
+}
 
 // The prunes read the same BF16 rows under the same condition.
 void set_prune_rows(const vmn_index* idx, vmn::PruneArgs& a) {
@@ -451,6 +469,7 @@ int build_fresh(vmn_index* idx, bool random_init) {
 }
 
 int set_tombstones(vmn_index* idx, const int32_t* dev_ids, int64_t m) {
+    ++idx->epoch;
     HIPCHK(idx->bad.reserve(64));
     HIPCHK(vmn::launch_fill_u64(idx->bad.as<unsigned long long>(), 1, 0ull, idx->stream));
     HIPCHK(vmn::launch_tombstone(dev_ids, m, idx->n, idx->del_bits.as<uint32_t>(),
@@ -933,6 +952,7 @@ int insert_rows(vmn_index* idx, const float* vecs, int64_t n, int32_t* ids_out, 
                 bool padded) {
     int rc = set_device(idx);
     if (rc) return rc;
+    ++idx->epoch;
     const int64_t base = idx->n;
     const int DP = idx->DP;
     float* dst = idx->vecs.as<float>() + base * DP;
@@ -1138,6 +1158,7 @@ int vmn_consolidate(vmn_index* idx) {
     if (idx->n_deleted == 0) return VMN_OK;
     rc = set_device(idx);
     if (rc) return rc;
+    ++idx->epoch;
     const int64_t N = idx->n;
     const int R = idx->p.R;
     HIPCHK(idx->rcount.reserve((size_t)N * sizeof(int32_t)));
@@ -1299,11 +1320,16 @@ int vmn_ctx_capture(vmn_ctx* c, const float* queries, int64_t nq, int32_t k, int
     if (e != hipSuccess) return fail(VMN_ERR_HIP, "capture: %s", hipGetErrorString(e));
     c->graph = g;
     HIPCHK(hipGraphInstantiate(&c->exec, g, nullptr, nullptr, 0));
+    c->exec_epoch = c->idx->epoch;
     return VMN_OK;
 }
 
 int vmn_ctx_replay(vmn_ctx* c) {
     if (!c || !c->exec) return fail(VMN_ERR_STATE, "no captured search on this context");
+    if (c->exec_epoch != c->idx->epoch)
+        return fail(VMN_ERR_STATE,
+                    "the index changed since the capture (build, insert, delete or "
+                    "consolidation): capture the search again");
     int rc = set_device(c->idx);
     if (rc) return rc;
     HIPCHK(hipGraphLaunch(c->exec, c->stream));

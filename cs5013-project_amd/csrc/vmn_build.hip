@@ -303,16 +303,18 @@ __global__ __launch_bounds__(64) void parth_prune_kernel(PruneArgs a) {
             for (int t = 0; t < m; ++t) dup |= c_id[0][t] == (uint32_t)nb;
             pass = pass && !dup;
         }
-        PruneQ<DP, BF> q;
+        // FP32 rows: distances in the reference's sequential order
+        PruneQ<DP, BF, !BF> q;
         q.load(rows, node);
-        const int ne = row_extras<DP, BF>(rows, q, pass, nb, e_id, e_d);
+        const int ne = row_extras<DP, BF, !BF>(rows, q, pass, nb, e_id, e_d);
         const uint32_t my_id = l < ne ? (uint32_t)e_id[l] : 0u;
         const float my_d = l < ne ? e_d[l] : 0.0f;
         const int cnt = merge_sorted<CCAP, false>(c_id[0], c_d[0], m, my_id, my_d, ne, c_id[1],
                                                   c_d[1], CCAP);
         __syncthreads();
-        const int nk = prune_list<DP, true, BF, CCAP>(rows, c_id[1], c_d[1], cnt, a.alpha, a.R, kept,
-                                              act_id, act_pos, act_d, newrow, nullptr);
+        const int nk = prune_list<DP, true, BF, CCAP, !BF>(rows, c_id[1], c_d[1], cnt, a.alpha,
+                                                           a.R, kept, act_id, act_pos, act_d,
+                                                           newrow, nullptr);
         for (int j = l; j < W; j += 64) a.adj[node * W + j] = j < nk ? newrow[j] : -1;
         __syncthreads();
     }

@@ -122,6 +122,51 @@ __device__ __forceinline__ void gather_dists(const QReg<DP>& q, const float* __r
     }
 }
 
+// Distances from the row q (a padded row in global memory, the same for every
+// lane) to `cnt` rows whose ids sit in LDS ids[0..cnt), one lane per row,
+// summed in index order as one fused multiply-add chain acc = fma(d_i, d_i,
+// acc), d_i = q_i - x_i: the order of the reference's device distance<D>
+// (parth/src/device_funcs.cu:17-49, `dist += diff * diff` contracted by nvcc's
+// default --fmad=true). The padded tail adds fma(0, 0, acc) = acc. Equal to
+// the lane-tree sums bit for bit on integer data; on real-valued rows this is
+// the form whose results match the reference's. Must be called by the whole
+// wave; dist_out[s] (LDS) receives the distance of ids[s].
+template <int DP>
+__device__ __forceinline__ void gather_dists_seq(const float* __restrict__ q,
+                                                 const float* __restrict__ vecs,
+                                                 const int32_t* ids, float* dist_out, int cnt) {
+    constexpr int T = DP / 4;
+    constexpr int CH = 8;  // float4 loads in flight per lane
+    const float4* qq = reinterpret_cast<const float4*>(q);
+    for (int base = 0; base < cnt; base += 64) {
+        const int c = base + lane();
+        if (c < cnt) {
+            const float4* x = reinterpret_cast<const float4*>(vecs + (int64_t)ids[c] * DP);
+            float acc = 0.0f;
+#pragma unroll
+            for (int t0 = 0; t0 < T; t0 += CH) {
+                float4 xv[CH];
+#pragma unroll
+                for (int t = 0; t < CH; ++t)
+                    if (t0 + t < T) xv[t] = x[t0 + t];
+#pragma unroll
+                for (int t = 0; t < CH; ++t) {
+                    if (t0 + t < T) {
+                        const float4 qv = qq[t0 + t];
+                        const float a = qv.x - xv[t].x, b = qv.y - xv[t].y;
+                        const float e = qv.z - xv[t].z, f = qv.w - xv[t].w;
+                        acc = __builtin_fmaf(a, a, acc);
+                        acc = __builtin_fmaf(b, b, acc);
+                        acc = __builtin_fmaf(e, e, acc);
+                        acc = __builtin_fmaf(f, f, acc);
+                    }
+                }
+            }
+            dist_out[c] = acc;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Low-precision candidate rows on MFMA.
 //

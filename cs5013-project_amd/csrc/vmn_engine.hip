@@ -7,8 +7,10 @@
 // run in operation order (what one converged wavefront does), the vector
 // writes and the nearest-node searches run in parallel, and every search of a
 // batch sees the graph from before the batch's links. Searches and the
-// 1-nearest searches use the PARTH search kernel; the ground truth is the FP64
-// brute-force kernel restricted to live ids below the node count.
+// 1-nearest searches use the PARTH search kernel (FP32 distances summed in the
+// reference's order, or BF16 rows where those give the same values bit for
+// bit); the ground truth is the reference's FP32 brute force (first 10 live
+// ids below the node count by FP32 distance, then id).
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -87,6 +89,7 @@ int check_engine(const vmn_index* idx) {
 
 int mark(vmn_index* idx, const std::vector<int32_t>& ids, int set) {
     if (ids.empty()) return VMN_OK;
+    ++idx->epoch;
     HIPCHK(idx->eng_marks.reserve(ids.size() * sizeof(int32_t)));
     HIPCHK(hipMemcpyAsync(idx->eng_marks.p, ids.data(), ids.size() * sizeof(int32_t),
                           hipMemcpyHostToDevice, idx->stream));
@@ -167,6 +170,7 @@ int vmn_engine_insert(vmn_index* idx, const float* vecs, int64_t n, int32_t link
     if (n == 0) return VMN_OK;
     rc = set_device(idx);
     if (rc) return rc;
+    ++idx->epoch;
     // ids: every insert sees the count of the batch start
     const int64_t fc = idx->free_count;
     const int64_t popped = fc > 0 ? std::min<int64_t>(n, fc) : 0;
@@ -288,9 +292,13 @@ int vmn_engine_run(vmn_index* idx, const int32_t* types, const int32_t* ids, con
             rc = stage_rows(idx, qs.data(), nq, VMN_MEM_HOST, idx->qpad, &q);
             if (rc) return rc;
             HIPCHK(idx->eng_found.reserve((size_t)nq * K * sizeof(int32_t)));
+            // BF16 rows only when the queries are exact in BF16 too
+            int prec = VMN_PREC_FP32;
+            rc = query_prec(idx, q, nq, &prec);
+            if (rc) return rc;
             HIPCHK(hipEventRecord(ev0, idx->stream));
             rc = run_search(idx, q, nq, L, K, idx->eng_found.as<int32_t>(), nullptr, false,
-                            VMN_MODE_PARTH, false, 0, build_prec(idx));
+                            VMN_MODE_PARTH, false, 0, prec);
             if (rc) return rc;
             HIPCHK(hipEventRecord(ev1, idx->stream));
             const int64_t n = idx->n;
@@ -304,7 +312,7 @@ int vmn_engine_run(vmn_index* idx, const int32_t* types, const int32_t* ids, con
             HIPCHK(vmn::launch_gt(idx->vecs.as<float>(), n, DP, idx->del_bits.as<uint32_t>(), q, nq,
                                   K, splits, idx->eng_part_d.as<double>(),
                                   idx->eng_part_i.as<int32_t>(), idx->eng_gt_i.as<int32_t>(),
-                                  idx->eng_gt_d.as<double>(), idx->stream));
+                                  idx->eng_gt_d.as<double>(), idx->stream, true));
             h_found.resize((size_t)nq * K);
             h_gt.resize((size_t)nq * K);
             h_hops.resize((size_t)nq);

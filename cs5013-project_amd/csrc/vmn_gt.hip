@@ -1,8 +1,15 @@
 // Exact k-nearest neighbours in FP64 (the ground truth recall is scored
-// against). Replaces the reference's FP32 linear scan (parth/src/engine.cu:
-// 253-279 `bruteForceKernel`) and the external DiskANN compute_groundtruth it
+// against). Replaces the external DiskANN compute_groundtruth the reference
 // shells out to (vamana.cu:1644). Each distance is sum_k (double(q_k) -
 // double(x_k))^2; equal distances are ordered by id.
+//
+// F32 form: the parth engine's own ground truth (parth/src/engine.cu:253-279
+// `bruteForceKernel`), a CandidateList<10> over the live ids in increasing
+// order with FP32 distances summed as one fused multiply-add chain in index
+// order (distance<D>, parth/src/device_funcs.cu:17-49). Ties after existing
+// entries and replacement only on a strictly smaller distance make its list
+// the first k by (FP32 distance, id), which is the order kept here; the FP32
+// value is stored exactly as a double.
 //
 // Pass 1: one lane per (query, base split) keeps a sorted top-KP list in
 // registers while base rows stream through LDS 32 at a time. Pass 2: one
@@ -11,7 +18,7 @@
 
 namespace vmn {
 
-template <int DP, int KP>
+template <int DP, int KP, bool F32>
 __global__ __launch_bounds__(64) void gt_partial_kernel(const float* __restrict__ base, int64_t n,
                                                         const uint32_t* del_bits,
                                                         const float* __restrict__ queries,
@@ -47,17 +54,32 @@ __global__ __launch_bounds__(64) void gt_partial_kernel(const float* __restrict_
             if (del_bits && bit_test(del_bits, (uint32_t)id)) continue;
             double acc = 0.0;
             const float4* x = reinterpret_cast<const float4*>(tile + r * DP);
+            if constexpr (F32) {
+                float f = 0.0f;
 #pragma unroll
-            for (int t = 0; t < DP / 4; ++t) {
-                const float4 xv = x[t];
-                double a = (double)qv[t].x - (double)xv.x;
-                double b = (double)qv[t].y - (double)xv.y;
-                double c = (double)qv[t].z - (double)xv.z;
-                double d = (double)qv[t].w - (double)xv.w;
-                acc = fma(a, a, acc);
-                acc = fma(b, b, acc);
-                acc = fma(c, c, acc);
-                acc = fma(d, d, acc);
+                for (int t = 0; t < DP / 4; ++t) {
+                    const float4 xv = x[t];
+                    const float a = qv[t].x - xv.x, b = qv[t].y - xv.y;
+                    const float c = qv[t].z - xv.z, d = qv[t].w - xv.w;
+                    f = __builtin_fmaf(a, a, f);
+                    f = __builtin_fmaf(b, b, f);
+                    f = __builtin_fmaf(c, c, f);
+                    f = __builtin_fmaf(d, d, f);
+                }
+                acc = (double)f;
+            } else {
+#pragma unroll
+                for (int t = 0; t < DP / 4; ++t) {
+                    const float4 xv = x[t];
+                    double a = (double)qv[t].x - (double)xv.x;
+                    double b = (double)qv[t].y - (double)xv.y;
+                    double c = (double)qv[t].z - (double)xv.z;
+                    double d = (double)qv[t].w - (double)xv.w;
+                    acc = fma(a, a, acc);
+                    acc = fma(b, b, acc);
+                    acc = fma(c, c, acc);
+                    acc = fma(d, d, acc);
+                }
             }
             if (acc < bd[KP - 1]) {
                 // insert keeping (dist, id) order; ids arrive increasing
@@ -137,23 +159,27 @@ __global__ __launch_bounds__(64) void gt_merge_kernel(const double* part_d, cons
 template <int DP, int KP>
 static void gt_launch(const float* base, int64_t n, const uint32_t* del_bits, const float* queries,
                       int64_t nq, int k, int splits, double* part_d, int32_t* part_i,
-                      int32_t* out_ids, double* out_d, hipStream_t s) {
+                      int32_t* out_ids, double* out_d, bool f32, hipStream_t s) {
     dim3 g((unsigned)((nq + 63) / 64), (unsigned)splits);
-    hipLaunchKernelGGL((gt_partial_kernel<DP, KP>), g, dim3(64), 0, s, base, n, del_bits, queries,
-                       nq, splits, part_d, part_i);
+    if (f32)
+        hipLaunchKernelGGL((gt_partial_kernel<DP, KP, true>), g, dim3(64), 0, s, base, n, del_bits,
+                           queries, nq, splits, part_d, part_i);
+    else
+        hipLaunchKernelGGL((gt_partial_kernel<DP, KP, false>), g, dim3(64), 0, s, base, n, del_bits,
+                           queries, nq, splits, part_d, part_i);
     hipLaunchKernelGGL((gt_merge_kernel<KP>), dim3((unsigned)nq), dim3(64), 0, s, part_d, part_i,
                        nq, splits, k, out_ids, out_d);
 }
 
 hipError_t launch_gt(const float* base, int64_t n, int DP, const uint32_t* del_bits,
                      const float* queries, int64_t nq, int k, int splits, double* part_d,
-                     int32_t* part_i, int32_t* out_ids, double* out_d, hipStream_t s) {
+                     int32_t* part_i, int32_t* out_ids, double* out_d, hipStream_t s, bool f32) {
 #define VMN_GT_CASE(DPV)                                                                            \
     case DPV:                                                                                       \
         if (k <= 16) gt_launch<DPV, 16>(base, n, del_bits, queries, nq, k, splits, part_d, part_i, \
-                                        out_ids, out_d, s);                                         \
+                                        out_ids, out_d, f32, s);                                    \
         else gt_launch<DPV, 128>(base, n, del_bits, queries, nq, k, splits, part_d, part_i,        \
-                                 out_ids, out_d, s);                                                \
+                                 out_ids, out_d, f32, s);                                           \
         break;
     if (k > 128) return hipErrorInvalidValue;
     switch (DP) {

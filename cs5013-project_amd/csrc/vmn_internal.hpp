@@ -106,6 +106,11 @@ struct vmn_index {
     int64_t n_pend = 0;
     int waves_per_cu = 32;  // search grid = CUs x waves_per_cu (persistent work queue)
     int list_cap = 0;       // minimum beam-list capacity of the search kernel
+    // bumped by every change of the graph, tombstones or rows: a search
+    // captured as a HIP graph holds the entry point, the tombstone pointer
+    // and the buffers of its capture, so it may only be replayed while this
+    // is unchanged
+    uint64_t epoch = 0;
 };
 
 // A search context: its own stream and per-launch scratch, so several search
@@ -117,6 +122,7 @@ struct vmn_ctx {
     int64_t last_nq = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    uint64_t exec_epoch = 0;  // index epoch the captured search belongs to
 };
 
 namespace vmn_rt {
@@ -138,6 +144,9 @@ int check_exact(vmn_index* idx, int64_t from, int64_t to);
 // Traversal precision of the build, insert and engine searches: BF16 rows
 // while every row makes it exact, else FP32.
 int build_prec(const vmn_index* idx);
+// Traversal precision for query rows that are not index rows (engine
+// searches): BF16 when the index rows and these rows all make it exact.
+int query_prec(vmn_index* idx, const float* q, int64_t nq, int* prec);
 // Host copy of an adjacency given by a caller ([n x row_width], -1 padded):
 // each row keeps its entries up to the first -1, in order, without repeated
 // ids (and without i itself when drop_self), -1 padded to width R.

@@ -141,8 +141,11 @@ hipError_t launch_compact_nonzero(const int32_t* counts, int64_t n, int64_t* lis
                                   unsigned long long* nlist, hipStream_t s);
 
 // FP64 exact k-NN: partial top-k per (query, base split), then a merge.
+// f32: the parth engine's FP32 ground truth instead (sequential FP32 fused
+// multiply-add distances, first k by (distance, id)).
 hipError_t launch_gt(const float* base, int64_t n, int DP, const uint32_t* del_bits,
                      const float* queries, int64_t nq, int k, int splits, double* part_d,
-                     int32_t* part_i, int32_t* out_ids, double* out_d, hipStream_t s);
+                     int32_t* part_i, int32_t* out_ids, double* out_d, hipStream_t s,
+                     bool f32 = false);
 
 }  // namespace vmn

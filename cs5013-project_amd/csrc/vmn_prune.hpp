@@ -18,11 +18,14 @@ struct PruneRows {
 // squared norm <= 2^22: BF16 then holds the values exactly and every partial
 // sum of ||a||^2 + ||b||^2 - 2 a.b is an integer below 2^24, so the distance
 // equals the FP32 sum of squared differences bit for bit, from half the bytes.
-template <int DP, bool BF>
+// SEQ (FP32 rows only): sequential fused multiply-add chains over the
+// dimensions, one lane per candidate (gather_dists_seq), the summation order
+// of the reference's parth distance<D> (parth/src/device_funcs.cu:17-49).
+template <int DP, bool BF, bool SEQ = false>
 struct PruneQ;
 
 template <int DP>
-struct PruneQ<DP, false> {
+struct PruneQ<DP, false, false> {
     QReg<DP> q;
     __device__ __forceinline__ void load(const PruneRows& r, int64_t id) {
         q.load(r.f32 + id * DP, lane() & 7);
@@ -34,7 +37,17 @@ struct PruneQ<DP, false> {
 };
 
 template <int DP>
-struct PruneQ<DP, true> {
+struct PruneQ<DP, false, true> {
+    const float* row;
+    __device__ __forceinline__ void load(const PruneRows& r, int64_t id) { row = r.f32 + id * DP; }
+    __device__ __forceinline__ void dists(const PruneRows& r, const int32_t* ids, float* out,
+                                          int cnt) const {
+        gather_dists_seq<DP>(row, r.f32, ids, out, cnt);
+    }
+};
+
+template <int DP>
+struct PruneQ<DP, true, false> {
     QLow<DP> q;
     __device__ __forceinline__ void load(const PruneRows& r, int64_t id) {
         prep_query<DP, VMN_PREC_BF16>(q, r.f32 + id * DP, nullptr);
@@ -56,7 +69,7 @@ struct PruneQ<DP, true> {
 // one float rounding, compared without fusion. Stops once R are kept. The
 // d(p*, p') of one round are computed by the whole wave (8 rows at a time).
 // ---------------------------------------------------------------------------
-template <int DP, bool STRICT>
+template <int DP, bool STRICT, bool SEQ = false>
 __device__ int robust_prune(const PruneRows& rows, const uint32_t* c_id,
                             const float* c_d, int cnt, float alpha, int R, uint8_t* s_kept,
                             int32_t* s_act_id, int32_t* s_act_pos, float* s_act_d,
@@ -83,7 +96,7 @@ __device__ int robust_prune(const PruneRows& rows, const uint32_t* c_id,
         }
         ++nk;
         if (nk >= R) break;
-        PruneQ<DP, false> pq;
+        PruneQ<DP, false, SEQ> pq;
         pq.load(rows, ps);
         int nact = 0;
         for (int b = p + 1; b < cnt; b += 64) {
@@ -321,8 +334,8 @@ __device__ void occlude_mfma(const PruneRows& rows, const int32_t* members, int 
 }
 
 // robust_prune over FP32 rows, or robust_prune_mfma when the rows are exact in
-// BF16 (same output).
-template <int DP, bool STRICT, bool BF, int CCAP>
+// BF16 (same output). SEQ: FP32 distances as sequential chains (PruneQ).
+template <int DP, bool STRICT, bool BF, int CCAP, bool SEQ = false>
 __device__ __forceinline__ int prune_list(const PruneRows& rows, const uint32_t* c_id,
                                           const float* c_d, int cnt, float alpha, int R,
                                           uint8_t* s_kept, int32_t* s_act_id, int32_t* s_act_pos,
@@ -333,14 +346,14 @@ __device__ __forceinline__ int prune_list(const PruneRows& rows, const uint32_t*
                                                    reinterpret_cast<uint32_t*>(s_act_d), out_ids,
                                                    out_d);
     else
-        return robust_prune<DP, STRICT>(rows, c_id, c_d, cnt, alpha, R, s_kept, s_act_id,
-                                        s_act_pos, s_act_d, out_ids, out_d);
+        return robust_prune<DP, STRICT, SEQ>(rows, c_id, c_d, cnt, alpha, R, s_kept, s_act_id,
+                                             s_act_pos, s_act_d, out_ids, out_d);
 }
 
 // Distances from `node` to its current row entries that pass `keep`, appended
 // (compacted, adjacency order) to e_id/e_d. Returns the count.
-template <int DP, bool BF>
-__device__ __forceinline__ int row_extras(const PruneRows& rows, const PruneQ<DP, BF>& q,
+template <int DP, bool BF, bool SEQ = false>
+__device__ __forceinline__ int row_extras(const PruneRows& rows, const PruneQ<DP, BF, SEQ>& q,
                                           bool pass, int32_t nb, int32_t* e_id, float* e_d) {
     const uint64_t m = ballot(pass);
     const int ne = __popcll((unsigned long long)m);

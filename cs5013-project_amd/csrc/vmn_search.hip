@@ -34,12 +34,17 @@
 // Both modes skip tombstoned nodes while expanding (search.cuh:122,
 // engine.cu:76) and give a tombstoned start node distance 1e30 (engine.cu:45).
 //
-// PREC selects the distance used while traversing: FP32 rows (lane-tree sum),
-// or BF16 / FP8 rows on MFMA (vmn_device.hpp). With krr > 0 the first krr
-// entries of the final list are re-ranked by exact FP32 distance (stable on
-// ties) before the k results are written; this is the re-rank step the
-// reference runs on the worklist (utils.cuh:199-285, merge_and_rerank_kernel),
-// done here on ids inside the search kernel.
+// PREC selects the distance used while traversing: FP32 rows, or BF16 / FP8
+// rows on MFMA (vmn_device.hpp). FP32 distances are sequential fused
+// multiply-add chains over the dimensions in PARTH mode (the reference's
+// distance<D>, parth/src/device_funcs.cu:17-49, so real-valued rows give the
+// reference's results) and lane-tree sums in FRESH mode (the reference sums
+// 8 strided partials with float atomics, kernels.cuh:40-52, in no fixed
+// order). With krr > 0 the first krr entries of the final list are re-ranked
+// by exact FP32 distance (stable on ties) before the k results are written;
+// this is the re-rank step the reference runs on the worklist (utils.cuh:
+// 199-285, merge_and_rerank_kernel), done here on ids inside the search
+// kernel.
 #include "vmn_kernels.hpp"
 
 namespace vmn {
@@ -58,6 +63,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
     (PREC_ARG == VMN_PREC_FP8 && LCAP <= 128 && DP <= 128) ? 5 : 1, 8))) void search_kernel(SearchArgs a) {
     constexpr int PREC = PREC_ARG == VMN_PREC_BF16_TABLE_NORMS ? VMN_PREC_BF16 : PREC_ARG;
     constexpr bool TN = PREC_ARG == VMN_PREC_BF16_TABLE_NORMS;
+    // FP32 distances in the reference's summation order (PARTH)
+    constexpr bool SEQ = MODE == VMN_MODE_PARTH;
     __shared__ __align__(16) uint32_t s_id[2][LCAP];
     __shared__ __align__(16) float s_d[2][LCAP];
     __shared__ __align__(16) int32_t s_sid[64];
@@ -77,9 +84,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
         if (qi >= a.nq) return;
 
         // The FP32 query lives in registers through the walk only for FP32
-        // traversal; the low-precision forms load it for the re-rank.
+        // lane-tree traversal; the low-precision forms load it for the
+        // re-rank. PARTH-mode FP32 distances are sequential chains read from
+        // the query row itself.
+        const float* qrow = a.queries + qi * DP;
         QReg<DP> q;
-        if (PREC == VMN_PREC_FP32) q.load(a.queries + qi * DP, sub);
+        if (PREC == VMN_PREC_FP32 && !SEQ) q.load(qrow, sub);
         QLow<DP> ql;
         if (PREC != VMN_PREC_FP32) prep_query<DP, PREC>(ql, a.queries + qi * DP, a.fp8_scales, a.fp8_centre,
                                  reinterpret_cast<uint8_t*>(s_sid));
@@ -90,8 +100,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
         }
         const int32_t st = a.start;
         float sd;
-        if (PREC == VMN_PREC_FP32) {
+        if (PREC == VMN_PREC_FP32 && !SEQ) {
             sd = reduce8(q.partial(vecs + (int64_t)st * DP, sub));
+        } else if (PREC == VMN_PREC_FP32) {
+            if (l == 0) s_sid[0] = st;
+            __syncthreads();
+            gather_dists_seq<DP>(qrow, vecs, s_sid, s_sd, 1);
+            __syncthreads();
+            sd = s_sd[0];
         } else {
             if (l == 0) s_sid[0] = st;
             __syncthreads();
@@ -215,7 +231,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
             const int slot = __popcll((unsigned long long)(vm & lanemask_lt()));
             if (valid) s_sid[slot] = nb;
             __syncthreads();
-            if (PREC == VMN_PREC_FP32)
+            if (PREC == VMN_PREC_FP32 && SEQ)
+                gather_dists_seq<DP>(qrow, vecs, s_sid, s_sd, nnew);
+            else if (PREC == VMN_PREC_FP32)
                 gather_dists<DP>(q, vecs, s_sid, s_sd, nnew);
             else
                 gather_dists_low<DP, PREC, VMN_GATHER_TILES, TN>(ql, a.low_rows, a.low_norms, s_sid,
@@ -238,9 +256,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
             // exact FP32 re-rank of the first krr entries, stable on ties
             const int nr = cnt < a.krr ? cnt : a.krr;
             if (l < nr) s_sid[l] = (int32_t)(s_id[cur][l] & VMN_ID_MASK);
-            q.load(a.queries + qi * DP, sub);
+            if (!SEQ) q.load(qrow, sub);
             __syncthreads();
-            gather_dists<DP>(q, vecs, s_sid, s_sd, nr);
+            if (SEQ) gather_dists_seq<DP>(qrow, vecs, s_sid, s_sd, nr);
+            else gather_dists<DP>(q, vecs, s_sid, s_sd, nr);
             __syncthreads();
             const uint32_t rid = l < nr ? (uint32_t)s_sid[l] : 0u;
             const float rd = l < nr ? s_sd[l] : 0.0f;

@@ -204,6 +204,28 @@ def _torch_ready(x):
         torch.cuda.current_stream(x.device).synchronize()
 
 
+def _device_tensor(x, what, dtype, device, min_numel=None, width=None):
+    """Checks a tensor handed to the library as a raw device pointer: a
+    contiguous CUDA tensor of `dtype` on `device`, holding at least
+    `min_numel` elements (rows of `width` when given). Raises VamanaError
+    (VMN_ERR_ARG) before any library call otherwise: a wrong pointer would be
+    read or written out of bounds on the device."""
+    import torch
+    tdt = {np.float32: torch.float32, np.int32: torch.int32}[dtype]
+    if not _is_torch(x):
+        raise VamanaError(-1, f"{what} must be a torch tensor on cuda:{device}")
+    if not x.is_cuda or x.device.index != device:
+        raise VamanaError(-1, f"{what} is on {x.device}, the index is on cuda:{device}")
+    if x.dtype != tdt:
+        raise VamanaError(-1, f"{what} has dtype {x.dtype}, expected {tdt}")
+    if not x.is_contiguous():
+        raise VamanaError(-1, f"{what} must be contiguous")
+    if width is not None and (x.dim() != 2 or int(x.shape[1]) != width):
+        raise VamanaError(-1, f"{what} has shape {tuple(x.shape)}, expected [n, {width}]")
+    if min_numel is not None and x.numel() < min_numel:
+        raise VamanaError(-1, f"{what} holds {x.numel()} elements, needs {min_numel}")
+
+
 def _arg(x, dtype):
     """(pointer, mem kind, keepalive) for a numpy array or a torch tensor."""
     if _is_torch(x):
@@ -404,10 +426,14 @@ class Vamana:
         qp, mem, keep = _arg(queries, np.float32)
         if mem == MEM_DEVICE:
             import torch
+            dev = int(self.params.device)
+            _device_tensor(queries, "queries", np.float32, dev, width=int(self.params.dim))
             ids = ids_out if ids_out is not None else torch.empty((nq, k), dtype=torch.int32,
                                                                   device=queries.device)
             dists = dists_out if dists_out is not None else torch.empty(
                 (nq, k), dtype=torch.float32, device=queries.device)
+            _device_tensor(ids, "ids_out", np.int32, dev, min_numel=nq * k)
+            _device_tensor(dists, "dists_out", np.float32, dev, min_numel=nq * k)
             _check(lib().vmn_search_ex(self._h, qp, nq, k, L, pc, rerank,
                                        ctypes.c_void_p(ids.data_ptr()),
                                        ctypes.c_void_p(dists.data_ptr()), mem))
@@ -703,6 +729,11 @@ class SearchContext:
         return lib().vmn_ctx_stream(self._h)
 
     def _args(self, q, k, L, prec, rerank, ids, dists):
+        dev, dim = int(self.index.params.device), int(self.index.params.dim)
+        _device_tensor(q, "queries", np.float32, dev, width=dim)
+        nq = int(q.shape[0])
+        _device_tensor(ids, "ids", np.int32, dev, min_numel=nq * int(k))
+        _device_tensor(dists, "dists", np.float32, dev, min_numel=nq * int(k))
         _torch_ready(q)
         pc = PREC[prec] if isinstance(prec, str) else prec
         return (self._h, ctypes.c_void_p(q.data_ptr()), int(q.shape[0]), k, L, pc, rerank,

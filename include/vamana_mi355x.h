@@ -45,7 +45,10 @@ extern "C" {
 #define VMN_MEM_DEVICE 1
 
 /* Distance precision during traversal (vmn_search_ex). */
-#define VMN_PREC_FP32 0 /* FP32 rows, lane-tree sums (the reference's arithmetic) */
+#define VMN_PREC_FP32 0 /* FP32 rows; PARTH mode sums each distance as one fused multiply-add
+                          chain in index order (parth/src/device_funcs.cu:17-49), FRESH mode
+                          in a lane tree (the reference's order there is atomic arrival,
+                          vamana/include/kernels.cuh:40-52) */
 #define VMN_PREC_BF16 1 /* BF16 rows on MFMA, FP32 accumulation */
 #define VMN_PREC_FP8 2  /* FP8 E4M3 rows, centred on the per-dimension mean and scaled by the
                           centred absmax, on MFMA */
@@ -367,7 +370,10 @@ int vmn_server_destroy(vmn_server* srv);
 /* Capture one context search (fixed pointers and sizes) as a HIP graph of
  * kernel nodes (the work-queue counter is cleared by a kernel, not a memset
  * node), and launch it again on the context's stream. Replays re-read the
- * query buffer, so new queries written there are searched. */
+ * query buffer, so new queries written there are searched. The graph holds
+ * the index state of its capture (entry point, tombstones, buffers): after
+ * any build, load, insert, delete or consolidation of the index,
+ * vmn_ctx_replay returns VMN_ERR_STATE until the search is captured again. */
 int vmn_ctx_capture(vmn_ctx* ctx, const float* queries, int64_t nq, int32_t k, int32_t L,
                     int32_t prec, int32_t k_rerank, int32_t* ids_out, float* dists_out);
 int vmn_ctx_replay(vmn_ctx* ctx);

@@ -43,6 +43,19 @@
 extern "C" {
 
 // ---------------------------------------------------------------------------
+// Planted mistakes, for the tests that check that the fresh-mode checks can
+// fail (0 = none; never set outside those tests). Each value breaks one rule:
+//  1 prune keeps p' on alpha * d(p*, p') == d(node, p') (`<` for `<=`);
+//  2 search merge puts new neighbours before worklist entries on ties;
+//  3 search does not skip the query's own node id;
+//  4 reverse candidates capped at M + 1; 5 at M - 1;
+//  6 out-prune candidates put the current neighbours first on ties;
+//  7 reverse-prune candidates put the current neighbours first on ties.
+// ---------------------------------------------------------------------------
+static int g_mut = 0;
+void orc_set_mutation(int m) { g_mut = m; }
+
+// ---------------------------------------------------------------------------
 // Distances
 // ---------------------------------------------------------------------------
 
@@ -419,7 +432,7 @@ static void fresh_search_one(const float* vecs, int d, const int32_t* adj, int W
             const int32_t v = nb[j];
             if (v == -1) break;
             if (del && del[v]) continue;
-            if (v == self) continue;
+            if (v == self && g_mut != 3) continue;
             if (bloom[orc_bf_hash1((uint32_t)v)] && bloom[orc_bf_hash2((uint32_t)v)]) continue;
             news.push_back(v);
         }
@@ -436,7 +449,9 @@ static void fresh_search_one(const float* vecs, int d, const int32_t* adj, int W
         merged.reserve(wl.size() + add.size());
         size_t a = 0, b = 0;
         while ((a < wl.size() || b < add.size()) && (int)merged.size() < L) {
-            if (b >= add.size() || (a < wl.size() && wl[a].d <= add[b].d)) merged.push_back(wl[a++]);
+            if (b >= add.size() ||
+                (a < wl.size() && (g_mut == 2 ? wl[a].d < add[b].d : wl[a].d <= add[b].d)))
+                merged.push_back(wl[a++]);
             else merged.push_back(add[b++]);
         }
         wl.swap(merged);
@@ -490,7 +505,7 @@ static void prune_le(const float* vecs, int d, const std::vector<Item>& c, float
             if (st[j] != 0) continue;
             float dp = l2sq_m(pv, vecs + (int64_t)c[j].id * d, d, dist_mode);
             float lhs = dp * alpha;
-            if (lhs <= c[j].d) st[j] = 1;
+            if (g_mut == 1 ? lhs < c[j].d : lhs <= c[j].d) st[j] = 1;
         }
     }
 }
@@ -524,7 +539,9 @@ void orc_fresh_out_prune(const float* vecs, int d, int32_t* adj, int W, int R, f
         std::stable_sort(B.begin(), B.end(), [](const Item& a, const Item& b) { return a.d < b.d; });
         size_t a = 0, b = 0;
         while ((a < A.size() || b < B.size()) && (int)C.size() < P) {
-            if (b >= B.size() || (a < A.size() && A[a].d <= B[b].d)) C.push_back(A[a++]);
+            if (b >= B.size() ||
+                (a < A.size() && (g_mut == 6 ? A[a].d < B[b].d : A[a].d <= B[b].d)))
+                C.push_back(A[a++]);
             else C.push_back(B[b++]);
         }
         std::vector<int32_t> out;
@@ -559,7 +576,8 @@ void orc_fresh_reverse(const float* vecs, int64_t n, int d, int32_t* adj, int W,
         std::sort(A.begin(), A.end(), [](const Item& a, const Item& b) {
             return a.d < b.d || (a.d == b.d && a.id < b.id);
         });
-        if ((int)A.size() > M) A.resize(M);
+        const int Mc = g_mut == 4 ? M + 1 : g_mut == 5 ? M - 1 : M;
+        if ((int)A.size() > Mc) A.resize(Mc);
         std::vector<Item> B, C;
         int32_t* row = adj + node * W;
         const float* nv = vecs + node * d;
@@ -570,8 +588,10 @@ void orc_fresh_reverse(const float* vecs, int64_t n, int d, int32_t* adj, int W,
         }
         std::stable_sort(B.begin(), B.end(), [](const Item& a, const Item& b) { return a.d < b.d; });
         size_t a = 0, b = 0;
-        while ((a < A.size() || b < B.size()) && (int)C.size() < M) {
-            if (b >= B.size() || (a < A.size() && A[a].d <= B[b].d)) C.push_back(A[a++]);
+        while ((a < A.size() || b < B.size()) && (int)C.size() < Mc) {
+            if (b >= B.size() ||
+                (a < A.size() && (g_mut == 7 ? A[a].d < B[b].d : A[a].d <= B[b].d)))
+                C.push_back(A[a++]);
             else C.push_back(B[b++]);
         }
         std::vector<int32_t> out;

@@ -63,6 +63,7 @@ def lib():
         _lib.orc_fresh_build.argtypes = [P, i64, i32, i32, i32, P, i32, u64, i32, i32, i32, i32,
                                          P, P, i32]
         _lib.orc_fresh_build.restype = i64
+        _lib.orc_set_mutation.argtypes = [i32]
         _lib.orc_num_threads.restype = i32
         _lib.orc_set_num_threads.argtypes = [i32]
     return _lib
@@ -177,6 +178,44 @@ def fresh_build(vecs, R=64, L=128, alphas=(1.5,), seed=1234, P=600, M=500, max_h
     med = lib().orc_fresh_build(_p(vecs), n, d, R, L, _p(al), len(al), seed, P, M, max_hops,
                                 dist_mode, _p(adj), _p(ini), 0 if ini is None else ini.shape[1])
     return adj, int(med)
+
+
+def fresh_out_prune(vecs, adj, R, alpha, vis_ids, vis_d, vis_cnt, base=0, deleted=None,
+                    dist_mode=0):
+    """Out-prune of nodes base .. base + len(vis_cnt) - 1 (rows of `adj`
+    rewritten in place) from their visited sets [cnt, P]. Returns the reverse
+    pairs (pair_dst [cnt, R], -1 unused; pair_d [cnt, R])."""
+    vecs = _f32(vecs)
+    assert adj.dtype == np.int32 and adj.flags.c_contiguous
+    vi = np.ascontiguousarray(vis_ids, dtype=np.int32)
+    vd = _f32(vis_d)
+    vc = np.ascontiguousarray(vis_cnt, dtype=np.int32)
+    cnt, P = vi.shape
+    pdst = np.full((cnt, R), -1, np.int32)
+    pd = np.zeros((cnt, R), np.float32)
+    dm = None if deleted is None else np.ascontiguousarray(deleted, dtype=np.uint8)
+    lib().orc_fresh_out_prune(_p(vecs), vecs.shape[1], _p(adj), adj.shape[1], R, float(alpha),
+                              base, cnt, _p(vi), _p(vd), _p(vc), P, dist_mode, _p(pdst), _p(pd),
+                              _p(dm))
+    return pdst, pd
+
+
+def fresh_reverse(vecs, adj, R, alpha, pair_dst, pair_d, base=0, M=500, dist_mode=0):
+    """Reverse-edge prune of every node that received a pair (rows of `adj`
+    rewritten in place); pair row i belongs to source node base + i."""
+    vecs = _f32(vecs)
+    assert adj.dtype == np.int32 and adj.flags.c_contiguous
+    pdst = np.ascontiguousarray(pair_dst, dtype=np.int32)
+    pd = _f32(pair_d)
+    lib().orc_fresh_reverse(_p(vecs), vecs.shape[0], vecs.shape[1], _p(adj), adj.shape[1], R,
+                            float(alpha), base, pdst.shape[0], _p(pdst), _p(pd), M, dist_mode)
+    return adj
+
+
+def set_mutation(m):
+    """Plants mistake `m` in the fresh-mode restatement (0 = none); used only
+    by the tests that show the fresh-mode checks can fail."""
+    lib().orc_set_mutation(int(m))
 
 
 def fresh_insert(vecs, n_old, adj, start, L, alpha, deleted=None, P=600, M=500, max_hops=1 << 20,

@@ -122,3 +122,19 @@ def test_cpp_drivers_build_against_the_c_abi(tmp_path, name):
     exe = compile_example(name, str(tmp_path / name))
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 1 and "Usage:" in r.stdout
+
+
+def test_device_tensor_checks_without_gpu(vmn):
+    """The Python mirror refuses, before any library call, a tensor it would
+    hand over as a raw device pointer unless it is a contiguous CUDA tensor
+    of the right dtype, device, width and size (here: CPU tensors and numpy
+    arrays, which no device pointer may come from)."""
+    import importlib
+
+    import numpy as np
+    import torch
+    v = importlib.import_module("cs5013-project_amd.vamana")
+    for x in (torch.zeros((4, 128)), np.zeros((4, 128), np.float32),
+              torch.zeros((4, 128), dtype=torch.float64)):
+        with pytest.raises(vmn.VamanaError):
+            v._device_tensor(x, "queries", np.float32, 0, width=128)

@@ -1,12 +1,14 @@
 """HIP graph capture / replay of a batched search, search contexts, launch
-shapes: results identical to eager searches."""
+shapes: results identical to eager searches and to the oracle's search on the
+same graph (integer rows: BF16 distances are exact, so the BF16 walk is the
+FP32 walk)."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-def test_graph_replay_matches_eager(vmn, gpu):
+def test_graph_replay_matches_eager(vmn, orc, gpu):
     import torch
     base = vmn.datasets.integer_mixture(8000, 128, k=32, seed=9, stream=0)
     qa = vmn.datasets.integer_mixture(300, 128, k=32, seed=9, stream=1)
@@ -14,6 +16,10 @@ def test_graph_replay_matches_eager(vmn, gpu):
     idx = vmn.Vamana(base, mode="fresh", R=32, L_build=64, alphas=(1.0, 1.2), low_precision="bf16")
     ea, da = idx.searchPoints(qa, k=10, L=48, prec="bf16", rerank=32)
     eb, db = idx.searchPoints(qb, k=10, L=48, prec="bf16", rerank=32)
+    # the eager results are the oracle's
+    for qq, e, d in ((qa, ea, da), (qb, eb, db)):
+        oi, od, _, _ = orc.fresh_search(base, idx.adjacency, qq, idx.medoid, L=48, out_n=10)
+        assert np.array_equal(e, oi) and np.array_equal(d, od)
     ctx = vmn.SearchContext(idx)
     q = torch.from_numpy(qa).to(gpu)
     ids = torch.empty((300, 10), dtype=torch.int32, device=gpu)
@@ -68,3 +74,72 @@ def test_contexts_concurrent_and_launch_shapes(vmn, gpu):
     assert np.array_equal(x, ref) and np.array_equal(xd, refd)
     for c in ctxs:
         c.close()
+
+
+def test_replay_refused_after_updates_and_recapture_matches_oracle(vmn, orc, gpu):
+    """A captured search holds the index state of its capture (entry point,
+    tombstones). After a delete or a consolidation the replay is refused
+    (VMN_ERR_STATE); a new capture searches the updated index exactly like an
+    eager search and the oracle with the same tombstones."""
+    import torch
+    base = vmn.datasets.integer_mixture(6000, 128, k=24, seed=21, stream=0)
+    qs = vmn.datasets.integer_mixture(256, 128, k=24, seed=21, stream=1)
+    idx = vmn.Vamana(base, mode="fresh", R=32, L_build=64, alphas=(1.0, 1.2),
+                     low_precision="bf16")
+    ctx = vmn.SearchContext(idx)
+    q = torch.from_numpy(qs).to(gpu)
+    ids = torch.empty((256, 10), dtype=torch.int32, device=gpu)
+    d = torch.empty((256, 10), dtype=torch.float32, device=gpu)
+    ctx.capture(q, ids, d, k=10, L=40, prec="bf16", rerank=20)
+    st = torch.cuda.ExternalStream(ctx.stream)
+    st.synchronize()
+    ctx.replay()
+    st.synchronize()
+    # delete the entry point and a slice of rows: the old graph is refused
+    rng = np.random.default_rng(4)
+    kill = np.unique(np.concatenate([[idx.medoid], rng.choice(6000, 600, replace=False)]))
+    idx.deleteIds(kill.astype(np.int32))
+    with pytest.raises(vmn.VamanaError) as e:
+        ctx.replay()
+    assert e.value.code == -3
+    idx.consolidate()
+    with pytest.raises(vmn.VamanaError):
+        ctx.replay()
+    ctx.capture(q, ids, d, k=10, L=40, prec="bf16", rerank=20)
+    for _ in range(3):
+        ids.fill_(-7)
+        torch.cuda.synchronize()
+        ctx.replay()
+        st.synchronize()
+        e_ids, e_d = idx.searchPoints(qs, k=10, L=40, prec="bf16", rerank=20)
+        assert np.array_equal(ids.cpu().numpy(), e_ids) and np.array_equal(d.cpu().numpy(), e_d)
+    dead = idx.deleted.astype(np.uint8)
+    oi, od, _, _ = orc.fresh_search(base, idx.adjacency, qs, idx.medoid, L=40, out_n=10,
+                                    deleted=dead)
+    assert np.array_equal(e_ids, oi) and np.array_equal(e_d, od)
+    assert not np.isin(e_ids[e_ids >= 0], kill).any()
+    ctx.close()
+
+
+def test_context_rejects_bad_tensors(vmn, gpu):
+    """Search-context tensors are checked before any device access: wrong
+    dtype, width, device, layout or an output too small raise VamanaError."""
+    import torch
+    base = vmn.datasets.integer_mixture(2000, 128, k=8, seed=3, stream=0)
+    idx = vmn.Vamana(base, mode="fresh", R=32, L_build=32, alphas=(1.2,))
+    ctx = vmn.SearchContext(idx)
+    q = torch.zeros((64, 128), dtype=torch.float32, device=gpu)
+    ids = torch.empty((64, 10), dtype=torch.int32, device=gpu)
+    d = torch.empty((64, 10), dtype=torch.float32, device=gpu)
+    bad = [
+        (q.double(), ids, d), (q.cpu(), ids, d), (q[:, :96], ids, d), (q.t(), ids, d),
+        (q, ids[:32], d), (q, ids, d[:, :5]), (q, ids.float(), d), (q, ids.cpu(), d),
+    ]
+    for qq, ii, dd in bad:
+        with pytest.raises(vmn.VamanaError):
+            ctx.search(qq, ii, dd, k=10, L=32)
+        with pytest.raises(vmn.VamanaError):
+            ctx.capture(qq, ii, dd, k=10, L=32)
+    ctx.search(q, ids, d, k=10, L=32)
+    torch.cuda.synchronize()
+    ctx.close()

@@ -101,19 +101,3 @@ def test_parth_search_recall_band_on_reference_graph(orc, sift):
     gt = sift["gt100"][:, :10]
     rec = np.mean([len(set(ids[i]) & set(gt[i])) / 10 for i in range(100)])
     assert 0.88 <= rec <= 0.95
-
-
-def test_bloom_hash_known_answers():
-    """FreshVamana bloom hashes (vamana/include/bloom_filter.cuh:11-30) as
-    closed forms over the 4 id bytes with 64-bit wrap-around, mod 39,988."""
-    def h(x, seed, mul):
-        v = seed
-        for s in (0, 8, 16, 24):
-            v = ((v ^ ((x >> s) & 0xFF)) * mul) % (1 << 64)
-        return v % 39988
-    kat1 = {0: h(0, 0xcbf29ce4, 0x01000193), 3732: h(3732, 0xcbf29ce4, 0x01000193)}
-    # hand-evaluated first step for x = 0: 0xcbf29ce4 * 0x01000193
-    assert (0xcbf29ce4 * 0x01000193) % (1 << 64) == 0xcbf29ce4 * 0x01000193
-    assert kat1[0] == ((((0xcbf29ce4 * 0x01000193) * 0x01000193) * 0x01000193 * 0x01000193)
-                       % (1 << 64)) % 39988
-    assert h(0, 0x84222325, 0x1B3) == (0x84222325 * 0x1B3 ** 4 % (1 << 64)) % 39988
