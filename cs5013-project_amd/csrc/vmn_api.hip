@@ -177,18 +177,10 @@ void set_prune_rows(const vmn_index* idx, vmn::PruneArgs& a) {
     a.bf16_norms = idx->exact_bf16 ? idx->bf16_norms.as<float>() : nullptr;
 }
 
-int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int32_t* out_ids,
-               float* out_d, bool queries_are_nodes, int mode, bool want_vis, int64_t node_base,
-               int prec, int krr, vmn_ctx* ctx) {
-    hipStream_t stream = ctx ? ctx->stream : idx->stream;
-    DevBuf& counter = ctx ? ctx->counter : idx->counter;
-    DevBuf& stat_hops = ctx ? ctx->stat_hops : idx->stat_hops;
-    DevBuf& stat_ndist = ctx ? ctx->stat_ndist : idx->stat_ndist;
-    HIPCHK(counter.reserve(64));
-    HIPCHK(vmn::launch_fill_u64(counter.as<unsigned long long>(), 1, 0ull, stream));
-    HIPCHK(stat_hops.reserve((size_t)nq * sizeof(int32_t)));
-    HIPCHK(stat_ndist.reserve((size_t)nq * sizeof(int32_t)));
-    vmn::SearchArgs a{};
+void search_args(const vmn_index* idx, vmn::SearchArgs& a, const float* q, int64_t nq, int L,
+                 int out_n, int32_t* out_ids, float* out_d, bool queries_are_nodes, int mode,
+                 int64_t node_base, int prec, int krr) {
+    a = vmn::SearchArgs{};
     a.mode = mode;
     a.DP = idx->DP;
     a.vecs = idx->vecs.as<float>();
@@ -206,18 +198,7 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
     a.out_n = out_n;
     a.out_ids = out_ids;
     a.out_d = out_d;
-    if (want_vis) {
-        HIPCHK(idx->vis_ids.reserve((size_t)nq * idx->p.P * sizeof(int32_t)));
-        HIPCHK(idx->vis_d.reserve((size_t)nq * idx->p.P * sizeof(float)));
-        HIPCHK(idx->vis_cnt.reserve((size_t)nq * sizeof(int32_t)));
-        a.vis_ids = idx->vis_ids.as<int32_t>();
-        a.vis_d = idx->vis_d.as<float>();
-        a.vis_cnt = idx->vis_cnt.as<int32_t>();
-    }
     a.P = idx->p.P;
-    a.stat_hops = stat_hops.as<int32_t>();
-    a.stat_ndist = stat_ndist.as<int32_t>();
-    a.counter = counter.as<unsigned long long>();
     a.prec = prec;
     a.krr = krr;
     if (prec == VMN_PREC_BF16) {
@@ -234,8 +215,38 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
     // BF16 row norms are read from the table (an extra 4-byte load per
     // candidate) instead of being summed from the row registers
     a.table_norms = nq <= (int64_t)idx->cus * 4;
-    const int64_t want = std::min<int64_t>(nq, (int64_t)idx->cus * idx->waves_per_cu);
-    HIPCHK(vmn::launch_search(a, (int)std::max<int64_t>(want, 1), stream));
+}
+
+int search_grid(const vmn_index* idx, int64_t nq) {
+    return (int)std::max<int64_t>(std::min<int64_t>(nq, (int64_t)idx->cus * idx->waves_per_cu), 1);
+}
+
+int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int32_t* out_ids,
+               float* out_d, bool queries_are_nodes, int mode, bool want_vis, int64_t node_base,
+               int prec, int krr, vmn_ctx* ctx) {
+    hipStream_t stream = ctx ? ctx->stream : idx->stream;
+    DevBuf& counter = ctx ? ctx->counter : idx->counter;
+    DevBuf& stat_hops = ctx ? ctx->stat_hops : idx->stat_hops;
+    DevBuf& stat_ndist = ctx ? ctx->stat_ndist : idx->stat_ndist;
+    HIPCHK(counter.reserve(64));
+    HIPCHK(vmn::launch_fill_u64(counter.as<unsigned long long>(), 1, 0ull, stream));
+    HIPCHK(stat_hops.reserve((size_t)nq * sizeof(int32_t)));
+    HIPCHK(stat_ndist.reserve((size_t)nq * sizeof(int32_t)));
+    vmn::SearchArgs a;
+    search_args(idx, a, q, nq, L, out_n, out_ids, out_d, queries_are_nodes, mode, node_base, prec,
+                krr);
+    if (want_vis) {
+        HIPCHK(idx->vis_ids.reserve((size_t)nq * idx->p.P * sizeof(int32_t)));
+        HIPCHK(idx->vis_d.reserve((size_t)nq * idx->p.P * sizeof(float)));
+        HIPCHK(idx->vis_cnt.reserve((size_t)nq * sizeof(int32_t)));
+        a.vis_ids = idx->vis_ids.as<int32_t>();
+        a.vis_d = idx->vis_d.as<float>();
+        a.vis_cnt = idx->vis_cnt.as<int32_t>();
+    }
+    a.stat_hops = stat_hops.as<int32_t>();
+    a.stat_ndist = stat_ndist.as<int32_t>();
+    a.counter = counter.as<unsigned long long>();
+    HIPCHK(vmn::launch_search(a, search_grid(idx, nq), stream));
     if (ctx) ctx->last_nq = nq;
     else idx->last_nq = nq;
     return VMN_OK;

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/vamana_mi355x.h"
+#include "vmn_kernels.hpp"
 
 namespace vmn_rt {
 
@@ -159,6 +160,13 @@ int apply_staged(vmn_index* idx);
 // with the staged rows merged in.
 int search_into(vmn_index* idx, const float* q, int64_t nq, int k, int L, int prec, int krr,
                 int32_t* ri, float* rd);
+// Search kernel arguments for `nq` padded query rows [nq x DP] from
+// idx->medoid, without the work counter, statistics and visited-set buffers
+// (left null), and the persistent grid for nq queries.
+void search_args(const vmn_index* idx, vmn::SearchArgs& a, const float* q, int64_t nq, int L,
+                 int out_n, int32_t* out_ids, float* out_d, bool queries_are_nodes, int mode,
+                 int64_t node_base, int prec, int krr);
+int search_grid(const vmn_index* idx, int64_t nq);
 // Search kernel over `nq` padded query rows [nq x DP] from idx->medoid.
 int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int32_t* out_ids,
                float* out_d, bool queries_are_nodes, int mode, bool want_vis,

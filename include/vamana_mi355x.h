@@ -198,6 +198,22 @@ int vmn_search_ex(vmn_index* idx, const float* queries, int64_t nq, int32_t k, i
                   int32_t prec, int32_t k_rerank, int32_t* ids_out, float* dists_out,
                   int32_t mem);
 
+/* vmn_search_ex on device buffers that allocates nothing: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * queries [nq x dim], ids_out [nq x k] and dists_out [nq x k] (may be NULL)
+ * are device pointers on the index's device; the call is asynchronous on
+ * vmn_stream(). Same results as vmn_search_ex. Its scratch (the work-queue
+ * counter and, when dim is not a multiple of 32, the zero-padded queries) is
+ * one static 4 MiB device array per GPU, held until the process ends: with
+ * padding, nq * padded_dim * 4 must fit in 4 MiB - 256 bytes (8,191 queries
+ * at dim 100), else VMN_ERR_CAPACITY. Every call on one GPU shares that
+ * array, so calls whose kernels overlap on one GPU (two streams, two indexes
+ * on one device) race: serialise them. Refused (VMN_ERR_STATE) while staged
+ * inserts are pending. Search statistics (vmn_search_stats) are not kept. */
+int vmn_search_noalloc(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int32_t L,
+                       int32_t prec, int32_t k_rerank, int32_t* ids_out, float* dists_out);
+
 /* vmn_search returning the k nearest rows themselves, vecs_out [nq x k x dim]
  * (zeros past the results found), like `searchPoints2` (vamana.cuh:127,
  * vamana_impl.cuh:99-180), which returns the worklist's vectors. */
