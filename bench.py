@@ -54,16 +54,19 @@ def parse():
     ap.add_argument("--sweep", default="10,20,40,50,60,70,80,100,120,150,200")
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--rerank", type=int, default=20,
-                    help="FP32 re-rank depth of the BF16 form (2k; BF16 rounding is small)")
+    ap.add_argument("--rerank", type=int, default=40,
+                    help="FP32 re-rank depth of the BF16 form (the top 4k = 40 candidates)")
     ap.add_argument("--rerank-fp8", type=int, default=40,
                     help="FP32 re-rank depth of the FP8 form (deeper: its distance error is larger)")
     ap.add_argument("--workload", default="search",
-                    choices=["search", "streaming", "serving", "copy", "engine"],
+                    choices=["search", "streaming", "serving", "copy", "gather", "engine"],
                     help="search: the headline metric; streaming: 10 rounds of 5%% deletes + "
                          "5%% inserts with consolidation, recall@10 per round; serving: the "
                          "continuous-batching request loop; copy: a 1 GiB device-to-device copy "
-                         "(the calibration kernel for the HBM traffic counters); engine: the "
+                         "(a calibration kernel for the HBM traffic counters); gather: the "
+                         "library's random-row gather probe over a 4 GiB table, 128 / 256 / "
+                         "512-byte rows, each row read once (the calibration for random row "
+                         "gathers); engine: the "
                          "parth engine's replay of its own SIFT workload file (batches of 50 "
                          "mixed operations), the reference's qps / recall_10 metric")
     ap.add_argument("--engine-reps", type=int, default=5)
@@ -73,6 +76,8 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=50_000.0, help="offered queries/s")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-L", type=int, default=64)
+    ap.add_argument("--serve-consolidate", type=int, default=20_000,
+                    help="serving: consolidate the tombstones after this many deletes")
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
 
@@ -93,6 +98,8 @@ def main():
         return serving(args, vmn, torch, dev)
     if args.workload == "copy":
         return copy_calibration(args, torch, dev)
+    if args.workload == "gather":
+        return gather_calibration(args, vmn, torch, dev)
     if args.workload == "engine":
         return engine_replay(args, vmn, torch)
     n, nq, d = args.n, args.nq, 128
@@ -188,15 +195,23 @@ def main():
         e1.synchronize()
         wall_serial = time.time() - t
         kern_ms = e0.elapsed_time(e1) / args.steps
-        # the timed steps: batches alternate between the two contexts
+        # the timed steps: batches alternate between the two contexts; HIP
+        # events on each context's stream bracket every launch, so each
+        # launch's duration is measured where it runs (overlapping the
+        # neighbouring launches on the other stream)
         for s_ in range(args.warmup):
             ctx[s_ % 2].replay()
         torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(args.steps)]
         t = time.time()
         for s_ in range(args.steps):
+            evs[s_][0].record(cst[s_ % 2])
             ctx[s_ % 2].replay()
+            evs[s_][1].record(cst[s_ % 2])
         torch.cuda.synchronize()
         wall = time.time() - t
+        launch_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
         rec = vmn.recall_at_k(ids.cpu().numpy(), gt, 10)
         rec2 = vmn.recall_at_k(ids2.cpu().numpy(), gt2, 10)
         kernel = idx.search_kernel_name(k=10, L=L, prec=prec, list_cap=tune["list_cap"])
@@ -209,14 +224,18 @@ def main():
                        "target_met": chosen[form][1],
                        "qps": round(nq * args.steps / wall, 1), "ms_per_step": wall * 1e3 / args.steps,
                        "qps_serial": round(nq * args.steps / wall_serial, 1),
-                       "kernel_ms": kern_ms, "bytes_per_launch": int(bytes_launch),
-                       "achieved_GBs": bytes_launch / (kern_ms / 1e3) / 1e9,
+                       "launch_ms": launch_ms, "bytes_per_launch": int(bytes_launch),
+                       "achieved_GBs": bytes_launch / (launch_ms / 1e3) / 1e9,
+                       "kernel_ms_serial": kern_ms,
+                       "achieved_GBs_serial": bytes_launch / (kern_ms / 1e3) / 1e9,
                        "launch": {"waves_per_cu": tune["waves_per_cu"], "list_cap": tune["list_cap"]},
                        "kernel": kernel, "rerank": rr,
                        "traffic": measured_traffic(form, n, nq, L, kernel, rr),
+                       "counters": measured_counters(form, n, nq, L, kernel, rr, launch_ms),
                        "hops_per_query": round(hops / nq, 1), "dists_per_query": round(nd / nq, 1)}
         log(f"timed {form:12s} L={L} {timed[form]['qps']:,.0f} QPS (serial {timed[form]['qps_serial']:,.0f})"
-            f" recall {rec:.4f}/{rec2:.4f} kernel {kern_ms:.3f} ms {timed[form]['achieved_GBs']:.0f} GB/s"
+            f" recall {rec:.4f}/{rec2:.4f} launch {launch_ms:.3f} ms overlapped, {kern_ms:.3f} ms"
+            f" serial, {timed[form]['achieved_GBs']:.0f} GB/s"
             f" (launch {tune['waves_per_cu']} waves/CU, cap {tune['list_cap']})")
     for c in ctx:
         c.close()
@@ -273,15 +292,34 @@ def main():
             "batches": f"K batches of {nq:,} queries alternating over 2 search contexts "
                        "(query sets: mixture streams 1 and 2)",
             "distance_error_vs_fp64": err, "sweep": sweeps},
-        "roofline": {"bound": "hbm", "achieved": round(head["achieved_GBs"], 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(head["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": head["traffic"],
-                     "kernel_ms": round(head["kernel_ms"], 4),
-                     "bytes_per_launch": head["bytes_per_launch"], "kernel": head["kernel"],
-                     "launch": head["launch"]},
+        "roofline": roofline(head),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def roofline(e):
+    """Roofline of one form's search kernel over the timed (overlapped) steps.
+    achieved = algorithmic bytes per launch / mean launch duration (HIP events
+    on the launch's own stream, neighbouring launches overlapping it);
+    traffic = counter HBM bytes per launch from the committed rocprofv3 passes
+    (when they cover this workload), and with it the HBM rate of the timed
+    steps, traffic / ms_per_step. The one-batch-at-a-time figures are under
+    "serial". Counter ratios measured for this kernel (L2 hit rate, MFMA busy)
+    come from the same committed profile."""
+    r = {"bound": "hbm", "achieved": round(e["achieved_GBs"], 1), "peak": HBM_PEAK_GBS,
+         "unit": "GB/s", "frac": round(e["achieved_GBs"] / HBM_PEAK_GBS, 4),
+         "traffic": e["traffic"], "kernel": e["kernel"], "launch": e["launch"],
+         "launch_ms": round(e["launch_ms"], 4), "bytes_per_launch": e["bytes_per_launch"]}
+    if e["traffic"]:
+        hbm = e["traffic"] / (e["ms_per_step"] / 1e3) / 1e9
+        r["hbm_GBs_per_step"] = round(hbm, 1)
+        r["hbm_frac_per_step"] = round(hbm / HBM_PEAK_GBS, 4)
+    r.update(e.get("counters") or {})
+    r["serial"] = {"kernel_ms": round(e["kernel_ms_serial"], 4),
+                   "achieved": round(e["achieved_GBs_serial"], 1),
+                   "frac": round(e["achieved_GBs_serial"] / HBM_PEAK_GBS, 4)}
+    return r
 
 
 def copy_calibration(args, torch, dev):
@@ -308,6 +346,51 @@ def copy_calibration(args, torch, dev):
                       "vs_baseline": None, "dtype": "f32", "data": "synthetic",
                       "config": {"workload": "copy 1 GiB device to device", "bytes": nbytes}}),
           flush=True)
+
+
+def gather_calibration(args, vmn, torch, dev):
+    """Random-row gathers of known size (vmn_gather_probe): 4 GiB table,
+    rows of 128, 256 and 512 bytes, 2^22 distinct rows per dispatch drawn from
+    a seeded permutation. FETCH_SIZE per dispatch against n_reads x row_bytes
+    + 4 n_reads gives the counter scale for the search kernels' access
+    pattern; the line reports the rows/s and bytes/s the probe reaches."""
+    import ctypes
+    L = vmn.lib()
+    table_bytes = 4 << 30
+    table = torch.zeros(table_bytes // 4, dtype=torch.int32, device=dev)
+    sink = torch.empty(1 << 22, dtype=torch.int32, device=dev)
+    g = torch.Generator(device="cpu").manual_seed(77)
+    n_reads = 1 << 22
+    out = {}
+    for rb in (128, 256, 512):
+        rows = table_bytes // rb
+        ids = torch.randperm(rows, generator=g)[:n_reads].to(torch.int32).to(dev)
+        torch.cuda.synchronize()
+        ms = ctypes.c_double()
+        times = []
+        for s_ in range(args.warmup + args.steps):
+            rc = L.vmn_gather_probe(ctypes.c_void_p(table.data_ptr()), rows, rb,
+                                    ctypes.c_void_p(ids.data_ptr()), n_reads,
+                                    ctypes.c_void_p(sink.data_ptr()), sink.numel(), 0,
+                                    ctypes.byref(ms))
+            if rc != 0:
+                raise vmn.VamanaError(rc, L.vmn_last_error().decode())
+            if s_ >= args.warmup:
+                times.append(ms.value)
+        t = float(np.median(times))
+        known = n_reads * rb + 4 * n_reads
+        out[str(rb)] = {"n_reads": n_reads, "known_bytes": known, "ms": round(t, 4),
+                        "GB_per_s": round(known / t / 1e6, 1),
+                        "G_rows_per_s": round(n_reads / t / 1e6, 2)}
+        log(f"gather probe {rb} B rows: {t:.3f} ms, {known / t / 1e6:,.0f} GB/s")
+    print(json.dumps({"metric": "random-row gather bandwidth (4 GiB table, rows read once)",
+                      "value": out["256"]["GB_per_s"], "unit": "GB/s", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": out["256"]["ms"], "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+                      "config": {"workload": "vmn_gather_probe: 2^22 distinct random rows of "
+                                             "128 / 256 / 512 B from a 4 GiB table",
+                                 "rows": out}}), flush=True)
 
 
 def log_uniform_batches(rng, total, lo=1000, hi=50000):
@@ -494,7 +577,8 @@ def serving(args, vmn, torch, dev):
     (R=64, L=100, alpha 1.0 then 1.2), BF16 rows with FP32 re-rank. Requests
     arrive as a Poisson stream at the offered query rate: 90% searches (batch
     log-uniform 1..256, k=10, L=64), 5% inserts, 5% deletes (batch log-uniform
-    1..256). Latency is arrival to completion on the server clock."""
+    1..256); the tombstones are consolidated every --serve-consolidate deletes.
+    Latency is arrival to completion on the server clock."""
     n, d, nq_gt = args.serve_n, 96, 10_000
     rng = np.random.default_rng(99)
     log(f"serving: generating {n}x{d} unit mixture")
@@ -519,7 +603,8 @@ def serving(args, vmn, torch, dev):
         kinds.append(vmn.REQ_SEARCH if r < 0.9 else (vmn.REQ_INSERT if r < 0.95 else vmn.REQ_DELETE))
         sizes.append(int(np.exp(rng.uniform(0, np.log(256)))))
         t_arr.append(tcur)
-    srv = vmn.Server(idx, k=10, L=args.serve_L, prec="bf16", rerank=40, max_batch=8192, n_slots=4)
+    srv = vmn.Server(idx, k=10, L=args.serve_L, prec="bf16", rerank=40, max_batch=8192, n_slots=4,
+                     consolidate_every=args.serve_consolidate)
     # warm-up, untimed: searches, and inserts / deletes at the largest request
     # size, so every buffer the engine and the update path grow is allocated
     # before the trace starts
@@ -602,6 +687,9 @@ def serving(args, vmn, torch, dev):
               "insert_ms_mean": round(eng["insert_s"] / max(eng["inserts"], 1) * 1e3, 3),
               "delete_ms_mean": round(eng["delete_s"] / max(eng["deletes"], 1) * 1e3, 3),
               "max_update_ms": round(eng["max_update_s"] * 1e3, 3),
+              "consolidations": eng["consolidations"],
+              "consolidate_every_deletes": args.serve_consolidate,
+              "failed_batches": eng["failed_batches"],
               "update_busy_frac": round((eng["insert_s"] + eng["delete_s"]) / (t_end - start), 4),
               "drain_frac": round(eng["drain_s"] / (t_end - start), 4)}
     log(f"serving: engine {engine}; update latency {upd}")
@@ -651,12 +739,10 @@ def serving(args, vmn, torch, dev):
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(form, n, nq, L, kernel, rerank):
-    """HBM bytes per launch of the search kernel for `form`, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (each scaled by what the counter
-    reads for a 1 GiB copy), when that profile covers this exact workload
-    (same n, nq, precision, L and re-rank depth; the beam-list capacity of the
-    instance does not change the traversal); otherwise None."""
+def _profile_entry(form, n, nq, L, kernel, rerank):
+    """The committed counter profile's entry for `form` when it covers this
+    exact workload (same n, nq, precision, L and re-rank depth; the beam-list
+    capacity of the kernel instance does not change the traversal), else None."""
     path = os.path.join(ROOT, "profiles", "search_1m_hbm_traffic.json")
     try:
         with open(path) as f:
@@ -668,7 +754,33 @@ def measured_traffic(form, n, nq, L, kernel, rerank):
     if not e or prof.get("n") != n or prof.get("nq") != nq or e.get("L") != L \
             or e.get("rerank") != rerank or not same(e.get("kernel", ""), kernel):
         return None
-    return int(e["traffic_bytes"])
+    return e
+
+
+def measured_traffic(form, n, nq, L, kernel, rerank):
+    """HBM bytes per launch of the search kernel for `form`, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by what the counters read
+    for random row gathers of known size), when that profile covers this exact
+    workload; otherwise None."""
+    e = _profile_entry(form, n, nq, L, kernel, rerank)
+    return None if e is None else int(e["traffic_bytes"])
+
+
+def measured_counters(form, n, nq, L, kernel, rerank, launch_ms):
+    """L2 hit rate and MFMA busy fraction of the search kernel for `form`, and
+    the MFMA rate its counted MFMA operations give over the measured launch
+    time (against the dense peak of its precision), from the same committed
+    profile; {} when the profile does not cover this workload."""
+    e = _profile_entry(form, n, nq, L, kernel, rerank)
+    if e is None or "l2_hit_rate" not in e:
+        return {}
+    out = {"l2_hit_rate": e["l2_hit_rate"], "mfma_busy": e["mfma_busy"]}
+    peak = {"bf16+rerank": 2500.0, "fp8+rerank": 5000.0}.get(form)
+    if peak and e.get("mfma_flop"):
+        tf = e["mfma_flop"] / (launch_ms / 1e3) / 1e12
+        out.update({"mfma_tflops": round(tf, 1), "mfma_peak_tflops": peak,
+                    "mfma_frac": round(tf / peak, 4)})
+    return out
 
 
 def cpu_model():

@@ -372,18 +372,32 @@ hipError_t launch_bf16_exact(const float* vecs, int64_t from, int64_t to, int DP
     return hipGetLastError();
 }
 
+// BF16 rows and their squared norms. The norm is summed in the order the
+// search kernel sums it from the row registers (gather_dists_low): four
+// chains, one per 8-element slice g of each 32-element k-block, each a
+// v_dot2_f32_bf16 chain over the element pairs (k-blocks ascending, pairs
+// ascending), then (c0 + c1) + (c2 + c3). So a distance read with the table's
+// norm (small batches) equals the one summed in registers (large batches)
+// bit for bit, and a query's result does not depend on its batch's size.
 __global__ void to_bf16_kernel(const float* __restrict__ vecs, int64_t n, int DP, __bf16* rows,
                                float* norms) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float s = 0.0f;
-    for (int k = 0; k < DP; ++k) {
-        const __bf16 h = to_bf16(vecs[i * DP + k]);
-        rows[i * DP + k] = h;
-        const float f = (float)h;
-        s += f * f;
+    float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int kb = 0; kb < DP / 32; ++kb) {
+        for (int g = 0; g < 4; ++g) {
+            for (int e = 0; e < 4; ++e) {
+                const int k = 32 * kb + 8 * g + 2 * e;
+                const __bf16 h0 = to_bf16(vecs[i * DP + k]);
+                const __bf16 h1 = to_bf16(vecs[i * DP + k + 1]);
+                rows[i * DP + k] = h0;
+                rows[i * DP + k + 1] = h1;
+                const bf16x2_t pr = {h0, h1};
+                c[g] = __builtin_amdgcn_fdot2_f32_bf16(pr, pr, c[g], false);
+            }
+        }
     }
-    norms[i] = s;
+    norms[i] = (c[0] + c[1]) + (c[2] + c[3]);
 }
 hipError_t launch_to_bf16(const float* vecs, int64_t n, int DP, void* rows, float* norms,
                           hipStream_t s) {

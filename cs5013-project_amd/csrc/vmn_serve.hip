@@ -107,7 +107,20 @@ struct vmn_server {
 
     void complete_slots() {
         for (Slot& s : slots) {
-            if (!s.busy || hipEventQuery(s.ev) != hipSuccess) continue;
+            if (!s.busy) continue;
+            const hipError_t q = hipEventQuery(s.ev);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) {
+                // a runtime error: the batch's requests fail with it and the
+                // slot is freed, so later requests and shutdown still proceed
+                const std::string msg = std::string("search batch: ") + hipGetErrorString(q);
+                for (Request* r : s.reqs) finish(r, VMN_ERR_HIP, msg.c_str());
+                s.reqs.clear();
+                s.busy = false;
+                std::lock_guard<std::mutex> g(m);
+                counters.failed_batches++;
+                continue;
+            }
             int64_t off = 0;
             for (Request* r : s.reqs) {
                 const int64_t cnt = r->n * p.k;

@@ -49,7 +49,8 @@ class ServerStats(ctypes.Structure):
                 ("max_batch_queries", ctypes.c_int64), ("inserts", ctypes.c_int64),
                 ("deletes", ctypes.c_int64), ("consolidations", ctypes.c_int64),
                 ("insert_s", ctypes.c_double), ("delete_s", ctypes.c_double),
-                ("max_update_s", ctypes.c_double), ("drain_s", ctypes.c_double)]
+                ("max_update_s", ctypes.c_double), ("drain_s", ctypes.c_double),
+                ("failed_batches", ctypes.c_int64)]
 
 
 class EngineResult(ctypes.Structure):
@@ -101,6 +102,7 @@ def lib():
         L.vmn_search_ex.argtypes = [P, P, i64, i32, i32, i32, i32, P, P, i32]
         L.vmn_search_vectors.argtypes = [P, P, i64, i32, i32, P, i32]
         L.vmn_search_noalloc.argtypes = [P, P, i64, i32, i32, i32, i32, P, P]
+        L.vmn_gather_probe.argtypes = [P, i64, i32, P, i64, P, i64, i32, P]
         L.vmn_get_lowp_rows.argtypes = [P, i32, P]
         L.vmn_get_adjacency.argtypes = [P, P, i32]
         L.vmn_size.argtypes = [P]

@@ -375,6 +375,8 @@ typedef struct vmn_server_stats {
     double insert_s, delete_s;                   /* engine time applying them (incl. consolidation) */
     double max_update_s;                         /* longest single update */
     double drain_s;                              /* time updates waited for launched searches */
+    int64_t failed_batches;                      /* search launches whose requests failed with a
+                                                    runtime error (VMN_ERR_HIP) */
 } vmn_server_stats;
 int vmn_server_get_stats(const vmn_server* srv, vmn_server_stats* out);
 /* Zeroes the counters (e.g. after a warm-up), maxima included. */
@@ -464,6 +466,18 @@ typedef struct vmn_engine_result {
 int vmn_engine_run(vmn_index* idx, const int32_t* types, const int32_t* ids, const float* vecs,
                    int64_t n_ops, int32_t batch_size, int32_t L, int32_t link_L,
                    vmn_engine_result* result, int32_t* found_out, int32_t* gt_out);
+
+/* Measurement helper, no counterpart in the reference: reads rows ids[0 ..
+ * n_reads) of `row_bytes` (128, 256 or 512) bytes each from `table` (device,
+ * table_rows rows; every id must be below table_rows) the way the search
+ * kernels gather candidate rows, and writes one word per lane to `sink`
+ * (device, sink_words >= 256). Synchronous on the null stream; *ms_out gets
+ * the kernel time. With distinct ids the bytes it must fetch are known
+ * (n_reads x row_bytes + 4 n_reads), which calibrates the HBM traffic
+ * counters for random row gathers. */
+int vmn_gather_probe(const void* table, int64_t table_rows, int32_t row_bytes, const int32_t* ids,
+                     int64_t n_reads, uint32_t* sink, int64_t sink_words, int32_t device,
+                     double* ms_out);
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* vmn_last_error(void);

@@ -42,10 +42,33 @@ def test_traffic_lookup_matches_its_workload_only(bench):
         assert bench.measured_traffic(form, n, nq, L, k.replace("<128,", "<96,"), rr) is None
     # the re-rank depths recorded are the bench defaults of each form
     assert {f: e["rerank"] for f, e in prof["forms"].items()} == {
-        "fp32": 0, "bf16+rerank": 20, "fp8+rerank": 40}
+        "fp32": 0, "bf16+rerank": 40, "fp8+rerank": 40}
     sys.argv, argv = ["bench.py"], sys.argv
     try:
         a = bench.parse()
     finally:
         sys.argv = argv
-    assert (a.rerank, a.rerank_fp8) == (20, 40)
+    assert (a.rerank, a.rerank_fp8) == (40, 40)
+
+
+def test_roofline_is_stated_on_the_timed_steps(bench):
+    """The roofline's `achieved` divides the algorithmic bytes by the mean
+    launch duration measured in the overlapped timed steps; the counter HBM
+    rate is traffic over ms_per_step; the one-batch-at-a-time figures sit
+    under `serial`; the counter ratios come from the committed profile."""
+    with open(os.path.join(ROOT, "profiles", "search_1m_hbm_traffic.json")) as f:
+        prof = json.load(f)
+    e = prof["forms"]["bf16+rerank"]
+    t = {"achieved_GBs": 4000.0, "traffic": e["traffic_bytes"], "kernel": e["kernel"],
+         "launch": {"waves_per_cu": 32, "list_cap": 0}, "launch_ms": 1.6,
+         "bytes_per_launch": e["algorithmic_bytes"], "ms_per_step": 0.85,
+         "kernel_ms_serial": 1.25, "achieved_GBs_serial": 6000.0,
+         "counters": bench.measured_counters("bf16+rerank", prof["n"], prof["nq"], e["L"],
+                                             e["kernel"], e["rerank"], 1.6)}
+    r = bench.roofline(t)
+    assert r["frac"] == 0.5 and r["serial"]["frac"] == 0.75
+    assert abs(r["hbm_GBs_per_step"] - e["traffic_bytes"] / 0.85e-3 / 1e9) < 0.1
+    assert r["l2_hit_rate"] == e["l2_hit_rate"] and r["mfma_busy"] == e["mfma_busy"]
+    assert abs(r["mfma_tflops"] - e["mfma_flop"] / 1.6e-3 / 1e12) < 0.1
+    assert bench.measured_counters("bf16+rerank", prof["n"], prof["nq"], e["L"] + 10,
+                                   e["kernel"], e["rerank"], 1.6) == {}

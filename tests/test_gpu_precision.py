@@ -129,3 +129,19 @@ def test_lowp_rerank_recall_on_real_valued_data(vmn, orc, gpu):
             # FP32 sums of 96 terms: ~1e-6 rounding; BF16 distances are off by ~1e-3
             assert np.allclose(rd, exact, rtol=1e-4, atol=1e-6), np.abs(rd - exact).max()
             assert (np.diff(rd, axis=1) >= 0).all()
+
+
+def test_bf16_results_do_not_depend_on_batch_size(vmn, gpu):
+    """On real-valued rows the BF16 distance depends on how the row norm is
+    summed. Small batches read it from the norms table, large ones sum it in
+    registers; the table is summed in the registers' order, so a query's ids
+    and distances are the same in a batch of 64 as in a batch of 3,000."""
+    base = vmn.datasets.unit_mixture(20000, 96, k=256, sigma=0.1, seed=29, stream=0)
+    q = vmn.datasets.unit_mixture(3000, 96, k=256, sigma=0.1, seed=29, stream=1)
+    idx = vmn.Vamana(base, mode="fresh", R=32, L_build=64, alphas=(1.0, 1.2),
+                     low_precision="bf16")
+    for rr in (0, 40):
+        big_i, big_d = idx.searchPoints(q, k=10, L=64, prec="bf16", rerank=rr)
+        for s in range(0, 3000, 64):
+            i, d = idx.searchPoints(q[s:s + 64], k=10, L=64, prec="bf16", rerank=rr)
+            assert np.array_equal(i, big_i[s:s + 64]) and np.array_equal(d, big_d[s:s + 64]), (rr, s)
