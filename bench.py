@@ -69,6 +69,8 @@ def parse():
                          "gathers); engine: the "
                          "parth engine's replay of its own SIFT workload file (batches of 50 "
                          "mixed operations), the reference's qps / recall_10 metric")
+    ap.add_argument("--contexts", type=int, default=2,
+                    help="search contexts (streams) the timed batches are issued round-robin over")
     ap.add_argument("--engine-reps", type=int, default=5)
     ap.add_argument("--rounds", type=int, default=10)
     ap.add_argument("--L-eval", type=int, default=60)
@@ -169,18 +171,26 @@ def main():
     # timed steps: every form at its chosen L. The launch shape is autotuned;
     # each context captures its batch as a HIP graph and replays it.
     timed = {}
-    ctx = [vmn.SearchContext(idx), vmn.SearchContext(idx)]
+    nctx = max(2, args.contexts)
+    ctx = [vmn.SearchContext(idx) for _ in range(nctx)]
     cst = [torch.cuda.ExternalStream(c.stream) for c in ctx]
     ids2 = torch.empty((nq, 10), dtype=torch.int32, device=dev)
     dists2 = torch.empty((nq, 10), dtype=torch.float32, device=dev)
+    # contexts past the first two search the two query sets again, into
+    # buffers of their own
+    more = [(torch.empty((nq, 10), dtype=torch.int32, device=dev),
+             torch.empty((nq, 10), dtype=torch.float32, device=dev)) for _ in range(nctx - 2)]
     for form in forms:
         L = chosen[form][0]["L"]
         prec, rr = forms[form]
         rr = min(rr, L)
         tune = vmn.tuning.launch_shape(idx, q_t, k=10, L=L, prec=prec, rerank=rr, reps=12,
-                                       overlap=True)
+                                       overlap=True, contexts=nctx)
         ctx[0].capture(q_t, ids, dists, k=10, L=L, prec=prec, rerank=rr)
         ctx[1].capture(q2_t, ids2, dists2, k=10, L=L, prec=prec, rerank=rr)
+        for c_, (i_, d_) in enumerate(more):
+            ctx[2 + c_].capture(q_t if c_ % 2 == 0 else q2_t, i_, d_, k=10, L=L, prec=prec,
+                                rerank=rr)
         # one batch at a time on one context: per-launch kernel time
         for _ in range(args.warmup):
             ctx[0].replay()
@@ -199,16 +209,16 @@ def main():
         # events on each context's stream bracket every launch, so each
         # launch's duration is measured where it runs (overlapping the
         # neighbouring launches on the other stream)
-        for s_ in range(args.warmup):
-            ctx[s_ % 2].replay()
+        for s_ in range(max(args.warmup, nctx)):
+            ctx[s_ % nctx].replay()
         torch.cuda.synchronize()
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                for _ in range(args.steps)]
         t = time.time()
         for s_ in range(args.steps):
-            evs[s_][0].record(cst[s_ % 2])
-            ctx[s_ % 2].replay()
-            evs[s_][1].record(cst[s_ % 2])
+            evs[s_][0].record(cst[s_ % nctx])
+            ctx[s_ % nctx].replay()
+            evs[s_][1].record(cst[s_ % nctx])
         torch.cuda.synchronize()
         wall = time.time() - t
         launch_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
@@ -289,8 +299,8 @@ def main():
             "bf16_vs_fp32_qps_serial": round(head["qps_serial"] / timed["fp32"]["qps_serial"], 3),
             "qps_serial": head["qps_serial"], "recall@10_set2": head["recall@10_set2"],
             "qps_host_buffers": round(nq / host_ms * 1e3, 1),
-            "batches": f"K batches of {nq:,} queries alternating over 2 search contexts "
-                       "(query sets: mixture streams 1 and 2)",
+            "batches": f"K batches of {nq:,} queries issued round-robin over {nctx} search "
+                       "contexts (query sets: mixture streams 1 and 2, alternating)",
             "distance_error_vs_fp64": err, "sweep": sweeps},
         "roofline": roofline(head),
         "cpu_baseline": cpu,
@@ -366,30 +376,34 @@ def gather_calibration(args, vmn, torch, dev):
         rows = table_bytes // rb
         ids = torch.randperm(rows, generator=g)[:n_reads].to(torch.int32).to(dev)
         torch.cuda.synchronize()
-        ms = ctypes.c_double()
-        times = []
-        for s_ in range(args.warmup + args.steps):
-            rc = L.vmn_gather_probe(ctypes.c_void_p(table.data_ptr()), rows, rb,
-                                    ctypes.c_void_p(ids.data_ptr()), n_reads,
-                                    ctypes.c_void_p(sink.data_ptr()), sink.numel(), 0,
-                                    ctypes.byref(ms))
-            if rc != 0:
-                raise vmn.VamanaError(rc, L.vmn_last_error().decode())
-            if s_ >= args.warmup:
-                times.append(ms.value)
-        t = float(np.median(times))
-        known = n_reads * rb + 4 * n_reads
-        out[str(rb)] = {"n_reads": n_reads, "known_bytes": known, "ms": round(t, 4),
+        for split in (0, 1):
+            ms = ctypes.c_double()
+            times = []
+            for s_ in range(args.warmup + args.steps):
+                rc = L.vmn_gather_probe(ctypes.c_void_p(table.data_ptr()), rows, rb, split,
+                                        ctypes.c_void_p(ids.data_ptr()), n_reads,
+                                        ctypes.c_void_p(sink.data_ptr()), sink.numel(), 0,
+                                        ctypes.byref(ms))
+                if rc != 0:
+                    raise vmn.VamanaError(rc, L.vmn_last_error().decode())
+                if s_ >= args.warmup:
+                    times.append(ms.value)
+            t = float(np.median(times))
+            known = n_reads * rb + 4 * n_reads
+            key = str(rb) + ("_split64" if split else "")
+            out[key] = {"n_reads": n_reads, "known_bytes": known, "ms": round(t, 4),
                         "GB_per_s": round(known / t / 1e6, 1),
                         "G_rows_per_s": round(n_reads / t / 1e6, 2)}
-        log(f"gather probe {rb} B rows: {t:.3f} ms, {known / t / 1e6:,.0f} GB/s")
+            log(f"gather probe {key} B rows: {t:.3f} ms, {known / t / 1e6:,.0f} GB/s")
     print(json.dumps({"metric": "random-row gather bandwidth (4 GiB table, rows read once)",
                       "value": out["256"]["GB_per_s"], "unit": "GB/s", "n_gpus": 1,
                       "steps": args.steps, "warmup": args.warmup,
                       "ms_per_step": out["256"]["ms"], "higher_is_better": True,
                       "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
                       "config": {"workload": "vmn_gather_probe: 2^22 distinct random rows of "
-                                             "128 / 256 / 512 B from a 4 GiB table",
+                                             "128 / 256 / 512 B from a 4 GiB table, each row in "
+                                             "one load instruction, or 64 B of 16 rows per "
+                                             "instruction (_split64)",
                                  "rows": out}}), flush=True)
 
 

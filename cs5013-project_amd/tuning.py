@@ -5,9 +5,9 @@ truth reaches a target (the reference hard-codes L_g = 199,
 vamana/include/constants.cuh:10). launch_shape(): times the batched search
 under each persistent-grid size (waves per CU) and beam-list capacity template,
 keeps the fastest, and installs it on the index. It times either one batch at
-a time on one search context, or batches alternating over two contexts, so
-that each batch's drain overlaps the next batch's start (the throughput of a
-stream of batches).
+a time on one search context, or batches issued round-robin over several
+contexts, so that each batch's drain overlaps the next batches' start (the
+throughput of a stream of batches).
 """
 import time
 
@@ -31,18 +31,18 @@ def _time(ctx, q, ids, d, k, L, prec, rerank, reps):
 
 
 def _time_overlapped(ctxs, q, outs, k, L, prec, rerank, reps):
-    """Milliseconds per batch over `reps` batches alternating between the
+    """Milliseconds per batch over `reps` batches issued round-robin over the
     contexts, each replaying its captured search (host wall clock between
     device-wide synchronisations)."""
     import torch
     for c, (ids, d) in zip(ctxs, outs):
         c.capture(q, ids, d, k, L, prec, rerank)
-    for r in range(2):
+    for r in range(len(ctxs)):
         ctxs[r].replay()
     torch.cuda.synchronize()
     t = time.perf_counter()
     for r in range(reps):
-        ctxs[r % 2].replay()
+        ctxs[r % len(ctxs)].replay()
     torch.cuda.synchronize()
     return (time.perf_counter() - t) * 1e3 / reps
 
@@ -61,13 +61,14 @@ def beam_width(index, queries, gt_ids, target, widths, k=10, prec="fp32", rerank
 
 
 def launch_shape(index, queries, k=10, L=64, prec="fp32", rerank=0,
-                 waves=(12, 16, 20, 24, 32), caps=(0, 128, 256), reps=3, overlap=False):
+                 waves=(12, 16, 20, 24, 32), caps=(0, 128, 256), reps=3, overlap=False,
+                 contexts=2):
     """Fastest (waves_per_cu, list_cap) for this batch; installs it and
     returns {"waves_per_cu", "list_cap", "ms", "table"}. overlap=True times
-    batches alternating over two search contexts (ms per batch of the
-    stream) instead of one batch at a time."""
+    batches issued round-robin over `contexts` search contexts (ms per batch
+    of the stream) instead of one batch at a time."""
     import torch
-    nctx = 2 if overlap else 1
+    nctx = max(2, contexts) if overlap else 1
     ctxs = [SearchContext(index) for _ in range(nctx)]
     nq = int(queries.shape[0])
     outs = [(torch.empty((nq, k), dtype=torch.int32, device=queries.device),

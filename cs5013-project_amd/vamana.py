@@ -102,7 +102,7 @@ def lib():
         L.vmn_search_ex.argtypes = [P, P, i64, i32, i32, i32, i32, P, P, i32]
         L.vmn_search_vectors.argtypes = [P, P, i64, i32, i32, P, i32]
         L.vmn_search_noalloc.argtypes = [P, P, i64, i32, i32, i32, i32, P, P]
-        L.vmn_gather_probe.argtypes = [P, i64, i32, P, i64, P, i64, i32, P]
+        L.vmn_gather_probe.argtypes = [P, i64, i32, i32, P, i64, P, i64, i32, P]
         L.vmn_get_lowp_rows.argtypes = [P, i32, P]
         L.vmn_get_adjacency.argtypes = [P, P, i32]
         L.vmn_size.argtypes = [P]

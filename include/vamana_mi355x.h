@@ -470,14 +470,15 @@ int vmn_engine_run(vmn_index* idx, const int32_t* types, const int32_t* ids, con
 /* Measurement helper, no counterpart in the reference: reads rows ids[0 ..
  * n_reads) of `row_bytes` (128, 256 or 512) bytes each from `table` (device,
  * table_rows rows; every id must be below table_rows) the way the search
- * kernels gather candidate rows, and writes one word per lane to `sink`
- * (device, sink_words >= 256). Synchronous on the null stream; *ms_out gets
- * the kernel time. With distinct ids the bytes it must fetch are known
- * (n_reads x row_bytes + 4 n_reads), which calibrates the HBM traffic
- * counters for random row gathers. */
-int vmn_gather_probe(const void* table, int64_t table_rows, int32_t row_bytes, const int32_t* ids,
-                     int64_t n_reads, uint32_t* sink, int64_t sink_words, int32_t device,
-                     double* ms_out);
+ * kernels gather candidate rows -- each row in one load instruction
+ * (split = 0), or 64 bytes of 16 rows per instruction (split = 1) -- and
+ * writes one word per lane to `sink` (device, sink_words >= 256). Synchronous
+ * on the null stream; *ms_out gets the kernel time. With distinct ids the
+ * bytes it must fetch are known (n_reads x row_bytes + 4 n_reads), which
+ * calibrates the HBM traffic counters for random row gathers. */
+int vmn_gather_probe(const void* table, int64_t table_rows, int32_t row_bytes, int32_t split,
+                     const int32_t* ids, int64_t n_reads, uint32_t* sink, int64_t sink_words,
+                     int32_t device, double* ms_out);
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* vmn_last_error(void);
