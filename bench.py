@@ -227,8 +227,12 @@ def main():
         kernel = idx.search_kernel_name(k=10, L=L, prec=prec, list_cap=tune["list_cap"])
         idx.searchPoints(q_t, k=10, L=L, prec=prec, rerank=rr)  # same traversal, for the counters
         hops, nd = idx.search_stats()
-        row_b = {"fp32": d * 4, "bf16": d * 2, "fp8": d + 4}[prec]
-        bytes_launch = (d * 4) * nq + hops * 4 * args.R + nd * row_b + nq * rr * d * 4 \
+        # per query: the FP32 query row, each expansion's adjacency row (FP8:
+        # and the row norms stored beside it), each measured candidate's row,
+        # the re-ranked FP32 rows, the k results
+        row_b = {"fp32": d * 4, "bf16": d * 2, "fp8": d}[prec]
+        adj_b = 4 * args.R * (2 if prec == "fp8" else 1)
+        bytes_launch = (d * 4) * nq + hops * adj_b + nd * row_b + nq * rr * d * 4 \
             + nq * 10 * 8
         timed[form] = {"L": L, "recall@10": round(rec, 4), "recall@10_set2": round(rec2, 4),
                        "target_met": chosen[form][1],

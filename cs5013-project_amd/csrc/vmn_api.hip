@@ -91,6 +91,8 @@ int alloc_graph(vmn_index* idx, int64_t n) {
     HIPCHK(idx->adj.reserve((size_t)cap * idx->p.R * sizeof(int32_t)));
     HIPCHK(idx->del_bits.reserve((size_t)((cap + 31) / 32) * sizeof(uint32_t)));
     HIPCHK(vmn::launch_fill_i32(idx->del_bits.as<int32_t>(), (cap + 31) / 32, 0, idx->stream));
+    if (idx->p.low_precision & VMN_STORE_FP8)
+        HIPCHK(idx->adj_fp8n.reserve((size_t)cap * idx->p.R * sizeof(float)));
     return VMN_OK;
 }
 
@@ -207,6 +209,7 @@ void search_args(const vmn_index* idx, vmn::SearchArgs& a, const float* q, int64
     } else if (prec == VMN_PREC_FP8) {
         a.low_rows = idx->fp8_rows.p;
         a.low_norms = idx->fp8_norms.as<float>();
+        a.adj_norms = idx->adj_fp8n.as<float>();
         a.fp8_scales = idx->fp8_scales.as<float>();
         a.fp8_centre = idx->fp8_centre.as<float>();
     }
@@ -215,6 +218,20 @@ void search_args(const vmn_index* idx, vmn::SearchArgs& a, const float* q, int64
     // BF16 row norms are read from the table (an extra 4-byte load per
     // candidate) instead of being summed from the row registers
     a.table_norms = nq <= (int64_t)idx->cus * 4;
+}
+
+int prepare_search(vmn_index* idx, int prec, bool sync) {
+    if (prec != VMN_PREC_FP8 || idx->adj_fp8n_epoch == idx->epoch) return VMN_OK;
+    // allocated with the graph (alloc_graph) for indexes that keep FP8 rows
+    if (idx->adj_fp8n.bytes < (size_t)idx->cap * idx->p.R * sizeof(float))
+        return fail(VMN_ERR_STATE, "FP8 norms array missing (index keeps no FP8 rows)");
+    HIPCHK(vmn::launch_adj_norms(idx->adj.as<int32_t>(), idx->n * idx->p.R,
+                                 idx->fp8_norms.as<float>(), idx->adj_fp8n.as<float>(),
+                                 idx->stream));
+    // searches on other streams (contexts) read it next
+    if (sync) HIPCHK(hipStreamSynchronize(idx->stream));
+    idx->adj_fp8n_epoch = idx->epoch;
+    return VMN_OK;
 }
 
 int search_grid(const vmn_index* idx, int64_t nq) {
@@ -228,6 +245,8 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
     DevBuf& counter = ctx ? ctx->counter : idx->counter;
     DevBuf& stat_hops = ctx ? ctx->stat_hops : idx->stat_hops;
     DevBuf& stat_ndist = ctx ? ctx->stat_ndist : idx->stat_ndist;
+    int rc = prepare_search(idx, prec, true);
+    if (rc) return rc;
     HIPCHK(counter.reserve(64));
     HIPCHK(vmn::launch_fill_u64(counter.as<unsigned long long>(), 1, 0ull, stream));
     HIPCHK(stat_hops.reserve((size_t)nq * sizeof(int32_t)));

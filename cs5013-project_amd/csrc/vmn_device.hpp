@@ -287,7 +287,9 @@ __device__ __forceinline__ uint32_t bperm(int src_byte, uint32_t v) {
 // ids[0..cnt); dist_out[s] (LDS) receives the distance of ids[s]. Two tiles
 // of 16 candidates are loaded before either is multiplied, so one memory round
 // trip serves 32 candidates. FP8 row norms (per-dimension scales) come from
-// `norms`. BF16 row norms come from the loaded registers (v_dot2_f32_bf16),
+// `cand_norms[s]` (LDS, the norm of ids[s], loaded with the adjacency row the
+// ids came from) when given, else from the table `norms` (a cache line per
+// candidate). BF16 row norms come from the loaded registers (v_dot2_f32_bf16),
 // or, with TN, from `norms` as well: one 4-byte load per tile column issued
 // with the row loads. That saves the dot products and two cross-lane sums on a
 // latency-bound batch, but costs a cache line per candidate, which a
@@ -301,7 +303,8 @@ __device__ __forceinline__ uint32_t bperm(int src_byte, uint32_t v) {
 template <int DP, int PREC, int NG = 2, bool TN = false>
 __device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* __restrict__ rows,
                                                  const float* __restrict__ norms,
-                                                 const int32_t* ids, float* dist_out, int cnt) {
+                                                 const int32_t* ids, float* dist_out, int cnt,
+                                                 const float* cand_norms = nullptr) {
     const int l = lane(), g = l >> 4, rr = l >> 2, sub = l & 3;
     constexpr int KB = QLow<DP>::KB;
     for (int base = 0; base < cnt; base += 16 * NG) {
@@ -363,7 +366,8 @@ __device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* 
 #pragma unroll
                     for (int st = 0; st < NS; ++st) v[t][st] = r[4 * st + sub];
                     // norm of tile column L%16: its id sits in lane 4*(L%16)
-                    x[t] = norms[__builtin_amdgcn_ds_bpermute((4 * (l & 15)) << 2, id[t])];
+                    x[t] = cand_norms ? cand_norms[base + 16 * t + (l & 15)]
+                                      : norms[__builtin_amdgcn_ds_bpermute((4 * (l & 15)) << 2, id[t])];
                 }
             }
             const bool hi = g & 1;
@@ -392,7 +396,7 @@ __device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* 
                     const unsigned long* r = reinterpret_cast<const unsigned long*>(reinterpret_cast<const uint8_t*>(rows) + (int64_t)oid[t] * DP);
 #pragma unroll
                     for (int kb = 0; kb < KB; ++kb) v[t][kb] = r[4 * kb + g];
-                    x[t] = norms[oid[t]];
+                    x[t] = cand_norms ? cand_norms[base + 16 * t + col] : norms[oid[t]];
                 }
             }
 #pragma unroll

@@ -107,6 +107,10 @@ struct vmn_index {
     int64_t n_pend = 0;
     int waves_per_cu = 32;  // search grid = CUs x waves_per_cu (persistent work queue)
     int list_cap = 0;       // minimum beam-list capacity of the search kernel
+    // FP8 searches: row norms beside the adjacency entries ([cap x R], the
+    // norm of adj[i] at i), valid for index epoch adj_fp8n_epoch
+    vmn_rt::DevBuf adj_fp8n;
+    uint64_t adj_fp8n_epoch = ~0ull;
     // bumped by every change of the graph, tombstones or rows: a search
     // captured as a HIP graph holds the entry point, the tombstone pointer
     // and the buffers of its capture, so it may only be replayed while this
@@ -167,6 +171,10 @@ void search_args(const vmn_index* idx, vmn::SearchArgs& a, const float* q, int64
                  int out_n, int32_t* out_ids, float* out_d, bool queries_are_nodes, int mode,
                  int64_t node_base, int prec, int krr);
 int search_grid(const vmn_index* idx, int64_t nq);
+// Makes the buffers a search of precision `prec` reads current (FP8: the
+// adjacency-aligned row norms, rebuilt on the index stream when the index
+// changed; sync waits for it, for searches on other streams).
+int prepare_search(vmn_index* idx, int prec, bool sync);
 // Search kernel over `nq` padded query rows [nq x DP] from idx->medoid.
 int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int32_t* out_ids,
                float* out_d, bool queries_are_nodes, int mode, bool want_vis,

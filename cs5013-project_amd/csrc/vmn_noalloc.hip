@@ -80,6 +80,11 @@ extern "C" int vmn_search_noalloc(vmn_index* idx, const float* queries, int64_t 
     unsigned char* base = nullptr;
     rc = scratch_ptr(idx->p.device, &base);
     if (rc) return rc;
+    // FP8 reads row norms laid out beside the adjacency (allocated with the
+    // graph): after an index update they are refreshed on the index stream,
+    // ahead of this search
+    rc = prepare_search(idx, prec, false);
+    if (rc) return rc;
     unsigned long long* counter = reinterpret_cast<unsigned long long*>(base);
     const float* q = queries;
     if (pad) {

@@ -69,6 +69,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
     __shared__ __align__(16) float s_d[2][LCAP];
     __shared__ __align__(16) int32_t s_sid[64];
     __shared__ float s_sd[64];
+    // FP8: the candidates' row norms, from the adjacency-aligned norms
+    __shared__ float s_sn[PREC_ARG == VMN_PREC_FP8 ? 64 : 1];
     __shared__ uint32_t s_bloom[MODE == VMN_MODE_FRESH ? VMN_BF_WORDS : 1];
     __shared__ int64_t s_q;
 
@@ -172,8 +174,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
                     ent[k] = i < cnt ? (s_id[cur][i] & VMN_ID_MASK) : 0xffffffffu;
                 }
             }
-            // neighbours up to the first -1
+            // neighbours up to the first -1 (FP8: with their row norms)
             int32_t nb = l < W ? a.adj[(int64_t)node * W + l] : -1;
+            float nbn = 0.0f;
+            if constexpr (PREC == VMN_PREC_FP8)
+                nbn = l < W ? a.adj_norms[(int64_t)node * W + l] : 0.0f;
             const uint64_t endm = ballot(l < W && nb == -1);
             const int deg = endm ? __ffsll((unsigned long long)endm) - 1 : W;
             bool valid = l < deg;
@@ -229,15 +234,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
             const uint64_t vm = ballot(valid);
             const int nnew = __popcll((unsigned long long)vm);
             const int slot = __popcll((unsigned long long)(vm & lanemask_lt()));
-            if (valid) s_sid[slot] = nb;
+            if (valid) {
+                s_sid[slot] = nb;
+                if constexpr (PREC == VMN_PREC_FP8) s_sn[slot] = nbn;
+            }
             __syncthreads();
             if (PREC == VMN_PREC_FP32 && SEQ)
                 gather_dists_seq<DP>(qrow, vecs, s_sid, s_sd, nnew);
             else if (PREC == VMN_PREC_FP32)
                 gather_dists<DP>(q, vecs, s_sid, s_sd, nnew);
             else
-                gather_dists_low<DP, PREC, VMN_GATHER_TILES, TN>(ql, a.low_rows, a.low_norms, s_sid,
-                                                                 s_sd, nnew);
+                gather_dists_low<DP, PREC, VMN_GATHER_TILES, TN>(
+                    ql, a.low_rows, a.low_norms, s_sid, s_sd, nnew,
+                    PREC == VMN_PREC_FP8 ? (const float*)s_sn : nullptr);
             __syncthreads();
             ndist += nnew;
             // A full list keeps its entries on ties, so a candidate at or beyond
@@ -324,7 +333,23 @@ static hipError_t launch_search_d(const SearchArgs& a, int grid, hipStream_t s) 
     return launch_search_l<DP, 512>(a, grid, s);
 }
 
+__global__ void adj_norms_kernel(const int32_t* __restrict__ adj, int64_t entries,
+                                 const float* __restrict__ norms, float* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= entries) return;
+    const int32_t v = adj[i];
+    out[i] = v >= 0 ? norms[v] : 0.0f;
+}
+hipError_t launch_adj_norms(const int32_t* adj, int64_t entries, const float* norms, float* out,
+                            hipStream_t s) {
+    if (entries <= 0) return hipSuccess;
+    hipLaunchKernelGGL(adj_norms_kernel, dim3((unsigned)((entries + 255) / 256)), dim3(256), 0, s,
+                       adj, entries, norms, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_search(const SearchArgs& a, int grid, hipStream_t s) {
+    if (a.prec == VMN_PREC_FP8 && !a.adj_norms) return hipErrorInvalidValue;
     switch (a.DP) {
         case 32: return launch_search_d<32>(a, grid, s);
         case 64: return launch_search_d<64>(a, grid, s);

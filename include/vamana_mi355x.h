@@ -51,7 +51,9 @@ extern "C" {
                           vamana/include/kernels.cuh:40-52) */
 #define VMN_PREC_BF16 1 /* BF16 rows on MFMA, FP32 accumulation */
 #define VMN_PREC_FP8 2  /* FP8 E4M3 rows, centred on the per-dimension mean and scaled by the
-                          centred absmax, on MFMA */
+                          centred absmax, on MFMA; their squared norms are kept beside the
+                          adjacency ([capacity x R] floats, read with each adjacency row and
+                          laid out again on the index stream after an update) */
 
 /* vmn_params.low_precision bits: which low-precision copies the index keeps. */
 #define VMN_STORE_BF16 1

@@ -145,3 +145,40 @@ def test_bf16_results_do_not_depend_on_batch_size(vmn, gpu):
         for s in range(0, 3000, 64):
             i, d = idx.searchPoints(q[s:s + 64], k=10, L=64, prec="bf16", rerank=rr)
             assert np.array_equal(i, big_i[s:s + 64]) and np.array_equal(d, big_d[s:s + 64]), (rr, s)
+
+
+def test_fp8_searches_after_updates_equal_a_reloaded_index(vmn, gpu, tmp_path):
+    """FP8 searches read the row norms laid out beside the adjacency, which
+    every update makes stale: after inserts, deletes and a consolidation the
+    results equal those of the same index saved and loaded again (whose
+    norms array is laid out from scratch), in eager searches and in a search
+    captured again after the update."""
+    import torch
+    base = vmn.datasets.integer_mixture(8000, 128, k=32, seed=37, stream=0)
+    q = vmn.datasets.integer_mixture(500, 128, k=32, seed=37, stream=1)
+    idx = vmn.Vamana(base, mode="fresh", R=32, L_build=64, alphas=(1.0, 1.2),
+                     low_precision="fp8", capacity=9500)
+    before, _ = idx.searchPoints(q, k=10, L=48, prec="fp8", rerank=40)
+    idx.insertPoints(vmn.datasets.integer_mixture(1000, 128, k=32, seed=37, stream=2))
+    idx.deleteIds(np.arange(0, 8000, 7, dtype=np.int32))
+    idx.consolidate()
+    after, ad = idx.searchPoints(q, k=10, L=48, prec="fp8", rerank=40)
+    assert not np.array_equal(before, after)
+    path = str(tmp_path / "idx.vmn")
+    idx.save(path)
+    again = vmn.Vamana.load(path)
+    ri, rd = again.searchPoints(q, k=10, L=48, prec="fp8", rerank=40)
+    assert np.array_equal(after, ri) and np.array_equal(ad, rd)
+    ctx = vmn.SearchContext(idx)
+    qt = torch.from_numpy(q).to(gpu)
+    ids = torch.empty((500, 10), dtype=torch.int32, device=gpu)
+    d = torch.empty((500, 10), dtype=torch.float32, device=gpu)
+    ctx.capture(qt, ids, d, k=10, L=48, prec="fp8", rerank=40)
+    idx.insertPoints(vmn.datasets.integer_mixture(200, 128, k=32, seed=37, stream=3))
+    ctx.capture(qt, ids, d, k=10, L=48, prec="fp8", rerank=40)
+    torch.cuda.synchronize()
+    ctx.replay()
+    torch.cuda.ExternalStream(ctx.stream).synchronize()
+    e_ids, e_d = idx.searchPoints(q, k=10, L=48, prec="fp8", rerank=40)
+    assert np.array_equal(ids.cpu().numpy(), e_ids) and np.array_equal(d.cpu().numpy(), e_d)
+    ctx.close()

@@ -12,7 +12,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 # include/vamana_mi355x.h
-VMN_PREC_FP32, VMN_PREC_BF16, VMN_MEM_DEVICE = 0, 1, 1
+VMN_PREC_FP32, VMN_PREC_BF16, VMN_PREC_FP8, VMN_MEM_DEVICE = 0, 1, 2, 1
 
 
 def _ptr(t):
@@ -26,8 +26,13 @@ def test_noalloc_search_equals_search_ex(vmn, gpu, dim):
     base = vmn.datasets.integer_mixture(6000, dim, k=16, seed=41, stream=0)
     qs = vmn.datasets.integer_mixture(3000, dim, k=16, seed=41, stream=1)
     idx = vmn.Vamana(base, mode="fresh", R=32, L_build=48, alphas=(1.0, 1.2),
-                     low_precision="bf16")
-    for prec, rr in ((VMN_PREC_FP32, 0), (VMN_PREC_BF16, 20)):
+                     low_precision="both", capacity=7000)
+    forms = ((VMN_PREC_FP32, 0), (VMN_PREC_BF16, 20), (VMN_PREC_FP8, 40))
+    for step, (prec, rr) in enumerate(forms + ((VMN_PREC_FP8, 40),)):
+        if step == len(forms):
+            # an update: the FP8 norms kept beside the adjacency are laid out
+            # again by the next search, here the allocation-free one
+            idx.insertPoints(vmn.datasets.integer_mixture(500, dim, k=16, seed=41, stream=2))
         for nq in (3000, 700):
             print(f"noalloc search dim {dim} prec {prec} nq {nq}", flush=True)
             q = torch.from_numpy(qs[:nq]).to(gpu)
