@@ -100,7 +100,9 @@ def test_fp8_rerank_recall_and_exact_distances(vmn, orc, gpu):
     ri, rd = idx.searchPoints(q, k=10, L=64, prec="fp8", rerank=40)
     rf = vmn.recall_at_k(fi, gt)
     r8 = vmn.recall_at_k(ri, gt)
-    assert r8 >= rf - 0.05, (r8, rf)
+    print(f"recall@10 FP32 {rf:.4f}, FP8 + re-rank 40 {r8:.4f}", flush=True)
+    # stated tolerance: FP8 with the FP32 re-rank of 40 within 0.005 of FP32
+    assert r8 >= rf - 0.005, (r8, rf)
     # re-ranked distances are the exact FP32 distances of the returned ids
     exact = ((base[ri].astype(np.float64) - q[:, None, :]) ** 2).sum(-1).astype(np.float32)
     assert np.array_equal(rd, exact)

@@ -111,13 +111,28 @@ def test_gpu_stream_of_updates_matches_oracle(vmn, orc, gpu):
 
 
 def test_gpu_delete_by_vector(vmn, gpu):
+    """vmn_delete_vectors (deletePoints, findPointsKernel kernels.cuh:905-933):
+    each vector tombstones the lowest live id whose row equals it exactly, or
+    reports -1. Rows are duplicated here so that the lowest-id rule decides,
+    and each answer is checked against an exact-match scan over the live rows;
+    deleting the same vectors again takes the next copy, then finds none."""
     v = _data(vmn, 3000, seed=11)
+    for a, b in ((10, 2900), (10, 1500), (20, 600), (30, 31)):
+        v[b] = v[a]
     idx = vmn.Vamana(v, mode="fresh", R=32, L_build=64, alphas=(1.2,))
-    probe = np.concatenate([v[[10, 20, 30]], np.full((1, 128), 999.0, np.float32)])
-    ids = idx.deletePoints(probe)
-    assert ids[3] == -1
-    for k, want in zip(ids[:3], (10, 20, 30)):
-        assert np.array_equal(v[k], v[want]) and idx.deleted[k]
+    probe = np.concatenate([v[[2900, 20, 31, 777]], np.full((1, 128), 999.0, np.float32)])
+    live = np.ones(3000, bool)
+    for rnd in range(4):
+        ids = idx.deletePoints(probe)
+        for j, p_ in enumerate(probe):
+            hit = np.flatnonzero(live & (v == p_).all(1))
+            want = int(hit[0]) if len(hit) else -1
+            assert ids[j] == want, (rnd, j, ids[j], want)
+            if want >= 0:
+                live[want] = False
+        assert np.array_equal(idx.deleted, ~live), rnd
+    # 10's row has three copies, 20's two, 30's two, 777's one
+    assert (~live).sum() == 3 + 2 + 2 + 1
     # deleting the same vectors again finds no live match unless a duplicate exists
     again = idx.deletePoints(probe[:3])
     for a, b in zip(again, ids[:3]):
