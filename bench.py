@@ -122,7 +122,10 @@ def main():
                      low_precision="both")
     torch.cuda.synchronize()
     build_s = time.time() - t
-    log(f"build {build_s:.2f} s = {n / build_s:.0f} points/s, medoid {idx.medoid}")
+    bst = idx.build_stats()
+    build_gbs = bst["algorithmic_bytes"] / build_s / 1e9
+    log(f"build {build_s:.2f} s = {n / build_s:.0f} points/s, medoid {idx.medoid}, "
+        f"{bst['algorithmic_bytes'] / 1e9:.1f} GB algorithmic = {build_gbs:,.0f} GB/s")
 
     log("FP64 ground truth")
     t = time.time()
@@ -296,6 +299,12 @@ def main():
             "alphas": list(alphas), "L_search": head["L"], "recall@10": head["recall@10"],
             "recall_target_met": head["target_met"], "build_s": round(build_s, 3),
             "build_points_per_s": round(n / build_s, 1),
+            "build_roofline": {"bound": "hbm", "bytes": int(bst["algorithmic_bytes"]),
+                               "achieved": round(build_gbs, 1), "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),
+                               "work": {k: v for k, v in bst.items()
+                                        if k != "algorithmic_bytes"},
+                               "time": "host wall clock around vmn_build, device synchronised"},
             "fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in timed["fp32"].items()},
             "fp8+rerank": {k: (round(v, 4) if isinstance(v, float) else v)
                            for k, v in timed["fp8+rerank"].items()},

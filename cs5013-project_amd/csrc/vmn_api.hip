@@ -265,6 +265,7 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
     a.stat_hops = stat_hops.as<int32_t>();
     a.stat_ndist = stat_ndist.as<int32_t>();
     a.counter = counter.as<unsigned long long>();
+    if (idx->build_count && !ctx) a.acc = idx->build_acc.as<unsigned long long>();
     HIPCHK(vmn::launch_search(a, search_grid(idx, nq), stream));
     if (ctx) ctx->last_nq = nq;
     else idx->last_nq = nq;
@@ -305,6 +306,7 @@ int build_parth(vmn_index* idx, bool random_init) {
         a.cand_d = idx->cand_d.as<float>();
         a.Lc = L;
         a.counter = idx->counter.as<unsigned long long>();
+        if (idx->build_count) a.acc = idx->build_acc.as<unsigned long long>();
         HIPCHK(vmn::launch_parth_prune(a, prune_grid(idx, n), idx->stream));
     }
     return VMN_OK;
@@ -404,6 +406,7 @@ int fresh_pass(vmn_index* idx, int64_t base, int64_t cnt, float alpha) {
     a.P = idx->p.P;
     a.del_bits = idx->n_deleted ? idx->del_bits.as<uint32_t>() : nullptr;
     a.M = idx->p.M;
+    if (idx->build_count) a.acc = idx->build_acc.as<unsigned long long>();
     int rc = VMN_OK;
     for (int64_t c0 = 0; c0 < cnt; c0 += chunk) {
         const int64_t cs = std::min(chunk, cnt - c0);
@@ -617,6 +620,38 @@ int vmn_destroy(vmn_index* idx) {
 
 namespace vmn_rt {
 
+// The build's algorithmic bytes from its work counters (the byte model of
+// vmn_build_stats in the header).
+static void set_build_stats(vmn_index* idx, const unsigned long long* acc, bool random_init) {
+    vmn_build_stats& b = idx->build_stats;
+    b = vmn_build_stats{};
+    const double n = (double)idx->n, DP = idx->DP, R = idx->p.R;
+    const bool bf = idx->exact_bf16;
+    b.searches = (int64_t)acc[VMN_ACC_SEARCHES];
+    b.expansions = (int64_t)acc[VMN_ACC_EXPANSIONS];
+    b.dist_evals = (int64_t)acc[VMN_ACC_DISTS];
+    b.prune_items = (int64_t)acc[VMN_ACC_ITEMS];
+    b.prune_rows = (int64_t)acc[VMN_ACC_ROWS];
+    b.reverse_pairs = (int64_t)acc[VMN_ACC_PAIRS];
+    b.list_entries = (int64_t)acc[VMN_ACC_LISTS];
+    b.reverse_entries = (int64_t)acc[VMN_ACC_REVS];
+    b.row_bytes = (int32_t)(bf ? 2 * DP : 4 * DP);
+    b.passes = idx->p.passes;
+    double by = 8.0 * n * DP;                        // medoid: centroid and distance passes
+    if (random_init) by += 4.0 * n * R;              // random initial graph
+    if ((idx->p.low_precision & VMN_STORE_BF16) || bf) by += n * (4 * DP + 2 * DP + 4);
+    if (idx->p.low_precision & VMN_STORE_FP8) by += n * (3 * 4 * DP + DP + 4);
+    by += (double)b.searches * 4 * DP;               // the searched nodes' own rows
+    by += (double)b.expansions * (4 * R + 8);        // adjacency rows, visited entries written
+    by += (double)b.dist_evals * b.row_bytes;        // candidate rows
+    by += (double)b.prune_rows * (b.row_bytes + (bf ? 4 : 0));  // prune rows (+ BF16 norms)
+    by += (double)b.list_entries * 8;                // candidate lists the prunes read
+    by += (double)b.prune_items * 8 * R;             // rows read and written by the prunes
+    by += (double)b.reverse_pairs * 24;              // pairs written, grouped, read
+    by += (double)b.reverse_entries * 8;             // reverse candidates the prunes read
+    b.algorithmic_bytes = by;
+}
+
 // vmn_build / vmn_build_from: `init` (host rows, -1 padded to R) replaces the
 // seeded random initial graph when not NULL.
 int build_common(vmn_index* idx, const float* vecs, int64_t n, int32_t mem,
@@ -643,12 +678,24 @@ int build_common(vmn_index* idx, const float* vecs, int64_t n, int32_t mem,
     idx->fp8_scaled = false;
     rc = refresh_lowp(idx, 0, n);
     if (rc) return rc;
+    HIPCHK(idx->build_acc.reserve(VMN_ACC_SLOTS * VMN_ACC_N * sizeof(unsigned long long)));
+    HIPCHK(vmn::launch_fill_u64(idx->build_acc.as<unsigned long long>(),
+                                VMN_ACC_SLOTS * VMN_ACC_N, 0ull, idx->stream));
+    idx->build_count = true;
     rc = idx->p.mode == VMN_MODE_PARTH ? build_parth(idx, !init) : build_fresh(idx, !init);
+    idx->build_count = false;
     if (rc) return rc;
+    unsigned long long slots[VMN_ACC_SLOTS * VMN_ACC_N] = {}, acc[VMN_ACC_N] = {};
+    HIPCHK(hipMemcpyAsync(slots, idx->build_acc.p, sizeof slots, hipMemcpyDeviceToHost,
+                          idx->stream));
     HIPCHK(hipStreamSynchronize(idx->stream));
+    for (int s = 0; s < VMN_ACC_SLOTS; ++s)
+        for (int k = 0; k < VMN_ACC_N; ++k) acc[k] += slots[s * VMN_ACC_N + k];
+    set_build_stats(idx, acc, !init);
     idx->built = true;
     return VMN_OK;
 }
+
 
 int check_build_args(const vmn_index* idx, const float* vecs, int64_t n) {
     if (!idx || !vecs) return fail(VMN_ERR_ARG, "NULL argument");
@@ -861,6 +908,13 @@ int vmn_get_adjacency(const vmn_index* idx, int32_t* out, int32_t mem) {
 
 int64_t vmn_size(const vmn_index* idx) { return idx ? idx->n : -1; }
 int64_t vmn_medoid(const vmn_index* idx) { return idx ? idx->medoid : -1; }
+int vmn_get_build_stats(const vmn_index* idx, vmn_build_stats* out) {
+    if (!idx || !out) return fail(VMN_ERR_ARG, "NULL argument");
+    if (!idx->built) return fail(VMN_ERR_STATE, "index not built");
+    *out = idx->build_stats;
+    return VMN_OK;
+}
+
 int32_t vmn_build_precision(const vmn_index* idx) {
     return idx ? build_prec(idx) : VMN_PREC_FP32;
 }

@@ -255,6 +255,18 @@ hipError_t launch_medoid(const float* vecs, int64_t n, int D, int DP, const uint
     return hipGetLastError();
 }
 
+// Build work counters of one pruned item (lane 0 adds them).
+__device__ __forceinline__ void count_item(unsigned long long* acc, int nread, int pairs,
+                                           int64_t lists, int64_t revs) {
+    if (!acc || lane() != 0) return;
+    unsigned long long* c = acc_slot(acc);
+    atomicAdd(&c[VMN_ACC_ITEMS], 1ull);
+    atomicAdd(&c[VMN_ACC_ROWS], (unsigned long long)nread);
+    if (pairs) atomicAdd(&c[VMN_ACC_PAIRS], (unsigned long long)pairs);
+    if (lists) atomicAdd(&c[VMN_ACC_LISTS], (unsigned long long)lists);
+    if (revs) atomicAdd(&c[VMN_ACC_REVS], (unsigned long long)revs);
+}
+
 // ---------------------------------------------------------------------------
 // PARTH prune: parth/src/build_graph.cu:173-264 (`robustPruneKernel`).
 // Candidates = the node's search list (minus -1 and the node itself, already
@@ -304,9 +316,10 @@ __global__ __launch_bounds__(64) void parth_prune_kernel(PruneArgs a) {
             pass = pass && !dup;
         }
         // FP32 rows: distances in the reference's sequential order
+        int nread = 0;
         PruneQ<DP, BF, !BF> q;
         q.load(rows, node);
-        const int ne = row_extras<DP, BF, !BF>(rows, q, pass, nb, e_id, e_d);
+        const int ne = row_extras<DP, BF, !BF>(rows, q, pass, nb, e_id, e_d, &nread);
         const uint32_t my_id = l < ne ? (uint32_t)e_id[l] : 0u;
         const float my_d = l < ne ? e_d[l] : 0.0f;
         const int cnt = merge_sorted<CCAP, false>(c_id[0], c_d[0], m, my_id, my_d, ne, c_id[1],
@@ -314,8 +327,9 @@ __global__ __launch_bounds__(64) void parth_prune_kernel(PruneArgs a) {
         __syncthreads();
         const int nk = prune_list<DP, true, BF, CCAP, !BF>(rows, c_id[1], c_d[1], cnt, a.alpha,
                                                            a.R, kept, act_id, act_pos, act_d,
-                                                           newrow, nullptr);
+                                                           newrow, nullptr, &nread);
         for (int j = l; j < W; j += 64) a.adj[node * W + j] = j < nk ? newrow[j] : -1;
+        count_item(a.acc, nread, 0, Lc, 0);
         __syncthreads();
     }
 }
@@ -384,9 +398,10 @@ __global__ __launch_bounds__(64) void fresh_out_prune_kernel(PruneArgs a) {
         const int32_t nb = l < W ? a.adj[node * W + l] : -1;
         const bool pass = nb != -1 && nb != (int32_t)node &&
                           !(a.del_bits && bit_test(a.del_bits, (uint32_t)nb));
+        int nread = 0;
         PruneQ<DP, BF> q;
         q.load(rows, node);
-        const int ne = row_extras<DP, BF>(rows, q, pass, nb, e_id, e_d);
+        const int ne = row_extras<DP, BF>(rows, q, pass, nb, e_id, e_d, &nread);
         const uint32_t my_id = l < ne ? (uint32_t)e_id[l] : 0u;
         const float my_d = l < ne ? e_d[l] : 0.0f;
         cnt = merge_sorted<CCAP, false>(c_id[cur], c_d[cur], cnt, my_id, my_d, ne, c_id[cur ^ 1],
@@ -394,7 +409,9 @@ __global__ __launch_bounds__(64) void fresh_out_prune_kernel(PruneArgs a) {
         cur ^= 1;
         __syncthreads();
         const int nk = prune_list<DP, false, BF, CCAP>(rows, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
-                                               kept, act_id, act_pos, act_d, newrow, newd);
+                                               kept, act_id, act_pos, act_d, newrow, newd,
+                                               &nread);
+        count_item(a.acc, nread, nk, vc, 0);
         int32_t* row = a.adj_out ? a.adj_out + i * W : a.adj + node * W;
         for (int j = l; j < W; j += 64) row[j] = j < nk ? newrow[j] : -1;
         for (int j = l; j < a.R; j += 64) {
@@ -458,9 +475,10 @@ __global__ __launch_bounds__(64) void fresh_rev_prune_kernel(PruneArgs a) {
         }
         const int32_t nb = l < W ? a.adj[node * W + l] : -1;
         const bool pass = nb != -1 && nb != (int32_t)node;
+        int nread = 0;
         PruneQ<DP, BF> q;
         q.load(rows, node);
-        const int ne = row_extras<DP, BF>(rows, q, pass, nb, e_id, e_d);
+        const int ne = row_extras<DP, BF>(rows, q, pass, nb, e_id, e_d, &nread);
         const uint32_t my_id = l < ne ? (uint32_t)e_id[l] : 0u;
         const float my_d = l < ne ? e_d[l] : 0.0f;
         cnt = merge_sorted<CCAP, false>(c_id[cur], c_d[cur], cnt, my_id, my_d, ne, c_id[cur ^ 1],
@@ -468,7 +486,9 @@ __global__ __launch_bounds__(64) void fresh_rev_prune_kernel(PruneArgs a) {
         cur ^= 1;
         __syncthreads();
         const int nk = prune_list<DP, false, BF, CCAP>(rows, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
-                                               kept, act_id, act_pos, act_d, newrow, nullptr);
+                                               kept, act_id, act_pos, act_d, newrow, nullptr,
+                                               &nread);
+        count_item(a.acc, nread, 0, 0, o1 - o0);
         for (int j = l; j < W; j += 64) a.adj[node * W + j] = j < nk ? newrow[j] : -1;
         __syncthreads();
     }

@@ -107,6 +107,10 @@ struct vmn_index {
     int64_t n_pend = 0;
     int waves_per_cu = 32;  // search grid = CUs x waves_per_cu (persistent work queue)
     int list_cap = 0;       // minimum beam-list capacity of the search kernel
+    // work counters of the last build (VMN_ACC_*), counted while build_count
+    vmn_rt::DevBuf build_acc;
+    bool build_count = false;
+    vmn_build_stats build_stats{};
     // FP8 searches: row norms beside the adjacency entries ([cap x R], the
     // norm of adj[i] at i), valid for index epoch adj_fp8n_epoch
     vmn_rt::DevBuf adj_fp8n;

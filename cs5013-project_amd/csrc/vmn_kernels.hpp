@@ -39,7 +39,27 @@ struct SearchArgs {
     int32_t* stat_hops;       // [nq] expansions
     int32_t* stat_ndist;      // [nq] distance evaluations
     unsigned long long* counter;  // work queue head, zero before launch
+    unsigned long long* acc;      // build work counters (VMN_ACC_*) or null
 };
+
+// Work counters the build kernels add to (u64 each), from which the build's
+// algorithmic bytes are summed.
+#define VMN_ACC_SEARCHES 0    // searches run
+#define VMN_ACC_EXPANSIONS 1  // nodes expanded (adjacency rows read)
+#define VMN_ACC_DISTS 2       // distance evaluations (candidate rows read)
+#define VMN_ACC_ITEMS 3       // nodes pruned
+#define VMN_ACC_ROWS 4        // rows the prunes read for distances
+#define VMN_ACC_PAIRS 5       // reverse pairs emitted by out-prunes
+#define VMN_ACC_LISTS 6       // candidate-list entries the prunes read (visited / search lists)
+#define VMN_ACC_REVS 7        // reverse candidates the reverse prunes read
+#define VMN_ACC_N 8
+// The counters are kept in VMN_ACC_SLOTS copies of VMN_ACC_N, one per
+// 64-byte line, each workgroup adding to the copy of its index modulo
+// VMN_ACC_SLOTS (so the atomics do not queue on one line); the host sums them.
+#define VMN_ACC_SLOTS 64
+__device__ __forceinline__ unsigned long long* acc_slot(unsigned long long* acc) {
+    return acc + (size_t)(blockIdx.x % VMN_ACC_SLOTS) * VMN_ACC_N;
+}
 
 hipError_t launch_search(const SearchArgs& a, int grid, hipStream_t s);
 // out[i] = norms[adj[i]] (0 where adj[i] is -1) for the n x W entries.
@@ -103,6 +123,7 @@ struct PruneArgs {
     int64_t* overflow;
     unsigned long long* overflow_n;
     const int64_t* items;     // optional: work item k is item items[k]
+    unsigned long long* acc;  // build work counters (VMN_ACC_*) or null
 };
 
 hipError_t launch_parth_prune(const PruneArgs& a, int grid, hipStream_t s);

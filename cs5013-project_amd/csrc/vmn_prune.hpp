@@ -73,7 +73,7 @@ template <int DP, bool STRICT, bool SEQ = false>
 __device__ int robust_prune(const PruneRows& rows, const uint32_t* c_id,
                             const float* c_d, int cnt, float alpha, int R, uint8_t* s_kept,
                             int32_t* s_act_id, int32_t* s_act_pos, float* s_act_d,
-                            int32_t* out_ids, float* out_d) {
+                            int32_t* out_ids, float* out_d, int* nread = nullptr) {
     const int l = lane();
     for (int i = l; i < cnt; i += 64) s_kept[i] = 1;
     __syncthreads();
@@ -112,6 +112,7 @@ __device__ int robust_prune(const PruneRows& rows, const uint32_t* c_id,
         }
         __syncthreads();
         pq.dists(rows, s_act_id, s_act_d, nact);
+        if (nread) *nread += nact;
         __syncthreads();
         for (int k = l; k < nact; k += 64) {
             const int j = s_act_pos[k];
@@ -143,8 +144,10 @@ template <int DP, bool STRICT, bool ORDERED>
 __device__ __forceinline__ void gram_masks(const PruneRows& rows, int32_t sid, int nrows,
                                            int nact, const int32_t* s_act_id,
                                            const int32_t* s_act_pos, const float* c_d,
-                                           float alpha, uint32_t* s_pm) {
+                                           float alpha, uint32_t* s_pm, int* nread = nullptr) {
     constexpr int KB = DP / 32;
+    // rows read: the 16 A rows and the active candidates' rows
+    if (nread) *nread += 16 + nact;
     const int l = lane(), g = l >> 4, rr = l >> 2, sub = l & 3, col = l & 15;
     const __bf16* bro = reinterpret_cast<const __bf16*>(rows.bf16);
     const float* nrm = rows.bf16_norms;
@@ -223,7 +226,7 @@ __device__ int robust_prune_mfma(const PruneRows& rows, const uint32_t* c_id, co
                                  int cnt, float alpha, int R, uint8_t* s_kept,
                                  int32_t* s_act_id, int32_t* s_act_pos, uint32_t* s_pm,
                                  int32_t* out_ids, float* out_d, int nk0 = 0,
-                                 bool init_kept = true) {
+                                 bool init_kept = true, int* nread = nullptr) {
     constexpr int NCH = (CCAP + 63) / 64;
     const int l = lane(), col = l & 15;
     if (init_kept) {
@@ -256,7 +259,7 @@ __device__ int robust_prune_mfma(const PruneRows& rows, const uint32_t* c_id, co
             // list repeat p0 and are ignored)
             const int32_t sid = (col > 0 && col - 1 < nact) ? s_act_id[col - 1] : id0;
             gram_masks<DP, STRICT, true>(rows, sid, ns, nact, s_act_id, s_act_pos, c_d, alpha,
-                                         s_pm);
+                                         s_pm, nread);
         }
         __syncthreads();
         uint32_t pm[NCH];
@@ -305,7 +308,7 @@ template <int DP>
 __device__ void occlude_mfma(const PruneRows& rows, const int32_t* members, int nm,
                              const uint32_t* c_id, const float* c_d, int cnt, float alpha,
                              uint8_t* s_kept, int32_t* s_act_id, int32_t* s_act_pos,
-                             uint32_t* s_pm) {
+                             uint32_t* s_pm, int* nread = nullptr) {
     const int l = lane(), col = l & 15;
     for (int g0 = 0; g0 < nm; g0 += 16) {
         int nact = 0;
@@ -325,7 +328,7 @@ __device__ void occlude_mfma(const PruneRows& rows, const int32_t* members, int 
         const int nrows = nm - g0 < 16 ? nm - g0 : 16;
         const int32_t sid = members[g0 + (col < nrows ? col : 0)];
         gram_masks<DP, false, false>(rows, sid, nrows, nact, s_act_id, s_act_pos, c_d, alpha,
-                                     s_pm);
+                                     s_pm, nread);
         __syncthreads();
         for (int k = l; k < nact; k += 64)
             if (s_pm[k]) s_kept[s_act_pos[k]] = 0;
@@ -339,24 +342,27 @@ template <int DP, bool STRICT, bool BF, int CCAP, bool SEQ = false>
 __device__ __forceinline__ int prune_list(const PruneRows& rows, const uint32_t* c_id,
                                           const float* c_d, int cnt, float alpha, int R,
                                           uint8_t* s_kept, int32_t* s_act_id, int32_t* s_act_pos,
-                                          float* s_act_d, int32_t* out_ids, float* out_d) {
+                                          float* s_act_d, int32_t* out_ids, float* out_d,
+                                          int* nread = nullptr) {
     if constexpr (BF)
         return robust_prune_mfma<DP, STRICT, CCAP>(rows, c_id, c_d, cnt, alpha, R, s_kept,
                                                    s_act_id, s_act_pos,
                                                    reinterpret_cast<uint32_t*>(s_act_d), out_ids,
-                                                   out_d);
+                                                   out_d, 0, true, nread);
     else
         return robust_prune<DP, STRICT, SEQ>(rows, c_id, c_d, cnt, alpha, R, s_kept, s_act_id,
-                                             s_act_pos, s_act_d, out_ids, out_d);
+                                             s_act_pos, s_act_d, out_ids, out_d, nread);
 }
 
 // Distances from `node` to its current row entries that pass `keep`, appended
 // (compacted, adjacency order) to e_id/e_d. Returns the count.
 template <int DP, bool BF, bool SEQ = false>
 __device__ __forceinline__ int row_extras(const PruneRows& rows, const PruneQ<DP, BF, SEQ>& q,
-                                          bool pass, int32_t nb, int32_t* e_id, float* e_d) {
+                                          bool pass, int32_t nb, int32_t* e_id, float* e_d,
+                                          int* nread = nullptr) {
     const uint64_t m = ballot(pass);
     const int ne = __popcll((unsigned long long)m);
+    if (nread) *nread += ne;
     if (pass) e_id[__popcll((unsigned long long)(m & lanemask_lt()))] = nb;
     __syncthreads();
     q.dists(rows, e_id, e_d, ne);

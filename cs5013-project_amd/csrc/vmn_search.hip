@@ -292,6 +292,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
             if (a.vis_cnt) a.vis_cnt[qi] = nexp < a.P ? nexp : a.P;
             if (a.stat_hops) a.stat_hops[qi] = nexp;
             if (a.stat_ndist) a.stat_ndist[qi] = ndist;
+            if (a.acc) {
+                unsigned long long* c = acc_slot(a.acc);
+                atomicAdd(&c[VMN_ACC_SEARCHES], 1ull);
+                atomicAdd(&c[VMN_ACC_EXPANSIONS], (unsigned long long)nexp);
+                atomicAdd(&c[VMN_ACC_DISTS], (unsigned long long)ndist);
+            }
         }
         __syncthreads();
     }

@@ -53,6 +53,16 @@ class ServerStats(ctypes.Structure):
                 ("failed_batches", ctypes.c_int64)]
 
 
+class BuildStats(ctypes.Structure):
+    """Mirror of `vmn_build_stats`."""
+    _fields_ = [("searches", ctypes.c_int64), ("expansions", ctypes.c_int64),
+                ("dist_evals", ctypes.c_int64), ("prune_items", ctypes.c_int64),
+                ("prune_rows", ctypes.c_int64), ("reverse_pairs", ctypes.c_int64),
+                ("list_entries", ctypes.c_int64), ("reverse_entries", ctypes.c_int64),
+                ("row_bytes", ctypes.c_int32), ("passes", ctypes.c_int32),
+                ("algorithmic_bytes", ctypes.c_double)]
+
+
 class EngineResult(ctypes.Structure):
     _fields_ = [("searches", ctypes.c_int64), ("inserts", ctypes.c_int64),
                 ("deletes", ctypes.c_int64), ("batches", ctypes.c_int64),
@@ -102,6 +112,7 @@ def lib():
         L.vmn_search_ex.argtypes = [P, P, i64, i32, i32, i32, i32, P, P, i32]
         L.vmn_search_vectors.argtypes = [P, P, i64, i32, i32, P, i32]
         L.vmn_search_noalloc.argtypes = [P, P, i64, i32, i32, i32, i32, P, P]
+        L.vmn_get_build_stats.argtypes = [P, ctypes.POINTER(BuildStats)]
         L.vmn_gather_probe.argtypes = [P, i64, i32, i32, P, i64, P, i64, i32, P]
         L.vmn_get_lowp_rows.argtypes = [P, i32, P]
         L.vmn_get_adjacency.argtypes = [P, P, i32]
@@ -608,6 +619,13 @@ class Vamana:
         pc = PREC[prec] if isinstance(prec, str) else prec
         _check(lib().vmn_get_lowp_rows(self._h, pc, out.ctypes.data_as(ctypes.c_void_p)))
         return out
+
+    def build_stats(self):
+        """Work counted by the last build's kernels and its algorithmic bytes
+        (vmn_build_stats)."""
+        st = BuildStats()
+        _check(lib().vmn_get_build_stats(self._h, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in BuildStats._fields_}
 
     def search_stats(self):
         e, d = ctypes.c_int64(), ctypes.c_int64()

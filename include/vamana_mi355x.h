@@ -293,6 +293,28 @@ int64_t vmn_medoid(const vmn_index* idx);
  * (greedySearch calls, vamana_impl.cuh:235, :325). */
 int32_t vmn_build_precision(const vmn_index* idx);
 
+/* Work of the last vmn_build / vmn_build_from, counted by its kernels, and
+ * the build's algorithmic bytes from them (the measure of build bandwidth;
+ * no counterpart in the reference). row_bytes is the bytes of one row the
+ * build read: 2 x padded dim while every row is exact in BF16, else 4 x. The
+ * byte sum is: 8 n DP (medoid: centroid and distance passes over the FP32
+ * rows) + 4 n R (random initial graph, when used) + the low-precision copies
+ * (BF16: n (6 DP + 4); FP8: n (13 DP + 4)) + per search its node's FP32 row
+ * (4 DP) + per expansion its adjacency row and visited entry (4 R + 8) + per
+ * distance evaluation a row (row_bytes) + per row a prune read for distances
+ * row_bytes (+ 4 for its BF16 norm) + 8 per candidate-list entry the prunes
+ * read + 8 R per pruned node (its row read and written) + 24 per reverse pair
+ * (written, grouped, read) + 8 per reverse candidate a reverse prune read. */
+typedef struct vmn_build_stats {
+    int64_t searches, expansions, dist_evals; /* the build's searches */
+    int64_t prune_items, prune_rows;          /* nodes pruned, rows read for their distances */
+    int64_t reverse_pairs;                    /* (p* <- node) pairs the out-prunes emitted */
+    int64_t list_entries, reverse_entries;    /* candidate-list / reverse-candidate entries read */
+    int32_t row_bytes, passes;
+    double algorithmic_bytes;
+} vmn_build_stats;
+int vmn_get_build_stats(const vmn_index* idx, vmn_build_stats* out);
+
 /* Sum over the queries of the last vmn_search: expansions (hops) and distance
  * evaluations. Synchronises the index's stream. */
 int vmn_search_stats(const vmn_index* idx, int64_t* expansions, int64_t* dist_evals);

@@ -154,3 +154,38 @@ def test_gpu_fresh_fp32_on_real_valued_rows_within_tolerance(vmn, orc, gpu):
           f"{same_q:.4f}, worst relative distance difference {worst:.2e}", flush=True)
     assert idx.medoid == omed
     assert same_rows >= 0.99 and same_q >= 0.995 and worst <= 5e-6
+
+
+def test_gpu_build_work_counters_equal_the_oracle_passes(vmn, orc, gpu):
+    """The build's work counters (vmn_get_build_stats), which its byte sum is
+    made of, against the oracle run pass by pass on the same data: searches,
+    expansions and distance evaluations of the build searches, visited-set
+    entries the out-prunes read, reverse pairs emitted and read, nodes
+    pruned."""
+    base = vmn.datasets.integer_mixture(5000, 128, k=32, seed=47, stream=0)
+    R, L, alphas = 32, 64, (1.0, 1.2)
+    idx = vmn.Vamana(base, mode="fresh", R=R, L_build=L, alphas=alphas)
+    st = idx.build_stats()
+    g, med = orc.random_init(5000, R, R, seed=1234), orc.medoid(base)
+    want = dict(searches=0, expansions=0, dist_evals=0, list_entries=0, reverse_pairs=0,
+                prune_items=0)
+    for a in alphas:
+        _, _, hops, nd, vi, vd, vc = orc.fresh_search(base, g, base, med, L, out_n=1,
+                                                      queries_are_nodes=True,
+                                                      want_visited=True)
+        out = g.copy()
+        pdst, pd = orc.fresh_out_prune(base, out, R, a, vi, vd, vc)
+        g = out.copy()
+        orc.fresh_reverse(base, g, R, a, pdst, pd, M=500)
+        want["searches"] += 5000
+        want["expansions"] += int(hops.sum())
+        want["dist_evals"] += int(nd.sum())
+        want["list_entries"] += int(vc.sum())
+        want["reverse_pairs"] += int((pdst >= 0).sum())
+        want["prune_items"] += 5000 + len(np.unique(pdst[pdst >= 0]))
+    assert np.array_equal(idx.adjacency, g)
+    got = {k: st[k] for k in want}
+    assert got == want, (got, want)
+    assert st["reverse_entries"] == want["reverse_pairs"]
+    assert st["row_bytes"] == 2 * 128 and st["passes"] == 2
+    assert st["prune_rows"] > 0 and st["algorithmic_bytes"] > 0
