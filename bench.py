@@ -5,11 +5,12 @@ A step is one batched search of 10,000 queries (1M x 128) in one persistent
 launch, inputs resident in HBM. The build and the recall sweep run first
 (untimed by the step clock, timed separately), then W warmup and K timed steps
 at the smallest beam width whose recall@10 reaches the target. The K steps are
-K batches issued alternately on two search contexts (streams), each replaying
-its own captured graph over its own 10,000-query set, so the drain of one
-batch overlaps the start of the next; `value` is the queries of the K batches
-over the wall time of the K steps. Each form's launch shape (waves per CU,
-beam-list capacity) is tuned on that same alternating stream of batches. The
+K batches issued round-robin over `--contexts` search contexts (streams,
+default 2), each replaying its own captured graph over its own 10,000-query
+set, so the drain of one batch overlaps the start of the next; `value` is the
+queries of the K batches over the wall time of the K steps. Each form's launch
+shape (waves per CU, beam-list capacity) is tuned on that same round-robin
+stream of batches. The
 same form timed one batch at a time on one context (no overlap) is reported as
 `qps_serial`, and the roofline uses that per-launch kernel time.
 
@@ -78,7 +79,7 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=50_000.0, help="offered queries/s")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-L", type=int, default=64)
-    ap.add_argument("--serve-consolidate", type=int, default=20_000,
+    ap.add_argument("--serve-consolidate", type=int, default=2_000,
                     help="serving: consolidate the tombstones after this many deletes")
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
@@ -208,7 +209,7 @@ def main():
         e1.synchronize()
         wall_serial = time.time() - t
         kern_ms = e0.elapsed_time(e1) / args.steps
-        # the timed steps: batches alternate between the two contexts; HIP
+        # the timed steps: batches go round-robin over the contexts; HIP
         # events on each context's stream bracket every launch, so each
         # launch's duration is measured where it runs (overlapping the
         # neighbouring launches on the other stream)
