@@ -189,7 +189,7 @@ def main():
         prec, rr = forms[form]
         rr = min(rr, L)
         tune = vmn.tuning.launch_shape(idx, q_t, k=10, L=L, prec=prec, rerank=rr, reps=12,
-                                       overlap=True, contexts=nctx)
+                                       overlap=True, contexts=nctx, queries_alt=q2_t)
         ctx[0].capture(q_t, ids, dists, k=10, L=L, prec=prec, rerank=rr)
         ctx[1].capture(q2_t, ids2, dists2, k=10, L=L, prec=prec, rerank=rr)
         for c_, (i_, d_) in enumerate(more):
@@ -246,7 +246,8 @@ def main():
                        "achieved_GBs": bytes_launch / (launch_ms / 1e3) / 1e9,
                        "kernel_ms_serial": kern_ms,
                        "achieved_GBs_serial": bytes_launch / (kern_ms / 1e3) / 1e9,
-                       "launch": {"waves_per_cu": tune["waves_per_cu"], "list_cap": tune["list_cap"]},
+                       "launch": {"waves_per_cu": tune["waves_per_cu"], "list_cap": tune["list_cap"],
+                                  "query_groups": tune["groups"]},
                        "kernel": kernel, "rerank": rr,
                        "traffic": measured_traffic(form, n, nq, L, kernel, rr),
                        "counters": measured_counters(form, n, nq, L, kernel, rr, launch_ms),
@@ -254,7 +255,8 @@ def main():
         log(f"timed {form:12s} L={L} {timed[form]['qps']:,.0f} QPS (serial {timed[form]['qps_serial']:,.0f})"
             f" recall {rec:.4f}/{rec2:.4f} launch {launch_ms:.3f} ms overlapped, {kern_ms:.3f} ms"
             f" serial, {timed[form]['achieved_GBs']:.0f} GB/s"
-            f" (launch {tune['waves_per_cu']} waves/CU, cap {tune['list_cap']})")
+            f" (launch {tune['waves_per_cu']} waves/CU, cap {tune['list_cap']},"
+            f" {tune['groups']} query groups)")
     for c in ctx:
         c.close()
 
@@ -263,6 +265,8 @@ def main():
     Lh = chosen["bf16+rerank"][0]["L"]
     rrh = min(args.rerank, Lh)
     idx.set_launch(timed["bf16+rerank"]["launch"]["waves_per_cu"], timed["bf16+rerank"]["launch"]["list_cap"])
+    if timed["bf16+rerank"]["launch"]["query_groups"] is not None:
+        idx.set_query_groups(timed["bf16+rerank"]["launch"]["query_groups"])
     idx.searchPoints(qs, k=10, L=Lh, prec="bf16", rerank=rrh)
     t = time.time()
     for _ in range(args.steps):

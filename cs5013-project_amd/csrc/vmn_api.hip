@@ -247,8 +247,23 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
     DevBuf& stat_ndist = ctx ? ctx->stat_ndist : idx->stat_ndist;
     int rc = prepare_search(idx, prec, true);
     if (rc) return rc;
-    HIPCHK(counter.reserve(64));
-    HIPCHK(vmn::launch_fill_u64(counter.as<unsigned long long>(), 1, 0ull, stream));
+    const int ng = groups_wanted(idx, nq) ? idx->query_groups : 0;
+    if (ng) {
+        bool fresh = false;
+        rc = ensure_centroids(idx, &fresh);
+        if (rc) return rc;
+        // computed on the index stream; a context's stream reads them next
+        if (fresh && ctx) HIPCHK(hipStreamSynchronize(idx->stream));
+    }
+    const int nlists = ng * VMN_GROUP_SUBLISTS;
+    HIPCHK(counter.reserve(ng ? (size_t)nlists * 256 : 64));
+    HIPCHK(vmn::launch_fill_u64(counter.as<unsigned long long>(), ng ? 32 * nlists : 1, 0ull, stream));
+    DevBuf& grp_order = ctx ? ctx->grp_order : idx->grp_order;
+    int64_t gstride = 0;
+    if (ng) {
+        rc = group_queries(idx, q, nq, grp_order, counter.as<unsigned long long>(), stream, &gstride);
+        if (rc) return rc;
+    }
     HIPCHK(stat_hops.reserve((size_t)nq * sizeof(int32_t)));
     HIPCHK(stat_ndist.reserve((size_t)nq * sizeof(int32_t)));
     vmn::SearchArgs a;
@@ -265,6 +280,11 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
     a.stat_hops = stat_hops.as<int32_t>();
     a.stat_ndist = stat_ndist.as<int32_t>();
     a.counter = counter.as<unsigned long long>();
+    if (ng) {
+        a.order = grp_order.as<int32_t>();
+        a.gstride = gstride;
+        a.ngroups = ng;
+    }
     if (idx->build_count && !ctx) a.acc = idx->build_acc.as<unsigned long long>();
     HIPCHK(vmn::launch_search(a, search_grid(idx, nq), stream));
     if (ctx) ctx->last_nq = nq;

@@ -47,6 +47,12 @@ __device__ __forceinline__ int readlane_i(int v, int l) {
 }
 
 // Sum over the 8 lanes of a row group (lanes 8g..8g+7).
+__device__ __forceinline__ float wave_min_f(float v) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+
 __device__ __forceinline__ float reduce8(float v) {
     v += __shfl_xor(v, 1);
     v += __shfl_xor(v, 2);

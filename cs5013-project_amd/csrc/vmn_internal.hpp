@@ -86,6 +86,13 @@ struct vmn_index {
     vmn_rt::DevBuf adj_shadow;  // out-pruned rows of a chunked FRESH pass
     vmn_rt::DevBuf centroid, medoid_key;
     int64_t last_nq = 0;
+    // query groups of large batches (one per XCD; 0 or 1: one queue), the
+    // k-means centroids they are formed by (ncent of them, computed at size
+    // cent_n) and the index stream's group lists
+    int query_groups = 8;
+    int ncent = 0;
+    int64_t cent_n = 0;
+    vmn_rt::DevBuf cent, cent_norm, grp_sample, grp_keys, grp_order;
     int64_t n_deleted = 0;  // tombstones set since the build (upper bound)
     vmn_rt::DevBuf staging, find_out, bad;
     vmn_rt::DevBuf bf16_rows, bf16_norms, fp8_rows, fp8_norms, fp8_scales, fp8_centre, fp8_absmax,
@@ -127,7 +134,7 @@ struct vmn_index {
 struct vmn_ctx {
     vmn_index* idx = nullptr;
     hipStream_t stream = nullptr;
-    vmn_rt::DevBuf counter, stat_hops, stat_ndist;
+    vmn_rt::DevBuf counter, stat_hops, stat_ndist, grp_order;
     int64_t last_nq = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -137,6 +144,14 @@ struct vmn_ctx {
 namespace vmn_rt {
 
 int set_device(const vmn_index* idx);
+// Query groups (one per XCD) for a batch of nq queries: wanted, the centroids
+// they come from (computed on the index stream, once), and the group lists of
+// a batch (written on `stream`; heads = 32 x groups x VMN_GROUP_SUBLISTS
+// zeroed counters; *stride receives the lists' stride).
+bool groups_wanted(const vmn_index* idx, int64_t nq);
+int ensure_centroids(vmn_index* idx, bool* computed);
+int group_queries(vmn_index* idx, const float* q, int64_t nq, DevBuf& order,
+                  unsigned long long* heads, hipStream_t stream, int64_t* stride);
 // Rows of dimension D (host or device) placed as [n x DP] rows on the device.
 int stage_rows(vmn_index* idx, const float* src, int64_t n, int32_t mem, DevBuf& scratch,
                const float** out);

@@ -55,12 +55,15 @@ __global__ void fill_zero_u32(unsigned int* p, int n) {
 
 // PREC_ARG is a VMN_PREC_* value, or VMN_PREC_BF16_TABLE_NORMS: BF16 rows with
 // the row norms read from the norms table (small batches, FRESH mode).
-// FP8 at row widths up to 128 with beam lists up to 128 entries is held to 5
-// waves per SIMD (96 VGPRs, no spills; the compiler's own choice takes 4):
-// the 10k-query batch runs 7% faster, single queries 3% slower.
+// FRESH mode at row widths up to 128 with beam lists up to 256 entries, and
+// PARTH-mode FP8 with lists up to 128, are held to 5 waves per SIMD (96
+// VGPRs, no spills; the compiler's own choice takes 4): large batches are
+// bandwidth-bound and run faster with the extra wave (FP8 10k-query batch
+// 7%), single queries about 3% slower. PARTH FP32 / BF16 would spill there.
 template <int DP, int LCAP, int MODE, int PREC_ARG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
-    (PREC_ARG == VMN_PREC_FP8 && LCAP <= 128 && DP <= 128) ? 5 : 1, 8))) void search_kernel(SearchArgs a) {
+    (DP <= 128 && ((MODE == VMN_MODE_FRESH && LCAP <= 256) ||
+                   (PREC_ARG == VMN_PREC_FP8 && LCAP <= 128))) ? 5 : 1, 8))) void search_kernel(SearchArgs a) {
     constexpr int PREC = PREC_ARG == VMN_PREC_BF16_TABLE_NORMS ? VMN_PREC_BF16 : PREC_ARG;
     constexpr bool TN = PREC_ARG == VMN_PREC_BF16_TABLE_NORMS;
     // FP32 distances in the reference's summation order (PARTH)
@@ -78,8 +81,45 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
     const float* __restrict__ vecs = a.vecs;
     const int W = a.W;
 
+    // query groups: this workgroup's walk over the sub-lists (lane 0), its
+    // own group's first; lists only empty during a launch, so the walk only
+    // moves forward
+    constexpr int SL = VMN_GROUP_SUBLISTS;
+    const int nlists = a.ngroups * SL;
+    const int g0 = a.ngroups > 0 ? (int)(blockIdx.x % (unsigned)a.ngroups) : 0;
+    const int s0 = a.ngroups > 0 ? (int)((blockIdx.x / (unsigned)a.ngroups) % SL) : 0;
+    int ft = 0;
+    unsigned long long fcnt = 0;  // entries of list ft, once read
+    bool fknown = false;
+
     for (;;) {
-        if (l == 0) s_q = (int64_t)atomicAdd(a.counter, 1ull);
+        if (l == 0) {
+            int64_t qsel = a.nq;
+            if (a.ngroups > 0) {
+                for (; ft < nlists; ++ft, fknown = false) {
+                    int g = g0 + ft / SL;
+                    if (g >= a.ngroups) g -= a.ngroups;
+                    const int L = g * SL + (s0 + ft) % SL;
+                    unsigned long long* h = a.counter + 32 * L;
+                    if (!fknown) {
+                        // first visit: an emptied list is passed over without
+                        // a read-modify-write
+                        fcnt = __hip_atomic_load(h + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (__hip_atomic_load(h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= fcnt)
+                            continue;
+                        fknown = true;
+                    }
+                    const unsigned long long c = atomicAdd(h, 1ull);
+                    if (c < fcnt) {
+                        qsel = a.order[(int64_t)L * a.gstride + (int64_t)c];
+                        break;
+                    }
+                }
+            } else {
+                qsel = (int64_t)atomicAdd(a.counter, 1ull);
+            }
+            s_q = qsel;
+        }
         __syncthreads();
         const int64_t qi = s_q;
         __syncthreads();
@@ -131,24 +171,35 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
         __syncthreads();
 
         int cur = 0, cnt = 1, nexp = 0, hops = 0, ndist = 1;
+        // adjacency row of the node predicted to be expanded next, loaded
+        // before the merge that decides it (pf: its id, -1 for none)
+        int32_t pf = -1, pf_nb = -1;
+        float pf_nbn = 0.0f;
         while (hops < a.max_hops) {
             ++hops;
             // closest unexpanded entry (its id and distance taken from the
-            // lane that read it)
-            int p = -1;
-            uint32_t e = 0u;
-            float nd = 0.0f;
-            for (int b = 0; b < cnt; b += 64) {
+            // lane that read it), and the one after it (p2), which is the
+            // closest once this one is expanded unless a new candidate beats it
+            int p = -1, p2 = -1;
+            uint32_t e = 0u, e2 = 0u;
+            float nd = 0.0f, nd2 = 0.0f;
+            for (int b = 0; b < cnt && p2 < 0; b += 64) {
                 const int i = b + l;
                 const uint32_t v = i < cnt ? s_id[cur][i] : VMN_VIS_BIT;
                 const float dv = i < cnt ? s_d[cur][i] : 0.0f;
-                const uint64_t m = ballot(!(v & VMN_VIS_BIT));
-                if (m) {
+                uint64_t m = ballot(!(v & VMN_VIS_BIT));
+                if (m && p < 0) {
                     const int t = __ffsll((unsigned long long)m) - 1;
                     p = b + t;
                     e = (uint32_t)readlane_i((int)v, t);
                     nd = readlane_f(dv, t);
-                    break;
+                    m &= m - 1;
+                }
+                if (m) {
+                    const int t = __ffsll((unsigned long long)m) - 1;
+                    p2 = b + t;
+                    e2 = (uint32_t)readlane_i((int)v, t);
+                    nd2 = readlane_f(dv, t);
                 }
             }
             if (p < 0) break;
@@ -175,10 +226,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
                 }
             }
             // neighbours up to the first -1 (FP8: with their row norms)
-            int32_t nb = l < W ? a.adj[(int64_t)node * W + l] : -1;
+            int32_t nb;
             float nbn = 0.0f;
-            if constexpr (PREC == VMN_PREC_FP8)
-                nbn = l < W ? a.adj_norms[(int64_t)node * W + l] : 0.0f;
+            if (node == pf) {
+                nb = pf_nb;
+                nbn = pf_nbn;
+            } else {
+                nb = l < W ? a.adj[(int64_t)node * W + l] : -1;
+                if constexpr (PREC == VMN_PREC_FP8)
+                    nbn = l < W ? a.adj_norms[(int64_t)node * W + l] : 0.0f;
+            }
             const uint64_t endm = ballot(l < W && nb == -1);
             const int deg = endm ? __ffsll((unsigned long long)endm) - 1 : W;
             bool valid = l < deg;
@@ -255,6 +312,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
             const uint32_t my_id = l < nnew ? (uint32_t)s_sid[l] : 0u;
             const float my_d = l < nnew ? s_sd[l] : 0.0f;
             const uint64_t km = ballot(l < nnew && my_d < thr);
+            // The merged list's closest unexpanded entry is the closest kept
+            // candidate (lowest slot on ties) if it is below p2's distance
+            // (list entries come first on ties), else p2: start loading its
+            // adjacency row now, so the load overlaps the merge.
+            {
+                const uint64_t bm = km & ballot(p2 < 0 || my_d < nd2);
+                pf = -1;
+                if (bm) {
+                    const float mn = wave_min_f(((bm >> l) & 1ull) ? my_d : INFINITY);
+                    const uint64_t mm = bm & ballot(my_d == mn);
+                    if (mm) pf = readlane_i((int)my_id, __ffsll((unsigned long long)mm) - 1);
+                } else if (p2 >= 0) {
+                    pf = (int32_t)(e2 & VMN_ID_MASK);
+                }
+                if (pf >= 0) {
+                    pf_nb = l < W ? a.adj[(int64_t)pf * W + l] : -1;
+                    if constexpr (PREC == VMN_PREC_FP8)
+                        pf_nbn = l < W ? a.adj_norms[(int64_t)pf * W + l] : 0.0f;
+                }
+            }
             cnt = merge_masked<LCAP>(s_id[cur], s_d[cur], cnt, my_id, my_d, km, s_id[cur ^ 1],
                                      s_d[cur ^ 1], a.L);
             cur ^= 1;

@@ -139,6 +139,8 @@ def lib():
         L.vmn_ctx_capture.argtypes = [P, P, i64, i32, i32, i32, i32, P, P]
         L.vmn_ctx_replay.argtypes = [P]
         L.vmn_set_launch.argtypes = [P, i32, i32]
+        L.vmn_set_query_groups.argtypes = [P, i32]
+        L.vmn_query_group_ids.argtypes = [P, ctypes.c_void_p, i64, i32, ctypes.c_void_p]
         L.vmn_default_server_params.argtypes = [ctypes.POINTER(ServerParams)]
         L.vmn_default_server_params.restype = None
         L.vmn_server_create.argtypes = [P, ctypes.POINTER(ServerParams), i32, ctypes.POINTER(P)]
@@ -183,7 +185,9 @@ def lib():
                      "vmn_delete_ids", "vmn_delete_vectors", "vmn_consolidate",
                      "vmn_get_deleted", "vmn_search_ex", "vmn_get_lowp_rows", "vmn_ctx_create",
                      "vmn_ctx_destroy", "vmn_ctx_search", "vmn_ctx_capture", "vmn_ctx_replay",
-                     "vmn_set_launch", "vmn_server_create", "vmn_server_submit",
+                     "vmn_set_launch", "vmn_set_query_groups", "vmn_query_group_ids",
+                     "vmn_server_create",
+                     "vmn_server_submit",
                      "vmn_server_poll", "vmn_server_destroy", "vmn_read_workload",
                      "vmn_engine_delete", "vmn_engine_insert", "vmn_engine_run",
                      "vmn_read_vectors", "vmn_write_vectors", "vmn_read_truthset",
@@ -601,6 +605,24 @@ class Vamana:
 
     def set_launch(self, waves_per_cu=32, list_cap=0):
         _check(lib().vmn_set_launch(self._h, waves_per_cu, list_cap))
+
+    def set_query_groups(self, groups=8):
+        """Query groups of large batches (one per XCD; 0 or 1: one queue).
+        Changes the order queries are searched in, never a result."""
+        _check(lib().vmn_set_query_groups(self._h, groups))
+
+    def query_group_ids(self, queries):
+        """Group of each query (uint8 numpy array), as a large batch uses."""
+        nq = int(queries.shape[0])
+        qp, mem, keep = _arg(queries, np.float32)
+        if mem == MEM_DEVICE:
+            import torch
+            out = torch.empty(nq, dtype=torch.uint8, device=queries.device)
+            _check(lib().vmn_query_group_ids(self._h, qp, nq, mem, ctypes.c_void_p(out.data_ptr())))
+            return out.cpu().numpy()
+        out = np.empty(nq, dtype=np.uint8)
+        _check(lib().vmn_query_group_ids(self._h, qp, nq, mem, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
 
     def search_kernel_name(self, k, L, prec="fp32", list_cap=0):
         """Name of the search kernel instance a search with these arguments

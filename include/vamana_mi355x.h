@@ -425,6 +425,20 @@ int vmn_ctx_replay(vmn_ctx* ctx);
  * beam-list capacity template (0: the smallest of 64/128/256/512 >= L). */
 int vmn_set_launch(vmn_index* idx, int32_t waves_per_cu, int32_t list_cap);
 
+/* Query groups of large batches (no counterpart in the reference; default 8,
+ * one per XCD of an MI355X in its default single-partition mode; 0 or 1: one
+ * queue). A batch of at least 8 queries per CU is split by the nearest of 64
+ * k-means centroids of the index's rows (computed on the index stream by the
+ * first such search, and again once the index has doubled), one group per
+ * XCD, and each workgroup searches its own XCD's group first, so the rows one
+ * L2 holds serve the queries that XCD runs. The grouping changes only the
+ * order in which queries are searched, never a result. */
+int vmn_set_query_groups(vmn_index* idx, int32_t groups);
+/* The group (0 .. groups-1) each of nq queries ([nq x dim]) would be searched
+ * in, one byte per query into gids_out (host or device, per mem). Synchronous. */
+int vmn_query_group_ids(vmn_index* idx, const float* queries, int64_t nq, int32_t mem,
+                        uint8_t* gids_out);
+
 /* ---- parth engine: workload files, free-list updates, workload replay ----
  *
  * The parth engine (parth/src/engine.cu:304-568, VamanaEngine) replays a
