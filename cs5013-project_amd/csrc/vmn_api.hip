@@ -255,13 +255,12 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
         // computed on the index stream; a context's stream reads them next
         if (fresh && ctx) HIPCHK(hipStreamSynchronize(idx->stream));
     }
-    const int nlists = ng * VMN_GROUP_SUBLISTS;
-    HIPCHK(counter.reserve(ng ? (size_t)nlists * 256 : 64));
-    HIPCHK(vmn::launch_fill_u64(counter.as<unsigned long long>(), ng ? 32 * nlists : 1, 0ull, stream));
+    const vmn::GroupLayout lay = ng ? group_layout(idx, nq) : vmn::GroupLayout{false, 0, 0};
+    HIPCHK(counter.reserve(ng ? (size_t)lay.nlists * 256 : 64));
+    HIPCHK(vmn::launch_fill_u64(counter.as<unsigned long long>(), ng ? 32 * lay.nlists : 1, 0ull, stream));
     DevBuf& grp_order = ctx ? ctx->grp_order : idx->grp_order;
-    int64_t gstride = 0;
     if (ng) {
-        rc = group_queries(idx, q, nq, grp_order, counter.as<unsigned long long>(), stream, &gstride);
+        rc = group_queries(idx, q, nq, grp_order, counter.as<unsigned long long>(), stream);
         if (rc) return rc;
     }
     HIPCHK(stat_hops.reserve((size_t)nq * sizeof(int32_t)));
@@ -282,8 +281,9 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
     a.counter = counter.as<unsigned long long>();
     if (ng) {
         a.order = grp_order.as<int32_t>();
-        a.gstride = gstride;
+        a.gstride = lay.stride;
         a.ngroups = ng;
+        a.nlists = lay.nlists;
     }
     if (idx->build_count && !ctx) a.acc = idx->build_acc.as<unsigned long long>();
     HIPCHK(vmn::launch_search(a, search_grid(idx, nq), stream));

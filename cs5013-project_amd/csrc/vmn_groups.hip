@@ -6,12 +6,16 @@
 // so the rows and adjacency lists one L2 holds serve the queries that XCD
 // runs: each group's searches walk fewer distinct parts of the graph. The
 // region is the nearest of C centroids (k-means over a sample of the index's
-// rows, computed once per index), and centroid c belongs to group
-// c * groups / C. This file computes the centroids and lists each group's
-// queries; the search kernel takes queries from its own group's list first and
-// from the others once that one is empty, so the groups need not be balanced.
-// The grouping decides only the order in which queries are searched: each
-// query's search, and so every result, is the same in any order.
+// rows, computed once per index), and centroid c belongs to group c * G / C.
+// This file computes the centroids and lists the batch's queries: one list per
+// centroid, in centroid order, so a group's lists are consecutive. The search
+// kernel's workgroups of one XCD walk the lists cyclically from their group's
+// first, together, so at any moment they search few regions; they go on to the
+// next groups' lists once their own are empty, so the groups need not be
+// balanced. (Batches of more than 65,536 queries, a build's, keep per-group
+// lists in a bounded layout instead: see list_layout.) The grouping decides
+// only the order in which queries are searched: each query's search, and so
+// every result, is the same in any order.
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -34,22 +38,22 @@ namespace {
 // centroid l % 16's (the A operand); the product tile D[m][n] (centroid m,
 // row n) sits in lane 16 (m / 4) + n, element m % 4, so every lane follows one
 // row and the four lanes of a row meet in two exchanges. Four centroid tiles
-// are accumulated at once, k-blocks outermost. With keys_out the centroid index is
-// written per row. Otherwise the row is appended to its group's list: group
-// g = index * G / C has S sub-lists, and workgroup b appends to sub-list
-// b % S (list L = g S + b % S: entries order[L * stride + ...], count at
-// heads[32 L + 1]), so the device-wide atomics on one count stay few. A
-// one-wavefront workgroup (WAVES = 1; it starts beside a persistent search
-// still holding the GPU) adds each group's rows with one atomic, all in
-// flight together; WAVES = 16 counts its 256 rows in LDS first, for batches
-// so large (a build's searches) that even those would queue.
+// are accumulated at once, k-blocks outermost. With keys_out the centroid
+// index is written per row. Otherwise the row is appended to list L of the
+// batch (entries order[L * stride + ...], count at heads[32 L + 1]): L is the
+// centroid (by_centroid), or, in the bounded layout, L = g S + b % S for group
+// g = index * G / C and workgroup b, S sub-lists per group. A one-wavefront
+// workgroup (WAVES = 1; it starts beside a persistent search still holding
+// the GPU) adds each list's rows with one atomic, the lists' atomics in flight
+// together; WAVES = 16 (bounded layout) counts its 256 rows per group in LDS
+// first, for batches so large that even those would queue.
 template <int KB, int WAVES>
 __device__ __forceinline__ void assign_body(const float* __restrict__ rows, int64_t nrows,
                                             const __bf16* __restrict__ cent,
                                             const float* __restrict__ cnorm, int C,
                                             int32_t* keys_out, int32_t* order,
                                             unsigned long long* heads, int G, int S,
-                                            int64_t gstride) {
+                                            int64_t gstride, bool by_centroid) {
     constexpr int DP = 32 * KB;
     __shared__ int s_cnt[WAVES > 1 ? 32 : 1];
     __shared__ unsigned long long s_base[WAVES > 1 ? 32 : 1];
@@ -119,21 +123,21 @@ __device__ __forceinline__ void assign_body(const float* __restrict__ rows, int6
     const int gr = (int)((int64_t)key * G / C);
     const int sub = (int)(blockIdx.x % (unsigned)S);
     if constexpr (WAVES == 1) {
-        // lane gg < G counts group gg's rows and adds them with one atomic;
-        // the G atomics are in flight together
+        // rows of one list meet in a mask; its lowest lane adds them with one
+        // atomic, the lists' atomics in flight together
+        const int Lr = !valid ? -1 : by_centroid ? key : gr * S + sub;
         uint64_t mine = 0;
-        int cnt = 0;
-        for (int gg = 0; gg < G; ++gg) {
-            const uint64_t m = __ballot(valid && gr == gg);
-            if (valid && gr == gg) mine = m;
-            if (l == gg) cnt = __popcll(m);
+        for (int j = 0; j < 16; ++j) {
+            const int Lj = __shfl(Lr, j);
+            const uint64_t m = __ballot(valid && Lr == Lj);
+            if (l == j) mine = m;
         }
+        const int lead = valid ? __ffsll((unsigned long long)mine) - 1 : 0;
         unsigned long long base = 0;
-        if (l < G && cnt > 0) base = atomicAdd(heads + 32 * (l * S + sub) + 1, (unsigned long long)cnt);
-        base = __shfl(base, valid ? gr : 0);
+        if (valid && lead == l) base = atomicAdd(heads + 32 * Lr + 1, (unsigned long long)__popcll(mine));
+        base = __shfl(base, lead);
         if (valid)
-            order[(int64_t)(gr * S + sub) * gstride + (int64_t)base + __popcll(mine & ((1ull << l) - 1ull))] =
-                (int32_t)r;
+            order[(int64_t)Lr * gstride + (int64_t)base + __popcll(mine & ((1ull << l) - 1ull))] = (int32_t)r;
     } else {
         __syncthreads();
         const int local = valid ? atomicAdd(&s_cnt[gr], 1) : 0;
@@ -149,8 +153,8 @@ __device__ __forceinline__ void assign_body(const float* __restrict__ rows, int6
 #define VMN_ASSIGN_ARGS                                                                            \
     const float *__restrict__ rows, int64_t nrows, const __bf16 *__restrict__ cent,                \
         const float *__restrict__ cnorm, int C, int32_t *keys_out, int32_t *order,                 \
-        unsigned long long *heads, int G, int S, int64_t gstride
-#define VMN_ASSIGN_PASS rows, nrows, cent, cnorm, C, keys_out, order, heads, G, S, gstride
+        unsigned long long *heads, int G, int S, int64_t gstride, bool by_centroid
+#define VMN_ASSIGN_PASS rows, nrows, cent, cnorm, C, keys_out, order, heads, G, S, gstride, by_centroid
 template <int KB>
 __global__ __launch_bounds__(64) void assign_kernel_one(VMN_ASSIGN_ARGS) {
     assign_body<KB, 1>(VMN_ASSIGN_PASS);
@@ -236,35 +240,44 @@ __global__ void cent_init_kernel(const float* __restrict__ smp, int S, int DP, i
 
 }  // namespace
 
-// Rows per workgroup of the assignment: one wavefront's 16, or 256 past
-// 65,536 rows.
-static int assign_rows_per_block(int64_t nrows) { return nrows > 65536 ? 256 : 16; }
+// Lists of a batch of nrows queries: one per centroid (C lists of nrows
+// entries each), or past 65,536 queries G x S sub-lists bounded by the rows of
+// the workgroups (of 256 rows) that append to each.
+constexpr int64_t kCentroidListsMax = 65536;
 
-int64_t group_list_stride(int64_t nrows, int S) {
-    const int per = assign_rows_per_block(nrows);
-    const int64_t blocks = (nrows + per - 1) / per;
-    return (blocks + S - 1) / S * per;
+GroupLayout list_layout(int64_t nrows, int C, int G, int S) {
+    GroupLayout y;
+    y.by_centroid = nrows <= kCentroidListsMax;
+    if (y.by_centroid) {
+        y.nlists = C;
+        y.stride = nrows;
+    } else {
+        const int64_t blocks = (nrows + 255) / 256;
+        y.nlists = G * S;
+        y.stride = (blocks + S - 1) / S * 256;
+    }
+    return y;
 }
 
 hipError_t launch_assign(const float* rows, int64_t nrows, int DP, const void* cent,
                          const float* cnorm, int C, int32_t* keys_out, int32_t* order,
                          unsigned long long* heads, int G, int S, hipStream_t s) {
     if (nrows <= 0) return hipSuccess;
-    if (!keys_out && (G < 1 || G > 32 || S < 1 || G * S > 256)) return hipErrorInvalidValue;
+    if (!keys_out && (G < 1 || G > 32 || S < 1 || G * S > 256 || C > 256)) return hipErrorInvalidValue;
     if (DP % 32 || DP < 32 || DP > 256) return hipErrorInvalidValue;
-    if (S < 1) S = 1;
-    const int per = assign_rows_per_block(nrows);
+    const GroupLayout y = keys_out ? GroupLayout{true, 0, 0} : list_layout(nrows, C, G, S);
+    const bool wide = !y.by_centroid;
+    const int per = wide ? 256 : 16;
     const unsigned blocks = (unsigned)((nrows + per - 1) / per);
-    const int64_t stride = group_list_stride(nrows, S);
     const __bf16* cb = static_cast<const __bf16*>(cent);
 #define VMN_ASSIGN(KB)                                                                              \
     do {                                                                                            \
-        if (per > 16)                                                                               \
+        if (wide)                                                                                   \
             hipLaunchKernelGGL((assign_kernel_wide<KB>), dim3(blocks), dim3(1024), 0, s, rows,      \
-                               nrows, cb, cnorm, C, keys_out, order, heads, G, S, stride);          \
+                               nrows, cb, cnorm, C, keys_out, order, heads, G, S, y.stride, false); \
         else                                                                                        \
             hipLaunchKernelGGL((assign_kernel_one<KB>), dim3(blocks), dim3(64), 0, s, rows, nrows,  \
-                               cb, cnorm, C, keys_out, order, heads, G, S, stride);                 \
+                               cb, cnorm, C, keys_out, order, heads, G, S, y.stride, true);         \
     } while (0)
     switch (DP / 32) {
         case 1: VMN_ASSIGN(1); break;
@@ -334,13 +347,17 @@ int ensure_centroids(vmn_index* idx, bool* computed) {
     return VMN_OK;
 }
 
+vmn::GroupLayout group_layout(const vmn_index* idx, int64_t nq) {
+    return vmn::list_layout(nq, idx->ncent, idx->query_groups, VMN_GROUP_SUBLISTS);
+}
+
 int group_queries(vmn_index* idx, const float* q, int64_t nq, DevBuf& order,
-                  unsigned long long* heads, hipStream_t stream, int64_t* stride) {
-    const int G = idx->query_groups, S = VMN_GROUP_SUBLISTS;
-    *stride = vmn::group_list_stride(nq, S);
-    HIPCHK(order.reserve((size_t)G * S * *stride * sizeof(int32_t)));
+                  unsigned long long* heads, hipStream_t stream) {
+    const vmn::GroupLayout y = group_layout(idx, nq);
+    HIPCHK(order.reserve((size_t)y.nlists * y.stride * sizeof(int32_t)));
     HIPCHK(vmn::launch_assign(q, nq, idx->DP, idx->cent.p, idx->cent_norm.as<float>(), idx->ncent,
-                              nullptr, order.as<int32_t>(), heads, G, S, stream));
+                              nullptr, order.as<int32_t>(), heads, idx->query_groups,
+                              VMN_GROUP_SUBLISTS, stream));
     return VMN_OK;
 }
 
@@ -368,25 +385,25 @@ extern "C" int vmn_query_group_ids(vmn_index* idx, const float* queries, int64_t
     const float* q = nullptr;
     rc = stage_rows(idx, queries, nq, mem, idx->qpad, &q);
     if (rc) return rc;
-    const int G = idx->query_groups, S = VMN_GROUP_SUBLISTS;
-    HIPCHK(idx->counter.reserve((size_t)G * S * 256));
+    const int G = idx->query_groups;
+    const vmn::GroupLayout y = group_layout(idx, nq);
+    HIPCHK(idx->counter.reserve((size_t)y.nlists * 256));
     unsigned long long* heads = idx->counter.as<unsigned long long>();
-    HIPCHK(vmn::launch_fill_u64(heads, 32 * G * S, 0ull, idx->stream));
-    int64_t stride = 0;
-    rc = group_queries(idx, q, nq, idx->grp_order, heads, idx->stream, &stride);
+    HIPCHK(vmn::launch_fill_u64(heads, 32 * y.nlists, 0ull, idx->stream));
+    rc = group_queries(idx, q, nq, idx->grp_order, heads, idx->stream);
     if (rc) return rc;
-    std::vector<unsigned long long> h((size_t)32 * G * S);
-    std::vector<int32_t> ord((size_t)G * S * stride);
+    std::vector<unsigned long long> h((size_t)32 * y.nlists);
+    std::vector<int32_t> ord((size_t)y.nlists * y.stride);
     HIPCHK(hipMemcpyAsync(h.data(), heads, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           idx->stream));
     HIPCHK(hipMemcpyAsync(ord.data(), idx->grp_order.p, ord.size() * sizeof(int32_t),
                           hipMemcpyDeviceToHost, idx->stream));
     HIPCHK(hipStreamSynchronize(idx->stream));
     std::vector<uint8_t> gid((size_t)nq, 0xff);
-    for (int L = 0; L < G * S; ++L)
-        for (unsigned long long i = 0; i < h[32 * L + 1] && i < (unsigned long long)stride; ++i) {
-            const int g = L / S;
-            const int32_t r = ord[(size_t)L * stride + i];
+    for (int L = 0; L < y.nlists; ++L)
+        for (unsigned long long i = 0; i < h[32 * L + 1] && i < (unsigned long long)y.stride; ++i) {
+            const int g = y.by_centroid ? (int)((int64_t)L * G / idx->ncent) : L / VMN_GROUP_SUBLISTS;
+            const int32_t r = ord[(size_t)L * y.stride + i];
             if (r < 0 || r >= nq || gid[r] != 0xff)
                 return fail(VMN_ERR_STATE, "query group lists are not a partition of the batch");
             gid[r] = (uint8_t)g;

@@ -145,13 +145,14 @@ namespace vmn_rt {
 
 int set_device(const vmn_index* idx);
 // Query groups (one per XCD) for a batch of nq queries: wanted, the centroids
-// they come from (computed on the index stream, once), and the group lists of
-// a batch (written on `stream`; heads = 32 x groups x VMN_GROUP_SUBLISTS
-// zeroed counters; *stride receives the lists' stride).
+// they come from (computed on the index stream, once), the batch's list
+// layout, and its lists (written on `stream`; heads = 32 x nlists zeroed
+// counters).
 bool groups_wanted(const vmn_index* idx, int64_t nq);
 int ensure_centroids(vmn_index* idx, bool* computed);
+vmn::GroupLayout group_layout(const vmn_index* idx, int64_t nq);
 int group_queries(vmn_index* idx, const float* q, int64_t nq, DevBuf& order,
-                  unsigned long long* heads, hipStream_t stream, int64_t* stride);
+                  unsigned long long* heads, hipStream_t stream);
 // Rows of dimension D (host or device) placed as [n x DP] rows on the device.
 int stage_rows(vmn_index* idx, const float* src, int64_t n, int32_t mem, DevBuf& scratch,
                const float** out);

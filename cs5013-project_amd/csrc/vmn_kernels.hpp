@@ -39,15 +39,15 @@ struct SearchArgs {
     int32_t* stat_hops;       // [nq] expansions
     int32_t* stat_ndist;      // [nq] distance evaluations
     unsigned long long* counter;  // work queue head(s), zero before launch
-    // Query groups: with ngroups > 0, group g's queries sit in its
-    // VMN_GROUP_SUBLISTS sub-lists L = g * VMN_GROUP_SUBLISTS + s, list L at
+    // Query groups: with ngroups > 0 the batch's queries sit in nlists lists
+    // ordered by group (a group's lists consecutive), list L at
     // order[L * gstride ...] with counter[32 L + 1] entries and its queue head
-    // at counter[32 L]. A workgroup takes from the sub-lists of group
-    // blockIdx.x % ngroups first and from the other groups' once those are
-    // empty.
+    // at counter[32 L]. Workgroup b walks the lists cyclically from the first
+    // of group b % ngroups.
     const int32_t* order;
     int64_t gstride;
     int ngroups;
+    int nlists;
     unsigned long long* acc;      // build work counters (VMN_ACC_*) or null
 };
 
@@ -150,17 +150,23 @@ hipError_t launch_merge_staged(const int32_t* gi, const float* gd, const int32_t
                                float* rd, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
-// Sub-lists per query group (separate counters keep the atomics of the
-// lists' appends and takes apart).
+// Sub-lists per query group in the bounded layout (separate counters keep
+// the atomics of the lists' appends and takes apart).
 #define VMN_GROUP_SUBLISTS 8
+// Lists of a grouped batch: one per centroid (by_centroid), or
+// groups x VMN_GROUP_SUBLISTS bounded ones; stride entries each.
+struct GroupLayout {
+    bool by_centroid;
+    int nlists;
+    int64_t stride;
+};
+GroupLayout list_layout(int64_t nrows, int C, int G, int S);
 // Nearest of C BF16 centroids per row: its index to keys_out, or the row
-// appended to a sub-list of its group (index * G / C) among S per group,
-// list L's entries at order[L * group_list_stride(nrows, S) ...], its count at
-// heads[32 L + 1].
+// appended to its list of list_layout(nrows, C, G, S) (entries at
+// order[L * stride ...], count at heads[32 L + 1]).
 hipError_t launch_assign(const float* rows, int64_t nrows, int DP, const void* cent,
                          const float* cnorm, int C, int32_t* keys_out, int32_t* order,
                          unsigned long long* heads, int G, int S, hipStream_t s);
-int64_t group_list_stride(int64_t nrows, int S);
 hipError_t launch_reverse_count(const int32_t* pair_dst, int64_t npairs, int32_t* counts,
                                 hipStream_t s);
 hipError_t launch_exclusive_scan(const int32_t* counts, int64_t n, int64_t* off, int64_t* tmp,

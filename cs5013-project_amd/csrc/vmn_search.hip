@@ -81,15 +81,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
     const float* __restrict__ vecs = a.vecs;
     const int W = a.W;
 
-    // query groups: this workgroup's walk over the sub-lists (lane 0), its
-    // own group's first; lists only empty during a launch, so the walk only
+    // query groups: this workgroup's cyclic walk over the lists (lane 0), from
+    // its group's first; lists only empty during a launch, so the walk only
     // moves forward
-    constexpr int SL = VMN_GROUP_SUBLISTS;
-    const int nlists = a.ngroups * SL;
-    const int g0 = a.ngroups > 0 ? (int)(blockIdx.x % (unsigned)a.ngroups) : 0;
-    const int s0 = a.ngroups > 0 ? (int)((blockIdx.x / (unsigned)a.ngroups) % SL) : 0;
+    const int nlists = a.ngroups > 0 ? a.nlists : 0;
+    const int fstart = a.ngroups > 0
+                           ? (int)(((int64_t)(blockIdx.x % (unsigned)a.ngroups) * nlists + a.ngroups - 1) / a.ngroups)
+                           : 0;
     int ft = 0;
-    unsigned long long fcnt = 0;  // entries of list ft, once read
+    unsigned long long fcnt = 0;  // entries of the list at step ft, once read
     bool fknown = false;
 
     for (;;) {
@@ -97,9 +97,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
             int64_t qsel = a.nq;
             if (a.ngroups > 0) {
                 for (; ft < nlists; ++ft, fknown = false) {
-                    int g = g0 + ft / SL;
-                    if (g >= a.ngroups) g -= a.ngroups;
-                    const int L = g * SL + (s0 + ft) % SL;
+                    const int L = fstart + ft < nlists ? fstart + ft : fstart + ft - nlists;
                     unsigned long long* h = a.counter + 32 * L;
                     if (!fknown) {
                         // first visit: an emptied list is passed over without
