@@ -298,6 +298,8 @@ hipError_t launch_assign(const float* rows, int64_t nrows, int DP, const void* c
 namespace vmn_rt {
 
 // Sample size and centroid count of the grouping's k-means, and its rounds.
+// 64 centroids: on the 1M x 128 mixture 32 leave FP32 batches 3% slower and
+// 128 / 256 leave BF16 batches 5% / 12% slower (more, smaller lists to walk).
 constexpr int kSample = 16384;
 constexpr int kCentroids = 64;
 constexpr int kRounds = 6;
