@@ -93,6 +93,8 @@ int alloc_graph(vmn_index* idx, int64_t n) {
     HIPCHK(vmn::launch_fill_i32(idx->del_bits.as<int32_t>(), (cap + 31) / 32, 0, idx->stream));
     if (idx->p.low_precision & VMN_STORE_FP8)
         HIPCHK(idx->adj_fp8n.reserve((size_t)cap * idx->p.R * sizeof(float)));
+    if (idx->p.low_precision & VMN_STORE_FP8)
+        HIPCHK(idx->adj_hash.reserve((size_t)cap * idx->p.R * sizeof(uint32_t)));
     return VMN_OK;
 }
 
@@ -213,6 +215,7 @@ void search_args(const vmn_index* idx, vmn::SearchArgs& a, const float* q, int64
         a.fp8_scales = idx->fp8_scales.as<float>();
         a.fp8_centre = idx->fp8_centre.as<float>();
     }
+    if (mode == VMN_MODE_FRESH && prec == VMN_PREC_FP8) a.adj_hash = idx->adj_hash.as<uint32_t>();
     a.lcap = idx->list_cap;
     // a batch that leaves most of the GPU idle is latency-bound: there the
     // BF16 row norms are read from the table (an extra 4-byte load per
@@ -220,17 +223,29 @@ void search_args(const vmn_index* idx, vmn::SearchArgs& a, const float* q, int64
     a.table_norms = nq <= (int64_t)idx->cus * 4;
 }
 
-int prepare_search(vmn_index* idx, int prec, bool sync) {
-    if (prec != VMN_PREC_FP8 || idx->adj_fp8n_epoch == idx->epoch) return VMN_OK;
-    // allocated with the graph (alloc_graph) for indexes that keep FP8 rows
-    if (idx->adj_fp8n.bytes < (size_t)idx->cap * idx->p.R * sizeof(float))
+int prepare_search(vmn_index* idx, int prec, int mode, bool sync) {
+    const bool norms = prec == VMN_PREC_FP8 && idx->adj_fp8n_epoch != idx->epoch;
+    // the FP8 FRESH search reads the neighbours' bloom cells
+    const bool hashes = mode == VMN_MODE_FRESH && prec == VMN_PREC_FP8 &&
+                        idx->adj_hash_epoch != idx->epoch;
+    if (!norms && !hashes) return VMN_OK;
+    // both allocated with the graph (alloc_graph); the norms for indexes that
+    // keep FP8 rows
+    if (norms && idx->adj_fp8n.bytes < (size_t)idx->cap * idx->p.R * sizeof(float))
         return fail(VMN_ERR_STATE, "FP8 norms array missing (index keeps no FP8 rows)");
-    HIPCHK(vmn::launch_adj_norms(idx->adj.as<int32_t>(), idx->n * idx->p.R,
-                                 idx->fp8_norms.as<float>(), idx->adj_fp8n.as<float>(),
-                                 idx->stream));
-    // searches on other streams (contexts) read it next
+    if (hashes && idx->adj_hash.bytes < (size_t)idx->cap * idx->p.R * sizeof(uint32_t))
+        return fail(VMN_ERR_STATE, "adjacency hash array missing");
+    if (norms)
+        HIPCHK(vmn::launch_adj_norms(idx->adj.as<int32_t>(), idx->n * idx->p.R,
+                                     idx->fp8_norms.as<float>(), idx->adj_fp8n.as<float>(),
+                                     idx->stream));
+    if (hashes)
+        HIPCHK(vmn::launch_adj_hash(idx->adj.as<int32_t>(), idx->n * idx->p.R,
+                                    idx->adj_hash.as<uint32_t>(), idx->stream));
+    // searches on other streams (contexts) read them next
     if (sync) HIPCHK(hipStreamSynchronize(idx->stream));
-    idx->adj_fp8n_epoch = idx->epoch;
+    if (norms) idx->adj_fp8n_epoch = idx->epoch;
+    if (hashes) idx->adj_hash_epoch = idx->epoch;
     return VMN_OK;
 }
 
@@ -245,7 +260,7 @@ int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int
     DevBuf& counter = ctx ? ctx->counter : idx->counter;
     DevBuf& stat_hops = ctx ? ctx->stat_hops : idx->stat_hops;
     DevBuf& stat_ndist = ctx ? ctx->stat_ndist : idx->stat_ndist;
-    int rc = prepare_search(idx, prec, true);
+    int rc = prepare_search(idx, prec, mode, true);
     if (rc) return rc;
     const int ng = groups_wanted(idx, nq) ? idx->query_groups : 0;
     if (ng) {
@@ -507,7 +522,10 @@ int fresh_pass(vmn_index* idx, int64_t base, int64_t cnt, float alpha) {
     a.node_list = idx->rlist.as<int64_t>();
     a.rev_src = idx->rsrc.as<int32_t>();
     a.rev_d = idx->rd.as<float>();
-    return tiered_prune(idx, a, (int64_t)nlist, true);
+    rc = tiered_prune(idx, a, (int64_t)nlist, true);
+    // the graph changed: the next search lays out its adjacency-side arrays again
+    ++idx->epoch;
+    return rc;
 }
 
 int build_fresh(vmn_index* idx, bool random_init) {

@@ -46,18 +46,64 @@ __device__ __forceinline__ int readlane_i(int v, int l) {
     return __builtin_amdgcn_readlane(v, l);
 }
 
-// Sum over the 8 lanes of a row group (lanes 8g..8g+7).
-__device__ __forceinline__ float wave_min_f(float v) {
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) v = fminf(v, __shfl_xor(v, o));
+// v_writelane_b32 (the compiler's intrinsic; it has no builtin here): `old`
+// with lane l replaced by the uniform value v.
+__device__ int writelane_i(int v, int l, int old) __asm("llvm.amdgcn.writelane.i32");
+
+// DPP lane moves within rows of 16 lanes: quad_perm [1,0,3,2] and [2,3,0,1]
+// read lanes l^1 and l^2; row_half_mirror reads lane 7 - l of the 8-lane
+// half, row_mirror lane 15 - l of the row. Every source lane is in the row,
+// so the moves fold into the instruction that consumes them.
+#define VMN_DPP_XOR1 0xB1
+#define VMN_DPP_XOR2 0x4E
+#define VMN_DPP_HALF_MIRROR 0x141
+#define VMN_DPP_MIRROR 0x140
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+
+// Order-preserving integer key of a distance: x < y iff fkey(x) < fkey(y)
+// (signed), equal values have equal keys (-0 is folded into +0 first).
+__device__ __forceinline__ int fkey(float x) {
+    const int b = __float_as_int(x + 0.0f);
+    return b ^ ((b >> 31) & 0x7fffffff);
+}
+
+// Minimum of k over the wave, returned uniform: within each row by DPP, then
+// over the four rows' results in scalar registers.
+__device__ __forceinline__ int wave_min_key(int k) {
+    k = min(k, dpp_i<VMN_DPP_XOR1>(k));
+    k = min(k, dpp_i<VMN_DPP_XOR2>(k));
+    k = min(k, dpp_i<VMN_DPP_HALF_MIRROR>(k));
+    k = min(k, dpp_i<VMN_DPP_MIRROR>(k));
+    return min(min(readlane_i(k, 0), readlane_i(k, 16)), min(readlane_i(k, 32), readlane_i(k, 48)));
+}
+
+// Sum over the 8 lanes of a row group (lanes 8g..8g+7), every lane of the
+// group receiving it: pairs l, l^1, then l, l^2, then the two quads (the
+// quads are uniform by then, so the mirror reads the l^4 quad's value).
+__device__ __forceinline__ float reduce8(float v) {
+    v += dpp_f<VMN_DPP_XOR1>(v);
+    v += dpp_f<VMN_DPP_XOR2>(v);
+    v += dpp_f<VMN_DPP_HALF_MIRROR>(v);
     return v;
 }
 
-__device__ __forceinline__ float reduce8(float v) {
-    v += __shfl_xor(v, 1);
-    v += __shfl_xor(v, 2);
-    v += __shfl_xor(v, 4);
-    return v;
+// (v_l + v_{l^16}) + (v_{l^32} + v_{l^48}) in every lane l, each sum taken
+// over the pair of 16-lane rows (then halves) that v_permlane16/32_swap put
+// side by side (an addition's operand order does not change its result).
+__device__ __forceinline__ float sum_rows4(float v) {
+    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    v = __uint_as_float(p[0]) + __uint_as_float(p[1]);
+    const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(q[0]) + __uint_as_float(q[1]);
 }
 
 // Query (or pivot) vector held in registers: lane sub (0..7) holds float4
@@ -243,8 +289,7 @@ __device__ __forceinline__ void prep_query(QLow<DP>& Q, const float* qf, const f
             }
         }
         // lanes of one 16-lane group hold the same slice; sum the 4 groups
-        part += __shfl_xor(part, 16);
-        part += __shfl_xor(part, 32);
+        part = sum_rows4(part);
         Q.qn = part;
         Q.t = 1.0f;
     } else {
@@ -352,10 +397,7 @@ __device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* 
                             for (int e = 0; e < 4; ++e) { const bf16x2_t p = __builtin_bit_cast(bf16x2_t, w[e]); x[t] = __builtin_amdgcn_fdot2_f32_bf16(p, p, x[t], false); }
                         }
                     }
-                    if constexpr (!TN) {
-                        x[t] += __shfl_xor(x[t], 16);
-                        x[t] += __shfl_xor(x[t], 32);
-                    }
+                    if constexpr (!TN) x[t] = sum_rows4(x[t]);
                 }
             }
         } else if constexpr (DP % 64 == 0) {
@@ -448,6 +490,13 @@ __host__ __device__ __forceinline__ uint32_t bf_hash2(uint32_t x) {
     h = (h ^ ((x >> 16) & 0xffu)) * 0x1B3ull;
     h = (h ^ ((x >> 24) & 0xffu)) * 0x1B3ull;
     return (uint32_t)(h % VMN_BF_CELLS);
+}
+
+// Both cells of id x, h1 | h2 << 16 (each below VMN_BF_CELLS < 2^16). The
+// FRESH search reads these from a table laid out beside the adjacency
+// (adj_hash_kernel) instead of hashing every neighbour on every hop.
+__host__ __device__ __forceinline__ uint32_t bf_hash_pair(uint32_t x) {
+    return bf_hash1(x) | (bf_hash2(x) << 16);
 }
 
 __device__ __forceinline__ bool bit_test(const uint32_t* bits, uint32_t i) {
@@ -544,43 +593,59 @@ __device__ __forceinline__ int merge_sorted(const uint32_t* src_id, const float*
 }
 
 // merge_sorted<CAP, false> for new candidates left in place: the lanes set
-// in `km` hold them (my_id / my_d), in lane order for ties. Same merged order
-// as compacting them to lanes 0.. and calling merge_sorted, without the
-// compaction's LDS round trip.
+// in `km` hold them (my_id / my_d, my_k = fkey(my_d)), in lane order for
+// ties. Same merged order as compacting them to lanes 0.. and calling
+// merge_sorted, without the compaction's LDS round trip. The candidates are
+// visited in lane order t, comparing order-preserving integer keys:
+//   * list entries at or below candidate t's key: the list entries placed
+//     before it (written into lane t);
+//   * list entry j counts the candidates at or above its key (placed after
+//     it); the others are placed before it;
+//   * candidate c counts the candidates before it: keys below c's, and equal
+//     keys from lower lanes. Lane c compares against thr = my_k + 1 (at or
+//     below) until t reaches c, when thr drops to my_k (strictly below).
 template <int CAP>
 __device__ __forceinline__ int merge_masked(const uint32_t* src_id, const float* src_d, int cnt,
-                                            uint32_t my_id, float my_d, uint64_t km,
+                                            uint32_t my_id, float my_d, int my_k, uint64_t km,
                                             uint32_t* dst_id, float* dst_d, int limit) {
     constexpr int PER = (CAP + 63) / 64;
     const int l = lane();
     float bdv[PER];
     uint32_t bidv[PER];
-    int less[PER];
+    int bk[PER], ab[PER];
+    uint64_t live[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
         const int bi = l + 64 * j;
         bdv[j] = bi < cnt ? src_d[bi] : 0.0f;
         bidv[j] = bi < cnt ? src_id[bi] : 0u;
-        less[j] = 0;
+        bk[j] = fkey(bdv[j]);
+        ab[j] = 0;
+        const int nl = cnt - 64 * j;
+        live[j] = nl >= 64 ? ~0ull : (nl > 0 ? (1ull << nl) - 1ull : 0ull);
     }
-    int rank = 0, ub = 0;
+    int rank = 0, ub = 0, thr = my_k + 1;
     for (uint64_t m = km; m; m &= m - 1) {
         const int t = __ffsll((unsigned long long)m) - 1;
-        const float dt = readlane_f(my_d, t);
+        const int kt = readlane_i(my_k, t);
         int c = 0;
 #pragma unroll
-        for (int j = 0; j < PER; ++j)
-            c += __popcll((unsigned long long)ballot((l + 64 * j) < cnt && bdv[j] <= dt));
-        if (l == t) ub = c;
-        rank += (dt < my_d) || (dt == my_d && t < l);
-#pragma unroll
-        for (int j = 0; j < PER; ++j) less[j] += dt < bdv[j];
+        for (int j = 0; j < PER; ++j) {
+            const bool at_or_below = bk[j] <= kt;
+            c += __popcll((unsigned long long)(ballot(at_or_below) & live[j]));
+            ab[j] += at_or_below ? 1 : 0;
+        }
+        ub = writelane_i(c, t, ub);
+        thr = writelane_i(kt, t, thr);
+        rank += kt < thr ? 1 : 0;
     }
+    const int nk = __popcll((unsigned long long)km);
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
         const int bi = l + 64 * j;
         if (bi < cnt) {
-            const int pos = bi + less[j];
+            // candidates with keys below entry j's: all but those at or above it
+            const int pos = bi + (nk - ab[j]);
             if (pos < limit) {
                 dst_id[pos] = bidv[j];
                 dst_d[pos] = bdv[j];
@@ -594,7 +659,7 @@ __device__ __forceinline__ int merge_masked(const uint32_t* src_id, const float*
             dst_d[pos] = my_d;
         }
     }
-    const int nc = cnt + __popcll((unsigned long long)km);
+    const int nc = cnt + nk;
     return nc < limit ? nc : limit;
 }
 

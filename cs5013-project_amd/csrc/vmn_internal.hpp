@@ -122,6 +122,10 @@ struct vmn_index {
     // norm of adj[i] at i), valid for index epoch adj_fp8n_epoch
     vmn_rt::DevBuf adj_fp8n;
     uint64_t adj_fp8n_epoch = ~0ull;
+    // FRESH searches: both bloom cells of every adjacency entry ([cap x R],
+    // bf_hash_pair(adj[i]) at i), valid for index epoch adj_hash_epoch
+    vmn_rt::DevBuf adj_hash;
+    uint64_t adj_hash_epoch = ~0ull;
     // bumped by every change of the graph, tombstones or rows: a search
     // captured as a HIP graph holds the entry point, the tombstone pointer
     // and the buffers of its capture, so it may only be replayed while this
@@ -194,7 +198,7 @@ int search_grid(const vmn_index* idx, int64_t nq);
 // Makes the buffers a search of precision `prec` reads current (FP8: the
 // adjacency-aligned row norms, rebuilt on the index stream when the index
 // changed; sync waits for it, for searches on other streams).
-int prepare_search(vmn_index* idx, int prec, bool sync);
+int prepare_search(vmn_index* idx, int prec, int mode, bool sync);
 // Search kernel over `nq` padded query rows [nq x DP] from idx->medoid.
 int run_search(vmn_index* idx, const float* q, int64_t nq, int L, int out_n, int32_t* out_ids,
                float* out_d, bool queries_are_nodes, int mode, bool want_vis,

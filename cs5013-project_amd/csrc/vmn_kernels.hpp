@@ -31,6 +31,7 @@ struct SearchArgs {
     const void* low_rows;     // [cap x DP] BF16 or FP8 rows
     const float* low_norms;   // [cap] ||x_hat||^2 of the low-precision rows
     const float* adj_norms;   // FP8: [cap x W] ||x_hat||^2 of adj[i] beside entry i (0 past -1)
+    const uint32_t* adj_hash; // FRESH: [cap x W] bf_hash_pair(adj[i]) beside entry i (0 past -1)
     const float* fp8_scales;  // [DP] per-dimension FP8 scales
     const float* fp8_centre;  // [DP] per-dimension centre the FP8 rows are coded about
     int krr;                  // > 0: exact FP32 re-rank of the first krr list entries (<= 64)
@@ -74,6 +75,8 @@ hipError_t launch_search(const SearchArgs& a, int grid, hipStream_t s);
 // out[i] = norms[adj[i]] (0 where adj[i] is -1) for the n x W entries.
 hipError_t launch_adj_norms(const int32_t* adj, int64_t entries, const float* norms, float* out,
                             hipStream_t s);
+// out[i] = bf_hash_pair(adj[i]) (0 where adj[i] is -1) for the n x W entries.
+hipError_t launch_adj_hash(const int32_t* adj, int64_t entries, uint32_t* out, hipStream_t s);
 // Clears *flag unless every row of [from, to) makes BF16 traversal exact.
 hipError_t launch_bf16_exact(const float* vecs, int64_t from, int64_t to, int DP, unsigned int* flag,
                              hipStream_t s);

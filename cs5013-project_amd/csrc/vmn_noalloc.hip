@@ -83,7 +83,7 @@ extern "C" int vmn_search_noalloc(vmn_index* idx, const float* queries, int64_t 
     // FP8 reads row norms laid out beside the adjacency (allocated with the
     // graph): after an index update they are refreshed on the index stream,
     // ahead of this search
-    rc = prepare_search(idx, prec, false);
+    rc = prepare_search(idx, prec, idx->p.mode, false);
     if (rc) return rc;
     unsigned long long* counter = reinterpret_cast<unsigned long long*>(base);
     const float* q = queries;

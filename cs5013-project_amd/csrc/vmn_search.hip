@@ -68,6 +68,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
     constexpr bool TN = PREC_ARG == VMN_PREC_BF16_TABLE_NORMS;
     // FP32 distances in the reference's summation order (PARTH)
     constexpr bool SEQ = MODE == VMN_MODE_PARTH;
+    // FRESH bloom cells of the neighbours: read from the table beside the
+    // adjacency by the FP8 form (its hops are short of ALU time), hashed in
+    // registers by the FP32 and BF16 forms (short of HBM bandwidth: the
+    // table's 4 bytes per neighbour cost them more than the hashing)
+    constexpr bool HTAB = MODE == VMN_MODE_FRESH && PREC == VMN_PREC_FP8;
     __shared__ __align__(16) uint32_t s_id[2][LCAP];
     __shared__ __align__(16) float s_d[2][LCAP];
     __shared__ __align__(16) int32_t s_sid[64];
@@ -172,6 +177,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
         // adjacency row of the node predicted to be expanded next, loaded
         // before the merge that decides it (pf: its id, -1 for none)
         int32_t pf = -1, pf_nb = -1;
+        uint32_t pf_nh = 0u;
         float pf_nbn = 0.0f;
         while (hops < a.max_hops) {
             ++hops;
@@ -223,14 +229,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
                     ent[k] = i < cnt ? (s_id[cur][i] & VMN_ID_MASK) : 0xffffffffu;
                 }
             }
-            // neighbours up to the first -1 (FP8: with their row norms)
+            // neighbours up to the first -1 (FRESH: with their bloom cells;
+            // FP8: with their row norms)
             int32_t nb;
+            uint32_t nh = 0u;
             float nbn = 0.0f;
             if (node == pf) {
                 nb = pf_nb;
+                nh = pf_nh;
                 nbn = pf_nbn;
             } else {
                 nb = l < W ? a.adj[(int64_t)node * W + l] : -1;
+                if constexpr (HTAB) nh = l < W ? a.adj_hash[(int64_t)node * W + l] : 0u;
                 if constexpr (PREC == VMN_PREC_FP8)
                     nbn = l < W ? a.adj_norms[(int64_t)node * W + l] : 0.0f;
             }
@@ -272,13 +282,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
             } else {
                 valid = valid && !gone;
                 __syncthreads();
-                uint32_t h1 = 0, h2 = 0;
+                if constexpr (!HTAB) nh = valid ? bf_hash_pair((uint32_t)nb) : 0u;
+                const uint32_t h1 = nh & 0xffffu, h2 = nh >> 16;
                 bool seen = true;
-                if (valid) {
-                    h1 = bf_hash1((uint32_t)nb);
-                    h2 = bf_hash2((uint32_t)nb);
-                    seen = bit_test(s_bloom, h1) && bit_test(s_bloom, h2);
-                }
+                // both cells read at once (no short circuit)
+                if (valid) seen = bit_test(s_bloom, h1) & bit_test(s_bloom, h2);
                 __syncthreads();
                 valid = valid && !seen;
                 if (valid) {
@@ -309,6 +317,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
             const float thr = cnt == a.L ? s_d[cur][cnt - 1] : INFINITY;
             const uint32_t my_id = l < nnew ? (uint32_t)s_sid[l] : 0u;
             const float my_d = l < nnew ? s_sd[l] : 0.0f;
+            const int my_k = fkey(my_d);
             const uint64_t km = ballot(l < nnew && my_d < thr);
             // The merged list's closest unexpanded entry is the closest kept
             // candidate (lowest slot on ties) if it is below p2's distance
@@ -318,19 +327,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
                 const uint64_t bm = km & ballot(p2 < 0 || my_d < nd2);
                 pf = -1;
                 if (bm) {
-                    const float mn = wave_min_f(((bm >> l) & 1ull) ? my_d : INFINITY);
-                    const uint64_t mm = bm & ballot(my_d == mn);
+                    const int mn = wave_min_key(((bm >> l) & 1ull) ? my_k : 0x7fffffff);
+                    const uint64_t mm = bm & ballot(my_k == mn);
                     if (mm) pf = readlane_i((int)my_id, __ffsll((unsigned long long)mm) - 1);
                 } else if (p2 >= 0) {
                     pf = (int32_t)(e2 & VMN_ID_MASK);
                 }
                 if (pf >= 0) {
                     pf_nb = l < W ? a.adj[(int64_t)pf * W + l] : -1;
+                    if constexpr (HTAB) pf_nh = l < W ? a.adj_hash[(int64_t)pf * W + l] : 0u;
                     if constexpr (PREC == VMN_PREC_FP8)
                         pf_nbn = l < W ? a.adj_norms[(int64_t)pf * W + l] : 0.0f;
                 }
             }
-            cnt = merge_masked<LCAP>(s_id[cur], s_d[cur], cnt, my_id, my_d, km, s_id[cur ^ 1],
+            cnt = merge_masked<LCAP>(s_id[cur], s_d[cur], cnt, my_id, my_d, my_k, km, s_id[cur ^ 1],
                                      s_d[cur ^ 1], a.L);
             cur ^= 1;
             __syncthreads();
@@ -429,8 +439,23 @@ hipError_t launch_adj_norms(const int32_t* adj, int64_t entries, const float* no
     return hipGetLastError();
 }
 
+__global__ void adj_hash_kernel(const int32_t* __restrict__ adj, int64_t entries, uint32_t* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= entries) return;
+    const int32_t v = adj[i];
+    out[i] = v >= 0 ? bf_hash_pair((uint32_t)v) : 0u;
+}
+hipError_t launch_adj_hash(const int32_t* adj, int64_t entries, uint32_t* out, hipStream_t s) {
+    if (entries <= 0) return hipSuccess;
+    hipLaunchKernelGGL(adj_hash_kernel, dim3((unsigned)((entries + 255) / 256)), dim3(256), 0, s,
+                       adj, entries, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_search(const SearchArgs& a, int grid, hipStream_t s) {
     if (a.prec == VMN_PREC_FP8 && !a.adj_norms) return hipErrorInvalidValue;
+    if (a.mode == VMN_MODE_FRESH && a.prec == VMN_PREC_FP8 && !a.adj_hash)
+        return hipErrorInvalidValue;
     switch (a.DP) {
         case 32: return launch_search_d<32>(a, grid, s);
         case 64: return launch_search_d<64>(a, grid, s);
