@@ -228,14 +228,16 @@ def main():
         launch_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
         rec = vmn.recall_at_k(ids.cpu().numpy(), gt, 10)
         rec2 = vmn.recall_at_k(ids2.cpu().numpy(), gt2, 10)
-        kernel = idx.search_kernel_name(k=10, L=L, prec=prec, list_cap=tune["list_cap"])
+        kernel = idx.search_kernel_name(k=10, L=L, prec=prec, list_cap=tune["list_cap"],
+                                        tiles=tune["tiles"])
         idx.searchPoints(q_t, k=10, L=L, prec=prec, rerank=rr)  # same traversal, for the counters
         hops, nd = idx.search_stats()
         # per query: the FP32 query row, each expansion's adjacency row (FP8:
-        # and the row norms stored beside it), each measured candidate's row,
-        # the re-ranked FP32 rows, the k results
+        # and the row norms stored beside it, and in FRESH mode the bloom
+        # cells), each measured candidate's row, the re-ranked FP32 rows, the
+        # k results
         row_b = {"fp32": d * 4, "bf16": d * 2, "fp8": d}[prec]
-        adj_b = 4 * args.R * (2 if prec == "fp8" else 1)
+        adj_b = 4 * args.R * ((3 if args.mode == "fresh" else 2) if prec == "fp8" else 1)
         bytes_launch = (d * 4) * nq + hops * adj_b + nd * row_b + nq * rr * d * 4 \
             + nq * 10 * 8
         timed[form] = {"L": L, "recall@10": round(rec, 4), "recall@10_set2": round(rec2, 4),
@@ -247,7 +249,7 @@ def main():
                        "kernel_ms_serial": kern_ms,
                        "achieved_GBs_serial": bytes_launch / (kern_ms / 1e3) / 1e9,
                        "launch": {"waves_per_cu": tune["waves_per_cu"], "list_cap": tune["list_cap"],
-                                  "query_groups": tune["groups"]},
+                                  "tiles": tune["tiles"], "query_groups": tune["groups"]},
                        "kernel": kernel, "rerank": rr,
                        "traffic": measured_traffic(form, n, nq, L, kernel, rr),
                        "counters": measured_counters(form, n, nq, L, kernel, rr, launch_ms),
@@ -255,7 +257,7 @@ def main():
         log(f"timed {form:12s} L={L} {timed[form]['qps']:,.0f} QPS (serial {timed[form]['qps_serial']:,.0f})"
             f" recall {rec:.4f}/{rec2:.4f} launch {launch_ms:.3f} ms overlapped, {kern_ms:.3f} ms"
             f" serial, {timed[form]['achieved_GBs']:.0f} GB/s"
-            f" (launch {tune['waves_per_cu']} waves/CU, cap {tune['list_cap']},"
+            f" (launch {tune['waves_per_cu']} waves/CU, cap {tune['list_cap']}, {tune['tiles']} tiles,"
             f" {tune['groups']} query groups)")
     for c in ctx:
         c.close()
@@ -265,6 +267,7 @@ def main():
     Lh = chosen["bf16+rerank"][0]["L"]
     rrh = min(args.rerank, Lh)
     idx.set_launch(timed["bf16+rerank"]["launch"]["waves_per_cu"], timed["bf16+rerank"]["launch"]["list_cap"])
+    idx.set_tiles(timed["bf16+rerank"]["launch"]["tiles"])
     if timed["bf16+rerank"]["launch"]["query_groups"] is not None:
         idx.set_query_groups(timed["bf16+rerank"]["launch"]["query_groups"])
     idx.searchPoints(qs, k=10, L=Lh, prec="bf16", rerank=rrh)

@@ -217,6 +217,9 @@ void search_args(const vmn_index* idx, vmn::SearchArgs& a, const float* q, int64
     }
     if (mode == VMN_MODE_FRESH && prec == VMN_PREC_FP8) a.adj_hash = idx->adj_hash.as<uint32_t>();
     a.lcap = idx->list_cap;
+    // the tile shape applies where its kernels exist (FP32 gathers 16 rows a round)
+    const int need = L > idx->list_cap ? L : idx->list_cap;
+    a.tiles = prec != VMN_PREC_FP32 && idx->tiles == 4 && vmn::search_tiles4(idx->DP, need, mode) ? 4 : 2;
     // a batch that leaves most of the GPU idle is latency-bound: there the
     // BF16 row norms are read from the table (an extra 4-byte load per
     // candidate) instead of being summed from the row registers
@@ -1455,6 +1458,13 @@ int vmn_ctx_replay(vmn_ctx* c) {
     int rc = set_device(c->idx);
     if (rc) return rc;
     HIPCHK(hipGraphLaunch(c->exec, c->stream));
+    return VMN_OK;
+}
+
+int vmn_set_tiles(vmn_index* idx, int32_t tiles) {
+    if (!idx) return fail(VMN_ERR_ARG, "NULL index");
+    if (tiles != 0 && tiles != 2 && tiles != 4) return fail(VMN_ERR_ARG, "tiles must be 0, 2 or 4 (%d)", tiles);
+    idx->tiles = tiles == 0 ? 2 : tiles;
     return VMN_OK;
 }
 

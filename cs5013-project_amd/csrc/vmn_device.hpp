@@ -20,10 +20,9 @@
 #include <stdint.h>
 
 #define VMN_WAVE 64
-// MFMA tiles of 16 candidates whose rows are in flight together
-#ifndef VMN_GATHER_TILES
+// MFMA tiles of 16 candidates whose rows are in flight together (the prunes'
+// and the search's default; the search autotuner may take 4)
 #define VMN_GATHER_TILES 2
-#endif
 #define VMN_VIS_BIT 0x80000000u
 #define VMN_ID_MASK 0x7fffffffu
 

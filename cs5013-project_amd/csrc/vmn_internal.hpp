@@ -114,6 +114,7 @@ struct vmn_index {
     int64_t n_pend = 0;
     int waves_per_cu = 32;  // search grid = CUs x waves_per_cu (persistent work queue)
     int list_cap = 0;       // minimum beam-list capacity of the search kernel
+    int tiles = 2;          // low-precision search: MFMA tiles gathered per round (2 or 4)
     // work counters of the last build (VMN_ACC_*), counted while build_count
     vmn_rt::DevBuf build_acc;
     bool build_count = false;

@@ -37,6 +37,7 @@ struct SearchArgs {
     int krr;                  // > 0: exact FP32 re-rank of the first krr list entries (<= 64)
     int lcap;                 // minimum list capacity template (0: smallest >= L)
     int table_norms;          // BF16, FRESH: row norms from low_norms (latency-bound batches)
+    int tiles;                // BF16 / FP8: MFMA tiles of 16 candidates gathered per round (2 or 4)
     int32_t* stat_hops;       // [nq] expansions
     int32_t* stat_ndist;      // [nq] distance evaluations
     unsigned long long* counter;  // work queue head(s), zero before launch
@@ -72,6 +73,9 @@ __device__ __forceinline__ unsigned long long* acc_slot(unsigned long long* acc)
 }
 
 hipError_t launch_search(const SearchArgs& a, int grid, hipStream_t s);
+// Whether search kernels with 4 gather tiles exist for padded width DP, beam
+// capacity need (max of L and the list-capacity minimum) and mode.
+bool search_tiles4(int DP, int need, int mode);
 // out[i] = norms[adj[i]] (0 where adj[i] is -1) for the n x W entries.
 hipError_t launch_adj_norms(const int32_t* adj, int64_t entries, const float* norms, float* out,
                             hipStream_t s);

@@ -60,10 +60,17 @@ __global__ void fill_zero_u32(unsigned int* p, int n) {
 // VGPRs, no spills; the compiler's own choice takes 4): large batches are
 // bandwidth-bound and run faster with the extra wave (FP8 10k-query batch
 // 7%), single queries about 3% slower. PARTH FP32 / BF16 would spill there.
-template <int DP, int LCAP, int MODE, int PREC_ARG>
+// NG is the low-precision forms' tile shape: MFMA tiles of 16 candidates
+// gathered per round. 2 (32 candidates) is every form's default; 4 (64
+// candidates, held to 4 waves per SIMD, 128 VGPRs) fetches a whole hop's new
+// candidates in one round (half of all hops need two rounds of 32), which
+// shortens the latency of small batches; instantiated for FRESH BF16 / FP8
+// at row widths 64 to 128 with lists up to 256 (the autotuner's choice).
+template <int DP, int LCAP, int MODE, int PREC_ARG, int NG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
-    (DP <= 128 && ((MODE == VMN_MODE_FRESH && LCAP <= 256) ||
-                   (PREC_ARG == VMN_PREC_FP8 && LCAP <= 128))) ? 5 : 1, 8))) void search_kernel(SearchArgs a) {
+    NG == 4 ? 4
+            : (DP <= 128 && ((MODE == VMN_MODE_FRESH && LCAP <= 256) ||
+                             (PREC_ARG == VMN_PREC_FP8 && LCAP <= 128))) ? 5 : 1, 8))) void search_kernel(SearchArgs a) {
     constexpr int PREC = PREC_ARG == VMN_PREC_BF16_TABLE_NORMS ? VMN_PREC_BF16 : PREC_ARG;
     constexpr bool TN = PREC_ARG == VMN_PREC_BF16_TABLE_NORMS;
     // FP32 distances in the reference's summation order (PARTH)
@@ -307,7 +314,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
             else if (PREC == VMN_PREC_FP32)
                 gather_dists<DP>(q, vecs, s_sid, s_sd, nnew);
             else
-                gather_dists_low<DP, PREC, VMN_GATHER_TILES, TN>(
+                gather_dists_low<DP, PREC, NG, TN>(
                     ql, a.low_rows, a.low_norms, s_sid, s_sd, nnew,
                     PREC == VMN_PREC_FP8 ? (const float*)s_sn : nullptr);
             __syncthreads();
@@ -388,21 +395,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
     }
 }
 
+// Kernel instances with 4 gather tiles exist for these shapes.
+template <int DP, int LCAP, int MODE>
+constexpr bool tiles4_shape() {
+    return MODE == VMN_MODE_FRESH && DP >= 64 && DP <= 128 && LCAP <= 256;
+}
+
+bool search_tiles4(int DP, int need, int mode) {
+    return mode == VMN_MODE_FRESH && DP >= 64 && DP <= 128 && need <= 256;
+}
+
+template <int DP, int LCAP, int MODE, int PREC_ARG>
+static void launch_lowp(const SearchArgs& a, int grid, hipStream_t s) {
+    if constexpr (tiles4_shape<DP, LCAP, MODE>()) {
+        if (a.tiles == 4) {
+            hipLaunchKernelGGL((search_kernel<DP, LCAP, MODE, PREC_ARG, 4>), dim3(grid), dim3(64), 0, s, a);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((search_kernel<DP, LCAP, MODE, PREC_ARG, 2>), dim3(grid), dim3(64), 0, s, a);
+}
+
 template <int DP, int LCAP, int MODE>
 static hipError_t launch_search_p(const SearchArgs& a, int grid, hipStream_t s) {
     switch (a.prec) {
         case VMN_PREC_FP32:
-            hipLaunchKernelGGL((search_kernel<DP, LCAP, MODE, VMN_PREC_FP32>), dim3(grid), dim3(64), 0, s, a);
+            hipLaunchKernelGGL((search_kernel<DP, LCAP, MODE, VMN_PREC_FP32, 2>), dim3(grid), dim3(64), 0, s, a);
             break;
         case VMN_PREC_BF16:
             if (MODE == VMN_MODE_FRESH && a.table_norms)
-                hipLaunchKernelGGL((search_kernel<DP, LCAP, MODE, MODE == VMN_MODE_FRESH ? VMN_PREC_BF16_TABLE_NORMS : VMN_PREC_BF16>),
-                                   dim3(grid), dim3(64), 0, s, a);
+                launch_lowp<DP, LCAP, MODE, MODE == VMN_MODE_FRESH ? VMN_PREC_BF16_TABLE_NORMS : VMN_PREC_BF16>(a, grid, s);
             else
-                hipLaunchKernelGGL((search_kernel<DP, LCAP, MODE, VMN_PREC_BF16>), dim3(grid), dim3(64), 0, s, a);
+                launch_lowp<DP, LCAP, MODE, VMN_PREC_BF16>(a, grid, s);
             break;
         case VMN_PREC_FP8:
-            hipLaunchKernelGGL((search_kernel<DP, LCAP, MODE, VMN_PREC_FP8>), dim3(grid), dim3(64), 0, s, a);
+            launch_lowp<DP, LCAP, MODE, VMN_PREC_FP8>(a, grid, s);
             break;
         default: return hipErrorInvalidValue;
     }

@@ -3,9 +3,10 @@
 beam_width(): the smallest beam width L whose recall@k against exact ground
 truth reaches a target (the reference hard-codes L_g = 199,
 vamana/include/constants.cuh:10). launch_shape(): times the batched search
-under each persistent-grid size (waves per CU), beam-list capacity template
-and query grouping (one group per XCD, or one queue), keeps the fastest, and
-installs it on the index. It times either one batch at a time on one search
+under each persistent-grid size (waves per CU), beam-list capacity template,
+tile shape (MFMA tiles gathered per round by the BF16 / FP8 forms) and query
+grouping (one group per XCD, or one queue), keeps the fastest, and installs
+it on the index. It times either one batch at a time on one search
 context, or batches issued round-robin over several contexts, so that each
 batch's drain overlaps the next batches' start (the throughput of a stream of
 batches).
@@ -63,9 +64,11 @@ def beam_width(index, queries, gt_ids, target, widths, k=10, prec="fp32", rerank
 
 def launch_shape(index, queries, k=10, L=64, prec="fp32", rerank=0,
                  waves=(12, 16, 20, 24, 32), caps=(0, 128, 256), reps=3, overlap=False,
-                 contexts=2, groups=(0, 8), queries_alt=None):
-    """Fastest (waves_per_cu, list_cap, query groups) for this batch; installs
-    it and returns {"waves_per_cu", "list_cap", "groups", "ms", "table"}.
+                 contexts=2, groups=(0, 8), queries_alt=None, tiles=(2, 4)):
+    """Fastest (waves_per_cu, list_cap, tiles, query groups) for this batch;
+    installs it and returns {"waves_per_cu", "list_cap", "tiles", "groups",
+    "ms", "table"}. Tile shapes are tried for the BF16 / FP8 forms where the
+    4-tile kernels exist (FRESH, dimensions 33 to 128, lists up to 256).
     overlap=True times batches issued round-robin over `contexts` search
     contexts (ms per batch of the stream) instead of one batch at a time; with
     queries_alt the contexts alternate between the two query sets (a stream of
@@ -90,14 +93,21 @@ def launch_shape(index, queries, k=10, L=64, prec="fp32", rerank=0,
         for cap in caps:
             if cap and cap < L:
                 continue
-            for w in waves:
-                index.set_launch(w, cap)
-                if overlap:
-                    ms = _time_overlapped(ctxs, qs, outs, k, L, prec, rr, reps)
-                else:
-                    ms = _time(ctxs[0], queries, outs[0][0], outs[0][1], k, L, prec, rr, reps)
-                table.append({"waves_per_cu": w, "list_cap": cap, "groups": grp, "ms": round(ms, 4)})
+            # a tile shape other than 2 is tried only where its kernels exist
+            # (elsewhere the search runs 2 tiles whatever is set)
+            two = index.search_kernel_name(k, L, prec, cap, 2)
+            for tl in [t for t in tiles if t == 2 or index.search_kernel_name(k, L, prec, cap, t) != two]:
+                index.set_tiles(tl)
+                for w in waves:
+                    index.set_launch(w, cap)
+                    if overlap:
+                        ms = _time_overlapped(ctxs, qs, outs, k, L, prec, rr, reps)
+                    else:
+                        ms = _time(ctxs[0], queries, outs[0][0], outs[0][1], k, L, prec, rr, reps)
+                    table.append({"waves_per_cu": w, "list_cap": cap, "tiles": tl, "groups": grp,
+                                  "ms": round(ms, 4)})
     best = min(table, key=lambda t: t["ms"])
+    index.set_tiles(best["tiles"])
     index.set_launch(best["waves_per_cu"], best["list_cap"])
     if best["groups"] is not None:
         index.set_query_groups(best["groups"])

@@ -139,6 +139,7 @@ def lib():
         L.vmn_ctx_capture.argtypes = [P, P, i64, i32, i32, i32, i32, P, P]
         L.vmn_ctx_replay.argtypes = [P]
         L.vmn_set_launch.argtypes = [P, i32, i32]
+        L.vmn_set_tiles.argtypes = [P, i32]
         L.vmn_set_query_groups.argtypes = [P, i32]
         L.vmn_query_group_ids.argtypes = [P, ctypes.c_void_p, i64, i32, ctypes.c_void_p]
         L.vmn_default_server_params.argtypes = [ctypes.POINTER(ServerParams)]
@@ -185,7 +186,8 @@ def lib():
                      "vmn_delete_ids", "vmn_delete_vectors", "vmn_consolidate",
                      "vmn_get_deleted", "vmn_search_ex", "vmn_get_lowp_rows", "vmn_ctx_create",
                      "vmn_ctx_destroy", "vmn_ctx_search", "vmn_ctx_capture", "vmn_ctx_replay",
-                     "vmn_set_launch", "vmn_set_query_groups", "vmn_query_group_ids",
+                     "vmn_set_launch", "vmn_set_tiles", "vmn_set_query_groups",
+                     "vmn_query_group_ids",
                      "vmn_server_create",
                      "vmn_server_submit",
                      "vmn_server_poll", "vmn_server_destroy", "vmn_read_workload",
@@ -606,6 +608,11 @@ class Vamana:
     def set_launch(self, waves_per_cu=32, list_cap=0):
         _check(lib().vmn_set_launch(self._h, waves_per_cu, list_cap))
 
+    def set_tiles(self, tiles=2):
+        """Tile shape of the BF16 / FP8 searches: MFMA tiles of 16 candidates
+        gathered per round (2 or 4). Changes the speed, never a result."""
+        _check(lib().vmn_set_tiles(self._h, tiles))
+
     def set_query_groups(self, groups=8):
         """Query groups of large batches (one per XCD; 0 or 1: one queue).
         Changes the order queries are searched in, never a result."""
@@ -624,16 +631,19 @@ class Vamana:
         _check(lib().vmn_query_group_ids(self._h, qp, nq, mem, out.ctypes.data_as(ctypes.c_void_p)))
         return out
 
-    def search_kernel_name(self, k, L, prec="fp32", list_cap=0):
+    def search_kernel_name(self, k, L, prec="fp32", list_cap=0, tiles=2):
         """Name of the search kernel instance a search with these arguments
         launches, as a profiler prints it (template arguments: padded
-        dimension, beam-list capacity, mode, precision)."""
+        dimension, beam-list capacity, mode, precision, gather tiles)."""
         d = (int(self.params.dim) + 31) // 32 * 32
         dp = 256 if d > 192 else d
         need = max(int(L), int(list_cap))
         cap = next(c for c in (64, 128, 256, 512) if need <= c)
         pc = PREC[prec] if isinstance(prec, str) else int(prec)
-        return f"void vmn::search_kernel<{dp}, {cap}, {int(self.params.mode)}, {pc}>(vmn::SearchArgs)"
+        mode = int(self.params.mode)
+        ng = 4 if (tiles == 4 and pc != 0 and mode == MODE_FRESH and 64 <= dp <= 128
+                   and need <= 256) else 2
+        return f"void vmn::search_kernel<{dp}, {cap}, {mode}, {pc}, {ng}>(vmn::SearchArgs)"
 
     def lowp_rows(self, prec):
         """Rows as the BF16 / FP8 traversal sees them, decoded to float32."""

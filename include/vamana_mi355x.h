@@ -424,6 +424,13 @@ int vmn_ctx_replay(vmn_ctx* ctx);
  * waves_per_cu x CUs workgroups (one wavefront each) and the minimum
  * beam-list capacity template (0: the smallest of 64/128/256/512 >= L). */
 int vmn_set_launch(vmn_index* idx, int32_t waves_per_cu, int32_t list_cap);
+/* Tile shape of the BF16 / FP8 searches, also chosen by the autotuner (no
+ * counterpart in the reference): MFMA tiles of 16 candidates whose rows are
+ * gathered per round, 2 (the default; 0 selects it) or 4 (a hop's new
+ * candidates in one round, at 4 wavefronts per SIMD; FRESH mode, dimensions
+ * 33 to 128, beam capacity up to 256, else 2 is used). Results do not depend
+ * on it. */
+int vmn_set_tiles(vmn_index* idx, int32_t tiles);
 
 /* Query groups of large batches (no counterpart in the reference; default 8,
  * one per XCD of an MI355X in its default single-partition mode; 0 or 1: one

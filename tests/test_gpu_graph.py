@@ -76,6 +76,43 @@ def test_contexts_concurrent_and_launch_shapes(vmn, gpu):
         c.close()
 
 
+@pytest.mark.parametrize("dim", [128, 96])
+def test_tile_shapes_give_the_same_results(vmn, orc, gpu, dim):
+    """The BF16 / FP8 searches gathering 4 MFMA tiles a round return what 2
+    tiles return (BF16 on integer rows: the oracle's FP32 search), for small
+    batches (row norms from the table) and large ones, with beam lists of 64
+    and 256; a mode without 4-tile kernels runs 2 whatever is set."""
+    import torch
+    base = vmn.datasets.integer_mixture(6000, dim, k=24, seed=31, stream=0)
+    qs = vmn.datasets.integer_mixture(2300, dim, k=24, seed=31, stream=1)
+    idx = vmn.Vamana(base, mode="fresh", R=32, L_build=64, alphas=(1.0, 1.2), low_precision="both")
+    q = torch.from_numpy(qs).to(gpu)
+    for L in (60, 150):
+        for prec in ("bf16", "fp8"):
+            for nq in (7, 2300):
+                out = {}
+                for tiles in (2, 4):
+                    idx.set_tiles(tiles)
+                    i_, d_ = idx.searchPoints(q[:nq], k=10, L=L, prec=prec, rerank=40)
+                    out[tiles] = (i_.cpu().numpy(), d_.cpu().numpy())
+                assert np.array_equal(out[2][0], out[4][0]) and np.array_equal(out[2][1], out[4][1]), \
+                    (L, prec, nq)
+                if prec == "bf16" and nq == 7:
+                    oi, _, _, _ = orc.fresh_search(base, idx.adjacency, qs[:nq], idx.medoid, L=L,
+                                                   out_n=10)
+                    assert np.array_equal(out[4][0], oi), L
+    assert idx.search_kernel_name(10, 60, "bf16", 0, tiles=4).endswith(", 1, 1, 4>(vmn::SearchArgs)")
+    assert idx.search_kernel_name(10, 60, "fp32", 0, tiles=4).endswith(", 1, 0, 2>(vmn::SearchArgs)")
+    with pytest.raises(Exception):
+        idx.set_tiles(3)
+    idx.set_tiles(0)
+    parth = vmn.Vamana(base[:3000], mode="parth", R=32, L_build=64, low_precision="bf16")
+    ref = parth.searchPoints(qs[:200], k=10, L=60, prec="bf16")
+    parth.set_tiles(4)
+    got = parth.searchPoints(qs[:200], k=10, L=60, prec="bf16")
+    assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])
+
+
 def test_replay_refused_after_updates_and_recapture_matches_oracle(vmn, orc, gpu):
     """A captured search holds the index state of its capture (entry point,
     tombstones). After a delete or a consolidation the replay is refused

@@ -125,8 +125,10 @@ def test_launch_tuner_chooses_grouping(vmn, gpu, data):
     ref = _search(idx, torch, gpu, q, 0, L=60, prec="bf16", rerank=40)
     t = vmn.tuning.launch_shape(idx, q, k=10, L=60, prec="bf16", rerank=40, waves=(20, 24),
                                 caps=(0,), reps=2, overlap=True, queries_alt=q2)
-    assert len(t["table"]) == 4 and {r["groups"] for r in t["table"]} == {0, 8}
-    assert t["groups"] in (0, 8)
+    # waves x groups x tile shapes (the BF16 form at d = 128, L = 60 has 4-tile kernels)
+    assert len(t["table"]) == 8 and {r["groups"] for r in t["table"]} == {0, 8}
+    assert {r["tiles"] for r in t["table"]} == {2, 4}
+    assert t["groups"] in (0, 8) and t["tiles"] in (2, 4)
     # the installed choice gives the same results
     ids = torch.full((NQ, 10), -7, dtype=torch.int32, device=gpu)
     d = torch.empty((NQ, 10), dtype=torch.float32, device=gpu)
