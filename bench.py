@@ -777,7 +777,8 @@ def serving(args, vmn, torch, dev):
 def _profile_entry(form, n, nq, L, kernel, rerank):
     """The committed counter profile's entry for `form` when it covers this
     exact workload (same n, nq, precision, L and re-rank depth; the beam-list
-    capacity of the kernel instance does not change the traversal), else None."""
+    capacity and the tile shape of the kernel instance do not change the
+    traversal), else None."""
     path = os.path.join(ROOT, "profiles", "search_1m_hbm_traffic.json")
     try:
         with open(path) as f:
@@ -785,7 +786,9 @@ def _profile_entry(form, n, nq, L, kernel, rerank):
     except (OSError, ValueError):
         return None
     e = prof.get("forms", {}).get(form)
-    same = lambda a, b: a.split(",")[0] == b.split(",")[0] and a.split(",")[2:] == b.split(",")[2:]
+    # template arguments <dim, capacity, mode, precision, tiles>: compare dim, mode, precision
+    same = lambda a, b: [a.split(",")[i].strip() for i in (0, 2, 3)] == \
+        [b.split(",")[i].strip() for i in (0, 2, 3)]
     if not e or prof.get("n") != n or prof.get("nq") != nq or e.get("L") != L \
             or e.get("rerank") != rerank or not same(e.get("kernel", ""), kernel):
         return None

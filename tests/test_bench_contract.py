@@ -34,6 +34,9 @@ def test_traffic_lookup_matches_its_workload_only(bench):
         # another beam-list capacity instance runs the same traversal
         k256 = k.replace("<128, 64,", "<128, 256,")
         assert bench.measured_traffic(form, n, nq, L, k256, rr) == e["traffic_bytes"]
+        # and so does the other tile shape
+        k4 = k.replace(", 2>(", ", 4>(")
+        assert k4 != k and bench.measured_traffic(form, n, nq, L, k4, rr) == e["traffic_bytes"]
         # any other workload parameter: no figure
         assert bench.measured_traffic(form, n, nq, L, k, rr + 20) is None
         assert bench.measured_traffic(form, n, nq, L + 10, k, rr) is None
