@@ -7,7 +7,8 @@ Import with importlib (the directory name is not an identifier):
     ids, dists = index.searchPoints(queries, k=10, L=40)
 """
 from .vamana import (MODE_FRESH, MODE_PARTH, REQ_DELETE, REQ_INSERT, REQ_SEARCH, Params,
-                     SearchContext, Server, Vamana, VamanaError, default_params, groundtruth, lib,
+                     SearchContext, Server, Vamana, VamanaError, default_params, groundtruth,
+                     groundtruth_fallbacks, lib,
                      read_truthset, read_vectors, read_workload, recall_at_k,
                      write_truthset, write_vectors)
 from .engine import VamanaEngine
@@ -15,6 +16,6 @@ from . import datasets, tuning
 
 __all__ = ["Vamana", "VamanaEngine", "VamanaError", "Params", "SearchContext", "Server", "tuning",
            "read_workload", "read_vectors", "write_vectors", "read_truthset", "write_truthset",
-           "default_params", "groundtruth", "lib",
+           "default_params", "groundtruth", "groundtruth_fallbacks", "lib",
            "recall_at_k", "datasets", "MODE_PARTH", "MODE_FRESH", "REQ_SEARCH", "REQ_INSERT",
            "REQ_DELETE"]

@@ -984,6 +984,56 @@ int vmn_search_stats(const vmn_index* idx, int64_t* expansions, int64_t* dist_ev
     return VMN_OK;
 }
 
+// FP64 exact k nearest of nq padded queries among n padded rows (device), by
+// (distance, id), into device out_i / out_d. Widths up to 128 and k <= 64
+// take the MFMA-filtered path (same results; the queries its error bound
+// cannot settle go through the direct kernel); the rest the direct kernel.
+int exact_knn(vmn_index* idx, const float* base, int64_t n, const float* q, int64_t nq, int k,
+              int32_t* out_i, double* out_d) {
+    const int DP = idx->DP;
+    auto direct = [&](const float* qq, int64_t m, int32_t* oi, double* od) -> int {
+        int splits = (int)std::max<int64_t>(1, std::min<int64_t>(256, (int64_t)idx->cus * 256 / std::max<int64_t>(m, 1)));
+        splits = (int)std::min<int64_t>(splits, n);
+        const int KP = k <= 16 ? 16 : 128;
+        DevBuf pd, pi;
+        HIPCHK(pd.reserve((size_t)m * splits * KP * sizeof(double)));
+        HIPCHK(pi.reserve((size_t)m * splits * KP * sizeof(int32_t)));
+        HIPCHK(vmn::launch_gt(base, n, DP, nullptr, qq, m, k, splits, pd.as<double>(), pi.as<int32_t>(),
+                              oi, od, idx->stream));
+        HIPCHK(hipStreamSynchronize(idx->stream));
+        return VMN_OK;
+    };
+    idx->gt_fallback = 0;
+    if (DP > 128 || k > 64 || n < 4096) return direct(q, nq, out_i, out_d);
+    const int S = vmn::gt_fast_splits(n, nq, idx->cus);
+    DevBuf scratch, misc;
+    HIPCHK(scratch.reserve(vmn::gt_fast_scratch_bytes(n, nq, S)));
+    HIPCHK(misc.reserve(64));
+    int32_t* fb = nullptr;
+    HIPCHK(vmn::launch_gt_fast(base, n, DP, nullptr, q, nq, k, S, scratch.p, out_i, out_d,
+                               misc.as<unsigned int>(), &fb, idx->stream));
+    unsigned int nfb = 0;
+    HIPCHK(hipMemcpyAsync(&nfb, misc.as<unsigned int>() + 1, sizeof nfb, hipMemcpyDeviceToHost,
+                          idx->stream));
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    idx->gt_fallback = nfb;
+    if (!nfb) return VMN_OK;
+    DevBuf qf, fi, fd;
+    HIPCHK(qf.reserve((size_t)nfb * DP * sizeof(float)));
+    HIPCHK(fi.reserve((size_t)nfb * k * sizeof(int32_t)));
+    HIPCHK(fd.reserve((size_t)nfb * k * sizeof(double)));
+    HIPCHK(vmn::launch_gt_gather(q, fb, nfb, DP, qf.as<float>(), idx->stream));
+    int rc = direct(qf.as<float>(), nfb, fi.as<int32_t>(), fd.as<double>());
+    if (rc) return rc;
+    HIPCHK(vmn::launch_gt_scatter(fb, nfb, k, fi.as<int32_t>(), fd.as<double>(), out_i, out_d, idx->stream));
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    return VMN_OK;
+}
+
+thread_local int64_t g_gt_fallbacks = 0;
+
+int64_t vmn_groundtruth_fallbacks(void) { return g_gt_fallbacks; }
+
 int vmn_groundtruth(const float* base, int64_t n, int32_t dim, const float* queries, int64_t nq,
                     int32_t k, int32_t* ids_out, double* dists_out, int32_t mem, int32_t device) {
     if (!base || !queries || !ids_out) return fail(VMN_ERR_ARG, "NULL argument");
@@ -1008,12 +1058,7 @@ int vmn_groundtruth(const float* base, int64_t n, int32_t dim, const float* quer
     if (rc) return rc;
     rc = stage_rows(idx, queries, nq, mem, idx->qpad, &q);
     if (rc) return rc;
-    int splits = (int)std::max<int64_t>(1, std::min<int64_t>(256, (int64_t)idx->cus * 256 / std::max<int64_t>(nq, 1)));
-    splits = (int)std::min<int64_t>(splits, n);
-    const int KP = k <= 16 ? 16 : 128;
-    DevBuf pd, pi, oi, od;
-    HIPCHK(pd.reserve((size_t)nq * splits * KP * sizeof(double)));
-    HIPCHK(pi.reserve((size_t)nq * splits * KP * sizeof(int32_t)));
+    DevBuf oi, od;
     int32_t* dst_i = ids_out;
     double* dst_d = dists_out;
     if (mem == VMN_MEM_HOST) {
@@ -1022,8 +1067,9 @@ int vmn_groundtruth(const float* base, int64_t n, int32_t dim, const float* quer
         dst_i = oi.as<int32_t>();
         dst_d = od.as<double>();
     }
-    HIPCHK(vmn::launch_gt(b, n, idx->DP, nullptr, q, nq, k, splits, pd.as<double>(), pi.as<int32_t>(),
-                          dst_i, dst_d, idx->stream));
+    rc = exact_knn(idx, b, n, q, nq, k, dst_i, dst_d);
+    g_gt_fallbacks = idx->gt_fallback;
+    if (rc) return rc;
     if (mem == VMN_MEM_HOST) {
         HIPCHK(hipMemcpyAsync(ids_out, dst_i, (size_t)nq * k * sizeof(int32_t), hipMemcpyDeviceToHost,
                               idx->stream));
@@ -1032,8 +1078,6 @@ int vmn_groundtruth(const float* base, int64_t n, int32_t dim, const float* quer
                                   hipMemcpyDeviceToHost, idx->stream));
     }
     HIPCHK(hipStreamSynchronize(idx->stream));
-    pd.release();
-    pi.release();
     oi.release();
     od.release();
     return VMN_OK;

@@ -115,6 +115,8 @@ struct vmn_index {
     int waves_per_cu = 32;  // search grid = CUs x waves_per_cu (persistent work queue)
     int list_cap = 0;       // minimum beam-list capacity of the search kernel
     int tiles = 2;          // low-precision search: MFMA tiles gathered per round (2 or 4)
+    // queries of the last exact k-NN that its MFMA filter left to the direct kernel
+    int64_t gt_fallback = 0;
     // work counters of the last build (VMN_ACC_*), counted while build_count
     vmn_rt::DevBuf build_acc;
     bool build_count = false;

@@ -205,5 +205,21 @@ hipError_t launch_gt(const float* base, int64_t n, int DP, const uint32_t* del_b
                      const float* queries, int64_t nq, int k, int splits, double* part_d,
                      int32_t* part_i, int32_t* out_ids, double* out_d, hipStream_t s,
                      bool f32 = false);
+// The FP64 ground truth of launch_gt for DP <= 128 and k <= 64 through an FP32
+// MFMA filter with a per-query error bound (same results): S row splits
+// (gt_fast_splits), scratch of gt_fast_scratch_bytes, misc = 2 device words.
+// Queries the bound cannot settle are listed in *fb_list_out (count in
+// misc[1] once the stream reaches it) for launch_gt.
+int gt_fast_splits(int64_t n, int64_t nq, int cus);
+size_t gt_fast_scratch_bytes(int64_t n, int64_t nq, int S);
+hipError_t launch_gt_fast(const float* base, int64_t n, int DP, const uint32_t* del_bits,
+                          const float* queries, int64_t nq, int k, int S, void* scratch,
+                          int32_t* out_ids, double* out_d, unsigned int* misc,
+                          int32_t** fb_list_out, hipStream_t s);
+// dst[i] = src row rows[i] (m rows of DP floats); out row rows[i] = result row i.
+hipError_t launch_gt_gather(const float* src, const int32_t* rows, int64_t m, int DP, float* dst,
+                            hipStream_t s);
+hipError_t launch_gt_scatter(const int32_t* rows, int64_t m, int k, const int32_t* ids, const double* d,
+                             int32_t* out_ids, double* out_d, hipStream_t s);
 
 }  // namespace vmn

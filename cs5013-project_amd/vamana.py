@@ -140,6 +140,8 @@ def lib():
         L.vmn_ctx_replay.argtypes = [P]
         L.vmn_set_launch.argtypes = [P, i32, i32]
         L.vmn_set_tiles.argtypes = [P, i32]
+        L.vmn_groundtruth_fallbacks.argtypes = []
+        L.vmn_groundtruth_fallbacks.restype = ctypes.c_int64
         L.vmn_set_query_groups.argtypes = [P, i32]
         L.vmn_query_group_ids.argtypes = [P, ctypes.c_void_p, i64, i32, ctypes.c_void_p]
         L.vmn_default_server_params.argtypes = [ctypes.POINTER(ServerParams)]
@@ -744,6 +746,12 @@ def groundtruth(base, queries, k, device=0):
                                  ids.ctypes.data_as(ctypes.c_void_p),
                                  d.ctypes.data_as(ctypes.c_void_p), bm, device))
     return ids, d
+
+
+def groundtruth_fallbacks():
+    """Queries of this thread's last groundtruth() call that the MFMA filter's
+    error bound left to the direct FP64 kernel (results are the same)."""
+    return int(lib().vmn_groundtruth_fallbacks())
 
 
 def recall_at_k(found, gt_ids, k=10, gt_dists=None):

@@ -328,6 +328,12 @@ void* vmn_stream(const vmn_index* idx);
  * compute_groundtruth (vamana.cu:1644). Runs on `device`; synchronous. */
 int vmn_groundtruth(const float* base, int64_t n, int32_t dim, const float* queries, int64_t nq,
                     int32_t k, int32_t* ids_out, double* dists_out, int32_t mem, int32_t device);
+/* Queries of the calling thread's last vmn_groundtruth whose candidates the
+ * MFMA filter (FP32 q.x with a per-query error bound, dimensions up to 128,
+ * k <= 64, n >= 4096) could not settle and the direct FP64 kernel computed;
+ * either way the results are the direct kernel's. No counterpart in the
+ * reference. */
+int64_t vmn_groundtruth_fallbacks(void);
 
 /* Mean recall@recall_at of nq result rows (found [nq x k_found], -1 padded)
  * against ground truth rows gt [nq x k_gt] (k_gt >= recall_at): the share of
