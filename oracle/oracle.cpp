@@ -892,6 +892,164 @@ int64_t orc_engine_run(float* vecs, int32_t* adj, uint8_t* del, int64_t cap, int
     return n_search;
 }
 
+// ---------------------------------------------------------------------------
+// BANG exact search (bang_exact/src/parANN.cu, the schedule its host loop
+// :500-668 drives). Per query and round:
+//  * neighbor_filtering_new (:996-1052): candidates are, in round 1, the entry
+//    point (:1017-1030), then the parent's neighbours in adjacency order whose
+//    single bloom cell hashFn1_d(x) (:1055-1064, FNV-1a over the four bytes,
+//    modulo 399,887) is clear; their cells are set. The reference tests and
+//    sets cells from concurrent threads, so two neighbours of one parent with
+//    the same cell may both pass there; here the first in adjacency order does
+//    and the later one is dropped. Each candidate is one distance computation
+//    (the `bang_counter` total, :519-526, :596-599).
+//  * compute_neighborDist_par (:1118-1158): the FP32 distance of each
+//    candidate, summed by 8 threads (thread s adds (x_i - q_i)^2 for
+//    i = s, s + 8, ... as one fused multiply-add chain) and combined by
+//    cub::WarpReduce<float, 8> as ((p0+p1)+(p2+p3))+((p4+p5)+(p6+p7)).
+//  * compute_BestLSets_par_sort_msort_new (:1475-1669): a stable merge sort of
+//    the candidates by distance (ties in arrival order), then round 1 takes the
+//    first min(count, L) as the list with only the entry point visited; later
+//    rounds merge the first nb sorted candidates into the list of B entries,
+//    nb = max(#leading candidates closer than the list's last entry (among
+//    the first min(L, count)), min(L - B, count)), new entries before list
+//    entries of equal distance, the list cut at min(B + nb, L); the first
+//    unvisited list entry is marked visited and becomes the next parent.
+//  * A round that yields no candidate ends the query: the reference skips the
+//    merge then and leaves the list bound of its parent scan unset
+//    (:1588-1590 vs :1641); its printed totals for SIFT10K (641,853
+//    computations, 47 rounds) are those of this reading, while scanning the
+//    whole list gives 642,920.
+//  * The host loop stops after round 4L + 19 (MAX_PARENTS_PERQUERY - 1,
+//    :49-50, :665-668); a query stops when it has no parent.
+// Results: the first k list entries (parANN.cu:1252-1254 copies d_BestLSets),
+// -1 / +inf past the list's end. out_stats [nq x 2]: computations, rounds run.
+// Rows whose `del` flag is set are never candidates (the reference has no
+// tombstones). Mutations: 8 scans the whole list in a round without
+// candidates, 9 puts list entries before new ones on ties, 10 drops the
+// min(L - B, count) term of nb.
+// ---------------------------------------------------------------------------
+uint32_t orc_bang_hash(uint32_t x) {
+    uint64_t h = 0xcbf29ce4ull;
+    h = (h ^ (x & 0xffu)) * 0x01000193ull;
+    h = (h ^ ((x >> 8) & 0xffu)) * 0x01000193ull;
+    h = (h ^ ((x >> 16) & 0xffu)) * 0x01000193ull;
+    h = (h ^ ((x >> 24) & 0xffu)) * 0x01000193ull;
+    return (uint32_t)(h % 399887u);
+}
+
+float orc_bang_l2sq(const float* q, const float* x, int d) {
+    float p[8];
+    for (int s = 0; s < 8; ++s) {
+        float acc = 0.0f;
+        for (int i = s; i < d; i += 8) {
+            float diff = x[i] - q[i];
+            acc = fmaf(diff, diff, acc);
+        }
+        p[s] = acc;
+    }
+    return ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+}
+
+void orc_bang_search(const float* vecs, int64_t n, int d, const int32_t* adj, int R,
+                     const uint8_t* del, int64_t entry, const float* queries, int64_t nq, int L,
+                     int k, int32_t* out_ids, float* out_dists, int32_t* out_stats) {
+    (void)n;
+    const int max_rounds = 4 * L + 19;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t qi = 0; qi < nq; ++qi) {
+        const float* q = queries + qi * d;
+        std::vector<uint8_t> bloom(399887, 0);
+        std::vector<Item> list;  // sorted by distance; vis = expanded
+        std::vector<int32_t> cand;
+        std::vector<std::pair<float, int32_t>> sc;  // (distance, arrival)
+        int64_t parent = entry;
+        int comps = 0, rounds = 0;
+        for (int it = 1; it <= max_rounds; ++it) {
+            rounds = it;
+            cand.clear();
+            if (it == 1) {
+                bloom[orc_bang_hash((uint32_t)entry)] = 1;
+                cand.push_back((int32_t)entry);
+            }
+            const int32_t* row = adj + parent * R;
+            for (int j = 0; j < R && row[j] >= 0; ++j) {
+                const int32_t x = row[j];
+                if (del && del[x]) continue;
+                const uint32_t h = orc_bang_hash((uint32_t)x);
+                if (bloom[h]) continue;
+                bloom[h] = 1;
+                cand.push_back(x);
+            }
+            const int nc = (int)cand.size();
+            comps += nc;
+            int bound = 0;
+            if (nc > 0) {
+                sc.clear();
+                for (int c = 0; c < nc; ++c)
+                    sc.emplace_back(orc_bang_l2sq(q, vecs + (int64_t)cand[c] * d, d), c);
+                std::stable_sort(sc.begin(), sc.end(),
+                                 [](const std::pair<float, int32_t>& a,
+                                    const std::pair<float, int32_t>& b) { return a.first < b.first; });
+                if (it == 1) {
+                    list.clear();
+                    for (int c = 0; c < std::min(nc, L); ++c) {
+                        const int32_t id = cand[sc[c].second];
+                        list.push_back({id, sc[c].first, id == entry});
+                    }
+                } else {
+                    const int B = (int)list.size();
+                    const float mx = list[B - 1].d;
+                    const int T = std::min(L, nc);
+                    int nb = 0;
+                    while (nb < T && sc[nb].first < mx) ++nb;
+                    if (g_mut != 10) nb = std::max(nb, std::min(L - B, nc));
+                    const int nb2 = std::min(B + nb, L);
+                    std::vector<Item> merged;
+                    merged.reserve(B + nb);
+                    int a = 0, b = 0;
+                    while ((int)merged.size() < nb2) {
+                        // new entry first unless the list entry is strictly closer
+                        const bool take_list =
+                            b >= nb ||
+                            (a < B && (g_mut == 9 ? list[a].d <= sc[b].first
+                                                  : list[a].d < sc[b].first));
+                        if (take_list) {
+                            merged.push_back(list[a++]);
+                        } else {
+                            merged.push_back({cand[sc[b].second], sc[b].first, false});
+                            ++b;
+                        }
+                    }
+                    list.swap(merged);
+                }
+                bound = (int)list.size();
+            } else if (g_mut == 8) {
+                bound = (int)list.size();
+            }
+            int next = -1;
+            for (int j = 0; j < bound; ++j) {
+                if (!list[j].vis) {
+                    list[j].vis = true;
+                    next = list[j].id;
+                    break;
+                }
+            }
+            if (next < 0) break;
+            parent = next;
+        }
+        for (int j = 0; j < k; ++j) {
+            const bool ok = j < (int)list.size();
+            out_ids[qi * k + j] = ok ? list[j].id : -1;
+            if (out_dists) out_dists[qi * k + j] = ok ? list[j].d : INFINITY;
+        }
+        if (out_stats) {
+            out_stats[qi * 2] = comps;
+            out_stats[qi * 2 + 1] = rounds;
+        }
+    }
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -64,6 +64,11 @@ def lib():
                                          P, P, i32]
         _lib.orc_fresh_build.restype = i64
         _lib.orc_set_mutation.argtypes = [i32]
+        _lib.orc_bang_hash.argtypes = [ctypes.c_uint32]
+        _lib.orc_bang_hash.restype = ctypes.c_uint32
+        _lib.orc_bang_l2sq.argtypes = [P, P, i32]
+        _lib.orc_bang_l2sq.restype = f32
+        _lib.orc_bang_search.argtypes = [P, i64, i32, P, i32, P, i64, P, i64, i32, i32, P, P, P]
         _lib.orc_num_threads.restype = i32
         _lib.orc_set_num_threads.argtypes = [i32]
     return _lib
@@ -254,6 +259,30 @@ def bruteforce_fp64(vecs, queries, k, deleted=None):
     lib().orc_bruteforce_fp64(_p(vecs), vecs.shape[0], vecs.shape[1], _p(dm), _p(queries), nq,
                               k, _p(ids), _p(dists))
     return ids, dists
+
+
+def bang_hash(x):
+    return int(lib().orc_bang_hash(x & 0xFFFFFFFF))
+
+
+def bang_l2sq(q, x):
+    q, x = _f32(q), _f32(x)
+    return float(lib().orc_bang_l2sq(_p(q), _p(x), q.shape[0]))
+
+
+def bang_search(vecs, adj, queries, entry, L, k=10, deleted=None):
+    """BANG exact search (oracle.cpp orc_bang_search): ids [nq x k], dists
+    [nq x k] and per-query (distance computations, rounds run) [nq x 2]."""
+    vecs, queries = _f32(vecs), _f32(queries)
+    adj = np.ascontiguousarray(adj, dtype=np.int32)
+    nq = queries.shape[0]
+    ids = np.empty((nq, k), dtype=np.int32)
+    dists = np.empty((nq, k), dtype=np.float32)
+    stats = np.empty((nq, 2), dtype=np.int32)
+    dm = None if deleted is None else np.ascontiguousarray(deleted, dtype=np.uint8)
+    lib().orc_bang_search(_p(vecs), vecs.shape[0], vecs.shape[1], _p(adj), adj.shape[1], _p(dm),
+                          int(entry), _p(queries), nq, L, k, _p(ids), _p(dists), _p(stats))
+    return ids, dists, stats
 
 
 def engine_run(vecs, adj, n, cap, types, ids, opvecs, batch, L, link_L=128, start=0,

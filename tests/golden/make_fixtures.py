@@ -29,6 +29,13 @@ Outputs (committed):
   parth_workloads.npz  the parth engine's workload files parth/data/
                {sift,custom_sift,toy,vis}_workload.bin as <name>_types, <name>_ids
                (int32) and <name>_vecs (float32 [ops x dim])
+  bang_sift10k.npz     gt_ids / gt_dists: the first 400 rows (10 per row) of
+               data/sift10k/sift10k_groundtruth.bin, the rows BANG's recall of its
+               400-query run reads (bang_exact/src/main.cu:87-137)
+  bang_expected.json   what BANG's SIFT10K run printed (bang_exact/output.txt):
+               per run the rounds ("iterations = "), total_recall and total
+               computations, plus L, k and the entry point of that build
+               (parANN.cuh:40-41,122-128) and numQueries from the run's first line
   parth_metrics.json   per workload: the reference engine's own result
                (parth/vis_outputs/<dir>/metrics.json: qps, recall_10), the batch
                size and MAX_N of its config (parth/config_<name>.json) and the
@@ -144,6 +151,29 @@ def main(ref):
     assert (ddeg == 64).all()
     np.savez_compressed(os.path.join(HERE, "diskann_graph.npz"),
                         adj=drec[:, 516:].copy().view(np.int32))
+    t_all = body[: n_t * k_t].reshape(n_t, k_t)
+    d_all = body[n_t * k_t: 2 * n_t * k_t].view(np.float32).reshape(n_t, k_t)
+    np.savez_compressed(os.path.join(HERE, "bang_sift10k.npz"),
+                        gt_ids=t_all[:400].astype(np.int32), gt_dists=d_all[:400].copy())
+    text = open(os.path.join(ref, "bang_exact/output.txt")).read()
+    runs, cur = [], {}
+    for line in text.splitlines():
+        if line.startswith("iterations = "):
+            cur = {"rounds": int(line.split("=")[1])}
+        elif line.startswith("total_recall = "):
+            cur["total_recall"] = float(line.split()[2])
+            cur["num_queries"] = int(line.split()[5])
+        elif "total computations:" in line:
+            cur["computations"] = int(line.rsplit(":", 1)[1])
+            cur["L"] = int(line.split()[0])
+            cur["recall_pct"] = float(line.split()[1])
+            runs.append(cur)
+    first = text.splitlines()[0]
+    assert first.startswith("[medoidID=")
+    entry = int(first.split("medoidID=")[1].split(",")[0])
+    with open(os.path.join(HERE, "bang_expected.json"), "w") as f:
+        json.dump({"source": "bang_exact/output.txt", "entry": entry, "k": 10,
+                   "dim": 128, "bloom_cells": 399887, "runs": runs}, f, indent=1)
     wl, metrics = {}, {}
     for name, dim, vis_dir in (("sift", 128, "sift"), ("custom_sift", 128, "custom_sift"),
                                ("toy", 2, "toy"), ("vis", 2, "step_by_step")):
