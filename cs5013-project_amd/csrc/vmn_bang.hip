@@ -1,0 +1,338 @@
+// BANG's exact search schedule (bang_exact/src/parANN.cu:96-978) over the
+// index's graph, as one kernel: a wavefront owns one query and runs all of its
+// rounds, with the query's bloom filter, candidates and best-L list in LDS.
+//
+// The reference drives the rounds from the host: per round a memset, three
+// kernels (neighbor_filtering_new :996-1052, compute_neighborDist_par
+// :1118-1158, compute_BestLSets_par_sort_msort_new :1475-1669), a blocking copy
+// of the per-query candidate counts (:594-599) and one of the continue flag
+// (:652-654), over a 399,888-byte global bloom filter per query (:346). Here
+// nothing leaves the device between rounds and a query ends as soon as it has
+// no parent; its bloom filter is 399,887 bits (12,500 words, 50 KB) in LDS,
+// cleared at the query's start, so three queries share a CU.
+//
+// Per round (semantics of the reference, order fixed where it is racy there):
+//  * candidates: in round 1 the entry point, then the parent's neighbours in
+//    adjacency order (up to the first -1) whose bloom cell hashFn1_d
+//    (:1055-1064) is clear; their cells are set. Neighbours of one parent that
+//    share a cell: the first in adjacency order is kept (the reference's
+//    threads race there). Tombstoned rows are never candidates.
+//  * distance: FP32, as compute_neighborDist_par sums it: eight fused
+//    multiply-add chains over dimensions s, s + 8, ... (s = 0..7) combined as
+//    ((p0+p1)+(p2+p3))+((p4+p5)+(p6+p7)) (cub::WarpReduce<float, 8>). One lane
+//    computes a whole candidate (32 float4 loads of its row at dim 128), so a
+//    round's candidates are in flight together.
+//  * stable sort by distance, ties in arrival order; round 1 takes the first
+//    min(count, L) as the list (entry visited); later rounds merge the first
+//    nb candidates, nb = max(#leading candidates below the list's last
+//    distance, min(L - B, count)), new entries first on ties, cut at L;
+//  * the first unvisited list entry is the next parent; a round without
+//    candidates or parent ends the query, and so does round 4L + 19
+//    (MAX_PARENTS_PERQUERY - 1, :49-50, :665-668).
+// Outputs: the first k list entries (-1 / +inf past the list), and per query
+// the number of distance computations and of rounds run.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "vmn_device.hpp"
+#include "vmn_internal.hpp"
+#include "vmn_kernels.hpp"
+
+using namespace vmn_rt;
+
+namespace {
+
+constexpr uint32_t kCells = 399887u;  // BF_ENTRIES, parANN.cu:36
+constexpr int kWords = 12500;         // >= ceil(kCells / 32), a multiple of 4
+
+struct BangArgs {
+    const float* vecs;       // [n x DP]
+    const int32_t* adj;      // [n x R], -1 padded
+    const uint32_t* del;     // tombstone bits or nullptr
+    const float* q;          // [nq x DP]
+    int64_t nq;
+    int R, L, k, max_rounds;
+    int32_t entry;
+    int32_t* out_ids;        // [nq x k]
+    float* out_d;            // [nq x k] or nullptr
+    int32_t* out_stats;      // [nq x 2] or nullptr
+};
+
+__device__ __forceinline__ uint32_t bang_cell(uint32_t x) {
+    uint64_t h = 0xcbf29ce4ull;
+    h = (h ^ (x & 0xffu)) * 0x01000193ull;
+    h = (h ^ ((x >> 8) & 0xffu)) * 0x01000193ull;
+    h = (h ^ ((x >> 16) & 0xffu)) * 0x01000193ull;
+    h = (h ^ ((x >> 24) & 0xffu)) * 0x01000193ull;
+    return (uint32_t)(h % kCells);
+}
+
+// Squared L2 of one row against the query in LDS, summed in the reference's
+// order (8 strided chains, then the pairwise tree). Padded dimensions are
+// zero in both and leave every chain unchanged.
+template <int DP>
+__device__ __forceinline__ float bang_dist(const float* __restrict__ row, const float* qs) {
+    const float4* r = reinterpret_cast<const float4*>(row);
+    const float4* qv = reinterpret_cast<const float4*>(qs);
+    float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f, p4 = 0.f, p5 = 0.f, p6 = 0.f, p7 = 0.f;
+#pragma unroll
+    for (int j = 0; j < DP / 8; ++j) {
+        const float4 a = r[2 * j], b = r[2 * j + 1];
+        const float4 qa = qv[2 * j], qb = qv[2 * j + 1];
+        float t;
+        t = a.x - qa.x; p0 = __fmaf_rn(t, t, p0);
+        t = a.y - qa.y; p1 = __fmaf_rn(t, t, p1);
+        t = a.z - qa.z; p2 = __fmaf_rn(t, t, p2);
+        t = a.w - qa.w; p3 = __fmaf_rn(t, t, p3);
+        t = b.x - qb.x; p4 = __fmaf_rn(t, t, p4);
+        t = b.y - qb.y; p5 = __fmaf_rn(t, t, p5);
+        t = b.z - qb.z; p6 = __fmaf_rn(t, t, p6);
+        t = b.w - qb.w; p7 = __fmaf_rn(t, t, p7);
+    }
+    return ((p0 + p1) + (p2 + p3)) + ((p4 + p5) + (p6 + p7));
+}
+
+// Entries of the ascending arr[0, n) below x (LE: at or below x): the
+// lower_bound_d / upper_bound_d positions of parANN.cu:1672-1700.
+template <bool LE>
+__device__ __forceinline__ int count_below(const float* arr, int n, float x) {
+    int lo = 0;
+    for (int step = n > 0 ? 1 << (31 - __clz(n)) : 0; step > 0; step >>= 1) {
+        if (lo + step <= n) {
+            const float v = arr[lo + step - 1];
+            if (LE ? v <= x : v < x) lo += step;
+        }
+    }
+    return lo;
+}
+
+template <int DP, int LCAP>
+__global__ __launch_bounds__(64) void bang_kernel(BangArgs a) {
+    __shared__ __align__(16) uint32_t bloom[kWords];
+    __shared__ __align__(16) float qs[DP];
+    __shared__ int32_t sid[72];  // sorted candidates (at most R + 1 = 65)
+    __shared__ float sd[72];
+    __shared__ int32_t lid[2][LCAP];  // best-L list, two buffers for the merge
+    __shared__ float ld[2][LCAP];
+    __shared__ uint8_t lvis[2][LCAP];
+
+    const int l = vmn::lane();
+    const int64_t qi = blockIdx.x;
+    for (int i = l; i < kWords / 4; i += 64)
+        reinterpret_cast<uint4*>(bloom)[i] = make_uint4(0u, 0u, 0u, 0u);
+    for (int i = l; i < DP; i += 64) qs[i] = a.q[qi * DP + i];
+    __syncthreads();
+
+    const int32_t entry = a.entry;
+    int cur = 0, B = 0, comps = 0, rounds = 0;
+    int32_t parent = entry;
+    for (int it = 1; it <= a.max_rounds; ++it) {
+        rounds = it;
+        const bool first = it == 1;
+        float ed = 0.f;
+        if (first) {
+            const uint32_t c = bang_cell((uint32_t)entry);
+            if (l == 0) bloom[c >> 5] |= 1u << (c & 31);
+            ed = bang_dist<DP>(a.vecs + (size_t)entry * DP, qs);
+            __syncthreads();
+        }
+        // neighbour filtering (neighbor_filtering_new)
+        const int32_t nbr = l < a.R ? a.adj[(size_t)parent * a.R + l] : -1;
+        const uint64_t ends = vmn::ballot(nbr < 0);
+        const int deg = ends ? __ffsll((unsigned long long)ends) - 1 : 64;
+        bool want = l < deg;
+        if (want && a.del && vmn::bit_test(a.del, (uint32_t)nbr)) want = false;
+        const uint32_t c = want ? bang_cell((uint32_t)nbr) : 0u;
+        const uint32_t w = c >> 5, bit = 1u << (c & 31);
+        if (want) want = !(bloom[w] & bit);
+        uint32_t old = 0u;
+        if (want) old = atomicOr(&bloom[w], bit);
+        if (vmn::ballot(want && (old & bit))) {
+            // two neighbours with one cell: keep the first in adjacency order
+            bool later = false;
+            for (uint64_t m = vmn::ballot(want); m; m &= m - 1) {
+                const int t = __ffsll((unsigned long long)m) - 1;
+                later |= t < l && (uint32_t)vmn::readlane_i((int)c, t) == c;
+            }
+            want = want && !later;
+        }
+        const uint64_t km = vmn::ballot(want);
+        const int nc = __popcll((unsigned long long)km) + (first ? 1 : 0);
+        comps += nc;
+        if (nc == 0) break;
+
+        // distances (compute_neighborDist_par), one candidate per lane
+        float d = 0.f;
+        if (want) d = bang_dist<DP>(a.vecs + (size_t)nbr * DP, qs);
+
+        // stable sort by (distance, arrival): arrival is the entry, then lane order
+        int rank = first && ed <= d ? 1 : 0;
+        int erank = 0;
+        for (uint64_t m = km; m; m &= m - 1) {
+            const int t = __ffsll((unsigned long long)m) - 1;
+            const float dt = vmn::readlane_f(d, t);
+            rank += dt < d || (dt == d && t < l);
+            erank += dt < ed;
+        }
+        if (want) {
+            sid[rank] = nbr;
+            sd[rank] = d;
+        }
+        if (first && l == 0) {
+            sid[erank] = entry;
+            sd[erank] = ed;
+        }
+        __syncthreads();
+
+        // best-L merge (compute_BestLSets_par_sort_msort_new)
+        int nB;
+        if (first) {
+            nB = min(nc, a.L);
+            for (int i = l; i < nB; i += 64) {
+                lid[cur][i] = sid[i];
+                ld[cur][i] = sd[i];
+                lvis[cur][i] = sid[i] == entry;
+            }
+        } else {
+            const float mx = ld[cur][B - 1];
+            const int T = min(a.L, nc);
+            int nb = __popcll((unsigned long long)vmn::ballot(l < T && sd[l] < mx));
+            nb = max(nb, min(a.L - B, nc));
+            nB = min(B + nb, a.L);
+            const int nx = cur ^ 1;
+            if (l < nb) {
+                const float x = sd[l];
+                const int p = count_below<false>(ld[cur], B, x) + l;
+                if (p < nB) {
+                    lid[nx][p] = sid[l];
+                    ld[nx][p] = x;
+                    lvis[nx][p] = 0;
+                }
+            }
+            for (int j = l; j < B; j += 64) {
+                const float x = ld[cur][j];
+                const int p = count_below<true>(sd, nb, x) + j;
+                if (p < nB) {
+                    lid[nx][p] = lid[cur][j];
+                    ld[nx][p] = x;
+                    lvis[nx][p] = lvis[cur][j];
+                }
+            }
+            cur = nx;
+        }
+        __syncthreads();
+        B = nB;
+
+        // next parent: the first unvisited entry
+        int32_t next = -1;
+        for (int base = 0; base < B; base += 64) {
+            const int i = base + l;
+            const uint64_t m = vmn::ballot(i < B && !lvis[cur][i]);
+            if (m) {
+                const int t = base + __ffsll((unsigned long long)m) - 1;
+                next = lid[cur][t];
+                if (l == 0) lvis[cur][t] = 1;
+                break;
+            }
+        }
+        __syncthreads();
+        if (next < 0) break;
+        parent = next;
+    }
+
+    for (int j = l; j < a.k; j += 64) {
+        const bool ok = j < B;
+        a.out_ids[qi * a.k + j] = ok ? lid[cur][j] : -1;
+        if (a.out_d) a.out_d[qi * a.k + j] = ok ? ld[cur][j] : INFINITY;
+    }
+    if (a.out_stats && l == 0) {
+        a.out_stats[qi * 2] = comps;
+        a.out_stats[qi * 2 + 1] = rounds;
+    }
+}
+
+template <int DP>
+hipError_t launch_dp(const BangArgs& a, hipStream_t s) {
+    const dim3 grid((unsigned)a.nq), block(64);
+    if (a.L <= 64)
+        hipLaunchKernelGGL((bang_kernel<DP, 64>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((bang_kernel<DP, 256>), grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bang(const BangArgs& a, int DP, hipStream_t s) {
+    switch (DP) {
+        case 32: return launch_dp<32>(a, s);
+        case 64: return launch_dp<64>(a, s);
+        case 96: return launch_dp<96>(a, s);
+        case 128: return launch_dp<128>(a, s);
+        case 160: return launch_dp<160>(a, s);
+        case 192: return launch_dp<192>(a, s);
+        case 256: return launch_dp<256>(a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+extern "C" int vmn_bang_search(vmn_index* idx, const float* queries, int64_t nq, int32_t k,
+                               int32_t L, int64_t entry, int32_t* ids_out, float* dists_out,
+                               int32_t* stats_out, int32_t mem) {
+    if (!idx || !queries || !ids_out) return fail(VMN_ERR_ARG, "NULL argument");
+    if (!idx->built) return fail(VMN_ERR_STATE, "index has no graph");
+    if (nq < 0 || nq > 0x7fffffff) return fail(VMN_ERR_ARG, "nq = %lld out of range", (long long)nq);
+    if (k < 1 || k > L || L > 256)
+        return fail(VMN_ERR_ARG, "need 1 <= k <= L <= 256 (k=%d, L=%d)", k, L);
+    if (mem != VMN_MEM_HOST && mem != VMN_MEM_DEVICE) return fail(VMN_ERR_ARG, "bad mem %d", mem);
+    if (entry < 0) entry = idx->medoid;
+    if (entry < 0 || entry >= idx->n)
+        return fail(VMN_ERR_ARG, "entry point %lld not in [0, %lld)", (long long)entry,
+                    (long long)idx->n);
+    if (nq == 0) return VMN_OK;
+    int rc = set_device(idx);
+    if (rc) return rc;
+    const float* q = nullptr;
+    rc = stage_rows(idx, queries, nq, mem, idx->qpad, &q);
+    if (rc) return rc;
+    int32_t* oi = ids_out;
+    float* od = dists_out;
+    int32_t* os = stats_out;
+    if (mem == VMN_MEM_HOST) {
+        HIPCHK(idx->out_ids.reserve((size_t)nq * k * sizeof(int32_t)));
+        HIPCHK(idx->out_d.reserve((size_t)nq * k * sizeof(float)));
+        HIPCHK(idx->bang_stats.reserve((size_t)nq * 2 * sizeof(int32_t)));
+        oi = idx->out_ids.as<int32_t>();
+        od = dists_out ? idx->out_d.as<float>() : nullptr;
+        os = stats_out ? idx->bang_stats.as<int32_t>() : nullptr;
+    }
+    BangArgs a{};
+    a.vecs = idx->vecs.as<float>();
+    a.adj = idx->adj.as<int32_t>();
+    a.del = idx->n_deleted ? idx->del_bits.as<uint32_t>() : nullptr;
+    a.q = q;
+    a.nq = nq;
+    a.R = idx->p.R;
+    a.L = L;
+    a.k = k;
+    a.max_rounds = 4 * L + 19;
+    a.entry = (int32_t)entry;
+    a.out_ids = oi;
+    a.out_d = od;
+    a.out_stats = os;
+    HIPCHK(launch_bang(a, idx->DP, idx->stream));
+    if (mem == VMN_MEM_HOST) {
+        HIPCHK(hipMemcpyAsync(ids_out, oi, (size_t)nq * k * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              idx->stream));
+        if (dists_out)
+            HIPCHK(hipMemcpyAsync(dists_out, od, (size_t)nq * k * sizeof(float),
+                                  hipMemcpyDeviceToHost, idx->stream));
+        if (stats_out)
+            HIPCHK(hipMemcpyAsync(stats_out, os, (size_t)nq * 2 * sizeof(int32_t),
+                                  hipMemcpyDeviceToHost, idx->stream));
+        HIPCHK(hipStreamSynchronize(idx->stream));
+    }
+    return VMN_OK;
+}

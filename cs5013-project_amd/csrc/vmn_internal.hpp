@@ -129,6 +129,8 @@ struct vmn_index {
     // bf_hash_pair(adj[i]) at i), valid for index epoch adj_hash_epoch
     vmn_rt::DevBuf adj_hash;
     uint64_t adj_hash_epoch = ~0ull;
+    // per-query (distance computations, rounds) of a BANG search into host memory
+    vmn_rt::DevBuf bang_stats;
     // bumped by every change of the graph, tombstones or rows: a search
     // captured as a HIP graph holds the entry point, the tombstone pointer
     // and the buffers of its capture, so it may only be replayed while this

@@ -112,6 +112,7 @@ def lib():
         L.vmn_search_ex.argtypes = [P, P, i64, i32, i32, i32, i32, P, P, i32]
         L.vmn_search_vectors.argtypes = [P, P, i64, i32, i32, P, i32]
         L.vmn_search_noalloc.argtypes = [P, P, i64, i32, i32, i32, i32, P, P]
+        L.vmn_bang_search.argtypes = [P, P, i64, i32, i32, i64, P, P, P, i32]
         L.vmn_get_build_stats.argtypes = [P, ctypes.POINTER(BuildStats)]
         L.vmn_gather_probe.argtypes = [P, i64, i32, i32, P, i64, P, i64, i32, P]
         L.vmn_get_lowp_rows.argtypes = [P, i32, P]
@@ -196,7 +197,7 @@ def lib():
                      "vmn_engine_delete", "vmn_engine_insert", "vmn_engine_run",
                      "vmn_read_vectors", "vmn_write_vectors", "vmn_read_truthset",
                      "vmn_write_truthset", "vmn_save_bang_index", "vmn_load_bang_index",
-                     "vmn_search_vectors", "vmn_insert_staged"):
+                     "vmn_search_vectors", "vmn_insert_staged", "vmn_bang_search"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -470,6 +471,36 @@ class Vamana:
         return ids, dists
 
     search = searchPoints
+
+    def bang_search(self, queries, k=10, L=40, entry=-1):
+        """BANG's exact search schedule (bang_exact/src/parANN.cu:96-978) over
+        this graph from `entry` (-1: the index's medoid) with list size L.
+        Returns (ids [nq,k] int32, dists [nq,k] float32, stats [nq,2] int32 =
+        per-query distance computations and rounds run). numpy in -> numpy out;
+        CUDA tensor in -> CUDA tensors out (synchronised)."""
+        nq = int(queries.shape[0])
+        qp, mem, keep = _arg(queries, np.float32)
+        if mem == MEM_DEVICE:
+            import torch
+            dev = int(self.params.device)
+            _device_tensor(queries, "queries", np.float32, dev, width=int(self.params.dim))
+            ids = torch.empty((nq, k), dtype=torch.int32, device=queries.device)
+            dists = torch.empty((nq, k), dtype=torch.float32, device=queries.device)
+            stats = torch.empty((nq, 2), dtype=torch.int32, device=queries.device)
+            _check(lib().vmn_bang_search(self._h, qp, nq, k, L, entry,
+                                         ctypes.c_void_p(ids.data_ptr()),
+                                         ctypes.c_void_p(dists.data_ptr()),
+                                         ctypes.c_void_p(stats.data_ptr()), mem))
+            self.synchronize()
+            return ids, dists, stats
+        ids = np.empty((nq, k), dtype=np.int32)
+        dists = np.empty((nq, k), dtype=np.float32)
+        stats = np.empty((nq, 2), dtype=np.int32)
+        _check(lib().vmn_bang_search(self._h, qp, nq, k, L, entry,
+                                     ids.ctypes.data_as(ctypes.c_void_p),
+                                     dists.ctypes.data_as(ctypes.c_void_p),
+                                     stats.ctypes.data_as(ctypes.c_void_p), mem))
+        return ids, dists, stats
 
     def searchPoints2(self, queries, k=10, L=None):
         """The k nearest rows of each query, [nq, k, dim] (vamana.cuh:127)."""

@@ -184,6 +184,26 @@ int vmn_save_bang_index(const vmn_index* idx, const char* index_path, const char
 int vmn_load_bang_index(const char* index_path, const char* metadata_path, const vmn_params* p,
                         vmn_index** out);
 
+/* BANG's exact search schedule over the index's graph: replaces the search of
+ * `parANN` (bang_exact/src/parANN.cu:96-978: its host round loop :500-668 and
+ * the kernels neighbor_filtering_new :996-1052, compute_neighborDist_par
+ * :1118-1158, compute_BestLSets_par_sort_msort_new :1475-1669,
+ * compute_NearestNeighbours :1191-1255), with L (compile-time L, parANN.cuh:41),
+ * the entry point (MEDOID, parANN.cuh:125; entry < 0 takes vmn_medoid()) and
+ * recall_at (k) as arguments. Per query: a 399,887-cell bloom filter on one
+ * hash; each round's candidates are the entry point (round 1) and the parent's
+ * unseen neighbours; FP32 distances summed as the reference does; a stable
+ * sort and BANG's best-L merge; the first unvisited list entry is the next
+ * parent; a query ends at a round without candidates or parent, or after
+ * round 4L + 19. Writes the first k list entries per query (ids -1 and dists
+ * +inf past the list's end) and, in stats_out [nq x 2] (may be NULL), the
+ * distance computations and rounds of each query: their sum and maximum are
+ * BANG's "total computations" and "iterations". Tombstoned rows are never
+ * candidates. Needs 1 <= k <= L <= 256. `mem` as for vmn_search. */
+int vmn_bang_search(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int32_t L,
+                    int64_t entry, int32_t* ids_out, float* dists_out, int32_t* stats_out,
+                    int32_t mem);
+
 /* Batched greedy search of nq queries [nq x dim] from the medoid with beam L;
  * writes the k best ids (-1 padded) and squared L2 distances (may be NULL) per
  * query. Replaces `searchPoints` (vamana.cuh:96, vamana_impl.cuh:55-96) and
