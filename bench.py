@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--rerank-fp8", type=int, default=40,
                     help="FP32 re-rank depth of the FP8 form (deeper: its distance error is larger)")
     ap.add_argument("--workload", default="search",
-                    choices=["search", "streaming", "serving", "copy", "gather", "engine"],
+                    choices=["search", "streaming", "serving", "copy", "gather", "engine", "bang"],
                     help="search: the headline metric; streaming: 10 rounds of 5%% deletes + "
                          "5%% inserts with consolidation, recall@10 per round; serving: the "
                          "continuous-batching request loop; copy: a 1 GiB device-to-device copy "
@@ -69,7 +69,8 @@ def parse():
                          "512-byte rows, each row read once (the calibration for random row "
                          "gathers); engine: the "
                          "parth engine's replay of its own SIFT workload file (batches of 50 "
-                         "mixed operations), the reference's qps / recall_10 metric")
+                         "mixed operations), the reference's qps / recall_10 metric; bang: "
+                         "BANG's SIFT10K run (400 queries, L 40) on its exact search schedule")
     ap.add_argument("--contexts", type=int, default=2,
                     help="search contexts (streams) the timed batches are issued round-robin over")
     ap.add_argument("--engine-reps", type=int, default=5)
@@ -97,6 +98,16 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "streaming":
         return streaming(args, vmn, torch, dev)
+    if args.workload == "bang":
+        b = bang_section(vmn, args)
+        print(json.dumps({"metric": "BANG exact search QPS, SIFT10K, 400 queries, L 40",
+                          "value": b["qps"], "unit": "queries/s", "n_gpus": 1,
+                          "steps": max(args.steps, 20), "warmup": args.warmup,
+                          "ms_per_step": b["ms"], "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": b["vs_reference"], "dtype": "fp32",
+                          "data": "SIFT10K base and queries, DiskANN graph (tests/golden)",
+                          "config": b}), flush=True)
+        return
     if args.workload == "serving":
         return serving(args, vmn, torch, dev)
     if args.workload == "copy":
@@ -294,6 +305,8 @@ def main():
         err[prec] = {"rms_rel": float(np.sqrt(np.mean(rel ** 2))), "max_rel": float(np.abs(rel).max())}
         del rows
 
+    bang = bang_section(vmn, args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(idx, base, qs, gt, chosen["fp32"][0]["L"], args.cpu_seconds, args.mode)
@@ -334,11 +347,82 @@ def main():
             "qps_host_buffers": round(nq / host_ms * 1e3, 1),
             "batches": f"K batches of {nq:,} queries issued round-robin over {nctx} search "
                        "contexts (query sets: mixture streams 1 and 2, alternating)",
-            "distance_error_vs_fp64": err, "sweep": sweeps},
+            "distance_error_vs_fp64": err, "sweep": sweeps, "bang": bang},
         "roofline": roofline(head),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+BANG_REF_QPS = 65271.94  # bang_exact/output.txt:46, best warm run (A100 80 GB)
+
+
+def bang_section(vmn, args):
+    """BANG's SIFT10K run (bang_exact/output.txt): 400 queries over the DiskANN
+    graph tests/golden/diskann_graph.npz from entry 123 at L 40, k 10 (the
+    query file's 100 rows and 300 zero rows), on vmn_bang_search. Kernel time
+    from HIP events on the index stream around K back-to-back launches; also
+    the same schedule on 10,000 queries (the 100 tiled). Bytes: every distance
+    computation reads one 512-byte FP32 row, every round one 256-byte adjacency
+    row; SIFT10K's 5 MB of rows stay in cache, so this is not an HBM figure."""
+    import ctypes
+    import torch
+    V = importlib.import_module("cs5013-project_amd.vamana")
+    gold = os.path.join(ROOT, "tests", "golden")
+    s = np.load(os.path.join(gold, "sift10k.npz"))
+    t = np.load(os.path.join(gold, "bang_sift10k.npz"))
+    adj = np.load(os.path.join(gold, "diskann_graph.npz"))["adj"]
+    base = s["base"].astype(np.float32)
+    q = np.zeros((400, 128), np.float32)
+    q[:100] = s["query"].astype(np.float32)
+    dev = torch.device("cuda:0")
+    bidx = vmn.Vamana.from_graph(base, adj, medoid=3732, mode="fresh", R=64)
+    stream = torch.cuda.ExternalStream(bidx.stream)
+
+    def timed(qh, reps):
+        nq = qh.shape[0]
+        qd = torch.from_numpy(qh).to(dev)
+        ids = torch.empty((nq, 10), dtype=torch.int32, device=dev)
+        d = torch.empty((nq, 10), dtype=torch.float32, device=dev)
+        st = torch.empty((nq, 2), dtype=torch.int32, device=dev)
+        ptr = [ctypes.c_void_p(x.data_ptr()) for x in (qd, ids, d, st)]
+
+        def launch():
+            V._check(V.lib().vmn_bang_search(bidx._h, ptr[0], nq, 10, 40, 123, ptr[1], ptr[2],
+                                             ptr[3], 1))
+        for _ in range(args.warmup):
+            launch()
+        bidx.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            launch()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps, ids.cpu().numpy(), st.cpu().numpy()
+
+    ms, ids, st = timed(q, max(args.steps, 20))
+    comps, rounds = int(st[:, 0].sum()), int(st[:, 1].sum())
+    total_recall = int(round(vmn.recall_at_k(ids, t["gt_ids"], 10, gt_dists=t["gt_dists"]) * 4000))
+    ms10, _, st10 = timed(np.tile(q[:100], (100, 1)), max(args.steps, 10))
+    gbs = (comps * 512 + rounds * 256) / ms / 1e6
+    log(f"BANG SIFT10K 400 queries L40: {ms:.4f} ms ({400 / ms * 1e3:,.0f} QPS, "
+        f"{400 / ms * 1e3 / BANG_REF_QPS:.1f}x BANG's best A100 run); {comps} computations, "
+        f"{int(st[:, 1].max())} rounds, total recall {total_recall}; 10,000 queries "
+        f"{ms10:.4f} ms ({1e4 / ms10 * 1e3:,.0f} QPS)")
+    bidx.close()
+    return {"workload": "BANG exact search, SIFT10K DiskANN graph, 400 queries (100 + 300 zero "
+                        "rows), L 40, k 10, entry 123",
+            "kernel": "bang_kernel<128, 64>", "ms": round(ms, 4), "qps": round(400 / ms * 1e3, 1),
+            "reference_qps": BANG_REF_QPS,
+            "vs_reference": round(400 / ms * 1e3 / BANG_REF_QPS, 2),
+            "computations": comps, "rounds": int(st[:, 1].max()), "total_recall": total_recall,
+            "matches_printed_run": (comps, int(st[:, 1].max()), total_recall) == (641853, 47, 2190),
+            "row_bytes_per_s_GB": round(gbs, 1),
+            "qps_10k_queries": round(1e4 / ms10 * 1e3, 1),
+            "computations_10k": int(st10[:, 0].sum()),
+            "time": "HIP events on the index stream around back-to-back launches"}
 
 
 def roofline(e):

@@ -9,7 +9,11 @@
 // (:652-654), over a 399,888-byte global bloom filter per query (:346). Here
 // nothing leaves the device between rounds and a query ends as soon as it has
 // no parent; its bloom filter is 399,887 bits (12,500 words, 50 KB) in LDS,
-// cleared at the query's start, so three queries share a CU.
+// cleared at the query's start, so three queries share a CU. The cells of ids
+// 0 .. 32,791 are all distinct (the first two ids with one cell are 16,573 and
+// 32,792), so on a graph of at most 32,792 nodes the filter is exactly the set
+// of ids seen: there it is a bitmap over ids, n bits (1.25 KB for SIFT10K),
+// and the CU holds as many queries as registers allow.
 //
 // Per round (semantics of the reference, order fixed where it is racy there):
 //  * candidates: in round 1 the entry point, then the parent's neighbours in
@@ -33,6 +37,7 @@
 // the number of distance computations and of rounds run.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -46,6 +51,7 @@ namespace {
 
 constexpr uint32_t kCells = 399887u;  // BF_ENTRIES, parANN.cu:36
 constexpr int kWords = 12500;         // >= ceil(kCells / 32), a multiple of 4
+constexpr int64_t kInjective = 32792;  // ids below this have distinct cells
 
 struct BangArgs {
     const float* vecs;       // [n x DP]
@@ -55,6 +61,8 @@ struct BangArgs {
     int64_t nq;
     int R, L, k, max_rounds;
     int32_t entry;
+    int words;               // filter words in LDS (a multiple of 4)
+    int by_id;               // filter indexed by id (n <= kInjective)
     int32_t* out_ids;        // [nq x k]
     float* out_d;            // [nq x k] or nullptr
     int32_t* out_stats;      // [nq x 2] or nullptr
@@ -67,6 +75,11 @@ __device__ __forceinline__ uint32_t bang_cell(uint32_t x) {
     h = (h ^ ((x >> 16) & 0xffu)) * 0x01000193ull;
     h = (h ^ ((x >> 24) & 0xffu)) * 0x01000193ull;
     return (uint32_t)(h % kCells);
+}
+
+// The filter bit of id x: its cell, or x itself where cells are distinct.
+__device__ __forceinline__ uint32_t filter_bit(uint32_t x, int by_id) {
+    return by_id ? x : bang_cell(x);
 }
 
 // Squared L2 of one row against the query in LDS, summed in the reference's
@@ -108,32 +121,37 @@ __device__ __forceinline__ int count_below(const float* arr, int n, float x) {
     return lo;
 }
 
+// At most 3 waves per SIMD: the register budget (up to 168) keeps a lane's 32
+// row loads of a round in flight together; at the compiler's default of 8 the
+// loads were issued in smaller groups and a round took a third longer.
 template <int DP, int LCAP>
-__global__ __launch_bounds__(64) void bang_kernel(BangArgs a) {
-    __shared__ __align__(16) uint32_t bloom[kWords];
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3))) void bang_kernel(
+    BangArgs a) {
+    extern __shared__ __align__(16) uint32_t bloom[];  // a.words words
     __shared__ __align__(16) float qs[DP];
     __shared__ int32_t sid[72];  // sorted candidates (at most R + 1 = 65)
     __shared__ float sd[72];
-    __shared__ int32_t lid[2][LCAP];  // best-L list, two buffers for the merge
-    __shared__ float ld[2][LCAP];
-    __shared__ uint8_t lvis[2][LCAP];
+    __shared__ int32_t lid[LCAP];  // best-L list
+    __shared__ float ld[LCAP];
+    __shared__ uint8_t lvis[LCAP];
+    constexpr int PER = LCAP / 64;
 
     const int l = vmn::lane();
     const int64_t qi = blockIdx.x;
-    for (int i = l; i < kWords / 4; i += 64)
+    for (int i = l; i < a.words / 4; i += 64)
         reinterpret_cast<uint4*>(bloom)[i] = make_uint4(0u, 0u, 0u, 0u);
     for (int i = l; i < DP; i += 64) qs[i] = a.q[qi * DP + i];
     __syncthreads();
 
     const int32_t entry = a.entry;
-    int cur = 0, B = 0, comps = 0, rounds = 0;
+    int B = 0, comps = 0, rounds = 0;
     int32_t parent = entry;
     for (int it = 1; it <= a.max_rounds; ++it) {
         rounds = it;
         const bool first = it == 1;
         float ed = 0.f;
         if (first) {
-            const uint32_t c = bang_cell((uint32_t)entry);
+            const uint32_t c = filter_bit((uint32_t)entry, a.by_id);
             if (l == 0) bloom[c >> 5] |= 1u << (c & 31);
             ed = bang_dist<DP>(a.vecs + (size_t)entry * DP, qs);
             __syncthreads();
@@ -144,7 +162,7 @@ __global__ __launch_bounds__(64) void bang_kernel(BangArgs a) {
         const int deg = ends ? __ffsll((unsigned long long)ends) - 1 : 64;
         bool want = l < deg;
         if (want && a.del && vmn::bit_test(a.del, (uint32_t)nbr)) want = false;
-        const uint32_t c = want ? bang_cell((uint32_t)nbr) : 0u;
+        const uint32_t c = want ? filter_bit((uint32_t)nbr, a.by_id) : 0u;
         const uint32_t w = c >> 5, bit = 1u << (c & 31);
         if (want) want = !(bloom[w] & bit);
         uint32_t old = 0u;
@@ -167,60 +185,87 @@ __global__ __launch_bounds__(64) void bang_kernel(BangArgs a) {
         float d = 0.f;
         if (want) d = bang_dist<DP>(a.vecs + (size_t)nbr * DP, qs);
 
-        // stable sort by (distance, arrival): arrival is the entry, then lane order
+        // stable sort by (distance, arrival): arrival is the entry, then lane
+        // order. After round 1 only the candidates that enter the list need a
+        // rank: while the list is full (nb = #candidates below its last
+        // distance, at most T) those are the ones below that distance, whose
+        // ranks among themselves are their ranks among all.
+        int nB, nb = 0;
+        float mx = 0.f;
+        uint64_t rm = km;
+        if (!first) {
+            mx = ld[B - 1];
+            const uint64_t below = vmn::ballot(want && d < mx);
+            const int nb0 = min(__popcll((unsigned long long)below), min(a.L, nc));
+            nb = max(nb0, min(a.L - B, nc));
+            if (nb == nb0) rm = below;
+        }
         int rank = first && ed <= d ? 1 : 0;
-        int erank = 0;
-        for (uint64_t m = km; m; m &= m - 1) {
+        for (uint64_t m = rm; m; m &= m - 1) {
             const int t = __ffsll((unsigned long long)m) - 1;
             const float dt = vmn::readlane_f(d, t);
             rank += dt < d || (dt == d && t < l);
-            erank += dt < ed;
         }
-        if (want) {
+        if (((rm >> l) & 1) && (first || rank < nb)) {
             sid[rank] = nbr;
             sd[rank] = d;
         }
-        if (first && l == 0) {
-            sid[erank] = entry;
-            sd[erank] = ed;
+        if (first) {
+            int erank = 0;
+            for (uint64_t m = km; m; m &= m - 1)
+                erank += vmn::readlane_f(d, __ffsll((unsigned long long)m) - 1) < ed;
+            if (l == 0) {
+                sid[erank] = entry;
+                sd[erank] = ed;
+            }
         }
         __syncthreads();
 
-        // best-L merge (compute_BestLSets_par_sort_msort_new)
-        int nB;
+        // best-L merge (compute_BestLSets_par_sort_msort_new): every position
+        // is taken from the old list before any entry is written
         if (first) {
             nB = min(nc, a.L);
             for (int i = l; i < nB; i += 64) {
-                lid[cur][i] = sid[i];
-                ld[cur][i] = sd[i];
-                lvis[cur][i] = sid[i] == entry;
+                lid[i] = sid[i];
+                ld[i] = sd[i];
+                lvis[i] = sid[i] == entry;
             }
         } else {
-            const float mx = ld[cur][B - 1];
-            const int T = min(a.L, nc);
-            int nb = __popcll((unsigned long long)vmn::ballot(l < T && sd[l] < mx));
-            nb = max(nb, min(a.L - B, nc));
             nB = min(B + nb, a.L);
-            const int nx = cur ^ 1;
+            int32_t oid[PER], nid = 0;
+            float od[PER], nd = 0.f;
+            uint8_t ov[PER];
+            int op[PER], np = nB;
             if (l < nb) {
-                const float x = sd[l];
-                const int p = count_below<false>(ld[cur], B, x) + l;
-                if (p < nB) {
-                    lid[nx][p] = sid[l];
-                    ld[nx][p] = x;
-                    lvis[nx][p] = 0;
+                nd = sd[l];
+                nid = sid[l];
+                np = count_below<false>(ld, B, nd) + l;
+            }
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int j = l + 64 * u;
+                op[u] = nB;
+                if (j < B) {
+                    oid[u] = lid[j];
+                    od[u] = ld[j];
+                    ov[u] = lvis[j];
+                    op[u] = count_below<true>(sd, nb, od[u]) + j;
                 }
             }
-            for (int j = l; j < B; j += 64) {
-                const float x = ld[cur][j];
-                const int p = count_below<true>(sd, nb, x) + j;
-                if (p < nB) {
-                    lid[nx][p] = lid[cur][j];
-                    ld[nx][p] = x;
-                    lvis[nx][p] = lvis[cur][j];
+            __syncthreads();
+            if (np < nB) {
+                lid[np] = nid;
+                ld[np] = nd;
+                lvis[np] = 0;
+            }
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                if (op[u] < nB) {
+                    lid[op[u]] = oid[u];
+                    ld[op[u]] = od[u];
+                    lvis[op[u]] = ov[u];
                 }
             }
-            cur = nx;
         }
         __syncthreads();
         B = nB;
@@ -229,11 +274,11 @@ __global__ __launch_bounds__(64) void bang_kernel(BangArgs a) {
         int32_t next = -1;
         for (int base = 0; base < B; base += 64) {
             const int i = base + l;
-            const uint64_t m = vmn::ballot(i < B && !lvis[cur][i]);
+            const uint64_t m = vmn::ballot(i < B && !lvis[i]);
             if (m) {
                 const int t = base + __ffsll((unsigned long long)m) - 1;
-                next = lid[cur][t];
-                if (l == 0) lvis[cur][t] = 1;
+                next = lid[t];
+                if (l == 0) lvis[t] = 1;
                 break;
             }
         }
@@ -244,8 +289,8 @@ __global__ __launch_bounds__(64) void bang_kernel(BangArgs a) {
 
     for (int j = l; j < a.k; j += 64) {
         const bool ok = j < B;
-        a.out_ids[qi * a.k + j] = ok ? lid[cur][j] : -1;
-        if (a.out_d) a.out_d[qi * a.k + j] = ok ? ld[cur][j] : INFINITY;
+        a.out_ids[qi * a.k + j] = ok ? lid[j] : -1;
+        if (a.out_d) a.out_d[qi * a.k + j] = ok ? ld[j] : INFINITY;
     }
     if (a.out_stats && l == 0) {
         a.out_stats[qi * 2] = comps;
@@ -256,10 +301,11 @@ __global__ __launch_bounds__(64) void bang_kernel(BangArgs a) {
 template <int DP>
 hipError_t launch_dp(const BangArgs& a, hipStream_t s) {
     const dim3 grid((unsigned)a.nq), block(64);
+    const size_t lds = (size_t)a.words * sizeof(uint32_t);
     if (a.L <= 64)
-        hipLaunchKernelGGL((bang_kernel<DP, 64>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((bang_kernel<DP, 64>), grid, block, lds, s, a);
     else
-        hipLaunchKernelGGL((bang_kernel<DP, 256>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((bang_kernel<DP, 256>), grid, block, lds, s, a);
     return hipGetLastError();
 }
 
@@ -319,6 +365,19 @@ extern "C" int vmn_bang_search(vmn_index* idx, const float* queries, int64_t nq,
     a.k = k;
     a.max_rounds = 4 * L + 19;
     a.entry = (int32_t)entry;
+    a.by_id = idx->n <= kInjective;
+    a.words = a.by_id ? (int)((idx->n + 127) / 128 * 4) : kWords;
+    if (a.by_id) {
+        // A batch smaller than the CUs can hold at once: pad each query's LDS
+        // so that a CU takes at most ceil(nq / CUs) of them (160 KB of LDS per
+        // CU, about 4 KB of it the kernel's own arrays) and the queries spread
+        // over all CUs instead of sharing SIMDs on a few.
+        const int64_t per_cu = std::max<int64_t>(1, (nq + idx->cus - 1) / std::max(1, idx->cus));
+        if (per_cu < 12) {
+            const int pad = (int)std::min<int64_t>(kWords, ((163840 / per_cu) - 4096) / 4 / 4 * 4);
+            a.words = std::max(a.words, pad);
+        }
+    }
     a.out_ids = oi;
     a.out_d = od;
     a.out_stats = os;

@@ -116,7 +116,7 @@ def compile_example(name, out):
     return out
 
 
-@pytest.mark.parametrize("name", ["parth_engine", "fresh_vamana"])
+@pytest.mark.parametrize("name", ["parth_engine", "fresh_vamana", "bang_search"])
 def test_cpp_drivers_build_against_the_c_abi(tmp_path, name):
     import subprocess
     exe = compile_example(name, str(tmp_path / name))

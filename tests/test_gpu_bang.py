@@ -60,17 +60,18 @@ def test_bang_gpu_other_list_sizes(orc, sift_bang, L, k, entry):
     same(got, orc.bang_search(b["base"], b["adj"], b["q"][:160], e, L, k=k))
 
 
-@pytest.mark.parametrize("dim", [32, 100, 128, 200])
-def test_bang_gpu_real_valued_rows(vmn, orc, gpu, dim):
+@pytest.mark.parametrize("dim,n", [(32, 3000), (100, 3000), (128, 3000), (200, 3000),
+                                   (128, 40000)])
+def test_bang_gpu_real_valued_rows(vmn, orc, gpu, dim, n):
     """Gaussian rows (FP32 rounding everywhere): the GPU sums each distance in
     the oracle's order, so results are bit-identical, also where the width is
-    padded (100 -> 128, 200 -> 256)."""
+    padded (100 -> 128, 200 -> 256). 40,000 nodes: past the ids whose bloom
+    cells are all distinct, the filter is the 399,887-cell one."""
     rng = np.random.default_rng(dim)
-    n = 3000
     v = rng.standard_normal((n, dim)).astype(np.float32)
     adj = orc.random_init(n, 24, 32, seed=dim)
     q = rng.standard_normal((96, dim)).astype(np.float32)
-    print(f"bang gaussian dim={dim}", flush=True)
+    print(f"bang gaussian dim={dim} n={n}", flush=True)
     idx = vmn.Vamana.from_graph(v, adj, medoid=5, mode="fresh", R=32)
     got = idx.bang_search(q, k=10, L=48, entry=5)
     same(got, orc.bang_search(v, adj, q, 5, 48, k=10))
@@ -133,6 +134,31 @@ def test_bang_gpu_device_tensors(orc, sift_bang):
     ids, d, st = b["idx"].bang_search(qd, k=10, L=40, entry=123)
     assert ids.is_cuda and st.is_cuda
     same((ids, d, st), orc.bang_search(b["base"], b["adj"], b["q"][:64], 123, 40, k=10))
+
+
+def test_cpp_bang_driver(vmn, sift_bang, tmp_path):
+    """examples/bang_search.cpp, BANG's command line over the C ABI, on the
+    reference's files rebuilt from the fixtures (the DiskANN disk index with
+    its metadata, the 100-row query file, the truthset's first 400 rows):
+    prints BANG's totals for its 400-query run."""
+    import subprocess
+    from test_abi import compile_example
+    b = sift_bang
+    ip, mp = str(tmp_path / "sift10k_index_disk.bin"), str(tmp_path / "sift10k_index_disk_metadata.bin")
+    b["idx"].save_bang_index(ip, mp)
+    qf, tf = str(tmp_path / "siftsmall_query.bin"), str(tmp_path / "sift10k_groundtruth.bin")
+    vmn.write_vectors(qf, b["q"][:100], "bin")
+    vmn.write_truthset(tf, b["gt_ids"], b["gt_dists"])
+    exe = compile_example("bang_search", str(tmp_path / "bang_search"))
+    print("bang_search driver", flush=True)
+    r = subprocess.run([exe, ".", ".", ip, qf, ".", ".", tf, "400", "1", "256", "512", "256",
+                        "10", "8", "1", "40", "123"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = r.stdout
+    assert "[medoidID=123, numQueries=400]" in out
+    assert "iterations = 47" in out
+    assert "total_recall = 2190.00 num_queries = 400 recall_at 10" in out
+    assert "40\t54.75\t total parents 0\t total computations: 641853" in out
 
 
 def test_bang_gpu_argument_checks(vmn, sift_bang):

@@ -59,6 +59,14 @@ def test_bang_hash_cells(orc):
     assert max(cells) < 399887 and len(cells) > 9800
 
 
+def test_bang_cells_are_distinct_below_32792(orc):
+    """The GPU filter indexes by id on graphs of at most 32,792 nodes: the
+    cells of ids 0 .. 32,791 are distinct, and 32,792 shares one with 16,573."""
+    cells = [orc.bang_hash(i) for i in range(32793)]
+    assert len(set(cells[:32792])) == 32792
+    assert cells[32792] == cells[16573]
+
+
 def test_bang_distance_is_exact_on_integer_rows(orc, sift):
     b = sift["base"][:50].astype(np.int64)
     q = sift["query"][0].astype(np.int64)
