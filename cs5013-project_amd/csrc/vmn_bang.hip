@@ -1,6 +1,7 @@
 // BANG's exact search schedule (bang_exact/src/parANN.cu:96-978) over the
 // index's graph, as one kernel: a wavefront owns one query and runs all of its
-// rounds, with the query's bloom filter, candidates and best-L list in LDS.
+// rounds, with the query's bloom filter in LDS and its best-L list in
+// registers (one entry per lane).
 //
 // The reference drives the rounds from the host: per round a memset, three
 // kernels (neighbor_filtering_new :996-1052, compute_neighborDist_par
@@ -30,8 +31,9 @@
 //    min(count, L) as the list (entry visited); later rounds merge the first
 //    nb candidates, nb = max(#leading candidates below the list's last
 //    distance, min(L - B, count)), new entries first on ties, cut at L;
-//  * the first unvisited list entry is the next parent; a round without
-//    candidates or parent ends the query, and so does round 4L + 19
+//  * the first unvisited list entry is the next parent (known before the
+//    merge lays the list out, so its adjacency row loads meanwhile); a round
+//    without candidates or parent ends the query, and so does round 4L + 19
 //    (MAX_PARENTS_PERQUERY - 1, :49-50, :665-668).
 // Outputs: the first k list entries (-1 / +inf past the list), and per query
 // the number of distance computations and of rounds run.
@@ -107,18 +109,21 @@ __device__ __forceinline__ float bang_dist(const float* __restrict__ row, const 
     return ((p0 + p1) + (p2 + p3)) + ((p4 + p5) + (p6 + p7));
 }
 
-// Entries of the ascending arr[0, n) below x (LE: at or below x): the
-// lower_bound_d / upper_bound_d positions of parANN.cu:1672-1700.
-template <bool LE>
-__device__ __forceinline__ int count_below(const float* arr, int n, float x) {
-    int lo = 0;
-    for (int step = n > 0 ? 1 << (31 - __clz(n)) : 0; step > 0; step >>= 1) {
-        if (lo + step <= n) {
-            const float v = arr[lo + step - 1];
-            if (LE ? v <= x : v < x) lo += step;
+// Element i of a list held one entry per lane (entry i in lane i % 64, slot
+// i / 64); i is uniform.
+template <int PER, class T>
+__device__ __forceinline__ T list_at(const T (&v)[PER], int i) {
+    T r{};
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (k == i / 64) {
+            if constexpr (sizeof(T) == 4 && !__is_same(T, float))
+                r = (T)vmn::readlane_i((int)v[k], i % 64);
+            else
+                r = vmn::readlane_f(v[k], i % 64);
         }
     }
-    return lo;
+    return r;
 }
 
 // At most 3 waves per SIMD: the register budget (up to 168) keeps a lane's 32
@@ -129,12 +134,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3))) void
     BangArgs a) {
     extern __shared__ __align__(16) uint32_t bloom[];  // a.words words
     __shared__ __align__(16) float qs[DP];
-    __shared__ int32_t sid[72];  // sorted candidates (at most R + 1 = 65)
-    __shared__ float sd[72];
-    __shared__ int32_t lid[LCAP];  // best-L list
-    __shared__ float ld[LCAP];
-    __shared__ uint8_t lvis[LCAP];
+    // staging: the sorted candidates (at most R + 1 = 65), then the merged list
+    constexpr int SZ = LCAP > 72 ? LCAP : 72;
+    __shared__ int32_t sid[SZ];
+    __shared__ float sd[SZ];
+    __shared__ uint8_t sv[SZ];
     constexpr int PER = LCAP / 64;
+    // the best-L list in registers: entry i in lane i % 64, slot i / 64
+    int32_t lid[PER];
+    float ld[PER];
+    bool lv[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        lid[k] = -1;
+        ld[k] = INFINITY;
+        lv[k] = true;
+    }
 
     const int l = vmn::lane();
     const int64_t qi = blockIdx.x;
@@ -145,7 +160,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3))) void
 
     const int32_t entry = a.entry;
     int B = 0, comps = 0, rounds = 0;
-    int32_t parent = entry;
+    // the parent's adjacency row: loaded as soon as the parent is known,
+    // before the merge that lays out the new list
+    int32_t row = l < a.R ? a.adj[(size_t)entry * a.R + l] : -1;
     for (int it = 1; it <= a.max_rounds; ++it) {
         rounds = it;
         const bool first = it == 1;
@@ -157,7 +174,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3))) void
             __syncthreads();
         }
         // neighbour filtering (neighbor_filtering_new)
-        const int32_t nbr = l < a.R ? a.adj[(size_t)parent * a.R + l] : -1;
+        const int32_t nbr = row;
         const uint64_t ends = vmn::ballot(nbr < 0);
         const int deg = ends ? __ffsll((unsigned long long)ends) - 1 : 64;
         bool want = l < deg;
@@ -188,13 +205,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3))) void
         // stable sort by (distance, arrival): arrival is the entry, then lane
         // order. After round 1 only the candidates that enter the list need a
         // rank: while the list is full (nb = #candidates below its last
-        // distance, at most T) those are the ones below that distance, whose
-        // ranks among themselves are their ranks among all.
-        int nB, nb = 0;
-        float mx = 0.f;
+        // distance, at most min(L, count)) those are the ones below that
+        // distance, whose ranks among themselves are their ranks among all.
+        int nb = 0;
         uint64_t rm = km;
         if (!first) {
-            mx = ld[B - 1];
+            const float mx = list_at<PER>(ld, B - 1);
             const uint64_t below = vmn::ballot(want && d < mx);
             const int nb0 = min(__popcll((unsigned long long)below), min(a.L, nc));
             nb = max(nb0, min(a.L - B, nc));
@@ -221,76 +237,110 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3))) void
         }
         __syncthreads();
 
-        // best-L merge (compute_BestLSets_par_sort_msort_new): every position
-        // is taken from the old list before any entry is written
+        int32_t next = -1;
         if (first) {
-            nB = min(nc, a.L);
-            for (int i = l; i < nB; i += 64) {
-                lid[i] = sid[i];
-                ld[i] = sd[i];
-                lvis[i] = sid[i] == entry;
+            // round 1: the first min(count, L) sorted candidates, the entry visited
+            B = min(nc, a.L);
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int i = l + 64 * k;
+                if (i < B) {
+                    lid[k] = sid[i];
+                    ld[k] = sd[i];
+                    lv[k] = sid[i] == entry;
+                }
             }
+            // next parent: the first unvisited entry
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const uint64_t m = vmn::ballot(l + 64 * k < B && !lv[k]);
+                if (m && next < 0) {
+                    const int t = __ffsll((unsigned long long)m) - 1;
+                    next = vmn::readlane_i(lid[k], t);
+                    if (l == t) lv[k] = true;
+                }
+            }
+            if (next >= 0) row = l < a.R ? a.adj[(size_t)next * a.R + l] : -1;
         } else {
-            nB = min(B + nb, a.L);
-            int32_t oid[PER], nid = 0;
-            float od[PER], nd = 0.f;
-            uint8_t ov[PER];
-            int op[PER], np = nB;
+            // best-L merge (compute_BestLSets_par_sort_msort_new): the nb sorted
+            // candidates, new entries before list entries on ties, cut at L
+            float xs = 0.f;
+            int32_t xid = 0;
             if (l < nb) {
-                nd = sd[l];
-                nid = sid[l];
-                np = count_below<false>(ld, B, nd) + l;
+                xs = sd[l];
+                xid = sid[l];
+            }
+            // The next parent is the merged list's first unvisited entry: the
+            // closest new entry if it is at or below the first unvisited list
+            // entry u (new entries come first on ties), else u. Its adjacency
+            // row is loaded now, so the load overlaps the merge.
+            int u = -1;
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const uint64_t m = vmn::ballot(l + 64 * k < B && !lv[k]);
+                if (m && u < 0) u = 64 * k + __ffsll((unsigned long long)m) - 1;
+            }
+            const bool take_new =
+                nb > 0 && (u < 0 || vmn::readlane_f(xs, 0) <= list_at<PER>(ld, u));
+            next = take_new ? vmn::readlane_i(xid, 0) : (u >= 0 ? list_at<PER>(lid, u) : -1);
+            if (next >= 0) row = l < a.R ? a.adj[(size_t)next * a.R + l] : -1;
+
+            // positions: new entry t at #(list entries below it) + t, list
+            // entry j at j + #(new entries at or below it)
+            const int nB = min(B + nb, a.L);
+            int np = l;
+            int cle[PER];
+#pragma unroll
+            for (int k = 0; k < PER; ++k) cle[k] = 0;
+            for (int t = 0; t < nb; ++t) {
+                const float xt = vmn::readlane_f(xs, t);
+                int less = 0;
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    less += __popcll((unsigned long long)vmn::ballot(l + 64 * k < B && ld[k] < xt));
+                    cle[k] += xt <= ld[k];
+                }
+                np = vmn::writelane_i(less + t, t, np);
+            }
+            __syncthreads();
+            if (l < nb && np < nB) {
+                sid[np] = xid;
+                sd[np] = xs;
+                sv[np] = take_new && l == 0;
             }
 #pragma unroll
-            for (int u = 0; u < PER; ++u) {
-                const int j = l + 64 * u;
-                op[u] = nB;
-                if (j < B) {
-                    oid[u] = lid[j];
-                    od[u] = ld[j];
-                    ov[u] = lvis[j];
-                    op[u] = count_below<true>(sd, nb, od[u]) + j;
+            for (int k = 0; k < PER; ++k) {
+                const int j = l + 64 * k;
+                const int p = j + cle[k];
+                if (j < B && p < nB) {
+                    sid[p] = lid[k];
+                    sd[p] = ld[k];
+                    sv[p] = lv[k] || (!take_new && j == u);
                 }
             }
             __syncthreads();
-            if (np < nB) {
-                lid[np] = nid;
-                ld[np] = nd;
-                lvis[np] = 0;
-            }
+            B = nB;
 #pragma unroll
-            for (int u = 0; u < PER; ++u) {
-                if (op[u] < nB) {
-                    lid[op[u]] = oid[u];
-                    ld[op[u]] = od[u];
-                    lvis[op[u]] = ov[u];
+            for (int k = 0; k < PER; ++k) {
+                const int i = l + 64 * k;
+                if (i < B) {
+                    lid[k] = sid[i];
+                    ld[k] = sd[i];
+                    lv[k] = sv[i];
                 }
-            }
-        }
-        __syncthreads();
-        B = nB;
-
-        // next parent: the first unvisited entry
-        int32_t next = -1;
-        for (int base = 0; base < B; base += 64) {
-            const int i = base + l;
-            const uint64_t m = vmn::ballot(i < B && !lvis[i]);
-            if (m) {
-                const int t = base + __ffsll((unsigned long long)m) - 1;
-                next = lid[t];
-                if (l == 0) lvis[t] = 1;
-                break;
             }
         }
         __syncthreads();
         if (next < 0) break;
-        parent = next;
     }
 
-    for (int j = l; j < a.k; j += 64) {
-        const bool ok = j < B;
-        a.out_ids[qi * a.k + j] = ok ? lid[j] : -1;
-        if (a.out_d) a.out_d[qi * a.k + j] = ok ? ld[j] : INFINITY;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int j = l + 64 * k;
+        if (j < a.k) {
+            a.out_ids[qi * a.k + j] = j < B ? lid[k] : -1;
+            if (a.out_d) a.out_d[qi * a.k + j] = j < B ? ld[k] : INFINITY;
+        }
     }
     if (a.out_stats && l == 0) {
         a.out_stats[qi * 2] = comps;
@@ -381,7 +431,17 @@ extern "C" int vmn_bang_search(vmn_index* idx, const float* queries, int64_t nq,
     a.out_ids = oi;
     a.out_d = od;
     a.out_stats = os;
-    HIPCHK(launch_bang(a, idx->DP, idx->stream));
+    // one workgroup per query; a dispatch's grid counts work-items in 32 bits
+    constexpr int64_t kChunk = (int64_t)1 << 24;
+    for (int64_t q0 = 0; q0 < nq; q0 += kChunk) {
+        BangArgs c = a;
+        c.nq = std::min(kChunk, nq - q0);
+        c.q = q + q0 * idx->DP;
+        c.out_ids = oi + q0 * k;
+        c.out_d = od ? od + q0 * k : nullptr;
+        c.out_stats = os ? os + q0 * 2 : nullptr;
+        HIPCHK(launch_bang(c, idx->DP, idx->stream));
+    }
     if (mem == VMN_MEM_HOST) {
         HIPCHK(hipMemcpyAsync(ids_out, oi, (size_t)nq * k * sizeof(int32_t), hipMemcpyDeviceToHost,
                               idx->stream));

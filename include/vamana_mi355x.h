@@ -199,7 +199,8 @@ int vmn_load_bang_index(const char* index_path, const char* metadata_path, const
  * +inf past the list's end) and, in stats_out [nq x 2] (may be NULL), the
  * distance computations and rounds of each query: their sum and maximum are
  * BANG's "total computations" and "iterations". Tombstoned rows are never
- * candidates. Needs 1 <= k <= L <= 256. `mem` as for vmn_search. */
+ * candidates; staged rows (vmn_insert_staged) are not searched. Needs
+ * 1 <= k <= L <= 256. `mem` as for vmn_search. */
 int vmn_bang_search(vmn_index* idx, const float* queries, int64_t nq, int32_t k, int32_t L,
                     int64_t entry, int32_t* ids_out, float* dists_out, int32_t* stats_out,
                     int32_t mem);
