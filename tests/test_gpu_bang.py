@@ -77,6 +77,31 @@ def test_bang_gpu_real_valued_rows(vmn, orc, gpu, dim, n):
     same(got, orc.bang_search(v, adj, q, 5, 48, k=10))
 
 
+@pytest.mark.parametrize("dim,n", [(128, 10000), (100, 3000), (128, 40000)])
+def test_bang_gpu_large_batches(vmn, orc, sift_bang, dim, n):
+    """Batches of 2,048 and more queries read each round's rows 8 lanes per
+    candidate (chain s in lane s, the same sums): bit-identical to the oracle
+    on SIFT10K and on Gaussian rows (padded width; 40,000 nodes: the
+    399,887-cell filter)."""
+    if n == 10000:
+        b = sift_bang
+        v, adj, entry, idx = b["base"], b["adj"], 123, b["idx"]
+        q = np.concatenate([np.tile(b["q"][:100], (20, 1)), np.zeros((48, 128), np.float32)])
+        L = 40
+    else:
+        rng = np.random.default_rng(n + dim)
+        v = rng.standard_normal((n, dim)).astype(np.float32)
+        adj = orc.random_init(n, 24, 32, seed=dim)
+        q = rng.standard_normal((2048, dim)).astype(np.float32)
+        entry, L = 5, 48
+        idx = vmn.Vamana.from_graph(v, adj, medoid=entry, mode="fresh", R=32)
+    print(f"bang large batch dim={dim} n={n} nq={q.shape[0]}", flush=True)
+    got = idx.bang_search(q, k=10, L=L, entry=entry)
+    same(got, orc.bang_search(v, adj, q, entry, L, k=10))
+    if n == 10000:
+        assert int(got[2][:2000, 0].sum()) == 20 * int(got[2][:100, 0].sum())
+
+
 def test_bang_gpu_tombstones(vmn, orc, sift_bang):
     b = sift_bang
     idx = vmn.Vamana.from_graph(b["base"], b["adj"], medoid=3732, mode="fresh", R=64)
