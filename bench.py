@@ -414,7 +414,7 @@ def bang_section(vmn, args):
     bidx.close()
     return {"workload": "BANG exact search, SIFT10K DiskANN graph, 400 queries (100 + 300 zero "
                         "rows), L 40, k 10, entry 123",
-            "kernel": "bang_kernel<128, 64>", "ms": round(ms, 4), "qps": round(400 / ms * 1e3, 1),
+            "kernel": "bang_kernel<128, 64, 8>", "ms": round(ms, 4), "qps": round(400 / ms * 1e3, 1),
             "reference_qps": BANG_REF_QPS,
             "vs_reference": round(400 / ms * 1e3 / BANG_REF_QPS, 2),
             "computations": comps, "rounds": int(st[:, 1].max()), "total_recall": total_recall,

@@ -126,21 +126,17 @@ __device__ __forceinline__ T list_at(const T (&v)[PER], int i) {
     return r;
 }
 
-// At most 3 waves per SIMD: the register budget (up to 168) keeps a round's
-// row loads in flight together; at the compiler's default of 8 the loads were
-// issued in smaller groups and a round took a third longer.
-//
-// G8 picks how a round's candidates are read. false: one lane per candidate
-// loads its whole row (32 float4 at dim 128), every load instruction touching
-// 64 rows: one memory round trip, the shortest round while few queries share
-// the GPU. true: the candidates in arrival order, 8 lanes each, lane s loading
-// dimensions s, s + 8, ... and summing chain s; a load instruction covers 8
-// rows in 32-byte runs, 4 candidates' loads per lane in flight: a third less
-// time at 4,096 and more queries, when the gathers of many waves share L1 and
-// L2.
-template <int DP, int LCAP, bool G8>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3))) void bang_kernel(
-    BangArgs a) {
+// A round's candidates are read in arrival order, 8 lanes each: lane s loads
+// dimensions s, s + 8, ... of its candidate's row and sums chain s, so a load
+// instruction covers 8 rows in 32-byte runs (one lane loading a whole row
+// touches 64 rows per instruction and was a third slower once many waves
+// gather at once). NP candidates' loads per lane are in flight: 8 (all of a
+// round's at most 64 candidates, one memory round trip) with up to 256
+// registers at 2 waves per SIMD, or 4 with up to 168 at 3 waves per SIMD for
+// batches that fill the GPU several times over.
+template <int DP, int LCAP, int NP>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, NP == 8 ? 2 : 3))) void
+bang_kernel(BangArgs a) {
     extern __shared__ __align__(16) uint32_t bloom[];  // a.words words
     __shared__ __align__(16) float qs[DP];
     // staging: the sorted candidates (at most R + 1 = 65), then the merged list
@@ -167,15 +163,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3))) void
     for (int i = l; i < DP; i += 64) qs[i] = a.q[qi * DP + i];
     __syncthreads();
 
-    // G8: lane s = l & 7 of each 8-lane group sums chain s, dims s, s + 8, ...
+    // lane s = l & 7 of each 8-lane group sums chain s: dims s, s + 8, ...
     const int gs = l & 7, gg = l >> 3;
-    float qv[G8 ? DP / 8 : 1];
-    if constexpr (G8) {
+    float qv[DP / 8];
 #pragma unroll
-        for (int j = 0; j < DP / 8; ++j) qv[j] = qs[8 * j + gs];
-    }
-    __shared__ int32_t cid[G8 ? 64 : 1];  // G8: the round's candidates in arrival order
-    __shared__ float cdist[G8 ? 64 : 1];
+    for (int j = 0; j < DP / 8; ++j) qv[j] = qs[8 * j + gs];
+    __shared__ int32_t cid[64];  // the round's candidates in arrival order
+    __shared__ float cdist[64];
     const int32_t entry = a.entry;
     int B = 0, comps = 0, rounds = 0;
     // the parent's adjacency row: loaded as soon as the parent is known,
@@ -217,47 +211,42 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3))) void
         comps += nc;
         if (nc == 0) break;
 
-        // distances (compute_neighborDist_par); the lanes stay in arrival order
-        float d = 0.f;
-        if constexpr (!G8) {
-            if (want) d = bang_dist<DP>(a.vecs + (size_t)nbr * DP, qs);
-        } else {
-            if (want) cid[__popcll((unsigned long long)(km & vmn::lanemask_lt()))] = nbr;
-            __syncthreads();
-            for (int p0 = 0; p0 < nn; p0 += 32) {
-                float x[4][DP / 8];
+        // distances (compute_neighborDist_par), 8 lanes per candidate
+        if (want) cid[__popcll((unsigned long long)(km & vmn::lanemask_lt()))] = nbr;
+        __syncthreads();
+        for (int p0 = 0; p0 < nn; p0 += 8 * NP) {
+            float x[NP][DP / 8];
 #pragma unroll
-                for (int p = 0; p < 4; ++p) {
-                    if (p0 + 8 * p < nn) {
-                        const int cc = p0 + 8 * p + gg;
-                        const float* r = a.vecs + (size_t)cid[cc < nn ? cc : 0] * DP + gs;
+            for (int p = 0; p < NP; ++p) {
+                if (p0 + 8 * p < nn) {
+                    const int cc = p0 + 8 * p + gg;
+                    const float* r = a.vecs + (size_t)cid[cc < nn ? cc : 0] * DP + gs;
 #pragma unroll
-                        for (int j = 0; j < DP / 8; ++j) x[p][j] = r[8 * j];
-                    }
-                }
-#pragma unroll
-                for (int p = 0; p < 4; ++p) {
-                    if (p0 + 8 * p < nn) {
-                        float acc = 0.f;
-#pragma unroll
-                        for (int j = 0; j < DP / 8; ++j) {
-                            const float t = x[p][j] - qv[j];
-                            acc = __fmaf_rn(t, t, acc);
-                        }
-                        // ((p0+p1)+(p2+p3))+((p4+p5)+(p6+p7)) in every lane of the group
-                        acc = vmn::reduce8(acc);
-                        const int cc = p0 + 8 * p + gg;
-                        if (gs == 0 && cc < nn) cdist[cc] = acc;
-                    }
+                    for (int j = 0; j < DP / 8; ++j) x[p][j] = r[8 * j];
                 }
             }
-            __syncthreads();
-            // one candidate per lane again, in arrival order
-            want = l < nn;
-            nbr = want ? cid[l] : -1;
-            d = want ? cdist[l] : 0.f;
-            km = vmn::ballot(want);
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                if (p0 + 8 * p < nn) {
+                    float acc = 0.f;
+#pragma unroll
+                    for (int j = 0; j < DP / 8; ++j) {
+                        const float t = x[p][j] - qv[j];
+                        acc = __fmaf_rn(t, t, acc);
+                    }
+                    // ((p0+p1)+(p2+p3))+((p4+p5)+(p6+p7)) in every lane of the group
+                    acc = vmn::reduce8(acc);
+                    const int cc = p0 + 8 * p + gg;
+                    if (gs == 0 && cc < nn) cdist[cc] = acc;
+                }
+            }
         }
+        __syncthreads();
+        // one candidate per lane again, in arrival order
+        want = l < nn;
+        nbr = want ? cid[l] : -1;
+        const float d = want ? cdist[l] : 0.f;
+        km = vmn::ballot(want);
 
         // stable sort by (distance, arrival): arrival is the entry, then lane
         // order. After round 1 only the candidates that enter the list need a
@@ -405,35 +394,36 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 3))) void
     }
 }
 
-template <int DP, bool G8>
+template <int DP, int NP>
 hipError_t launch_dp(const BangArgs& a, hipStream_t s) {
     const dim3 grid((unsigned)a.nq), block(64);
     const size_t lds = (size_t)a.words * sizeof(uint32_t);
     if (a.L <= 64)
-        hipLaunchKernelGGL((bang_kernel<DP, 64, G8>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((bang_kernel<DP, 64, NP>), grid, block, lds, s, a);
     else
-        hipLaunchKernelGGL((bang_kernel<DP, 256, G8>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((bang_kernel<DP, 256, NP>), grid, block, lds, s, a);
     return hipGetLastError();
 }
 
-template <bool G8>
-hipError_t launch_g(const BangArgs& a, int DP, hipStream_t s) {
+template <int NP>
+hipError_t launch_np(const BangArgs& a, int DP, hipStream_t s) {
     switch (DP) {
-        case 32: return launch_dp<32, G8>(a, s);
-        case 64: return launch_dp<64, G8>(a, s);
-        case 96: return launch_dp<96, G8>(a, s);
-        case 128: return launch_dp<128, G8>(a, s);
-        case 160: return launch_dp<160, G8>(a, s);
-        case 192: return launch_dp<192, G8>(a, s);
-        case 256: return launch_dp<256, G8>(a, s);
+        case 32: return launch_dp<32, NP>(a, s);
+        case 64: return launch_dp<64, NP>(a, s);
+        case 96: return launch_dp<96, NP>(a, s);
+        case 128: return launch_dp<128, NP>(a, s);
+        case 160: return launch_dp<160, NP>(a, s);
+        case 192: return launch_dp<192, NP>(a, s);
+        case 256: return launch_dp<256, NP>(a, s);
         default: return hipErrorInvalidValue;
     }
 }
 
-// 8 lanes per candidate from 6 queries per CU (measured crossover: 1,024
-// queries on 256 CUs favour one lane per candidate, 2,048 favour 8)
+// All of a round's loads in flight up to 16 queries per CU (4,096 on 256 CUs:
+// 0.273 against 0.288 ms), 4 candidates' loads at 3 waves per SIMD beyond
+// (10,000 queries: 0.560 against 0.634 ms)
 hipError_t launch_bang(const BangArgs& a, int DP, int cus, hipStream_t s) {
-    return a.nq >= 6 * (int64_t)cus ? launch_g<true>(a, DP, s) : launch_g<false>(a, DP, s);
+    return a.nq <= 16 * (int64_t)cus ? launch_np<8>(a, DP, s) : launch_np<4>(a, DP, s);
 }
 
 }  // namespace
