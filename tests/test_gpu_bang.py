@@ -137,6 +137,19 @@ def test_bang_gpu_neighbours_sharing_a_cell(vmn, orc, gpu):
     assert pair[0] not in got[0][0] and pair[1] in got[0][0]
 
 
+def test_bang_gpu_entry_without_neighbours(vmn, orc, gpu):
+    n = 100
+    v = np.arange(n, dtype=np.float32)[:, None].repeat(32, 1)
+    adj = np.full((n, 64), -1, np.int32)
+    adj[1:-1, 0] = np.arange(2, n)
+    q = np.full((3, 32), 7.0, np.float32)
+    print("bang entry without neighbours", flush=True)
+    idx = vmn.Vamana.from_graph(v, adj, medoid=0, mode="fresh", R=64)
+    got = idx.bang_search(q, k=5, L=8, entry=0)
+    same(got, orc.bang_search(v, adj, q, 0, 8, k=5))
+    assert (got[2] == 1).all() and (got[0][:, 0] == 0).all() and (got[0][:, 1:] == -1).all()
+
+
 def test_bang_gpu_round_cap(vmn, orc, gpu):
     n = 300
     v = np.arange(n, dtype=np.float32)[:, None].repeat(32, 1)

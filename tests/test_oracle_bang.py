@@ -150,6 +150,18 @@ def test_bang_merge_puts_new_entries_first_on_ties(orc):
     assert ids9[0].tolist() == [1, 2]
 
 
+def test_bang_entry_without_neighbours(orc):
+    """An entry point with an empty row: round 1 computes the entry alone, the
+    list holds it visited, and the query ends after that round."""
+    v = np.arange(8, dtype=np.float32)[:, None].repeat(4, 1)
+    adj = np.full((8, 64), -1, np.int32)
+    adj[1, :3] = [2, 3, 4]
+    q = np.full((2, 4), 5.0, np.float32)
+    ids, dist, st = orc.bang_search(v, adj, q, 0, L=4, k=3)
+    assert ids[:, 0].tolist() == [0, 0] and (ids[:, 1:] == -1).all()
+    assert dist[:, 0].tolist() == [100.0, 100.0] and st.tolist() == [[1, 1], [1, 1]]
+
+
 def test_bang_full_beam_on_complete_graph_is_exact_knn(orc):
     """With every node adjacent to every other and L = n, the list ends up
     holding all n nodes in distance order: the results are the exact k-NN."""
