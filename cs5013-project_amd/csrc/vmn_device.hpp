@@ -662,4 +662,48 @@ __device__ __forceinline__ int merge_masked(const uint32_t* src_id, const float*
     return nc < limit ? nc : limit;
 }
 
+// merge_masked for a list held in registers, entry i in lane i (cnt <= 64;
+// lanes at or past cnt hold VMN_VIS_BIT and are not entries): the same
+// positions, the merged list scattered to dst (LDS, 64 entries) and read back
+// into the registers. Returns the new count.
+__device__ __forceinline__ int merge_regs(uint32_t& r_id, float& r_d, int cnt, uint32_t my_id,
+                                          float my_d, int my_k, uint64_t km, uint32_t* dst_id,
+                                          float* dst_d, int limit) {
+    const int l = lane();
+    const int bk = fkey(r_d);
+    const uint64_t live = cnt >= 64 ? ~0ull : ((1ull << cnt) - 1ull);
+    int ab = 0, rank = 0, ub = 0, thr = my_k + 1;
+    for (uint64_t m = km; m; m &= m - 1) {
+        const int t = __ffsll((unsigned long long)m) - 1;
+        const int kt = readlane_i(my_k, t);
+        const bool at_or_below = bk <= kt;
+        const int c = __popcll((unsigned long long)(ballot(at_or_below) & live));
+        ab += at_or_below ? 1 : 0;
+        ub = writelane_i(c, t, ub);
+        thr = writelane_i(kt, t, thr);
+        rank += kt < thr ? 1 : 0;
+    }
+    const int nk = __popcll((unsigned long long)km);
+    if (l < cnt) {
+        // candidates with keys below this entry's: all but those at or above it
+        const int pos = l + (nk - ab);
+        if (pos < limit) {
+            dst_id[pos] = r_id;
+            dst_d[pos] = r_d;
+        }
+    }
+    if ((km >> l) & 1ull) {
+        const int pos = ub + rank;
+        if (pos < limit) {
+            dst_id[pos] = my_id;
+            dst_d[pos] = my_d;
+        }
+    }
+    const int nc = cnt + nk < limit ? cnt + nk : limit;
+    __syncthreads();
+    r_id = l < nc ? dst_id[l] : VMN_VIS_BIT;
+    r_d = l < nc ? dst_d[l] : 0.0f;
+    return nc;
+}
+
 }  // namespace vmn

@@ -80,6 +80,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
     // registers by the FP32 and BF16 forms (short of HBM bandwidth: the
     // table's 4 bytes per neighbour cost them more than the hashing)
     constexpr bool HTAB = MODE == VMN_MODE_FRESH && PREC == VMN_PREC_FP8;
+    // FRESH lists of up to 64 entries live in registers, entry i in lane i:
+    // the select reads them with a ballot, the merge scatters through LDS once
+    constexpr bool RL = MODE == VMN_MODE_FRESH && LCAP == 64;
     __shared__ __align__(16) uint32_t s_id[2][LCAP];
     __shared__ __align__(16) float s_d[2][LCAP];
     __shared__ __align__(16) int32_t s_sid[64];
@@ -180,6 +183,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
         }
         __syncthreads();
 
+        uint32_t r_id = l == 0 ? (uint32_t)st : VMN_VIS_BIT;
+        float r_d = l == 0 ? sd : 0.0f;
         int cur = 0, cnt = 1, nexp = 0, hops = 0, ndist = 1;
         // adjacency row of the node predicted to be expanded next, loaded
         // before the merge that decides it (pf: its id, -1 for none)
@@ -196,8 +201,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
             float nd = 0.0f, nd2 = 0.0f;
             for (int b = 0; b < cnt && p2 < 0; b += 64) {
                 const int i = b + l;
-                const uint32_t v = i < cnt ? s_id[cur][i] : VMN_VIS_BIT;
-                const float dv = i < cnt ? s_d[cur][i] : 0.0f;
+                const uint32_t v = i < cnt ? (RL ? r_id : s_id[cur][i]) : VMN_VIS_BIT;
+                const float dv = i < cnt ? (RL ? r_d : s_d[cur][i]) : 0.0f;
                 uint64_t m = ballot(!(v & VMN_VIS_BIT));
                 if (m && p < 0) {
                     const int t = __ffsll((unsigned long long)m) - 1;
@@ -218,8 +223,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
             ++nexp;
             const int32_t node = (int32_t)(e & VMN_ID_MASK);
             __syncthreads();
+            if (RL && l == p) r_id |= VMN_VIS_BIT;
             if (l == 0) {
-                s_id[cur][p] = e | VMN_VIS_BIT;
+                if (!RL) s_id[cur][p] = e | VMN_VIS_BIT;
                 if (MODE == VMN_MODE_FRESH && a.vis_ids && nexp <= a.P) {
                     a.vis_ids[qi * a.P + nexp - 1] = node;
                     a.vis_d[qi * a.P + nexp - 1] = nd;
@@ -321,7 +327,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
             ndist += nnew;
             // A full list keeps its entries on ties, so a candidate at or beyond
             // its last distance can never enter: drop those before ranking.
-            const float thr = cnt == a.L ? s_d[cur][cnt - 1] : INFINITY;
+            const float thr =
+                cnt == a.L ? (RL ? readlane_f(r_d, cnt - 1) : s_d[cur][cnt - 1]) : INFINITY;
             const uint32_t my_id = l < nnew ? (uint32_t)s_sid[l] : 0u;
             const float my_d = l < nnew ? s_sd[l] : 0.0f;
             const int my_k = fkey(my_d);
@@ -347,9 +354,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
                         pf_nbn = l < W ? a.adj_norms[(int64_t)pf * W + l] : 0.0f;
                 }
             }
-            cnt = merge_masked<LCAP>(s_id[cur], s_d[cur], cnt, my_id, my_d, my_k, km, s_id[cur ^ 1],
-                                     s_d[cur ^ 1], a.L);
-            cur ^= 1;
+            if constexpr (RL) {
+                cnt = merge_regs(r_id, r_d, cnt, my_id, my_d, my_k, km, s_id[0], s_d[0], a.L);
+            } else {
+                cnt = merge_masked<LCAP>(s_id[cur], s_d[cur], cnt, my_id, my_d, my_k, km,
+                                         s_id[cur ^ 1], s_d[cur ^ 1], a.L);
+                cur ^= 1;
+            }
+            __syncthreads();
+        }
+        if constexpr (RL) {
+            // the list back to LDS for the re-rank and the output
+            if (l < cnt) {
+                s_id[0][l] = r_id;
+                s_d[0][l] = r_d;
+            }
             __syncthreads();
         }
 
