@@ -77,12 +77,14 @@ def test_bang_gpu_real_valued_rows(vmn, orc, gpu, dim, n):
     same(got, orc.bang_search(v, adj, q, 5, 48, k=10))
 
 
-@pytest.mark.parametrize("dim,n", [(128, 10000), (100, 3000), (128, 40000)])
+@pytest.mark.parametrize("dim,n", [(128, 10000), (100, 3000), (128, 40000), (200, 3000)])
 def test_bang_gpu_large_batches(vmn, orc, sift_bang, dim, n):
-    """Batches of 2,048 and more queries read each round's rows 8 lanes per
-    candidate (chain s in lane s, the same sums): bit-identical to the oracle
-    on SIFT10K and on Gaussian rows (padded width; 40,000 nodes: the
-    399,887-cell filter)."""
+    """Batches of 2,048 queries (all of a round's row loads in flight) and
+    4,608 (past 16 queries per CU: 4 candidates' loads per lane group): the
+    rows are read 8 lanes per candidate, chain s in lane s, the same sums.
+    Bit-identical to the oracle on SIFT10K and on Gaussian rows (padded
+    widths 100 -> 128 and 200 -> 256; 40,000 nodes: the 399,887-cell
+    filter)."""
     if n == 10000:
         b = sift_bang
         v, adj, entry, idx = b["base"], b["adj"], 123, b["idx"]
@@ -92,7 +94,7 @@ def test_bang_gpu_large_batches(vmn, orc, sift_bang, dim, n):
         rng = np.random.default_rng(n + dim)
         v = rng.standard_normal((n, dim)).astype(np.float32)
         adj = orc.random_init(n, 24, 32, seed=dim)
-        q = rng.standard_normal((2048, dim)).astype(np.float32)
+        q = rng.standard_normal((4608 if dim == 200 or n == 40000 else 2048, dim)).astype(np.float32)
         entry, L = 5, 48
         idx = vmn.Vamana.from_graph(v, adj, medoid=entry, mode="fresh", R=32)
     print(f"bang large batch dim={dim} n={n} nq={q.shape[0]}", flush=True)
