@@ -24,9 +24,9 @@
 //    threads race there). Tombstoned rows are never candidates.
 //  * distance: FP32, as compute_neighborDist_par sums it: eight fused
 //    multiply-add chains over dimensions s, s + 8, ... (s = 0..7) combined as
-//    ((p0+p1)+(p2+p3))+((p4+p5)+(p6+p7)) (cub::WarpReduce<float, 8>). One lane
-//    computes a whole candidate (32 float4 loads of its row at dim 128), so a
-//    round's candidates are in flight together.
+//    ((p0+p1)+(p2+p3))+((p4+p5)+(p6+p7)) (cub::WarpReduce<float, 8>): 8 lanes
+//    per candidate, lane s loading and summing chain s, the tree in three DPP
+//    adds, several candidates' loads in flight per lane group.
 //  * stable sort by distance, ties in arrival order; round 1 takes the first
 //    min(count, L) as the list (entry visited); later rounds merge the first
 //    nb candidates, nb = max(#leading candidates below the list's last
