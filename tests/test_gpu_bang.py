@@ -104,6 +104,22 @@ def test_bang_gpu_large_batches(vmn, orc, sift_bang, dim, n):
         assert int(got[2][:2000, 0].sum()) == 20 * int(got[2][:100, 0].sum())
 
 
+def test_bang_gpu_on_a_built_graph(vmn, orc, gpu):
+    """BANG's schedule over a FRESH graph the library built (100,000 integer
+    mixture rows, R 64; past 32,792 nodes, so the 399,887-cell filter with its
+    collisions), entry at the index's medoid, L 40 and 80: bit-identical to
+    the oracle run on the same adjacency."""
+    ds = vmn.datasets
+    base = ds.integer_mixture(100000, 128, k=64, sigma=20.0, seed=77, stream=0)
+    q = ds.integer_mixture(3000, 128, k=64, sigma=20.0, seed=77, stream=1)
+    idx = vmn.Vamana(base, mode="fresh", R=64, L_build=64, alphas=(1.0, 1.2))
+    adj = idx.adjacency
+    for L in (40, 80):
+        print(f"bang built graph L={L}", flush=True)
+        got = idx.bang_search(q, k=10, L=L)
+        same(got, orc.bang_search(base, adj, q, idx.medoid, L, k=10))
+
+
 def test_bang_gpu_tombstones(vmn, orc, sift_bang):
     b = sift_bang
     idx = vmn.Vamana.from_graph(b["base"], b["adj"], medoid=3732, mode="fresh", R=64)
