@@ -662,34 +662,61 @@ __device__ __forceinline__ int merge_masked(const uint32_t* src_id, const float*
     return nc < limit ? nc : limit;
 }
 
-// merge_masked for a list held in registers, entry i in lane i (cnt <= 64;
-// lanes at or past cnt hold VMN_VIS_BIT and are not entries): the same
-// positions, the merged list scattered to dst (LDS, 64 entries) and read back
-// into the registers. Returns the new count.
-__device__ __forceinline__ int merge_regs(uint32_t& r_id, float& r_d, int cnt, uint32_t my_id,
-                                          float my_d, int my_k, uint64_t km, uint32_t* dst_id,
-                                          float* dst_d, int limit) {
+// Element i (uniform) of a list held in registers, entry i in lane i % 64,
+// slot i / 64.
+template <int PER>
+__device__ __forceinline__ float reg_at(const float (&v)[PER], int i) {
+    float r = 0.0f;
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+        if (k == i / 64) r = readlane_f(v[k], i % 64);
+    return r;
+}
+
+// merge_masked for a list held in registers, entry i in lane i % 64, slot
+// i / 64 (entries at or past cnt hold VMN_VIS_BIT): the same positions, the
+// merged list scattered to dst (LDS, 64 PER entries) and read back into the
+// registers. Returns the new count.
+template <int PER>
+__device__ __forceinline__ int merge_regs(uint32_t (&r_id)[PER], float (&r_d)[PER], int cnt,
+                                          uint32_t my_id, float my_d, int my_k, uint64_t km,
+                                          uint32_t* dst_id, float* dst_d, int limit) {
     const int l = lane();
-    const int bk = fkey(r_d);
-    const uint64_t live = cnt >= 64 ? ~0ull : ((1ull << cnt) - 1ull);
-    int ab = 0, rank = 0, ub = 0, thr = my_k + 1;
+    int bk[PER], ab[PER];
+    uint64_t live[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        bk[j] = fkey(r_d[j]);
+        ab[j] = 0;
+        const int nl = cnt - 64 * j;
+        live[j] = nl >= 64 ? ~0ull : (nl > 0 ? (1ull << nl) - 1ull : 0ull);
+    }
+    int rank = 0, ub = 0, thr = my_k + 1;
     for (uint64_t m = km; m; m &= m - 1) {
         const int t = __ffsll((unsigned long long)m) - 1;
         const int kt = readlane_i(my_k, t);
-        const bool at_or_below = bk <= kt;
-        const int c = __popcll((unsigned long long)(ballot(at_or_below) & live));
-        ab += at_or_below ? 1 : 0;
+        int c = 0;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const bool at_or_below = bk[j] <= kt;
+            c += __popcll((unsigned long long)(ballot(at_or_below) & live[j]));
+            ab[j] += at_or_below ? 1 : 0;
+        }
         ub = writelane_i(c, t, ub);
         thr = writelane_i(kt, t, thr);
         rank += kt < thr ? 1 : 0;
     }
     const int nk = __popcll((unsigned long long)km);
-    if (l < cnt) {
-        // candidates with keys below this entry's: all but those at or above it
-        const int pos = l + (nk - ab);
-        if (pos < limit) {
-            dst_id[pos] = r_id;
-            dst_d[pos] = r_d;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int bi = l + 64 * j;
+        if (bi < cnt) {
+            // candidates with keys below this entry's: all but those at or above it
+            const int pos = bi + (nk - ab[j]);
+            if (pos < limit) {
+                dst_id[pos] = r_id[j];
+                dst_d[pos] = r_d[j];
+            }
         }
     }
     if ((km >> l) & 1ull) {
@@ -701,8 +728,12 @@ __device__ __forceinline__ int merge_regs(uint32_t& r_id, float& r_d, int cnt, u
     }
     const int nc = cnt + nk < limit ? cnt + nk : limit;
     __syncthreads();
-    r_id = l < nc ? dst_id[l] : VMN_VIS_BIT;
-    r_d = l < nc ? dst_d[l] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int i = l + 64 * j;
+        r_id[j] = i < nc ? dst_id[i] : VMN_VIS_BIT;
+        r_d[j] = i < nc ? dst_d[i] : 0.0f;
+    }
     return nc;
 }
 

@@ -56,10 +56,11 @@ __global__ void fill_zero_u32(unsigned int* p, int n) {
 // PREC_ARG is a VMN_PREC_* value, or VMN_PREC_BF16_TABLE_NORMS: BF16 rows with
 // the row norms read from the norms table (small batches, FRESH mode).
 // FRESH mode at row widths up to 128 with beam lists up to 256 entries, and
-// PARTH-mode FP8 with lists up to 128, are held to 5 waves per SIMD (96
-// VGPRs, no spills; the compiler's own choice takes 4): large batches are
+// PARTH-mode FP8 at width 128 with lists up to 128, are held to 5 waves per
+// SIMD (96 VGPRs; the compiler's own choice takes 4): large batches are
 // bandwidth-bound and run faster with the extra wave (FP8 10k-query batch
-// 7%), single queries about 3% slower. PARTH FP32 / BF16 would spill there.
+// 7%), single queries about 3% slower. PARTH FP32 / BF16, and PARTH FP8 at
+// widths 64 and 96, would spill there.
 // NG is the low-precision forms' tile shape: MFMA tiles of 16 candidates
 // gathered per round. 2 (32 candidates) is every form's default; 4 (64
 // candidates, held to 4 waves per SIMD, 128 VGPRs) fetches a whole hop's new
@@ -70,7 +71,7 @@ template <int DP, int LCAP, int MODE, int PREC_ARG, int NG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
     NG == 4 ? 4
             : (DP <= 128 && ((MODE == VMN_MODE_FRESH && LCAP <= 256) ||
-                             (PREC_ARG == VMN_PREC_FP8 && LCAP <= 128))) ? 5 : 1, 8))) void search_kernel(SearchArgs a) {
+                             (PREC_ARG == VMN_PREC_FP8 && LCAP <= 128 && DP == 128))) ? 5 : 1, 8))) void search_kernel(SearchArgs a) {
     constexpr int PREC = PREC_ARG == VMN_PREC_BF16_TABLE_NORMS ? VMN_PREC_BF16 : PREC_ARG;
     constexpr bool TN = PREC_ARG == VMN_PREC_BF16_TABLE_NORMS;
     // FP32 distances in the reference's summation order (PARTH)
@@ -80,9 +81,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
     // registers by the FP32 and BF16 forms (short of HBM bandwidth: the
     // table's 4 bytes per neighbour cost them more than the hashing)
     constexpr bool HTAB = MODE == VMN_MODE_FRESH && PREC == VMN_PREC_FP8;
-    // FRESH lists of up to 64 entries live in registers, entry i in lane i:
-    // the select reads them with a ballot, the merge scatters through LDS once
-    constexpr bool RL = MODE == VMN_MODE_FRESH && LCAP == 64;
+    // FRESH lists of up to 128 entries live in registers, entry i in lane
+    // i % 64, slot i / 64: the select reads them with ballots, the merge
+    // scatters through LDS once (longer lists, and the 4-tile table-norm
+    // form at 128, would spill)
+    constexpr bool RL = MODE == VMN_MODE_FRESH &&
+                        (LCAP == 64 || (LCAP == 128 && !(TN && NG == 4)));
+    constexpr int RPER = RL ? LCAP / 64 : 1;
     __shared__ __align__(16) uint32_t s_id[2][LCAP];
     __shared__ __align__(16) float s_d[2][LCAP];
     __shared__ __align__(16) int32_t s_sid[64];
@@ -183,8 +188,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
         }
         __syncthreads();
 
-        uint32_t r_id = l == 0 ? (uint32_t)st : VMN_VIS_BIT;
-        float r_d = l == 0 ? sd : 0.0f;
+        uint32_t r_id[RPER];
+        float r_d[RPER];
+#pragma unroll
+        for (int k = 0; k < RPER; ++k) {
+            r_id[k] = k == 0 && l == 0 ? (uint32_t)st : VMN_VIS_BIT;
+            r_d[k] = k == 0 && l == 0 ? sd : 0.0f;
+        }
         int cur = 0, cnt = 1, nexp = 0, hops = 0, ndist = 1;
         // adjacency row of the node predicted to be expanded next, loaded
         // before the merge that decides it (pf: its id, -1 for none)
@@ -199,10 +209,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
             int p = -1, p2 = -1;
             uint32_t e = 0u, e2 = 0u;
             float nd = 0.0f, nd2 = 0.0f;
-            for (int b = 0; b < cnt && p2 < 0; b += 64) {
-                const int i = b + l;
-                const uint32_t v = i < cnt ? (RL ? r_id : s_id[cur][i]) : VMN_VIS_BIT;
-                const float dv = i < cnt ? (RL ? r_d : s_d[cur][i]) : 0.0f;
+            auto scan = [&](int b, uint32_t v, float dv) {
                 uint64_t m = ballot(!(v & VMN_VIS_BIT));
                 if (m && p < 0) {
                     const int t = __ffsll((unsigned long long)m) - 1;
@@ -217,13 +224,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
                     e2 = (uint32_t)readlane_i((int)v, t);
                     nd2 = readlane_f(dv, t);
                 }
+            };
+            if constexpr (RL) {
+#pragma unroll
+                for (int k = 0; k < RPER; ++k) {
+                    if (64 * k < cnt && p2 < 0) {
+                        const bool in = 64 * k + l < cnt;
+                        scan(64 * k, in ? r_id[k] : VMN_VIS_BIT, in ? r_d[k] : 0.0f);
+                    }
+                }
+            } else {
+                for (int b = 0; b < cnt && p2 < 0; b += 64) {
+                    const int i = b + l;
+                    scan(b, i < cnt ? s_id[cur][i] : VMN_VIS_BIT, i < cnt ? s_d[cur][i] : 0.0f);
+                }
             }
             if (p < 0) break;
             if (MODE == VMN_MODE_PARTH && nexp >= a.visited_cap) break;
             ++nexp;
             const int32_t node = (int32_t)(e & VMN_ID_MASK);
             __syncthreads();
-            if (RL && l == p) r_id |= VMN_VIS_BIT;
+            if constexpr (RL) {
+#pragma unroll
+                for (int k = 0; k < RPER; ++k)
+                    if (64 * k + l == p) r_id[k] |= VMN_VIS_BIT;
+            }
             if (l == 0) {
                 if (!RL) s_id[cur][p] = e | VMN_VIS_BIT;
                 if (MODE == VMN_MODE_FRESH && a.vis_ids && nexp <= a.P) {
@@ -328,7 +353,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
             // A full list keeps its entries on ties, so a candidate at or beyond
             // its last distance can never enter: drop those before ranking.
             const float thr =
-                cnt == a.L ? (RL ? readlane_f(r_d, cnt - 1) : s_d[cur][cnt - 1]) : INFINITY;
+                cnt == a.L ? (RL ? reg_at<RPER>(r_d, cnt - 1) : s_d[cur][cnt - 1]) : INFINITY;
             const uint32_t my_id = l < nnew ? (uint32_t)s_sid[l] : 0u;
             const float my_d = l < nnew ? s_sd[l] : 0.0f;
             const int my_k = fkey(my_d);
@@ -355,7 +380,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
                 }
             }
             if constexpr (RL) {
-                cnt = merge_regs(r_id, r_d, cnt, my_id, my_d, my_k, km, s_id[0], s_d[0], a.L);
+                cnt = merge_regs<RPER>(r_id, r_d, cnt, my_id, my_d, my_k, km, s_id[0], s_d[0], a.L);
             } else {
                 cnt = merge_masked<LCAP>(s_id[cur], s_d[cur], cnt, my_id, my_d, my_k, km,
                                          s_id[cur ^ 1], s_d[cur ^ 1], a.L);
@@ -365,9 +390,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
         }
         if constexpr (RL) {
             // the list back to LDS for the re-rank and the output
-            if (l < cnt) {
-                s_id[0][l] = r_id;
-                s_d[0][l] = r_d;
+#pragma unroll
+            for (int k = 0; k < RPER; ++k) {
+                if (64 * k + l < cnt) {
+                    s_id[0][64 * k + l] = r_id[k];
+                    s_d[0][64 * k + l] = r_d[k];
+                }
             }
             __syncthreads();
         }
