@@ -221,12 +221,24 @@ __device__ __forceinline__ void gather_dists_seq(const float* __restrict__ q,
 // ---------------------------------------------------------------------------
 // Low-precision candidate rows on MFMA.
 //
-// A wave computes q . x for 16 candidates per v_mfma_f32_16x16x32_{bf16,fp8}:
-// the query fills all 16 rows of A, candidate c's row is column c of B.
-// Lane l supplies, for k-block kk, elements 32*kk + 8*(l/16) .. +7 of the
-// query (A) and of candidate (l % 16) (B); A and B use the same k split, so
-// the products pair up whatever order the hardware sums them in, and column
-// n of the result (lane n % 16) holds q . x_n in every row.
+// A wave computes q . x for 16 candidates per v_mfma_f32_16x16x32_{bf16,fp8}
+// straight from coalesced row loads: lane l holds 8 elements of candidate
+// l / 4, chunk c = l % 4 (elements 8c .. 8c + 7 of the 32-element k-block;
+// FP8: of a 16-byte load, one MFMA per half). The MFMA reads lane l's 8
+// elements as column l % 16, k-slice l / 16 of B, so column j's k-slices
+// come from four different candidates. The query operand A is therefore
+// block-diagonal: row i = 4a + c of A holds the query's chunk c in k-slice a
+// and zeros elsewhere (lane l carries it when l / 16 == (l / 4) % 4). Then
+// C[4a + c][4m + c] = (query chunk c) . (chunk c of candidate m + 4a), which
+// the MFMA leaves in register c of lane 16a + 4m + c: lane l holds the chunk
+// l % 4 partial of its own candidate l / 4 in register l % 4, accumulated
+// over the k-blocks, and the four lanes of a quad sum to q . x. No lane
+// permutes are needed between the loads and the MFMA.
+//
+// FP8 rows narrower than a multiple of 64 bytes load in operand layout
+// instead: lane l supplies, for k-block kk, elements 32*kk + 8*(l/16) .. +7
+// of the query (A, every row) and of candidate (l % 16) (B), and column n of
+// the result holds q . x_n.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
@@ -238,11 +250,16 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 // kernel-internal: BF16 traversal with row norms read from the norms table
 #define VMN_PREC_BF16_TABLE_NORMS 3
 
+// Block-diagonal A operand (lane l of A non-zero): see above.
+__device__ __forceinline__ bool diag_lane(int l) { return (l >> 4) == ((l >> 2) & 3); }
+
 template <int DP>
 struct QLow {
     static constexpr int KB = DP / 32;  // k-blocks of 32
-    bf16x8_t a[KB];                     // BF16 query slice
-    long a8[KB];                        // FP8 query slice (8 x e4m3 per lane)
+    // FP8 rows load coalesced (block-diagonal A) at multiples of 64 bytes
+    static constexpr bool DIAG8 = DP % 64 == 0;
+    bf16x8_t a[KB];                     // BF16 query operand (block-diagonal)
+    long a8[KB];                        // FP8 query operand (8 x e4m3 per lane)
     float qn;                           // ||q_hat||^2 (BF16) or ||q||^2 (FP8)
     float t;                            // FP8: query scale
 };
@@ -268,23 +285,24 @@ __device__ __forceinline__ float fp8_to_f32(uint32_t word, int byte) {
 // about the per-dimension centre c): q'_j = (q_j - c_j) * s_j,
 // t = max|q'| / 448, A = e4m3(q' / t), qn = ||q - c||^2. The FP8 codes are
 // made one element per lane per 64 (j = l + 64 r) and handed to the operand
-// layout (8 consecutive codes per lane and k-block) through `stage`, DP bytes
-// of LDS, so only a few values per lane are live at once.
+// layout (8 consecutive codes per lane and MFMA) through `stage`, DP bytes of
+// LDS, so only a few values per lane are live at once. qn is summed over the
+// natural k-slices (8 elements per lane and k-block) in either layout.
 template <int DP, int PREC>
 __device__ __forceinline__ void prep_query(QLow<DP>& Q, const float* qf, const float* scales,
                                            const float* centre = nullptr,
                                            uint8_t* stage = nullptr) {
     const int l = lane(), g = l >> 4;
+    const bool dg = diag_lane(l);
     if (PREC == VMN_PREC_BF16) {
         float part = 0.0f;
 #pragma unroll
         for (int kb = 0; kb < QLow<DP>::KB; ++kb) {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const __bf16 h = to_bf16(qf[32 * kb + 8 * g + e]);
-                Q.a[kb][e] = h;
-                const float hf = (float)h;
+                const float hf = (float)to_bf16(qf[32 * kb + 8 * g + e]);
                 part += hf * hf;
+                Q.a[kb][e] = dg ? to_bf16(qf[32 * kb + 8 * (l & 3) + e]) : to_bf16(0.0f);
             }
         }
         // lanes of one 16-lane group hold the same slice; sum the 4 groups
@@ -319,8 +337,12 @@ __device__ __forceinline__ void prep_query(QLow<DP>& Q, const float* qf, const f
         }
         __syncthreads();
 #pragma unroll
-        for (int kb = 0; kb < QLow<DP>::KB; ++kb)
-            Q.a8[kb] = *reinterpret_cast<const long*>(stage + 32 * kb + 8 * g);
+        for (int kb = 0; kb < QLow<DP>::KB; ++kb) {
+            if constexpr (QLow<DP>::DIAG8)
+                Q.a8[kb] = dg ? *reinterpret_cast<const long*>(stage + 64 * (kb >> 1) + 16 * (l & 3) + 8 * (kb & 1)) : 0l;
+            else
+                Q.a8[kb] = *reinterpret_cast<const long*>(stage + 32 * kb + 8 * g);
+        }
         __syncthreads();
         Q.qn = part;
         Q.t = t;
@@ -340,7 +362,7 @@ __device__ __forceinline__ uint32_t bperm(int src_byte, uint32_t v) {
 // `cand_norms[s]` (LDS, the norm of ids[s], loaded with the adjacency row the
 // ids came from) when given, else from the table `norms` (a cache line per
 // candidate). BF16 row norms come from the loaded registers (v_dot2_f32_bf16),
-// or, with TN, from `norms` as well: one 4-byte load per tile column issued
+// or, with TN, from `norms` as well: one 4-byte load per candidate issued
 // with the row loads. That saves the dot products and two cross-lane sums on a
 // latency-bound batch, but costs a cache line per candidate, which a
 // bandwidth-bound batch cannot afford.
@@ -348,15 +370,26 @@ __device__ __forceinline__ uint32_t bperm(int src_byte, uint32_t v) {
 // Loads are coalesced per row: in load step s, lane l reads 16 bytes of tile
 // row l/4 at byte 64*s + 16*(l%4), so each 4-lane quad reads 64 contiguous
 // bytes of one row (one L1 access, against four when lanes are laid out as
-// the MFMA operand wants). ds_bpermute then hands every dword to the lane
-// that holds it as the B operand (lane L: column L%16, k-slice L/16).
+// the MFMA operand wants), and the block-diagonal query operand multiplies
+// them where they are (see the section comment). A quad's lanes hold its
+// candidate's four chunk partials of the dot product (register l % 4) and of
+// the row norm; both are summed (c0 + c1) + (c2 + c3) across the quad, the
+// norm in the norms table's order.
 template <int DP, int PREC, int NG = 2, bool TN = false>
 __device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* __restrict__ rows,
                                                  const float* __restrict__ norms,
                                                  const int32_t* ids, float* dist_out, int cnt,
                                                  const float* cand_norms = nullptr) {
-    const int l = lane(), g = l >> 4, rr = l >> 2, sub = l & 3;
+    const int l = lane(), rr = l >> 2, sub = l & 3;
     constexpr int KB = QLow<DP>::KB;
+    // the quad's partial sums: (c0 + c1) + (c2 + c3) in every lane of the quad
+    auto quad_sum = [](float v) {
+        v += dpp_f<VMN_DPP_XOR1>(v);
+        return v + dpp_f<VMN_DPP_XOR2>(v);
+    };
+    auto own = [&](const f32x4_t& c) {
+        return sub == 0 ? c[0] : sub == 1 ? c[1] : sub == 2 ? c[2] : c[3];
+    };
     for (int base = 0; base < cnt; base += 16 * NG) {
         int32_t id[NG];
 #pragma unroll
@@ -370,7 +403,6 @@ __device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* 
         for (int t = 0; t < NG; ++t) { acc[t] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f}; x[t] = 0.0f; }
         if (PREC == VMN_PREC_BF16) {
             // step kb holds k-block kb (64 bytes of every row)
-            const int src = (4 * (l & 15) + g) << 2;
             uint4 v[NG][KB];
 #pragma unroll
             for (int t = 0; t < NG; ++t) {
@@ -378,8 +410,7 @@ __device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* 
                     const uint4* r = reinterpret_cast<const uint4*>(reinterpret_cast<const __bf16*>(rows) + (int64_t)id[t] * DP);
 #pragma unroll
                     for (int kb = 0; kb < KB; ++kb) v[t][kb] = r[4 * kb + sub];
-                    // tile column L%16's id sits in lane 4*(L%16)
-                    if constexpr (TN) x[t] = norms[__builtin_amdgcn_ds_bpermute((4 * (l & 15)) << 2, id[t])];
+                    if constexpr (TN) x[t] = norms[id[t]];
                 }
             }
 #pragma unroll
@@ -387,24 +418,21 @@ __device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* 
                 if (t == 0 || base + 16 * t < cnt) {
 #pragma unroll
                     for (int kb = 0; kb < KB; ++kb) {
-                        const uint32_t w[4] = {bperm(src, v[t][kb].x), bperm(src, v[t][kb].y),
-                                               bperm(src, v[t][kb].z), bperm(src, v[t][kb].w)};
-                        const uint4 u = {w[0], w[1], w[2], w[3]};
-                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Q.a[kb], __builtin_bit_cast(bf16x8_t, u), acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Q.a[kb], __builtin_bit_cast(bf16x8_t, v[t][kb]), acc[t], 0, 0, 0);
                         if constexpr (!TN) {
+                            const uint32_t w[4] = {v[t][kb].x, v[t][kb].y, v[t][kb].z, v[t][kb].w};
 #pragma unroll
                             for (int e = 0; e < 4; ++e) { const bf16x2_t p = __builtin_bit_cast(bf16x2_t, w[e]); x[t] = __builtin_amdgcn_fdot2_f32_bf16(p, p, x[t], false); }
                         }
                     }
-                    if constexpr (!TN) x[t] = sum_rows4(x[t]);
+                    if constexpr (!TN) x[t] = quad_sum(x[t]);
                 }
             }
-        } else if constexpr (DP % 64 == 0) {
-            // FP8: k-block kb is 32 bytes; step s = kb/2 holds k-blocks 2s, 2s+1.
-            // Lane L needs bytes 32*kb + 8*g .. +7 of row L%16: 16-byte chunk
-            // 2*kb + g/2 (lane 4*(L%16) + (2*kb + g/2) % 4 of step kb/2),
-            // dword pair 2*(g%2).
-            constexpr int NS = DP / 64 > 0 ? DP / 64 : 1;
+        } else if constexpr (QLow<DP>::DIAG8) {
+            // FP8: step s holds k-blocks 2s, 2s+1 (64 bytes of every row);
+            // MFMA kb multiplies the low (kb even) or high half of each
+            // lane's 16 bytes
+            constexpr int NS = DP / 64;
             uint4 v[NG][NS];
 #pragma unroll
             for (int t = 0; t < NG; ++t) {
@@ -412,28 +440,24 @@ __device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* 
                     const uint4* r = reinterpret_cast<const uint4*>(reinterpret_cast<const uint8_t*>(rows) + (int64_t)id[t] * DP);
 #pragma unroll
                     for (int st = 0; st < NS; ++st) v[t][st] = r[4 * st + sub];
-                    // norm of tile column L%16: its id sits in lane 4*(L%16)
-                    x[t] = cand_norms ? cand_norms[base + 16 * t + (l & 15)]
-                                      : norms[__builtin_amdgcn_ds_bpermute((4 * (l & 15)) << 2, id[t])];
+                    x[t] = cand_norms ? cand_norms[base + 16 * t + rr] : norms[id[t]];
                 }
             }
-            const bool hi = g & 1;
 #pragma unroll
             for (int t = 0; t < NG; ++t) {
                 if (t == 0 || base + 16 * t < cnt) {
 #pragma unroll
                     for (int kb = 0; kb < KB; ++kb) {
-                        const int src = (4 * (l & 15) + ((2 * kb + (g >> 1)) & 3)) << 2;
                         const uint4 u = v[t][kb >> 1];
-                        const uint32_t lo0 = bperm(src, u.x), lo1 = bperm(src, u.y), hi0 = bperm(src, u.z), hi1 = bperm(src, u.w);
-                        const unsigned long op = ((unsigned long)(hi ? hi1 : lo1) << 32) | (hi ? hi0 : lo0);
+                        const unsigned long op = (kb & 1) ? (((unsigned long)u.w << 32) | u.z)
+                                                          : (((unsigned long)u.y << 32) | u.x);
                         acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(Q.a8[kb], (long)op, acc[t], 0, 0, 0);
                     }
                 }
             }
         } else {
             // FP8 rows shorter than a multiple of 64 bytes: operand-shaped loads
-            const int col = l & 15;
+            const int col = l & 15, g = l >> 4;
             unsigned long v[NG][KB];
             int32_t oid[NG];
 #pragma unroll
@@ -454,13 +478,23 @@ __device__ __forceinline__ void gather_dists_low(const QLow<DP>& Q, const void* 
                         acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(Q.a8[kb], (long)v[t][kb], acc[t], 0, 0, 0);
                 }
             }
-        }
-        if (g == 0) {
-            const float ts = PREC == VMN_PREC_FP8 ? Q.t : 1.0f;
+            // column l % 16 of the result, row 0, in lanes 0..15
+            if (g == 0) {
 #pragma unroll
-            for (int t = 0; t < NG; ++t) {
-                const int c = base + 16 * t + l;
-                if (c < cnt) dist_out[c] = (Q.qn + x[t]) - 2.0f * (acc[t][0] * ts);
+                for (int t = 0; t < NG; ++t) {
+                    const int c = base + 16 * t + l;
+                    if (c < cnt) dist_out[c] = (Q.qn + x[t]) - 2.0f * (acc[t][0] * Q.t);
+                }
+            }
+            continue;
+        }
+        const float ts = PREC == VMN_PREC_FP8 ? Q.t : 1.0f;
+#pragma unroll
+        for (int t = 0; t < NG; ++t) {
+            if (t == 0 || base + 16 * t < cnt) {
+                const float dot = quad_sum(own(acc[t]));
+                const int c = base + 16 * t + rr;
+                if (sub == 0 && c < cnt) dist_out[c] = (Q.qn + x[t]) - 2.0f * (dot * ts);
             }
         }
     }
