@@ -295,18 +295,24 @@ __device__ __forceinline__ void prep_query(QLow<DP>& Q, const float* qf, const f
     const int l = lane(), g = l >> 4;
     const bool dg = diag_lane(l);
     if (PREC == VMN_PREC_BF16) {
+        // every lane converts chunk l % 4 of each k-block (the operand keeps
+        // it in the block-diagonal lanes); a quad's four chunk sums give qn
         float part = 0.0f;
 #pragma unroll
         for (int kb = 0; kb < QLow<DP>::KB; ++kb) {
+            bf16x8_t h8;
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const float hf = (float)to_bf16(qf[32 * kb + 8 * g + e]);
+                const __bf16 h = to_bf16(qf[32 * kb + 8 * (l & 3) + e]);
+                h8[e] = h;
+                const float hf = (float)h;
                 part += hf * hf;
-                Q.a[kb][e] = dg ? to_bf16(qf[32 * kb + 8 * (l & 3) + e]) : to_bf16(0.0f);
             }
+            Q.a[kb] = dg ? h8 : bf16x8_t{};
         }
-        // lanes of one 16-lane group hold the same slice; sum the 4 groups
-        part = sum_rows4(part);
+        // (c0 + c1) + (c2 + c3) over the quad's chunks
+        part += dpp_f<VMN_DPP_XOR1>(part);
+        part += dpp_f<VMN_DPP_XOR2>(part);
         Q.qn = part;
         Q.t = 1.0f;
     } else {
