@@ -662,19 +662,53 @@ hipError_t launch_reverse_scatter(const int32_t* pair_dst, const float* pair_d, 
 
 // getPrunableQueryIDs (vamana/include/kernels.cuh:539-564): wave-ballot
 // compaction of nodes with a non-zero count.
-__global__ void compact_nonzero_kernel(const int32_t* counts, int64_t n, int64_t* list,
-                                       unsigned long long* nlist) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool f = i < n && counts[i] != 0;
-    const uint64_t m = ballot(f);
-    unsigned long long base = 0;
-    if (lane() == 0 && m) base = atomicAdd(nlist, (unsigned long long)__popcll((unsigned long long)m));
-    base = __shfl(base, 0);
-    if (f) list[base + __popcll((unsigned long long)(m & lanemask_lt()))] = i;
+// list[0..nlist) = the indices i with counts[i] != 0, in no particular order
+// (nlist zero on entry). A workgroup of 256 takes a tile of 4,096 entries:
+// it counts them, reserves its range with one atomic, then writes them in
+// tile order. One atomic per wavefront queued them on the counter's L2 line
+// when most wavefronts hold a nonzero entry (10M entries: 1.2 ms).
+#define VMN_COMPACT_CHUNKS 16
+__global__ __launch_bounds__(256) void compact_nonzero_kernel(const int32_t* counts, int64_t n,
+                                                              int64_t* list, unsigned long long* nlist) {
+    __shared__ int s_w[4];
+    __shared__ unsigned long long s_base;
+    const int t = threadIdx.x, w = t >> 6;
+    const int64_t tile = (int64_t)blockIdx.x * (256 * VMN_COMPACT_CHUNKS);
+    int cnt = 0;
+#pragma unroll
+    for (int c = 0; c < VMN_COMPACT_CHUNKS; ++c) {
+        const int64_t i = tile + 256 * c + t;
+        cnt += (i < n && counts[i] != 0) ? 1 : 0;
+    }
+    // block total: wave sums, then thread 0
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if (lane() == 0) s_w[w] = cnt;
+    __syncthreads();
+    if (t == 0) {
+        const int tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        s_base = tot ? atomicAdd(nlist, (unsigned long long)tot) : 0ull;
+    }
+    __syncthreads();
+    unsigned long long base = s_base;
+    for (int c = 0; c < VMN_COMPACT_CHUNKS; ++c) {
+        const int64_t i = tile + 256 * c + t;
+        const bool f = i < n && counts[i] != 0;
+        const uint64_t m = ballot(f);
+        __syncthreads();  // s_w of the previous chunk has been read
+        if (lane() == 0) s_w[w] = __popcll((unsigned long long)m);
+        __syncthreads();
+        unsigned long long off = base;
+        for (int k = 0; k < w; ++k) off += (unsigned long long)s_w[k];
+        if (f) list[off + __popcll((unsigned long long)(m & lanemask_lt()))] = i;
+        base += (unsigned long long)(s_w[0] + s_w[1] + s_w[2] + s_w[3]);
+    }
 }
 hipError_t launch_compact_nonzero(const int32_t* counts, int64_t n, int64_t* list,
                                   unsigned long long* nlist, hipStream_t s) {
-    hipLaunchKernelGGL(compact_nonzero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+    if (n <= 0) return hipSuccess;
+    const int64_t per = 256 * VMN_COMPACT_CHUNKS;
+    hipLaunchKernelGGL(compact_nonzero_kernel, dim3((unsigned)((n + per - 1) / per)), dim3(256), 0, s,
                        counts, n, list, nlist);
     return hipGetLastError();
 }

@@ -51,26 +51,31 @@ hipError_t launch_find_vectors(const float* vecs, int64_t n, int D, int DP, cons
     return hipGetLastError();
 }
 
-// flags[p] = 1 for live nodes with at least one tombstoned out-neighbour.
+// flags[p] = 1 for live nodes with at least one tombstoned out-neighbour;
+// flags must be zero on entry. One lane per adjacency entry, so a wavefront
+// reads 256 contiguous bytes of the rows (one lane walking its own row made
+// each load instruction touch 64 rows: 10M x 64 rows took 11 ms). Lanes that
+// find a tombstone write 1 (every writer stores the same value).
 __global__ void cons_flags_kernel(const int32_t* adj, int64_t n, int W, const uint32_t* del_bits,
                                   int32_t* flags) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    int32_t f = 0;
-    if (!bit_test(del_bits, (uint32_t)p)) {
-        for (int j = 0; j < W; ++j) {
-            const int32_t v = adj[p * W + j];
-            if (v >= 0 && bit_test(del_bits, (uint32_t)v)) {
-                f = 1;
-                break;
-            }
-        }
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * W) return;
+    const int32_t v = adj[i];
+    if (v >= 0 && bit_test(del_bits, (uint32_t)v)) {
+        const int64_t p = i / W;
+        if (!bit_test(del_bits, (uint32_t)p)) flags[p] = 1;
     }
-    flags[p] = f;
+}
+__global__ void zero_i32_kernel(int32_t* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 hipError_t launch_cons_flags(const int32_t* adj, int64_t n, int W, const uint32_t* del_bits,
                              int32_t* flags, hipStream_t s) {
-    hipLaunchKernelGGL(cons_flags_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, adj,
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, flags, n);
+    const int64_t ne = n * W;
+    hipLaunchKernelGGL(cons_flags_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, adj,
                        n, W, del_bits, flags);
     return hipGetLastError();
 }
