@@ -734,6 +734,12 @@ def serving(args, vmn, torch, dev):
         kinds.append(vmn.REQ_SEARCH if r < 0.9 else (vmn.REQ_INSERT if r < 0.95 else vmn.REQ_DELETE))
         sizes.append(int(np.exp(rng.uniform(0, np.log(256)))))
         t_arr.append(tcur)
+    # launch shape (tile shape, waves per CU) tuned on a batch of the trace's
+    # typical size: 64 queries, one batch at a time
+    tune = vmn.tuning.launch_shape(idx, torch.from_numpy(pool[:64]).to(dev), k=10, L=args.serve_L,
+                                   prec="bf16", rerank=40, reps=20)
+    log(f"serving: launch tuned on 64-query batches: {tune['waves_per_cu']} waves/CU, "
+        f"{tune['tiles']} tiles, {tune['ms']:.3f} ms")
     srv = vmn.Server(idx, k=10, L=args.serve_L, prec="bf16", rerank=40, max_batch=8192, n_slots=4,
                      consolidate_every=args.serve_consolidate)
     # warm-up, untimed: searches, and inserts / deletes at the largest request
@@ -862,6 +868,7 @@ def serving(args, vmn, torch, dev):
                       "p90_ms": round(float(np.percentile(lat_ms, 90)), 3),
                       "p999_ms": round(float(np.percentile(lat_ms, 99.9)), 3),
                       "update_latency": upd, "engine": engine,
+                      "launch": {k: tune[k] for k in ("waves_per_cu", "list_cap", "tiles")},
                       "client_submit_lag_ms": {
                           "p99": round(float(np.percentile(lags, 99)) * 1e3, 3),
                           "max": round(float(np.max(lags)) * 1e3, 3)},
