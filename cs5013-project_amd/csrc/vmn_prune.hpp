@@ -336,8 +336,145 @@ __device__ void occlude_mfma(const PruneRows& rows, const int32_t* members, int 
     }
 }
 
-// robust_prune over FP32 rows, or robust_prune_mfma when the rows are exact in
-// BF16 (same output). SEQ: FP32 distances as sequential chains (PruneQ).
+// robust_prune over FP32 rows with the next NB kept candidates taken as
+// pivots per pass over the active rows (the FP32 counterpart of
+// robust_prune_mfma's 16-row Gram). Each row group holds the NB pivot rows in
+// registers (QReg layout) and loads two active rows per round trip; every
+// pivot-row distance is gather_dists's sum, in its order (8 lane partials,
+// reduce8), so each occlusion decision is the one-pivot-at-a-time loop's.
+// The decisions become an NB-bit mask per active candidate (bit j: pivot j,
+// which is p0 or active entry j - 1, occludes it and comes before it), and
+// the pivots are then taken in list order as robust_prune takes them: one
+// still kept at its turn is a p*, is output, and drops what its bit marks;
+// one an earlier p* dropped is skipped. The next pass scans from after the
+// last pivot. One gather of the active rows serves up to NB rounds.
+template <int DP, bool STRICT, int CCAP>
+__device__ int robust_prune_batch(const PruneRows& rows, const uint32_t* c_id, const float* c_d,
+                                  int cnt, float alpha, int R, uint8_t* s_kept,
+                                  int32_t* s_act_id, int32_t* s_act_pos, uint32_t* s_pm,
+                                  int32_t* out_ids, float* out_d, int nk0 = 0,
+                                  bool init_kept = true, int* nread = nullptr) {
+    constexpr int NB = DP <= 128 ? 4 : 2;
+    constexpr int NCH = (CCAP + 63) / 64;
+    constexpr int T = DP / 32;
+    const int l = lane(), sub = l & 7, grp = l >> 3;
+    if (init_kept) {
+        for (int i = l; i < cnt; i += 64) s_kept[i] = 1;
+    }
+    __syncthreads();
+    int nk = nk0, from = 0;
+    while (nk < R) {
+        // kept candidates from `from` on: the first is p0, the others the active list
+        int nall = 0, p0 = -1;
+        for (int b = from; b < cnt; b += 64) {
+            const int j = b + l;
+            const bool f = j < cnt && s_kept[j];
+            const uint64_t m = ballot(f);
+            const int r = nall + __popcll((unsigned long long)(m & lanemask_lt()));
+            if (p0 < 0 && m) p0 = b + __ffsll((unsigned long long)m) - 1;
+            if (f && r > 0) {
+                s_act_id[r - 1] = (int32_t)(c_id[j] & VMN_ID_MASK);
+                s_act_pos[r - 1] = j;
+            }
+            nall += __popcll((unsigned long long)m);
+        }
+        if (nall == 0) break;
+        const int nact = nall - 1;
+        const int ns = nall < NB ? nall : NB;
+        const int32_t id0 = (int32_t)(c_id[p0] & VMN_ID_MASK);
+        __syncthreads();
+        if (nact > 0) {
+            if (nread) *nread += ns + nact;
+            QReg<DP> pv[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const int32_t pid = j == 0 ? id0 : (j - 1 < nact ? s_act_id[j - 1] : id0);
+                if (j < ns) pv[j].load(rows.f32 + (int64_t)pid * DP, sub);
+            }
+            for (int base = 0; base < nact; base += 16) {
+                const int r0 = base + grp, r1 = base + 8 + grp;
+                float4 x0[T], x1[T];
+                const float4* row0 = reinterpret_cast<const float4*>(rows.f32 + (int64_t)s_act_id[r0 < nact ? r0 : 0] * DP);
+                const float4* row1 = reinterpret_cast<const float4*>(rows.f32 + (int64_t)s_act_id[r1 < nact ? r1 : 0] * DP);
+                const bool two = base + 8 < nact;
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    x0[t] = row0[sub + 8 * t];
+                    if (two) x1[t] = row1[sub + 8 * t];
+                }
+                uint32_t b0 = 0u, b1 = 0u;
+                const float dn0 = r0 < nact ? c_d[s_act_pos[r0]] : 0.0f;
+                const float dn1 = r1 < nact ? c_d[s_act_pos[r1]] : 0.0f;
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    if (j >= ns) continue;
+                    float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+                    for (int t = 0; t < T; ++t) {
+                        float a = x0[t].x - pv[j].v[t].x, b = x0[t].y - pv[j].v[t].y;
+                        float c = x0[t].z - pv[j].v[t].z, d = x0[t].w - pv[j].v[t].w;
+                        s0 += a * a; s0 += b * b; s0 += c * c; s0 += d * d;
+                        if (two) {
+                            a = x1[t].x - pv[j].v[t].x; b = x1[t].y - pv[j].v[t].y;
+                            c = x1[t].z - pv[j].v[t].z; d = x1[t].w - pv[j].v[t].w;
+                            s1 += a * a; s1 += b * b; s1 += c * c; s1 += d * d;
+                        }
+                    }
+                    s0 = reduce8(s0);
+                    s1 = reduce8(s1);
+                    const float l0 = alpha * s0, l1 = alpha * s1;
+                    const bool o0 = STRICT ? (l0 < dn0) : (l0 <= dn0);
+                    const bool o1 = STRICT ? (l1 < dn1) : (l1 <= dn1);
+                    // pivot j >= 1 is active entry j - 1: it drops only entries after it
+                    b0 |= (uint32_t)(o0 && r0 >= j) << j;
+                    b1 |= (uint32_t)(o1 && r1 >= j) << j;
+                }
+                if (sub == 0 && r0 < nact) s_pm[r0] = b0;
+                if (sub == 0 && two && r1 < nact) s_pm[r1] = b1;
+            }
+        }
+        __syncthreads();
+        uint32_t pm[NCH];
+        uint64_t keep[NCH];
+#pragma unroll
+        for (int m = 0; m < NCH; ++m) {
+            const int k = 64 * m + l;
+            pm[m] = k < nact ? s_pm[k] : 0u;
+            keep[m] = ballot(k < nact);
+        }
+        bool done = false;
+        for (int j = 0; j < ns; ++j) {
+            // pivot j >= 1 is active entry j - 1: still kept?
+            if (j > 0 && !((keep[0] >> (j - 1)) & 1ull)) continue;
+            if (l == 0) {
+                out_ids[nk] = j == 0 ? id0 : s_act_id[j - 1];
+                if (out_d) out_d[nk] = c_d[j == 0 ? p0 : s_act_pos[j - 1]];
+            }
+            ++nk;
+            if (nk >= R) {
+                done = true;
+                break;
+            }
+#pragma unroll
+            for (int m = 0; m < NCH; ++m) keep[m] &= ~ballot((pm[m] >> j) & 1u);
+        }
+        if (done) break;
+        // drops back to the list; the next scan starts after the last pivot
+#pragma unroll
+        for (int m = 0; m < NCH; ++m) {
+            const int k = 64 * m + l;
+            if (k < nact && !((keep[m] >> l) & 1ull)) s_kept[s_act_pos[k]] = 0;
+        }
+        from = (ns == 1 ? p0 : s_act_pos[ns - 2]) + 1;
+        __syncthreads();
+    }
+    __syncthreads();
+    return nk;
+}
+
+// robust_prune_batch over FP32 rows, or robust_prune_mfma when the rows are
+// exact in BF16 (same output). SEQ: FP32 distances as sequential chains, one
+// p* per round (robust_prune with PruneQ).
 template <int DP, bool STRICT, bool BF, int CCAP, bool SEQ = false>
 __device__ __forceinline__ int prune_list(const PruneRows& rows, const uint32_t* c_id,
                                           const float* c_d, int cnt, float alpha, int R,
@@ -349,6 +486,11 @@ __device__ __forceinline__ int prune_list(const PruneRows& rows, const uint32_t*
                                                    s_act_id, s_act_pos,
                                                    reinterpret_cast<uint32_t*>(s_act_d), out_ids,
                                                    out_d, 0, true, nread);
+    else if constexpr (!SEQ)
+        return robust_prune_batch<DP, STRICT, CCAP>(rows, c_id, c_d, cnt, alpha, R, s_kept,
+                                                    s_act_id, s_act_pos,
+                                                    reinterpret_cast<uint32_t*>(s_act_d), out_ids,
+                                                    out_d, 0, true, nread);
     else
         return robust_prune<DP, STRICT, SEQ>(rows, c_id, c_d, cnt, alpha, R, s_kept, s_act_id,
                                              s_act_pos, s_act_d, out_ids, out_d, nread);
