@@ -472,6 +472,82 @@ __device__ int robust_prune_batch(const PruneRows& rows, const uint32_t* c_id, c
     return nk;
 }
 
+// occlude_mfma over FP32 rows: every node of members[0..nm) drops the kept
+// candidates it occludes (alpha * d(member, c) <= d(node, c)), NB members per
+// pass over the kept candidates' rows (two rows per row group and round
+// trip), each distance gather_dists's sum in its order. The members all
+// apply, in any order, so a candidate stays exactly when no member occludes
+// it.
+template <int DP>
+__device__ void occlude_fp32(const PruneRows& rows, const int32_t* members, int nm,
+                             const uint32_t* c_id, const float* c_d, int cnt, float alpha,
+                             uint8_t* s_kept, int32_t* s_act_id, int32_t* s_act_pos,
+                             int* nread = nullptr) {
+    constexpr int NB = DP <= 128 ? 4 : 2;
+    constexpr int T = DP / 32;
+    const int l = lane(), sub = l & 7, grp = l >> 3;
+    for (int g0 = 0; g0 < nm; g0 += NB) {
+        int nact = 0;
+        for (int b = 0; b < cnt; b += 64) {
+            const int j = b + l;
+            const bool f = j < cnt && s_kept[j];
+            const uint64_t m = ballot(f);
+            const int k = nact + __popcll((unsigned long long)(m & lanemask_lt()));
+            if (f) {
+                s_act_id[k] = (int32_t)(c_id[j] & VMN_ID_MASK);
+                s_act_pos[k] = j;
+            }
+            nact += __popcll((unsigned long long)m);
+        }
+        if (nact == 0) break;
+        __syncthreads();
+        const int nrows = nm - g0 < NB ? nm - g0 : NB;
+        if (nread) *nread += nrows + nact;
+        QReg<DP> pv[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            if (j < nrows) pv[j].load(rows.f32 + (int64_t)members[g0 + j] * DP, sub);
+        for (int base = 0; base < nact; base += 16) {
+            const int r0 = base + grp, r1 = base + 8 + grp;
+            float4 x0[T], x1[T];
+            const float4* row0 = reinterpret_cast<const float4*>(rows.f32 + (int64_t)s_act_id[r0 < nact ? r0 : 0] * DP);
+            const float4* row1 = reinterpret_cast<const float4*>(rows.f32 + (int64_t)s_act_id[r1 < nact ? r1 : 0] * DP);
+            const bool two = base + 8 < nact;
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                x0[t] = row0[sub + 8 * t];
+                if (two) x1[t] = row1[sub + 8 * t];
+            }
+            const float dn0 = r0 < nact ? c_d[s_act_pos[r0]] : 0.0f;
+            const float dn1 = r1 < nact ? c_d[s_act_pos[r1]] : 0.0f;
+            bool o0 = false, o1 = false;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                if (j >= nrows) continue;
+                float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    float a = x0[t].x - pv[j].v[t].x, b = x0[t].y - pv[j].v[t].y;
+                    float c = x0[t].z - pv[j].v[t].z, d = x0[t].w - pv[j].v[t].w;
+                    s0 += a * a; s0 += b * b; s0 += c * c; s0 += d * d;
+                    if (two) {
+                        a = x1[t].x - pv[j].v[t].x; b = x1[t].y - pv[j].v[t].y;
+                        c = x1[t].z - pv[j].v[t].z; d = x1[t].w - pv[j].v[t].w;
+                        s1 += a * a; s1 += b * b; s1 += c * c; s1 += d * d;
+                    }
+                }
+                s0 = reduce8(s0);
+                s1 = reduce8(s1);
+                o0 |= alpha * s0 <= dn0;
+                o1 |= alpha * s1 <= dn1;
+            }
+            if (sub == 0 && r0 < nact && o0) s_kept[s_act_pos[r0]] = 0;
+            if (sub == 0 && two && r1 < nact && o1) s_kept[s_act_pos[r1]] = 0;
+        }
+        __syncthreads();
+    }
+}
+
 // robust_prune_batch over FP32 rows, or robust_prune_mfma when the rows are
 // exact in BF16 (same output). SEQ: FP32 distances as sequential chains, one
 // p* per round (robust_prune with PruneQ).

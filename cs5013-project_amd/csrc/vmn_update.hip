@@ -160,72 +160,13 @@ __global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
                                                     reinterpret_cast<uint32_t*>(act_d), newrow,
                                                     nullptr, nK, false);
         } else {
-            for (int t = 0; t < nK && cnt > 0; ++t) {
-                int nact = 0;
-                for (int b = 0; b < cnt; b += 64) {
-                    const int j = b + l;
-                    const bool f = j < cnt && kept[j];
-                    const uint64_t m = ballot(f);
-                    const int k = nact + __popcll((unsigned long long)(m & lanemask_lt()));
-                    if (f) {
-                        act_id[k] = (int32_t)(c_id[cur][j] & VMN_ID_MASK);
-                        act_pos[k] = j;
-                    }
-                    nact += __popcll((unsigned long long)m);
-                }
-                if (nact == 0) break;
-                QReg<DP> kq;
-                kq.load(a.vecs + (int64_t)k_id[t] * DP, sub);
-                __syncthreads();
-                gather_dists<DP>(kq, a.vecs, act_id, act_d, nact);
-                __syncthreads();
-                for (int k = l; k < nact; k += 64) {
-                    const int j = act_pos[k];
-                    if (a.alpha * act_d[k] <= c_d[cur][j]) kept[j] = 0;
-                }
-                __syncthreads();
-            }
+            occlude_fp32<DP>(rows, k_id, nK, c_id[cur], c_d[cur], cnt, a.alpha, kept, act_id,
+                             act_pos);
             // then the remaining candidates in order, as robust prune takes them
-            int from = 0;
-            while (nk < a.R) {
-                int pi = -1;
-                for (int b = from; b < cnt; b += 64) {
-                    const int j = b + l;
-                    const uint64_t m = ballot(j < cnt && kept[j]);
-                    if (m) {
-                        pi = b + __ffsll((unsigned long long)m) - 1;
-                        break;
-                    }
-                }
-                if (pi < 0) break;
-                const int32_t ps = (int32_t)(c_id[cur][pi] & VMN_ID_MASK);
-                if (l == 0) newrow[nk] = ps;
-                ++nk;
-                if (nk >= a.R) break;
-                int nact = 0;
-                for (int b = pi + 1; b < cnt; b += 64) {
-                    const int j = b + l;
-                    const bool f = j < cnt && kept[j];
-                    const uint64_t m = ballot(f);
-                    const int k = nact + __popcll((unsigned long long)(m & lanemask_lt()));
-                    if (f) {
-                        act_id[k] = (int32_t)(c_id[cur][j] & VMN_ID_MASK);
-                        act_pos[k] = j;
-                    }
-                    nact += __popcll((unsigned long long)m);
-                }
-                QReg<DP> pq;
-                pq.load(a.vecs + (int64_t)ps * DP, sub);
-                __syncthreads();
-                gather_dists<DP>(pq, a.vecs, act_id, act_d, nact);
-                __syncthreads();
-                for (int k = l; k < nact; k += 64) {
-                    const int j = act_pos[k];
-                    if (a.alpha * act_d[k] <= c_d[cur][j]) kept[j] = 0;
-                }
-                __syncthreads();
-                from = pi + 1;
-            }
+            nk = robust_prune_batch<DP, false, CCAP>(rows, c_id[cur], c_d[cur], cnt, a.alpha, a.R,
+                                                     kept, act_id, act_pos,
+                                                     reinterpret_cast<uint32_t*>(act_d), newrow,
+                                                     nullptr, nK, false);
         }
         __syncthreads();
         for (int j = l; j < W; j += 64) a.adj[p * W + j] = j < nk ? newrow[j] : -1;

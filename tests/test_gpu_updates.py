@@ -211,3 +211,26 @@ def test_gpu_heavy_delete_consolidation_tiers(vmn, orc, gpu):
     idx.consolidate()
     orc.consolidate(v, adj, dele, alpha=1.2)
     assert np.array_equal(idx.adjacency, adj)
+
+
+@pytest.mark.parametrize("frac", [0.15, 0.6])
+def test_gpu_consolidation_on_fp32_rows_matches_oracle(vmn, orc, gpu, frac):
+    """Rows that are not exact in BF16 (half-integers up to 255.5) take the
+    FP32 occlusion checks and fill, several members or pivots per pass over
+    the candidates' rows. Their differences are integers, so every FP32 sum
+    is exact in any order and the oracle fixes the graph. At 60% deleted most
+    rows have more than 8 tombstoned neighbours, so the full-pool tier runs
+    too."""
+    v = _data(vmn, 4000, seed=11) + np.float32(0.5)
+    idx = vmn.Vamana(v, mode="fresh", R=32, L_build=64, alphas=(1.0, 1.2))
+    assert idx.build_precision == "fp32"
+    adj, med = orc.fresh_build(v, R=32, L=64, alphas=(1.0, 1.2))
+    assert np.array_equal(idx.adjacency, adj)
+    kill = np.random.default_rng(12).choice(4000, int(4000 * frac), replace=False).astype(np.int32)
+    dele = np.zeros(4000, np.uint8)
+    dele[kill] = 1
+    idx.deleteIds(kill)
+    idx.consolidate()
+    orc.consolidate(v, adj, dele, alpha=1.2)
+    assert np.array_equal(idx.adjacency, adj)
+    assert idx.medoid == orc.medoid_live(v, dele)
