@@ -1332,8 +1332,10 @@ int vmn_consolidate(vmn_index* idx) {
     const int R = idx->p.R;
     HIPCHK(idx->rcount.reserve((size_t)N * sizeof(int32_t)));
     HIPCHK(idx->rlist.reserve((size_t)N * sizeof(int64_t)));
+    // the node list's buffer holds the tombstone summary until the list is made
     HIPCHK(vmn::launch_cons_flags(idx->adj.as<int32_t>(), N, R, idx->del_bits.as<uint32_t>(),
-                                  idx->rcount.as<int32_t>(), idx->stream));
+                                  idx->rcount.as<int32_t>(), idx->rlist.as<uint32_t>(), idx->cus,
+                                  idx->stream));
     rc = zero_counter(idx);
     if (rc) return rc;
     HIPCHK(vmn::launch_compact_nonzero(idx->rcount.as<int32_t>(), N, idx->rlist.as<int64_t>(),

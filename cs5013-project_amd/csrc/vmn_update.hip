@@ -55,28 +55,59 @@ hipError_t launch_find_vectors(const float* vecs, int64_t n, int D, int DP, cons
 // flags must be zero on entry. One lane per adjacency entry, so a wavefront
 // reads 256 contiguous bytes of the rows (one lane walking its own row made
 // each load instruction touch 64 rows: 10M x 64 rows took 11 ms). Lanes that
-// find a tombstone write 1 (every writer stores the same value).
-__global__ void cons_flags_kernel(const int32_t* adj, int64_t n, int W, const uint32_t* del_bits,
-                                  int32_t* flags) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n * W) return;
-    const int32_t v = adj[i];
-    if (v >= 0 && bit_test(del_bits, (uint32_t)v)) {
-        const int64_t p = i / W;
-        if (!bit_test(del_bits, (uint32_t)p)) flags[p] = 1;
+// find a tombstone write 1 (every writer stores the same value). The
+// tombstone test first reads a summary bit per 64 nodes held in LDS (set when
+// any of the 64 is tombstoned; 19.5 KB at 10M nodes), so with few tombstones
+// almost no entry reads the full bitmap; `summary` null: every entry reads it.
+__global__ void del_summary_kernel(const uint32_t* del_bits, int64_t nwords, uint32_t* summary) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // 64-node block
+    if (2 * b >= nwords) return;
+    const uint32_t w = del_bits[2 * b] | (2 * b + 1 < nwords ? del_bits[2 * b + 1] : 0u);
+    if (w) atomicOr(&summary[b >> 5], 1u << (b & 31));
+}
+__global__ __launch_bounds__(256) void cons_flags_kernel(const int32_t* adj, int64_t n, int W,
+                                                         const uint32_t* del_bits,
+                                                         const uint32_t* summary, int nsum,
+                                                         int32_t* flags) {
+    extern __shared__ uint32_t s_sum[];
+    for (int i = threadIdx.x; i < nsum; i += blockDim.x) s_sum[i] = summary[i];
+    __syncthreads();
+    const int64_t ne = n * W, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += stride) {
+        const int32_t v = adj[i];
+        if (v < 0) continue;
+        const uint32_t blk = (uint32_t)v >> 6;
+        if (summary && !((s_sum[blk >> 5] >> (blk & 31)) & 1u)) continue;
+        if (bit_test(del_bits, (uint32_t)v)) {
+            const int64_t p = i / W;
+            if (!bit_test(del_bits, (uint32_t)p)) flags[p] = 1;
+        }
     }
 }
 __global__ void zero_i32_kernel(int32_t* p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }
+// summary: scratch of (n + 2047) / 2048 words, or null
 hipError_t launch_cons_flags(const int32_t* adj, int64_t n, int W, const uint32_t* del_bits,
-                             int32_t* flags, hipStream_t s) {
+                             int32_t* flags, uint32_t* summary, int cus, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(zero_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, flags, n);
+    // the summary is read into 48 KB of LDS at most (25M nodes)
+    const int64_t nblk = (n + 63) / 64, nsum64 = (nblk + 31) / 32;
+    const int nsum = summary && nsum64 <= 12288 ? (int)nsum64 : 0;
+    if (nsum) {
+        hipLaunchKernelGGL(zero_i32_kernel, dim3((unsigned)((nsum + 255) / 256)), dim3(256), 0, s,
+                           reinterpret_cast<int32_t*>(summary), (int64_t)nsum);
+        const int64_t nwords = (n + 31) / 32;
+        hipLaunchKernelGGL(del_summary_kernel, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, s,
+                           del_bits, nwords, summary);
+    }
     const int64_t ne = n * W;
-    hipLaunchKernelGGL(cons_flags_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, adj,
-                       n, W, del_bits, flags);
+    const int64_t want = (ne + 255) / 256, cap = (int64_t)(cus > 0 ? cus : 256) * 8;
+    const unsigned grid = (unsigned)(want < cap ? want : cap);
+    hipLaunchKernelGGL(cons_flags_kernel, dim3(grid), dim3(256), (size_t)nsum * 4, s, adj, n, W,
+                       del_bits, nsum ? summary : nullptr, nsum, flags);
     return hipGetLastError();
 }
 

@@ -30,7 +30,7 @@ hipError_t launch_tombstone(const int32_t* ids, int64_t m, int64_t n, uint32_t* 
 hipError_t launch_find_vectors(const float* vecs, int64_t n, int D, int DP, const uint32_t* del_bits,
                                const float* qv, int64_t nq, uint32_t* out, hipStream_t s);
 hipError_t launch_cons_flags(const int32_t* adj, int64_t n, int W, const uint32_t* del_bits,
-                             int32_t* flags, hipStream_t s);
+                             int32_t* flags, uint32_t* summary, int cus, hipStream_t s);
 hipError_t launch_consolidate(const ConsArgs& a, int grid, hipStream_t s);
 hipError_t launch_clear_deleted_rows(int32_t* adj, int64_t n, int W, const uint32_t* del_bits,
                                      hipStream_t s);
