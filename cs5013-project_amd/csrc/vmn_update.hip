@@ -233,14 +233,31 @@ hipError_t launch_consolidate(const ConsArgs& a, int grid, hipStream_t s) {
 }
 
 // Rows of tombstoned nodes are emptied once consolidation no longer reads them.
+// Rows of tombstoned nodes become empty: one thread per 32-node word of the
+// tombstone bitmap writes -1 over the rows of its set bits (only those rows
+// are written; one thread per adjacency entry took 0.7 ms at 10M x 64).
 __global__ void clear_deleted_rows_kernel(int32_t* adj, int64_t n, int W, const uint32_t* del_bits) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n * W) return;
-    if (bit_test(del_bits, (uint32_t)(i / W))) adj[i] = -1;
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w * 32 >= n) return;
+    uint32_t bits = del_bits[w];
+    while (bits) {
+        const int b = __ffs(bits) - 1;
+        bits &= bits - 1;
+        const int64_t node = w * 32 + b;
+        if (node >= n) break;
+        int32_t* row = adj + node * W;
+        if ((W & 3) == 0) {
+            for (int j = 0; j < W; j += 4) *reinterpret_cast<int4*>(row + j) = make_int4(-1, -1, -1, -1);
+        } else {
+            for (int j = 0; j < W; ++j) row[j] = -1;
+        }
+    }
 }
 hipError_t launch_clear_deleted_rows(int32_t* adj, int64_t n, int W, const uint32_t* del_bits,
                                      hipStream_t s) {
-    hipLaunchKernelGGL(clear_deleted_rows_kernel, dim3((unsigned)((n * W + 255) / 256)), dim3(256),
+    if (n <= 0) return hipSuccess;
+    const int64_t nwords = (n + 31) / 32;
+    hipLaunchKernelGGL(clear_deleted_rows_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256),
                        0, s, adj, n, W, del_bits);
     return hipGetLastError();
 }
