@@ -72,15 +72,24 @@ __global__ __launch_bounds__(256) void cons_flags_kernel(const int32_t* adj, int
     extern __shared__ uint32_t s_sum[];
     for (int i = threadIdx.x; i < nsum; i += blockDim.x) s_sum[i] = summary[i];
     __syncthreads();
+    // four entries per lane and step, their loads issued together
     const int64_t ne = n * W, stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += stride) {
-        const int32_t v = adj[i];
-        if (v < 0) continue;
-        const uint32_t blk = (uint32_t)v >> 6;
-        if (summary && !((s_sum[blk >> 5] >> (blk & 31)) & 1u)) continue;
-        if (bit_test(del_bits, (uint32_t)v)) {
-            const int64_t p = i / W;
-            if (!bit_test(del_bits, (uint32_t)p)) flags[p] = 1;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < ne; i0 += 4 * stride) {
+        int32_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t i = i0 + u * stride;
+            v[u] = i < ne ? adj[i] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (v[u] < 0) continue;
+            const uint32_t blk = (uint32_t)v[u] >> 6;
+            if (summary && !((s_sum[blk >> 5] >> (blk & 31)) & 1u)) continue;
+            if (bit_test(del_bits, (uint32_t)v[u])) {
+                const int64_t p = (i0 + u * stride) / W;
+                if (!bit_test(del_bits, (uint32_t)p)) flags[p] = 1;
+            }
         }
     }
 }
