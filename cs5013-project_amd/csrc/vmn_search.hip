@@ -324,7 +324,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(
                 const uint32_t h1 = nh & 0xffffu, h2 = nh >> 16;
                 bool seen = true;
                 // both cells read at once (no short circuit)
-                if (valid) seen = bit_test(s_bloom, h1) & bit_test(s_bloom, h2);
+                if (valid) seen = ((int)bit_test(s_bloom, h1) & (int)bit_test(s_bloom, h2)) != 0;
                 __syncthreads();
                 valid = valid && !seen;
                 if (valid) {

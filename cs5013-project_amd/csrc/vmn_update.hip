@@ -145,7 +145,7 @@ __global__ __launch_bounds__(64) void consolidate_kernel(ConsArgs a) {
     __shared__ __align__(16) int32_t k_id[64];
     __shared__ int32_t newrow[64];
     __shared__ int64_t s_q;
-    const int l = threadIdx.x, sub = l & 7;
+    const int l = threadIdx.x;
     const int W = a.W;
     for (;;) {
         const int64_t it = next_item(a.counter, &s_q);
