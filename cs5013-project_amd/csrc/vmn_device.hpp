@@ -95,16 +95,6 @@ __device__ __forceinline__ float reduce8(float v) {
     return v;
 }
 
-// (v_l + v_{l^16}) + (v_{l^32} + v_{l^48}) in every lane l, each sum taken
-// over the pair of 16-lane rows (then halves) that v_permlane16/32_swap put
-// side by side (an addition's operand order does not change its result).
-__device__ __forceinline__ float sum_rows4(float v) {
-    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
-    v = __uint_as_float(p[0]) + __uint_as_float(p[1]);
-    const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
-    return __uint_as_float(q[0]) + __uint_as_float(q[1]);
-}
-
 // Query (or pivot) vector held in registers: lane sub (0..7) holds float4
 // indices sub, sub+8, sub+16, ... of the padded row.
 template <int DP>
